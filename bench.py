@@ -1,0 +1,190 @@
+"""Benchmark: Msamples/s of the MI355X wavefront path tracer on the Disney-grid
+configuration (8x8 level-6 icospheres = 5,242,882 triangles on a ground quad,
+2048x1024 sky+sun envmap, 512x512, 256 spp, max depth 12), plain path_mis.
+
+A step renders spp_per_step samples per pixel of the view (sample offsets
+advance from step to step, so K steps of 32 spp form the full 256-spp render).
+Inputs are resident in HBM before the timed region; the timed region is K
+render calls, each ending in a stream synchronisation.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 is not supported: this
+build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pbrt-renderer-_amd"))
+
+METRIC = "Msamples/s; time-to-relMSE≤1e-3, guided vs plain, on 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES = 80, 48
+# per extension ray besides the traversal: ray read 32 B, hit write 16 B, queue index 4 B
+EXT_IO_BYTES = 32 + 16 + 4
+# per shadow ray besides the traversal: origin/direction read 32 B
+SHADOW_IO_BYTES = 32
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, cam, spp, depth):
+    """The CPU oracle (path_mis restated on the CPU, Nori BVH2) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    o = orc.OracleScene(scene)
+    log(f"[bench] oracle scene (Nori BVH2) built in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    _, stats = o.render(cam, spp, depth, seed=1, nthreads=threads)
+    dt = time.time() - t0
+    return {"value": float(stats[0]) / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{cam.width}x{cam.height} at {spp} spp (first {spp} samples of every pixel), "
+                      f"{int(stats[0])} samples in {dt:.2f}s"}
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the closest-hit kernel from the committed rocprof counter summary."""
+    path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp-per-step", type=int, default=32)
+    ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--subdiv", type=int, default=6)
+    ap.add_argument("--batch", type=int, default=1 << 21)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-spp", type=int, default=32)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (use --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import nori_gpu as ng
+
+    t0 = time.time()
+    scene = ng.generate_scene(2, subdiv=args.subdiv)
+    cam = scene.camera
+    log(f"[bench] scene: {len(scene.indices)} triangles, envmap {scene.env_rgb.shape[1]}x{scene.env_rgb.shape[0]}, "
+        f"{time.time() - t0:.1f}s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, cam, args.cpu_spp, args.depth)
+        log(f"[bench] cpu baseline {cpu['value']:.3f} Msamples/s on {cpu['cores']} threads")
+
+    gpu = ng.NoriGPU(0)
+    t0 = time.time()
+    gpu.upload(scene)
+    info = gpu.bvh_info()
+    log(f"[bench] upload: BVH8 {info.nodes} nodes depth {info.max_depth} ({info.node_bytes / 1e6:.1f} MB nodes, "
+        f"{info.tri_bytes / 1e6:.1f} MB triangles), build {info.build_ms:.0f} ms, total {time.time() - t0:.1f}s")
+    gpu.set_option("batch", args.batch)
+    spp = args.spp_per_step
+
+    # Traversal counts of this workload (instrumented kernels, untimed): average
+    # BVH8 nodes and triangles per extension and per shadow ray.
+    gpu.set_option("count_traversal", 1)
+    gpu.reset_film()
+    gpu.render(ng.make_view(cam, 4, args.depth, sample_offset=10 ** 6), seed=99, readback=False)
+    cs = gpu.stats()
+    gpu.set_option("count_traversal", 0)
+    ext_nodes = cs.ext_nodes / max(cs.extension_rays, 1)
+    ext_tris = cs.ext_tris / max(cs.extension_rays, 1)
+    sh_nodes = cs.shadow_nodes / max(cs.shadow_rays, 1)
+    sh_tris = cs.shadow_tris / max(cs.shadow_rays, 1)
+    log(f"[bench] per extension ray {ext_nodes:.2f} nodes {ext_tris:.2f} tris; per shadow ray {sh_nodes:.2f} nodes "
+        f"{sh_tris:.2f} tris")
+
+    gpu.set_option("timing_mode", 1)
+    gpu.reset_film()
+    for w in range(args.warmup):
+        gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=(10 ** 6) + 64 + w * spp), seed=7,
+                   readback=False)
+        log(f"[bench] warmup step {w + 1}/{args.warmup} done")
+    gpu.reset_film()
+
+    acc = {"samples": 0, "ext": 0, "sh": 0, "ms_ext": 0.0, "ms_sh": 0.0, "ms_shade": 0.0, "launch_ext": 0,
+           "ms_gpu": 0.0}
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=k * spp), seed=7, readback=False)
+        s = gpu.stats()
+        acc["samples"] += s.samples
+        acc["ext"] += s.extension_rays
+        acc["sh"] += s.shadow_rays
+        acc["ms_ext"] += s.ms_trace_closest
+        acc["ms_sh"] += s.ms_trace_shadow
+        acc["ms_shade"] += s.ms_shade
+        acc["launch_ext"] += s.launches_closest
+        acc["ms_gpu"] += s.ms_total
+        log(f"[bench] step {k + 1}/{args.steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s "
+            f"(extend {s.ms_trace_closest:.1f} ms, shadow {s.ms_trace_shadow:.1f} ms, shade {s.ms_shade:.1f} ms)")
+    wall = time.perf_counter() - t0
+
+    value = acc["samples"] / wall / 1e6
+    ext_bytes = acc["ext"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
+    achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
+    traffic = traffic_from_profiles()
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Disney icosphere grid 8x8, level {args.subdiv}, {len(scene.indices)} triangles, "
+                        f"2048x1024 sky+sun envmap, {cam.width}x{cam.height}, {args.steps * spp} spp, "
+                        f"max depth {args.depth}, plain path_mis",
+            "spp_per_step": spp,
+            "batch_paths": args.batch,
+        },
+        "mrays_per_s": round((acc["ext"] + acc["sh"]) / wall / 1e6, 2),
+        "gpu_ms": {"extend": round(acc["ms_ext"], 2), "shadow": round(acc["ms_sh"], 2),
+                   "shade_raygen": round(acc["ms_shade"], 2), "render_total": round(acc["ms_gpu"], 2)},
+        "traversal": {"ext_nodes_per_ray": round(ext_nodes, 3), "ext_tris_per_ray": round(ext_tris, 3),
+                      "shadow_nodes_per_ray": round(sh_nodes, 3), "shadow_tris_per_ray": round(sh_tris, 3)},
+        "roofline": {
+            "kernel": "k_extend (BVH8 closest-hit traversal)",
+            "bound": "hbm",
+            "achieved": None if achieved is None else round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": round(ext_bytes / max(acc["launch_ext"], 1)),
+            "avg_launch_ms": round(acc["ms_ext"] / max(acc["launch_ext"], 1), 4),
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        result["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(result), flush=True)
+    gpu.close()
+
+
+if __name__ == "__main__":
+    main()

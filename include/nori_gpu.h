@@ -1,0 +1,219 @@
+/*
+ * nori_gpu.h — C ABI of the MI355X wavefront path tracer (libnori_gpu.so).
+ *
+ * The library replaces the GPU half of the reference renderer: the D3D12/DXR
+ * host (src/gpu/DXRApp*.cpp) and the RayGen megakernel it dispatches
+ * (shaders/Shaders.hlsl:25-651), and it also serves Nori's CPU-side
+ * Accel::rayIntersect (include/accel.h:62-83) for batches of rays.  Plain
+ * pointers and sizes only; every function returns 0 on success or a negative
+ * NORI_E_* code, and nori_gpu_last_error() returns the message.  A context owns
+ * its device memory and one HIP stream and is not thread-safe.  The caller owns
+ * every host buffer it passes.
+ */
+#ifndef NORI_GPU_H
+#define NORI_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NORI_OK 0
+#define NORI_E_ARG -1      /* invalid argument                        */
+#define NORI_E_HIP -2      /* HIP runtime error                       */
+#define NORI_E_STATE -3    /* call out of order (e.g. no scene yet)   */
+#define NORI_E_LIMIT -4    /* size beyond a documented limit          */
+#define NORI_E_NODEVICE -5 /* no GPU / HIP device unavailable        */
+
+/* Material types, the `type` field of GPUMaterial (shaders/Material.hlsli:18-194). */
+#define NORI_MAT_DIFFUSE 0
+#define NORI_MAT_MIRROR 1
+#define NORI_MAT_DIELECTRIC 2
+#define NORI_MAT_MICROFACET 3
+#define NORI_MAT_DISNEY 4
+#define NORI_MAT_HAIR 5
+
+#define NORI_NO_TEXTURE 0xFFFFFFFFu
+
+/* Per-mesh material record; field-for-field the 144-byte GPUMaterial of
+ * src/gpu/DXRApp.h:72-111 / shaders/Common.hlsli:55-89.  indexOffset and
+ * vertexOffset give where the mesh starts in the flat buffers (indices in
+ * nori_scene_desc are already global, so kernels do not add them). */
+typedef struct nori_material {
+    uint32_t type;
+    float albedo[3];
+    float intIOR, extIOR, alpha;
+    uint32_t isEmitter;
+    float radiance[3];
+    uint32_t indexOffset, vertexOffset, indexCount, vertexCount;
+    float surfaceArea;
+    uint32_t emitterCdfOffset;
+    uint32_t albedoTexIndex, normalTexIndex, roughnessTexIndex, metallicTexIndex, specularTexIndex,
+        subsurfaceTexIndex, alphaTexIndex;
+    float roughness, metallic, specular, specularTint, sheen, sheenTint, subsurface, clearcoat, clearcoatGloss,
+        anisotropic;
+    float betaN;
+    float emitterSelectionProb;
+} nori_material;
+
+/* Flat scene, as src/gpu/DXRApp_Scene.cpp:172-451 packs it (SoA, one material
+ * per mesh).  All arrays are copied by nori_gpu_upload_scene. */
+typedef struct nori_scene_desc {
+    uint32_t num_vertices, num_triangles, num_meshes;
+    const float *positions;         /* 3 * num_vertices                        */
+    const float *normals;           /* 3 * num_vertices, or NULL               */
+    const float *uvs;               /* 2 * num_vertices, or NULL               */
+    const float *tangents;          /* 3 * num_vertices (hair), or NULL        */
+    const uint32_t *indices;        /* 3 * num_triangles, global vertex ids    */
+    const uint32_t *mesh_tri_offset;/* num_meshes: first global triangle       */
+    const uint32_t *mesh_tri_count; /* num_meshes                              */
+    const nori_material *materials; /* num_meshes                              */
+    uint32_t env_width, env_height; /* 0 = no environment map                  */
+    const float *env_rgb;           /* env_height * env_width * 3, row 0 = +y  */
+    float env_scale;
+} nori_scene_desc;
+
+/* Nori's perspective camera (src/cpu/perspective.cpp:21-123): horizontal fov,
+ * optional thin lens. */
+typedef struct nori_camera {
+    float to_world[16]; /* row-major camera-to-world */
+    float fov_deg;
+    uint32_t width, height;
+    float near_clip, far_clip, lens_radius, focal_distance;
+} nori_camera;
+
+/* One view of a render: spp samples starting at sample index sample_offset,
+ * paths of at most max_depth segments, Russian roulette from bounce rr_depth
+ * (shaders/Shaders.hlsl:108,620-626). */
+typedef struct nori_view {
+    nori_camera camera;
+    uint32_t spp, sample_offset, max_depth, rr_depth;
+} nori_view;
+
+/* Ray for the batch trace entry: origin, mint, direction, maxt (32 bytes). */
+typedef struct nori_ray {
+    float o[3];
+    float mint;
+    float d[3];
+    float maxt;
+} nori_ray;
+
+/* Closest hit: t, global primitive id (0xFFFFFFFF = miss), barycentrics u, v of
+ * src/cpu/mesh.cpp:51-89.  Equal t is broken by the smaller primitive id. */
+typedef struct nori_hit {
+    float t;
+    uint32_t prim;
+    float u, v;
+} nori_hit;
+
+typedef struct nori_ctx nori_ctx;
+
+/* Create a context on HIP device `device`. */
+int nori_gpu_create(nori_ctx **ctx, int device);
+void nori_gpu_destroy(nori_ctx *ctx);
+const char *nori_gpu_last_error(const nori_ctx *ctx);
+
+/* Build the compressed BVH8, the emitter tables and the envmap alias table on
+ * the host and upload everything.  Replaces DXRApp::OnInit's scene half
+ * (src/gpu/DXRApp.cpp:58-79 -> DXRApp_Scene.cpp:37-451, DXRApp_Textures.cpp:181-390). */
+int nori_gpu_upload_scene(nori_ctx *ctx, const nori_scene_desc *scene);
+
+/* Render views[0..nviews) with the path_mis integrator and ADD the samples to
+ * each view's device film (RGBA float: radiance sum, sample count).  If
+ * accum_rgba is not NULL the films are copied to it afterwards, concatenated in
+ * view order (4 floats per pixel).  Replaces the DispatchRays frame loop
+ * (src/gpu/DXRApp.cpp:199-246) running RayGen (shaders/Shaders.hlsl:25-651). */
+int nori_gpu_render(nori_ctx *ctx, const nori_view *views, int nviews, uint64_t seed, float *accum_rgba);
+
+/* Zero the device films. */
+int nori_gpu_reset_film(nori_ctx *ctx);
+
+/* Batch closest-hit (any_hit = 0) or occlusion (any_hit = 1) queries; rays and
+ * hits are host arrays.  For any_hit, prim is 0 on a hit and 0xFFFFFFFF else.
+ * The batch form of Accel::rayIntersect (src/cpu/accel.cpp:429-544). */
+int nori_gpu_trace(nori_ctx *ctx, const nori_ray *rays, int n, nori_hit *hits, int any_hit);
+
+/* Same, on device pointers (no copies, no synchronisation). */
+int nori_gpu_trace_device(nori_ctx *ctx, const nori_ray *d_rays, int n, nori_hit *d_hits, int any_hit);
+
+/* BSDF batch evaluation (BSDF::eval/pdf/sample, include/bsdf.h:111-139, with the
+ * GPU material semantics of shaders/Material.hlsli:18-194).  One material per
+ * query, local frame.  op 0: eval -> rgb; op 1: pdf -> pdf; op 2: sample with
+ * u[3] -> rgb = f*cos/pdf, wo, pdf (ops 0 and 1 both write rgb and pdf).
+ * Host arrays; NULL outputs are skipped. */
+#define NORI_BSDF_EVAL 0
+#define NORI_BSDF_PDF 1
+#define NORI_BSDF_SAMPLE 2
+int nori_gpu_bsdf_batch(nori_ctx *ctx, int op, int n, const nori_material *mats, const float *wi3, const float *wo3,
+                        const float *u3, float *rgb3, float *pdf, float *wo3_out);
+
+/* Envmap alias sampling of the uploaded scene: for each i, a 32-bit integer r,
+ * a coin and two jitters -> direction, radiance, solid-angle pdf and texel id.
+ * The pdf equals EnvmapPdfDirection (shaders/Envmap.hlsli:109-135). */
+int nori_gpu_envmap_sample(nori_ctx *ctx, int n, const uint32_t *r, const float *coin, const float *jx,
+                           const float *jy, float *dir3, float *rgb3, float *pdf, uint32_t *texel);
+
+/* Envmap evaluation and pdf for directions (shaders/Envmap.hlsli:15-24,109-135). */
+int nori_gpu_envmap_eval(nori_ctx *ctx, int n, const float *dir3, float *rgb3, float *pdf);
+
+/* pcg32 stream check: n outputs of seed(initstate, initseq) advanced by `advance`. */
+int nori_gpu_pcg32(nori_ctx *ctx, uint64_t initstate, uint64_t initseq, uint64_t advance, int n, uint32_t *out);
+
+/* Counters and timings of the last nori_gpu_render call.  The per-kernel
+ * times are filled when option "timing_mode" is 1 (HIP events around every
+ * launch on the context's stream); the traversal counts when option
+ * "count_traversal" is 1 (BVH8 nodes visited and triangles tested, summed). */
+typedef struct nori_render_stats {
+    uint64_t samples, extension_rays, shadow_rays, bounces;
+    double ms_total;          /* render time (HIP events on the context stream) */
+    double ms_trace_closest;  /* summed kernel time of closest-hit traversal    */
+    double ms_trace_shadow;   /* summed kernel time of shadow traversal         */
+    double ms_shade;          /* summed kernel time of raygen and shading       */
+    uint64_t launches_closest, launches_shadow;
+    uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
+} nori_render_stats;
+int nori_gpu_render_stats(const nori_ctx *ctx, nori_render_stats *out);
+
+/* Sizes of the uploaded acceleration structure. */
+typedef struct nori_bvh_info {
+    uint32_t nodes, triangles, max_depth, leaf_tris_max;
+    uint64_t node_bytes, tri_bytes;
+    double build_ms;
+} nori_bvh_info;
+int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
+
+/* Options: "batch" (paths per wavefront batch, default 2^21), "timing_mode"
+ * (0/1), "count_traversal" (0/1). */
+int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
+
+/* Host-only BVH8 build (the builder nori_gpu_upload_scene runs), for checks
+ * and build timing without a GPU.  If nodes / tris are not NULL they receive
+ * info->nodes * 80 and info->triangles * 48 bytes (call once with NULL to size). */
+int nori_bvh8_build_host(const float *positions, uint32_t num_vertices, const uint32_t *indices,
+                         uint32_t num_triangles, nori_bvh_info *info, void *nodes, void *tris);
+
+/* -------------------------------------------------------------- scenes */
+/* Seeded procedural scenes of the benchmark configurations, built on the host.
+ * config 1: Cornell box (5 quads, 2 icospheres, ceiling light).
+ * config 2: grid of Disney icospheres on a ground quad under a sky+sun envmap.
+ * Options left 0 take the configuration's defaults. */
+typedef struct nori_scene_options {
+    int config;
+    uint64_t seed;
+    uint32_t subdiv;        /* icosphere subdivision level                  */
+    uint32_t grid;          /* spheres per grid side (config 2)            */
+    uint32_t env_width, env_height;
+    uint32_t width, height; /* image size of the default camera            */
+} nori_scene_options;
+
+typedef struct nori_scene nori_scene;
+int nori_scene_generate(const nori_scene_options *opt, nori_scene **out);
+/* Pointers stay valid until nori_scene_free. */
+int nori_scene_get_desc(const nori_scene *s, nori_scene_desc *desc);
+int nori_scene_get_camera(const nori_scene *s, nori_camera *cam);
+void nori_scene_free(nori_scene *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker or the timed CPU baseline.
+The oracle restates the reference renderer on the CPU; see the headers of the
+C++ sources for the reference file:line each part follows.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        L.orc_pcg32.argtypes = [u64, u64, u64, u32, vp]
+        L.orc_pcg32_sample_floats.argtypes = [u64, u32, u32, u32, vp]
+        L.orc_pcg_hash.argtypes, L.orc_pcg_hash.restype = [u32], u32
+        L.orc_scene_create.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, u32, u32, vp, ctypes.c_float]
+        L.orc_scene_create.restype = vp
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_bvh_nodes.argtypes, L.orc_scene_bvh_nodes.restype = [vp], u32
+        L.orc_trace.argtypes = [vp, vp, u32, i32, i32, vp, vp, vp, vp, i32, vp]
+        L.orc_render.argtypes = [vp, vp, u32, u32, u32, u32, u64, i32, i32, vp, vp]
+        L.orc_camera_rays.argtypes = [vp, u32, vp, vp]
+        L.orc_alias_build.argtypes = [vp, u32, vp, vp]
+        L.orc_env_tables.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_env_sample.argtypes = [vp, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_env_eval_pdf.argtypes = [vp, u32, vp, vp, vp]
+        L.orc_bsdf_batch.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def pcg32(initstate, initseq, n, advance=0):
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32(initstate, initseq, advance, n, _p(out))
+    return out
+
+
+def sample_floats(seed, pixel, sample, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_pcg32_sample_floats(seed, pixel, sample, n, _p(out))
+    return out
+
+
+def pcg_hash(v):
+    return lib().orc_pcg_hash(v)
+
+
+def alias_build(w):
+    w = np.ascontiguousarray(w, np.float32)
+    prob, alias = np.zeros(len(w), np.float32), np.zeros(len(w), np.uint32)
+    lib().orc_alias_build(_p(w), len(w), _p(prob), _p(alias))
+    return prob, alias
+
+
+def bsdf_batch(op, mats, wi, wo=None, u=None, precision=64):
+    """op 0 eval, 1 pdf, 2 sample -> (rgb, pdf, wo) as float64 arrays."""
+    n = len(mats)
+    mats = np.ascontiguousarray(mats)
+    wi = np.ascontiguousarray(wi, np.float32)
+    wo = None if wo is None else np.ascontiguousarray(wo, np.float32)
+    u = None if u is None else np.ascontiguousarray(u, np.float32)
+    rgb, pdf, woo = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3))
+    lib().orc_bsdf_batch(op, precision, n, _p(mats), _p(wi), _p(wo), _p(u), _p(rgb), _p(pdf), _p(woo))
+    return rgb, pdf, woo
+
+
+class OracleScene:
+    """The oracle's own copy of a flat scene: Nori BVH2, emitter CDFs, envmap tables."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        L = lib()
+        env = scene.env_rgb
+        tri_mesh = scene.tri_mesh
+        self._keep = (scene, tri_mesh)
+        self.h = L.orc_scene_create(
+            len(scene.positions), _p(scene.positions), _p(scene.normals), len(scene.indices), _p(scene.indices),
+            _p(tri_mesh), len(scene.materials), _p(scene.materials), _p(scene.mesh_tri_offset),
+            _p(scene.mesh_tri_count), 0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
+            _p(env), scene.env_scale)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def trace(self, rays, any_hit=False, brute=False, nthreads=0, counters=None):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        t, prim = np.zeros(n, np.float32), np.zeros(n, np.uint32)
+        u, v = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        cnt = np.zeros(2, np.uint64)
+        lib().orc_trace(self.h, _p(rays), n, int(any_hit), int(brute), _p(t), _p(prim), _p(u), _p(v), nthreads,
+                        _p(cnt))
+        if counters is not None:
+            counters[:] = cnt
+        return t, prim, u, v
+
+    def render(self, camera, spp, max_depth, rr_depth=3, seed=0, sample_offset=0, env_cdf=False, nthreads=0):
+        w, h = camera.width, camera.height
+        acc = np.zeros((w * h, 4), np.float32)
+        stats = np.zeros(3, np.uint64)
+        lib().orc_render(self.h, ctypes.byref(camera), spp, sample_offset, max_depth, rr_depth, seed, int(env_cdf),
+                         nthreads, _p(acc), _p(stats))
+        return acc, stats
+
+    def env_tables(self):
+        H, W = self.scene.env_rgb.shape[:2]
+        marg, cond = np.zeros(H + 1, np.float32), np.zeros(H * (W + 1), np.float32)
+        ppix, prob, alias = np.zeros(H * W, np.float32), np.zeros(H * W, np.float32), np.zeros(H * W, np.uint32)
+        lib().orc_env_tables(self.h, _p(marg), _p(cond), _p(ppix), _p(prob), _p(alias))
+        return marg, cond.reshape(H, W + 1), ppix, prob, alias
+
+    def env_sample(self, r, coin, jx, jy, mode=0):
+        n = len(coin)
+        r = np.ascontiguousarray(r if r is not None else np.zeros(n), np.uint32)
+        coin, jx, jy = (np.ascontiguousarray(a, np.float32) for a in (coin, jx, jy))
+        d, L = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+        pdf, tex = np.zeros(n, np.float32), np.zeros(n, np.uint32)
+        lib().orc_env_sample(self.h, mode, n, _p(r), _p(coin), _p(jx), _p(jy), _p(d), _p(L), _p(pdf), _p(tex))
+        return d, L, pdf, tex
+
+    def env_eval(self, dirs):
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        n = len(dirs)
+        L, pdf = np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+        lib().orc_env_eval_pdf(self.h, n, _p(dirs), _p(L), _p(pdf))
+        return L, pdf
+
+
+def camera_rays(camera, samples4):
+    samples4 = np.ascontiguousarray(samples4, np.float32).reshape(-1, 4)
+    rays = np.zeros((len(samples4), 8), np.float32)
+    lib().orc_camera_rays(ctypes.byref(camera), len(samples4), _p(samples4), _p(rays))
+    return rays
+
+
+def rel_mse(img, ref, eps=1e-2):
+    """relMSE of validation_tests/compare_exr.py:101-103: mean((img-ref)^2 / (ref^2 + eps))."""
+    img, ref = np.asarray(img, np.float64), np.asarray(ref, np.float64)
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))

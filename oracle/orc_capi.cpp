@@ -1,0 +1,180 @@
+// CPU oracle — test infrastructure only.
+//
+// Plain-C entry points of the oracle library (oracle/liboracle.so), loaded with
+// ctypes by tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py.
+// Nothing in the product links or loads this library.
+#include "orc_render.h"
+#include <cstring>
+#include <thread>
+
+using namespace orc;
+
+namespace {
+struct Handle {
+    SceneData s;
+};
+} // namespace
+
+extern "C" {
+
+void orc_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out) {
+    Pcg32 r;
+    r.seed(initstate, initseq);
+    r.advance(advance);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next_uint();
+}
+
+void orc_pcg32_sample_floats(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t n, float *out) {
+    Pcg32 r = sample_stream(seed, pixel, sample);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next_float();
+}
+
+uint32_t orc_pcg_hash(uint32_t v) { return pcg_hash(v); }
+
+void *orc_scene_create(uint32_t nv, const float *pos, const float *nrm, uint32_t nt, const uint32_t *idx,
+                       const uint32_t *tri_mesh, uint32_t nmesh, const void *mats, const uint32_t *mesh_tri_offset,
+                       const uint32_t *mesh_tri_count, uint32_t env_w, uint32_t env_h, const float *env_rgb,
+                       float env_scale) {
+    Handle *h = new Handle();
+    SceneData &s = h->s;
+    s.nv = nv; s.nt = nt; s.nmesh = nmesh;
+    s.pos.assign(pos, pos + 3 * (size_t)nv);
+    if (nrm) s.nrm.assign(nrm, nrm + 3 * (size_t)nv);
+    s.idx.assign(idx, idx + 3 * (size_t)nt);
+    s.tri_mesh.assign(tri_mesh, tri_mesh + nt);
+    s.mats.resize(nmesh);
+    std::memcpy(s.mats.data(), mats, sizeof(MatRec) * nmesh);
+    s.mesh_tri_offset.assign(mesh_tri_offset, mesh_tri_offset + nmesh);
+    s.mesh_tri_count.assign(mesh_tri_count, mesh_tri_count + nmesh);
+    if (env_w && env_h && env_rgb) s.env.build(env_w, env_h, env_rgb, env_scale);
+    s.finalize();
+    return h;
+}
+
+void orc_scene_destroy(void *h) { delete (Handle *)h; }
+
+uint32_t orc_scene_bvh_nodes(void *hp) {
+    uint32_t n = 0;
+    for (auto &nd : ((Handle *)hp)->s.bvh.nodes) n += nd.used;
+    return n;
+}
+
+// rays: n x {ox,oy,oz,mint, dx,dy,dz,maxt}.  brute=1 tests every triangle.
+void orc_trace(void *hp, const float *rays, uint32_t n, int any_hit, int brute, float *t, uint32_t *prim, float *u,
+               float *v, int nthreads, uint64_t *counters /* nodes, tris; may be null */) {
+    const SceneData &s = ((Handle *)hp)->s;
+    int nth = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    std::vector<uint64_t> cn(2 * nth, 0);
+    auto work = [&](int tid) {
+        for (uint32_t i = tid; i < n; i += nth) {
+            Ray r = make_ray(rays + 8 * (size_t)i);
+            Hit hh = brute ? brute_trace(s.bvh, r, any_hit != 0)
+                           : s.bvh.trace(r, any_hit != 0, &cn[2 * tid], &cn[2 * tid + 1]);
+            t[i] = hh.t; prim[i] = hh.prim; u[i] = hh.u; v[i] = hh.v;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k) th.emplace_back(work, k);
+    for (auto &x : th) x.join();
+    if (counters) {
+        counters[0] = counters[1] = 0;
+        for (int k = 0; k < nth; ++k) { counters[0] += cn[2 * k]; counters[1] += cn[2 * k + 1]; }
+    }
+}
+
+void orc_render(void *hp, const Camera *cam, uint32_t spp, uint32_t sample_offset, uint32_t max_depth,
+                uint32_t rr_depth, uint64_t seed, int env_cdf, int nthreads, float *accum, uint64_t *stats3) {
+    RenderParams p{spp, sample_offset, max_depth, rr_depth, seed, env_cdf, nthreads};
+    RenderStats st;
+    render(((Handle *)hp)->s, *cam, p, accum, &st);
+    if (stats3) { stats3[0] = st.samples; stats3[1] = st.extension_rays; stats3[2] = st.shadow_rays; }
+}
+
+void orc_camera_rays(const Camera *cam, uint32_t n, const float *samples4, float *rays8) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Ray r;
+        const float *s4 = samples4 + 4 * (size_t)i;
+        camera_ray(*cam, s4[0], s4[1], s4[2], s4[3], r);
+        float *o = rays8 + 8 * (size_t)i;
+        o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.mint;
+        o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = r.maxt;
+    }
+}
+
+// ---------------------------------------------------------------- envmap
+void orc_alias_build(const float *w, uint32_t n, float *prob, uint32_t *alias) { build_alias(w, n, prob, alias); }
+
+void orc_env_tables(void *hp, float *marginal, float *conditional, float *ppix, float *prob, uint32_t *alias) {
+    const Envmap &e = ((Handle *)hp)->s.env;
+    if (marginal) std::memcpy(marginal, e.marginal.data(), e.marginal.size() * 4);
+    if (conditional) std::memcpy(conditional, e.conditional.data(), e.conditional.size() * 4);
+    if (ppix) std::memcpy(ppix, e.ppix.data(), e.ppix.size() * 4);
+    if (prob) std::memcpy(prob, e.alias_prob.data(), e.alias_prob.size() * 4);
+    if (alias) std::memcpy(alias, e.alias_idx.data(), e.alias_idx.size() * 4);
+}
+
+// mode 0: alias (r, coin, jx, jy); mode 1: reference CDF (u1 = coin, u2 = jx).
+void orc_env_sample(void *hp, int mode, uint32_t n, const uint32_t *r, const float *coin, const float *jx,
+                    const float *jy, float *dir3, float *L3, float *pdf, uint32_t *texel) {
+    const Envmap &e = ((Handle *)hp)->s.env;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3f d, L;
+        float p;
+        if (mode == 0) {
+            e.sample_alias(r[i], coin[i], jx[i], jy[i], d, L, p);
+            if (texel) texel[i] = e.alias_pick(r[i], coin[i]);
+        } else {
+            e.sample_cdf(coin[i], jx[i], d, L, p);
+            if (texel) texel[i] = 0xFFFFFFFFu;
+        }
+        dir3[3 * i] = d.x; dir3[3 * i + 1] = d.y; dir3[3 * i + 2] = d.z;
+        L3[3 * i] = L.x; L3[3 * i + 1] = L.y; L3[3 * i + 2] = L.z;
+        pdf[i] = p;
+    }
+}
+
+void orc_env_eval_pdf(void *hp, uint32_t n, const float *dir3, float *L3, float *pdf) {
+    const Envmap &e = ((Handle *)hp)->s.env;
+    for (uint32_t i = 0; i < n; ++i) {
+        V3f d(dir3[3 * i], dir3[3 * i + 1], dir3[3 * i + 2]);
+        V3f L = e.eval(d);
+        if (L3) { L3[3 * i] = L.x; L3[3 * i + 1] = L.y; L3[3 * i + 2] = L.z; }
+        if (pdf) pdf[i] = e.pdf(d);
+    }
+}
+
+// ---------------------------------------------------------------- BSDFs
+// op 0 eval (f rgb), 1 pdf, 2 sample (weight rgb, wo, pdf).  One material record
+// per query.  precision 64 evaluates in double from the float inputs.
+void orc_bsdf_batch(int op, int precision, uint32_t n, const void *mats, const float *wi3, const float *wo3,
+                    const float *u3, double *out_rgb, double *out_pdf, double *out_wo) {
+    const MatRec *mr = (const MatRec *)mats;
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        for (uint32_t i = 0; i < n; ++i) {
+            MatT<T> m = MatT<T>::from(mr[i]);
+            V3<T> wi{T(wi3[3 * i]), T(wi3[3 * i + 1]), T(wi3[3 * i + 2])};
+            if (op == 0 || op == 1) {
+                V3<T> wo{T(wo3[3 * i]), T(wo3[3 * i + 1]), T(wo3[3 * i + 2])};
+                if (op == 0) {
+                    V3<T> f = material_eval(wi, wo, m);
+                    out_rgb[3 * i] = f.x; out_rgb[3 * i + 1] = f.y; out_rgb[3 * i + 2] = f.z;
+                } else {
+                    out_pdf[i] = material_pdf(wi, wo, m);
+                }
+            } else {
+                T u[3] = {T(u3[3 * i]), T(u3[3 * i + 1]), T(u3[3 * i + 2])};
+                V3<T> wo;
+                T pdf;
+                V3<T> w = material_sample(wi, u, m, wo, pdf);
+                out_rgb[3 * i] = w.x; out_rgb[3 * i + 1] = w.y; out_rgb[3 * i + 2] = w.z;
+                out_pdf[i] = pdf;
+                out_wo[3 * i] = wo.x; out_wo[3 * i + 1] = wo.y; out_wo[3 * i + 2] = wo.z;
+            }
+        }
+    };
+    if (precision == 64) run(0.0);
+    else run(0.0f);
+}
+
+} // extern "C"

@@ -1,0 +1,123 @@
+// CPU oracle — test infrastructure only.
+//
+// Ray/triangle intersection and Nori's BVH2, restated.
+//   Möller–Trumbore          src/cpu/mesh.cpp:51-89
+//   slab test                include/bbox.h:360-392
+//   binned + serial SAH build src/cpu/accel.cpp:55-323 (run serially here)
+//   stack traversal          src/cpu/accel.cpp:429-488
+//
+// Canonical hit (shared with the GPU path): the smallest t over all triangles
+// accepted by the Möller–Trumbore test with mint <= t <= maxt; equal t is broken
+// by the smallest global primitive index.  Nori's own leaf loop keeps the last
+// hit at equal t, which depends on its (TBB-ordered) leaf order; the explicit
+// tie-break makes the result independent of the acceleration structure.
+#pragma once
+#include "orc_math.h"
+#include <vector>
+
+namespace orc {
+
+struct Ray {
+    V3f o, d, drcp;
+    float mint, maxt;
+};
+
+struct Hit {
+    float t, u, v;
+    uint32_t prim; // 0xFFFFFFFF = miss
+};
+
+// src/cpu/mesh.cpp:51-89, operating on (p0, e1 = p1-p0, e2 = p2-p0).
+inline bool intersect_tri(const V3f &p0, const V3f &e1, const V3f &e2, const Ray &r, float &u, float &v, float &t) {
+    V3f pvec = cross(r.d, e2);
+    float det = dot(e1, pvec);
+    if (det > -1e-8f && det < 1e-8f) return false;
+    float inv_det = 1.0f / det;
+    V3f tvec = r.o - p0;
+    u = dot(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    V3f qvec = cross(tvec, e1);
+    v = dot(r.d, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qvec) * inv_det;
+    return t >= r.mint && t <= r.maxt;
+}
+
+struct BBox {
+    V3f mn, mx;
+    BBox() : mn(3.4028235e38f), mx(-3.4028235e38f) {}
+    void expand(const V3f &p) {
+        mn = V3f(std::min(mn.x, p.x), std::min(mn.y, p.y), std::min(mn.z, p.z));
+        mx = V3f(std::max(mx.x, p.x), std::max(mx.y, p.y), std::max(mx.z, p.z));
+    }
+    void expand(const BBox &b) {
+        mn = V3f(std::min(mn.x, b.mn.x), std::min(mn.y, b.mn.y), std::min(mn.z, b.mn.z));
+        mx = V3f(std::max(mx.x, b.mx.x), std::max(mx.y, b.mx.y), std::max(mx.z, b.mx.z));
+    }
+    float area() const {
+        V3f d = mx - mn;
+        float r = 0.0f;
+        for (int i = 0; i < 3; ++i) {
+            float term = 1.0f;
+            for (int j = 0; j < 3; ++j)
+                if (i != j) term *= d[j];
+            r += term;
+        }
+        return r * 2.0f;
+    }
+    int largest_axis() const {
+        V3f e = mx - mn;
+        if (e.x >= e.y && e.x >= e.z) return 0;
+        if (e.y >= e.x && e.y >= e.z) return 1;
+        return 2;
+    }
+    // include/bbox.h:360-392
+    bool hit(const Ray &r) const {
+        float nearT = -INFINITY, farT = INFINITY;
+        for (int i = 0; i < 3; ++i) {
+            float o = r.o[i], lo = mn[i], hi = mx[i];
+            if (r.d[i] == 0) {
+                if (o < lo || o > hi) return false;
+            } else {
+                float t1 = (lo - o) * r.drcp[i], t2 = (hi - o) * r.drcp[i];
+                if (t1 > t2) std::swap(t1, t2);
+                nearT = std::max(t1, nearT);
+                farT = std::min(t2, farT);
+                if (!(nearT <= farT)) return false;
+            }
+        }
+        return r.mint <= farT && nearT <= r.maxt;
+    }
+};
+
+struct Bvh2 {
+    struct Node {
+        BBox box;
+        uint32_t right_or_start; // inner: right child; leaf: first index
+        uint32_t size;           // leaf: triangle count
+        uint8_t leaf, used;
+    };
+    std::vector<V3f> p0, e1, e2; // per-triangle precomputation
+    std::vector<V3f> centroid;
+    std::vector<BBox> tbox;
+    std::vector<Node> nodes;
+    std::vector<uint32_t> index;
+
+    void build(const float *pos, const uint32_t *idx, uint32_t nt);
+    Hit trace(const Ray &ray, bool any_hit, uint64_t *nodes_visited = nullptr, uint64_t *tris_tested = nullptr) const;
+};
+
+// Exhaustive test over every triangle (ground truth for small scenes).
+Hit brute_trace(const Bvh2 &tris, const Ray &ray, bool any_hit);
+
+inline Ray make_ray(const float *r8) {
+    Ray r;
+    r.o = V3f(r8[0], r8[1], r8[2]);
+    r.mint = r8[3];
+    r.d = V3f(r8[4], r8[5], r8[6]);
+    r.maxt = r8[7];
+    r.drcp = V3f(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+    return r;
+}
+
+} // namespace orc

@@ -1,0 +1,297 @@
+// CPU oracle — test infrastructure only.  See orc_render.h.
+#include "orc_render.h"
+#include <atomic>
+#include <thread>
+
+namespace orc {
+
+static const float PI_F = 3.14159265358979323846f;
+
+void SceneData::finalize() {
+    bvh.build(pos.data(), idx.data(), nt);
+    emitters.clear(); emit_cdf.clear(); emit_area.clear();
+    for (uint32_t m = 0; m < nmesh; ++m) {
+        if (!mats[m].isEmitter) continue;
+        // include/dpdf.h append/normalize over triangle areas (src/cpu/mesh.cpp:32-48)
+        std::vector<float> cdf(1, 0.0f);
+        for (uint32_t k = 0; k < mesh_tri_count[m]; ++k) {
+            uint32_t f = mesh_tri_offset[m] + k;
+            V3f p0 = vtx(idx[3 * f]), p1 = vtx(idx[3 * f + 1]), p2 = vtx(idx[3 * f + 2]);
+            float a = 0.5f * length(cross(p1 - p0, p2 - p0));
+            cdf.push_back(cdf.back() + a);
+        }
+        float sum = cdf.back();
+        if (sum > 0) {
+            float norm = 1.0f / sum;
+            for (size_t i = 1; i < cdf.size(); ++i) cdf[i] *= norm;
+            cdf.back() = 1.0f;
+        }
+        emitters.push_back(m);
+        emit_cdf.push_back(cdf);
+        emit_area.push_back(sum);
+    }
+}
+
+// Nori's perspective camera, src/cpu/perspective.cpp:50-123, with the
+// sample-to-camera matrix written in closed form: a sample (sx, sy) in [0,1]^2
+// maps to the camera-space direction ((1-2sx)tan(fov/2), (1-2sy)tan(fov/2)/aspect, 1).
+void camera_ray(const Camera &cam, float px, float py, float ax, float ay, Ray &ray) {
+    float sx = px * (1.0f / (float)cam.width), sy = py * (1.0f / (float)cam.height);
+    float aspect = (float)cam.width / (float)cam.height;
+    float th = std::tan(cam.fov_deg * (PI_F / 180.0f) * 0.5f);
+    V3f d = normalize(V3f((1.0f - 2.0f * sx) * th, (1.0f - 2.0f * sy) * th / aspect, 1.0f));
+    V3f o(0.0f);
+    if (cam.lens_radius > 0.0f) {
+        float r = std::sqrt(ax), phi = 2.0f * PI_F * ay; // squareToUniformDisk, src/cpu/warp.cpp:42-47
+        V3f pl(cam.lens_radius * (r * std::cos(phi)), cam.lens_radius * (r * std::sin(phi)), 0.0f);
+        float ft = cam.focal_distance / d.z;
+        V3f pf = d * ft;
+        o = pl;
+        d = normalize(pf - pl);
+    }
+    float invz = 1.0f / d.z;
+    const float *M = cam.to_world;
+    ray.o = V3f(M[0] * o.x + M[1] * o.y + M[2] * o.z + M[3], M[4] * o.x + M[5] * o.y + M[6] * o.z + M[7],
+                M[8] * o.x + M[9] * o.y + M[10] * o.z + M[11]);
+    ray.d = V3f(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z,
+                M[8] * d.x + M[9] * d.y + M[10] * d.z);
+    ray.mint = cam.near_clip * invz;
+    ray.maxt = cam.far_clip * invz;
+    ray.drcp = V3f(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
+}
+
+static inline Ray mk_ray(V3f o, V3f d, float mint, float maxt) {
+    Ray r;
+    r.o = o; r.d = d; r.mint = mint; r.maxt = maxt;
+    r.drcp = V3f(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    return r;
+}
+
+// shaders/Emitter.hlsli:17-31
+static uint32_t cdf_sample(const std::vector<float> &c, uint32_t n, float u) {
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (c[mid] <= u) lo = mid + 1;
+        else hi = mid;
+    }
+    return (uint32_t)std::max(0, (int)lo - 1);
+}
+
+static bool occluded(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st) {
+    ++st.shadow_rays;
+    Hit h = s.bvh.trace(mk_ray(o, d, 0.0f, maxt), true);
+    return h.prim != 0xFFFFFFFFu;
+}
+
+// RayGen body, shaders/Shaders.hlsl:55-627, for the surface materials of the
+// scene (diffuse, mirror, dielectric, microfacet, Disney).
+V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, const RenderParams &p, RenderStats &st) {
+    (void)cam;
+    V3f L(0.0f), T(1.0f);
+    float last_pdf = 0.0f, eta = 1.0f;
+    const uint32_t E = (uint32_t)s.emitters.size();
+    for (uint32_t bounce = 0; bounce < p.max_depth; ++bounce) {
+        ++st.extension_rays;
+        Hit h = s.bvh.trace(ray, false);
+        if (h.prim == 0xFFFFFFFFu) {
+            if (s.env.valid()) {
+                V3f env = s.env.eval(ray.d);
+                if (last_pdf == 0.0f) L += T * env;
+                else L += T * env * balance_heuristic(last_pdf, s.env.pdf(ray.d));
+            }
+            break;
+        }
+        V3f P = ray.o + ray.d * h.t;
+        uint32_t f = h.prim, mesh = s.tri_mesh[f];
+        const MatRec &mr = s.mats[mesh];
+        MatT<float> m = MatT<float>::from(mr);
+        V3f Ng = normalize(cross(s.bvh.e1[f], s.bvh.e2[f]));
+        V3f N = Ng;
+        if (!s.nrm.empty()) {
+            uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+            V3f n0(s.nrm[3 * i0], s.nrm[3 * i0 + 1], s.nrm[3 * i0 + 2]);
+            V3f n1(s.nrm[3 * i1], s.nrm[3 * i1 + 1], s.nrm[3 * i1 + 2]);
+            V3f n2(s.nrm[3 * i2], s.nrm[3 * i2 + 1], s.nrm[3 * i2 + 2]);
+            float w0 = 1.0f - h.u - h.v;
+            V3f n = n0 * w0 + n1 * h.u + n2 * h.v;
+            float l2 = dot(n, n);
+            N = l2 < 1e-8f ? Ng : normalize(n);
+        }
+        bool back = dot(Ng, ray.d) > 0.0f;
+        if (back) Ng = -Ng;
+        if (dot(N, Ng) < 0.0f) N = -N;
+
+        if (mr.isEmitter) {
+            V3f Le(mr.radiance[0], mr.radiance[1], mr.radiance[2]);
+            if (last_pdf == 0.0f) {
+                L += T * Le;
+            } else {
+                // shaders/Emitter.hlsli:91-103
+                V3f dv = P - ray.o;
+                float d2 = dot(dv, dv), pe = 0.0f;
+                if (d2 >= 1e-12f) {
+                    float dist = std::sqrt(d2);
+                    float cl = std::fabs(dot(N, -dv / dist));
+                    if (cl >= 1e-8f) {
+                        uint32_t k = 0;
+                        while (s.emitters[k] != mesh) ++k;
+                        pe = (1.0f / (s.emit_area[k] * (float)E)) * d2 / cl;
+                    }
+                }
+                L += T * Le * balance_heuristic(last_pdf, pe);
+            }
+            break;
+        }
+
+        V3f Tg, B;
+        build_onb(N, Tg, B);
+        {
+            V3f V = -ray.d;
+            float ndv = dot(N, V);
+            if (ndv <= 0.0f && dot(V, Ng) > 0.0f) {
+                N = normalize(N + V * (1e-3f - ndv));
+                build_onb(N, Tg, B);
+            }
+        }
+        V3f wi = to_local(-ray.d, Tg, B, N);
+
+        if (m.type == MAT_MIRROR) {
+            ray = mk_ray(P + Ng * 0.001f, reflect(ray.d, N), 0.0f, 1e20f);
+            last_pdf = 0.0f;
+        } else if (m.type == MAT_DIELECTRIC) {
+            V3f I = ray.d;
+            float etaI = back ? m.intIOR : m.extIOR, etaT = back ? m.extIOR : m.intIOR;
+            float Fr = fresnel_dielectric(dot(-I, N), etaI, etaT);
+            V3f nd;
+            bool refr = false;
+            if (rng.next_float() < Fr) {
+                nd = reflect(I, N);
+            } else {
+                nd = refract(I, N, etaI / etaT);
+                if (dot(nd, nd) < 0.001f) nd = reflect(I, N);
+                else refr = true;
+            }
+            V3f on = dot(nd, Ng) > 0.0f ? Ng : -Ng;
+            ray = mk_ray(P + on * 0.001f, nd, 0.0f, 1e20f);
+            last_pdf = 0.0f;
+            if (refr) eta *= etaI / etaT;
+        } else if (m.type == MAT_DIFFUSE || m.type == MAT_MICROFACET || m.type == MAT_DISNEY) {
+            V3f so = offset_ray_origin(P, Ng, Ng);
+            // Area light NEE, shaders/Emitter.hlsli:43-89, 105-153
+            if (E > 0) {
+                uint32_t pick = std::min((uint32_t)(rng.next_float() * (float)E), E - 1);
+                uint32_t em = s.emitters[pick];
+                uint32_t ntri = s.mesh_tri_count[em];
+                uint32_t k = std::min(cdf_sample(s.emit_cdf[pick], ntri + 1, rng.next_float()), ntri - 1);
+                uint32_t ef = s.mesh_tri_offset[em] + k;
+                V3f p0 = s.vtx(s.idx[3 * ef]), p1 = s.vtx(s.idx[3 * ef + 1]), p2 = s.vtx(s.idx[3 * ef + 2]);
+                float r1 = rng.next_float(), r2 = rng.next_float();
+                float sr1 = std::sqrt(r1);
+                V3f ep = (1.0f - sr1) * p0 + sr1 * (1.0f - r2) * p1 + sr1 * r2 * p2;
+                V3f en = normalize(cross(p1 - p0, p2 - p0));
+                float pdfA = 1.0f / (s.emit_area[pick] * (float)E);
+                V3f tl = ep - P;
+                float dist = length(tl);
+                V3f ww = tl / dist;
+                float ct = dot(N, ww), cl = dot(en, -ww);
+                if (ct > 0.0f && cl > 0.0f && !occluded(s, so, ww, dist - 0.001f, st)) {
+                    V3f wol = to_local(ww, Tg, B, N);
+                    V3f fv = material_eval(wi, wol, m);
+                    float pb = material_pdf(wi, wol, m);
+                    float pe = pdfA * dist * dist / cl;
+                    float w = balance_heuristic(pe, pb);
+                    V3f Le(s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]);
+                    L += T * (Le * fv * ct / std::max(pe, 1e-20f) * w);
+                }
+            }
+            // Envmap NEE, shaders/Emitter.hlsli:275-326
+            if (s.env.valid()) {
+                V3f dir, Le;
+                float pe;
+                if (p.env_cdf) {
+                    float u1 = rng.next_float(), u2 = rng.next_float();
+                    s.env.sample_cdf(u1, u2, dir, Le, pe);
+                } else {
+                    uint32_t r = rng.next_uint();
+                    float coin = rng.next_float(), jx = rng.next_float(), jy = rng.next_float();
+                    s.env.sample_alias(r, coin, jx, jy, dir, Le, pe);
+                }
+                float ct = dot(N, dir);
+                if (pe > 0.0f && ct > 0.0f && !occluded(s, so, dir, 1e20f, st)) {
+                    V3f wol = to_local(dir, Tg, B, N);
+                    V3f fv = material_eval(wi, wol, m);
+                    float pb = material_pdf(wi, wol, m);
+                    float w = balance_heuristic(pe, pb);
+                    L += T * (Le * fv * ct / std::max(pe, 1e-20f) * w);
+                }
+            }
+            float u[3];
+            int nd = material_sample_dims(m.type);
+            for (int i = 0; i < nd; ++i) u[i] = rng.next_float();
+            V3f wo;
+            float pdf;
+            V3f wgt = material_sample(wi, u, m, wo, pdf);
+            if (pdf <= 0.0f || (wgt.x == 0.0f && wgt.y == 0.0f && wgt.z == 0.0f)) break;
+            V3f ww = to_world(wo, Tg, B, N);
+            ray = mk_ray(offset_ray_origin(P, Ng, Ng), ww, 0.0f, 1e20f);
+            T *= wgt;
+            last_pdf = std::max(pdf, 1e-20f);
+        } else {
+            break;
+        }
+        // Russian roulette, shaders/Shaders.hlsl:620-626
+        if (bounce >= p.rr_depth) {
+            float q = std::min(maxc(T) * eta * eta, 0.95f);
+            if (rng.next_float() >= q) break;
+            T /= q;
+        }
+    }
+    if (!std::isfinite(L.x) || !std::isfinite(L.y) || !std::isfinite(L.z)) L = V3f(0.0f);
+    return L;
+}
+
+void render(const SceneData &s, const Camera &cam, const RenderParams &p, float *accum, RenderStats *stats) {
+    const uint32_t W = cam.width, H = cam.height, BS = 32;
+    const uint32_t bx = (W + BS - 1) / BS, by = (H + BS - 1) / BS, nblocks = bx * by;
+    std::atomic<uint32_t> next(0);
+    int nth = p.nthreads > 0 ? p.nthreads : (int)std::thread::hardware_concurrency();
+    std::vector<RenderStats> per(nth);
+    auto work = [&](int tid) {
+        RenderStats &st = per[tid];
+        while (true) {
+            uint32_t b = next.fetch_add(1);
+            if (b >= nblocks) break;
+            uint32_t x0 = (b % bx) * BS, y0 = (b / bx) * BS;
+            for (uint32_t y = y0; y < std::min(y0 + BS, H); ++y)
+                for (uint32_t x = x0; x < std::min(x0 + BS, W); ++x) {
+                    uint32_t pix = y * W + x;
+                    float *a = accum + 4 * (size_t)pix;
+                    for (uint32_t si = 0; si < p.spp; ++si) {
+                        Pcg32 rng = sample_stream(p.seed, pix, p.sample_offset + si);
+                        float jx = rng.next_float(), jy = rng.next_float();
+                        float ax = 0.5f, ay = 0.5f;
+                        if (cam.lens_radius > 0.0f) { ax = rng.next_float(); ay = rng.next_float(); }
+                        Ray ray;
+                        camera_ray(cam, (float)x + jx, (float)y + jy, ax, ay, ray);
+                        V3f L = path_mis_li(s, cam, ray, rng, p, st);
+                        a[0] += L.x; a[1] += L.y; a[2] += L.z; a[3] += 1.0f;
+                        ++st.samples;
+                    }
+                }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t) th.emplace_back(work, t);
+    for (auto &t : th) t.join();
+    if (stats) {
+        *stats = RenderStats{};
+        for (auto &q : per) {
+            stats->extension_rays += q.extension_rays;
+            stats->shadow_rays += q.shadow_rays;
+            stats->samples += q.samples;
+        }
+    }
+}
+
+} // namespace orc

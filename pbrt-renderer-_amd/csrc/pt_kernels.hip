@@ -1,0 +1,531 @@
+// Wavefront path tracer kernels for gfx950.
+//
+// The reference runs every path of a frame inside one DXR RayGen megakernel
+// (shaders/Shaders.hlsl:25-651).  Here one bounce of all live paths is three
+// launches over queues in HBM: k_extend (closest-hit traversal of the BVH8),
+// k_shade (surface interaction, next-event rays pushed to a shadow queue, BSDF
+// sampling, Russian roulette, compaction of the survivors) and k_shadow
+// (occlusion traversal, film accumulation).  Path state is structure-of-arrays
+// in float4s so every per-path access is a single 16-byte load or store.
+#include "bsdf_dev.h"
+#include "pt_kernels.h"
+#include "trace_dev.h"
+
+namespace ng {
+
+namespace {
+
+constexpr uint32_t MISS = 0xFFFFFFFFu;
+constexpr int TRACE_BLOCK = 64;
+constexpr int SHADE_BLOCK = 128;
+
+__device__ __forceinline__ uint64_t rng_load(float4 aux) {
+    return ((uint64_t)__float_as_uint(aux.w) << 32) | (uint64_t)__float_as_uint(aux.z);
+}
+__device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+__device__ __forceinline__ bool finite3(f3 c) { return isfinite(c.x) && isfinite(c.y) && isfinite(c.z); }
+
+__device__ __forceinline__ void film_add(float4 *film, uint32_t pixel, f3 c) {
+    if (!finite3(c)) return;
+    float *p = reinterpret_cast<float *>(film + pixel);
+    unsafeAtomicAdd(p + 0, c.x);
+    unsafeAtomicAdd(p + 1, c.y);
+    unsafeAtomicAdd(p + 2, c.z);
+}
+
+// Wave-aggregated queue append: one atomic per wave; returns this lane's slot.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    uint32_t leader = __builtin_ctzll(mask);
+    uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    uint64_t below = mask & ((1ull << lane) - 1ull);
+    return base + (uint32_t)__popcll(below);
+}
+
+// Nori perspective camera, src/cpu/perspective.cpp:86-123, with the
+// sample-to-camera transform in closed form.
+__device__ void camera_ray(const nori_camera &c, float px, float py, float ax, float ay, f3 &o, f3 &d, float &mint,
+                           float &maxt) {
+    float sx = px * (1.0f / (float)c.width), sy = py * (1.0f / (float)c.height);
+    float aspect = (float)c.width / (float)c.height;
+    float th = tanf(c.fov_deg * (PI / 180.0f) * 0.5f);
+    f3 dc = normalize(f3{(1.0f - 2.0f * sx) * th, (1.0f - 2.0f * sy) * th / aspect, 1.0f});
+    f3 oc = mk(0.0f);
+    if (c.lens_radius > 0.0f) {
+        float r = sqrtf(ax), s, cs;
+        sincosf(2.0f * PI * ay, &s, &cs);
+        f3 pl = f3{c.lens_radius * (r * cs), c.lens_radius * (r * s), 0.0f};
+        f3 pf = dc * (c.focal_distance / dc.z);
+        oc = pl;
+        dc = normalize(pf - pl);
+    }
+    float invz = 1.0f / dc.z;
+    const float *M = c.to_world;
+    o = f3{M[0] * oc.x + M[1] * oc.y + M[2] * oc.z + M[3], M[4] * oc.x + M[5] * oc.y + M[6] * oc.z + M[7],
+           M[8] * oc.x + M[9] * oc.y + M[10] * oc.z + M[11]};
+    d = f3{M[0] * dc.x + M[1] * dc.y + M[2] * dc.z, M[4] * dc.x + M[5] * dc.y + M[6] * dc.z,
+           M[8] * dc.x + M[9] * dc.y + M[10] * dc.z};
+    mint = c.near_clip * invz;
+    maxt = c.far_clip * invz;
+}
+
+// ------------------------------------------------------------------ kernels
+
+template <bool ANY>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_batch(SceneDev s, const float4 *__restrict__ rays,
+                                                             float4 *__restrict__ hits, uint32_t n) {
+    extern __shared__ uint2 stk[];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = rays[2 * i], b = rays[2 * i + 1];
+    TraceOut r = traverse_bvh8<ANY, false>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x);
+    uint32_t prim = ANY ? (r.prim == MISS ? MISS : 0u) : r.prim;
+    hits[i] = make_float4(r.t, __uint_as_float(prim), r.u, r.v);
+}
+
+__global__ void k_raygen(SceneDev s, PathsDev p, uint32_t *queue, ViewDev v, uint64_t seed, uint64_t first,
+                         uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t g = first + i;
+    uint32_t pixel = (uint32_t)(g % v.npix);
+    uint32_t sample = v.sample_offset + (uint32_t)(g / v.npix);
+    uint64_t rng = sample_stream(seed, pixel, sample);
+    float jx = pcg_float(rng), jy = pcg_float(rng);
+    float ax = 0.5f, ay = 0.5f;
+    if (v.cam.lens_radius > 0.0f) {
+        ax = pcg_float(rng);
+        ay = pcg_float(rng);
+    }
+    uint32_t x = pixel % v.cam.width, y = pixel / v.cam.width;
+    f3 o, d;
+    float mint, maxt;
+    camera_ray(v.cam, (float)x + jx, (float)y + jy, ax, ay, o, d, mint, maxt);
+    p.ray_o[i] = make_float4(o.x, o.y, o.z, mint);
+    p.ray_d[i] = make_float4(d.x, d.y, d.z, maxt);
+    p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p.aux[i] = make_float4(1.0f, __uint_as_float(pixel), __uint_as_float((uint32_t)rng),
+                           __uint_as_float((uint32_t)(rng >> 32)));
+    queue[i] = i;
+    unsafeAtomicAdd(reinterpret_cast<float *>(v.film + pixel) + 3, 1.0f);
+}
+
+// Adds this lane's traversal counts into stats[k], stats[k+1] (one atomic pair per wave).
+__device__ __forceinline__ void count_add(unsigned long long *stats, int k, uint32_t nodes, uint32_t tris) {
+    for (int off = 32; off > 0; off >>= 1) {
+        nodes += __shfl_xor(nodes, off);
+        tris += __shfl_xor(tris, off);
+    }
+    if (__lane_id() == 0) {
+        atomicAdd(stats + k, (unsigned long long)nodes);
+        atomicAdd(stats + k + 1, (unsigned long long)tris);
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, const uint32_t *__restrict__ queue,
+                                                        const uint32_t *__restrict__ count, unsigned long long *stats) {
+    extern __shared__ uint2 stk[];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *count;
+    if (COUNT ? (i - threadIdx.x % 64u) >= n : i >= n) return;
+    uint32_t nn = 0, nt = 0;
+    if (i < n) {
+        uint32_t slot = queue[i];
+        float4 a = p.ray_o[slot], b = p.ray_d[slot];
+        TraceOut r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x,
+                                                 blockDim.x, &nn, &nt);
+        p.hit[slot] = make_float4(r.t, __uint_as_float(r.prim), r.u, r.v);
+    }
+    if (COUNT) count_add(stats, 4, nn, nt);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
+                                                        float4 *film, unsigned long long *stats) {
+    extern __shared__ uint2 stk[];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = *count;
+    if (COUNT ? (i - threadIdx.x % 64u) >= n : i >= n) return;
+    uint32_t nn = 0, nt = 0;
+    if (i < n) {
+        float4 a = sh.o[i], b = sh.d[i];
+        TraceOut r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), 0.0f, a.w, stk + threadIdx.x,
+                                                blockDim.x, &nn, &nt);
+        if (r.prim == MISS) film_add(film, __float_as_uint(b.w), xyz(sh.c[i]));
+    }
+    if (COUNT) count_add(stats, 6, nn, nt);
+}
+
+__device__ __forceinline__ void push_shadow(ShadowDev &sh, uint32_t *cnt, bool pred, f3 o, float tmax, f3 d,
+                                            uint32_t pixel, f3 c) {
+    uint32_t k = wave_append(cnt, pred);
+    if (pred) {
+        sh.o[k] = make_float4(o.x, o.y, o.z, tmax);
+        sh.d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(pixel));
+        sh.c[k] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
+// One bounce of shaders/Shaders.hlsl:108-627 for every path in q_in.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, const uint32_t *__restrict__ q_in,
+                                                       const uint32_t *__restrict__ cnt_in, uint32_t *q_out,
+                                                       uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film,
+                                                       uint32_t bounce, uint32_t max_depth, uint32_t rr_depth) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *cnt_in) return;
+    const uint32_t slot = q_in[i];
+    const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], hit = p.hit[slot], th = p.thr[slot], ax = p.aux[slot];
+    const f3 o = xyz(ro), d = xyz(rd);
+    f3 T = xyz(th);
+    float last_pdf = th.w, eta = ax.x;
+    const uint32_t pixel = __float_as_uint(ax.y);
+    uint64_t rng = rng_load(ax);
+    const uint32_t prim = __float_as_uint(hit.y);
+
+    bool alive = false, sh_area = false, sh_env = false;
+    f3 new_o = mk(0.0f), new_d = mk(0.0f);
+    f3 so = mk(0.0f), a_dir = mk(0.0f), a_c = mk(0.0f), e_dir = mk(0.0f), e_c = mk(0.0f);
+    float a_tmax = 0.0f;
+
+    if (prim == MISS) {
+        if (s.has_env) {
+            f3 env = env_eval(s.env, d);
+            f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
+            film_add(film, pixel, c);
+        }
+    } else {
+        const f3 P = o + d * hit.x;
+        const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+        const f3 p0 = f3{s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
+        const f3 p1 = f3{s.pos[3 * i1], s.pos[3 * i1 + 1], s.pos[3 * i1 + 2]};
+        const f3 p2 = f3{s.pos[3 * i2], s.pos[3 * i2 + 1], s.pos[3 * i2 + 2]};
+        f3 Ng = normalize(cross_exact(sub_exact(p1, p0), sub_exact(p2, p0)));
+        f3 N = Ng;
+        if (s.nrm) {
+            f3 n0 = f3{s.nrm[3 * i0], s.nrm[3 * i0 + 1], s.nrm[3 * i0 + 2]};
+            f3 n1 = f3{s.nrm[3 * i1], s.nrm[3 * i1 + 1], s.nrm[3 * i1 + 2]};
+            f3 n2 = f3{s.nrm[3 * i2], s.nrm[3 * i2 + 1], s.nrm[3 * i2 + 2]};
+            f3 n = n0 * (1.0f - hit.z - hit.w) + n1 * hit.z + n2 * hit.w;
+            float l2 = dot(n, n);
+            N = l2 < 1e-8f ? Ng : normalize(n);
+        }
+        const bool back = dot(Ng, d) > 0.0f;
+        if (back) Ng = -Ng;
+        if (dot(N, Ng) < 0.0f) N = -N;
+        const uint32_t mesh = s.tri_mesh[prim];
+        const nori_material &mr = s.mats[mesh];
+        if (mr.isEmitter) {
+            f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
+            if (last_pdf == 0.0f) {
+                film_add(film, pixel, T * Le);
+            } else {
+                // shaders/Emitter.hlsli:91-103
+                f3 dv = P - o;
+                float d2 = dot(dv, dv), pe = 0.0f;
+                if (d2 >= 1e-12f) {
+                    float dist = sqrtf(d2);
+                    float cl = fabsf(dot(N, -dv / dist));
+                    if (cl >= 1e-8f) {
+                        uint32_t k = 0;
+                        while (k + 1 < s.n_emit && s.emitters[k] != mesh) ++k;
+                        pe = (1.0f / (s.emit_area[k] * (float)s.n_emit)) * d2 / cl;
+                    }
+                }
+                film_add(film, pixel, T * Le * balance_heuristic(last_pdf, pe));
+            }
+        } else {
+            f3 Tg, B;
+            build_onb(N, Tg, B);
+            {
+                f3 V = -d;
+                float ndv = dot(N, V);
+                if (ndv <= 0.0f && dot(V, Ng) > 0.0f) {
+                    N = normalize(N + V * (1e-3f - ndv));
+                    build_onb(N, Tg, B);
+                }
+            }
+            const f3 wi = to_local(-d, Tg, B, N);
+            const uint32_t type = mr.type;
+            if (type == NORI_MAT_MIRROR) {
+                new_o = P + Ng * 0.001f;
+                new_d = reflect(d, N);
+                last_pdf = 0.0f;
+                alive = true;
+            } else if (type == NORI_MAT_DIELECTRIC) {
+                float etaI = back ? mr.intIOR : mr.extIOR, etaT = back ? mr.extIOR : mr.intIOR;
+                float Fr = fresnel_dielectric(dot(-d, N), etaI, etaT);
+                f3 nd;
+                bool refr = false;
+                if (pcg_float(rng) < Fr) {
+                    nd = reflect(d, N);
+                } else {
+                    nd = refract(d, N, etaI / etaT);
+                    if (dot(nd, nd) < 0.001f) nd = reflect(d, N);
+                    else refr = true;
+                }
+                new_o = P + (dot(nd, Ng) > 0.0f ? Ng : -Ng) * 0.001f;
+                new_d = nd;
+                last_pdf = 0.0f;
+                if (refr) eta *= etaI / etaT;
+                alive = true;
+            } else if (type == NORI_MAT_DIFFUSE || type == NORI_MAT_MICROFACET || type == NORI_MAT_DISNEY) {
+                const Mat m = load_mat(mr);
+                so = offset_ray_origin(P, Ng, Ng);
+                // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
+                if (s.n_emit > 0) {
+                    uint32_t pick = min((uint32_t)(pcg_float(rng) * (float)s.n_emit), s.n_emit - 1);
+                    uint32_t em = s.emitters[pick];
+                    uint32_t ntri = s.mats[em].indexCount / 3;
+                    const float *cdf = s.emit_cdf + s.emit_cdf_off[pick];
+                    float uc = pcg_float(rng);
+                    uint32_t lo = 0, hi = ntri;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (cdf[mid] <= uc) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    uint32_t k = min((uint32_t)max(0, (int)lo - 1), ntri - 1);
+                    uint32_t ef = s.mats[em].indexOffset / 3 + k;
+                    uint32_t j0 = s.idx[3 * ef], j1 = s.idx[3 * ef + 1], j2 = s.idx[3 * ef + 2];
+                    f3 q0 = f3{s.pos[3 * j0], s.pos[3 * j0 + 1], s.pos[3 * j0 + 2]};
+                    f3 q1 = f3{s.pos[3 * j1], s.pos[3 * j1 + 1], s.pos[3 * j1 + 2]};
+                    f3 q2 = f3{s.pos[3 * j2], s.pos[3 * j2 + 1], s.pos[3 * j2 + 2]};
+                    float r1 = pcg_float(rng), r2 = pcg_float(rng);
+                    float sr1 = sqrtf(r1);
+                    f3 ep = (1.0f - sr1) * q0 + sr1 * (1.0f - r2) * q1 + sr1 * r2 * q2;
+                    f3 en = normalize(cross(q1 - q0, q2 - q0));
+                    float pdfA = 1.0f / (s.emit_area[pick] * (float)s.n_emit);
+                    f3 tl = ep - P;
+                    float dist = length(tl);
+                    f3 ww = tl / dist;
+                    float ct = dot(N, ww), cl = dot(en, -ww);
+                    if (ct > 0.0f && cl > 0.0f) {
+                        f3 f;
+                        float pb;
+                        material_eval_pdf(wi, to_local(ww, Tg, B, N), m, f, pb);
+                        float pe = pdfA * dist * dist / cl;
+                        float w = balance_heuristic(pe, pb);
+                        f3 Le = f3{s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]};
+                        a_c = T * (Le * f * (ct / fmaxf(pe, 1e-20f) * w));
+                        a_dir = ww;
+                        a_tmax = dist - 0.001f;
+                        sh_area = true;
+                    }
+                }
+                // Envmap NEE, shaders/Emitter.hlsli:275-326 (alias-table draw)
+                if (s.has_env) {
+                    uint32_t r = pcg_next(rng);
+                    float coin = pcg_float(rng), jx = pcg_float(rng), jy = pcg_float(rng);
+                    f3 dir, Le;
+                    float pe;
+                    env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
+                    float ct = dot(N, dir);
+                    if (pe > 0.0f && ct > 0.0f) {
+                        f3 f;
+                        float pb;
+                        material_eval_pdf(wi, to_local(dir, Tg, B, N), m, f, pb);
+                        float w = balance_heuristic(pe, pb);
+                        e_c = T * (Le * f * (ct / fmaxf(pe, 1e-20f) * w));
+                        e_dir = dir;
+                        sh_env = true;
+                    }
+                }
+                float u[3];
+                u[0] = pcg_float(rng);
+                u[1] = pcg_float(rng);
+                u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
+                f3 wo;
+                float pdf;
+                f3 wgt = material_sample(wi, u, m, wo, pdf);
+                if (pdf > 0.0f && !is_zero(wgt)) {
+                    new_o = so;
+                    new_d = to_world(wo, Tg, B, N);
+                    T *= wgt;
+                    last_pdf = fmaxf(pdf, 1e-20f);
+                    alive = true;
+                }
+            }
+        }
+    }
+    push_shadow(sh, cnt_sh, sh_area, so, a_tmax, a_dir, pixel, a_c);
+    push_shadow(sh, cnt_sh, sh_env, so, 1e20f, e_dir, pixel, e_c);
+
+    // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not
+    // need it: the path ends either way.
+    const bool more = bounce + 1 < max_depth;
+    if (alive && more && bounce >= rr_depth) {
+        float q = fminf(maxc(T) * eta * eta, 0.95f);
+        if (pcg_float(rng) >= q) alive = false;
+        else T = T * (1.0f / q);
+    }
+    alive = alive && more;
+    uint32_t k = wave_append(cnt_out, alive);
+    if (alive) {
+        p.ray_o[slot] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+        p.ray_d[slot] = make_float4(new_d.x, new_d.y, new_d.z, 1e20f);
+        p.thr[slot] = make_float4(T.x, T.y, T.z, last_pdf);
+        p.aux[slot] = make_float4(eta, ax.y, __uint_as_float((uint32_t)rng), __uint_as_float((uint32_t)(rng >> 32)));
+        q_out[k] = slot;
+    }
+}
+
+__global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (clear_mask & 0x10u)
+        for (int k = 0; k < 8; ++k) c.stats[k] = 0;
+    if (ext_from < 2) {
+        c.stats[0] += c.cnt[ext_from];
+        c.stats[3] += 1;
+    }
+    if (add_shadow) c.stats[1] += c.cnt[2];
+    for (int k = 0; k < 4; ++k)
+        if (clear_mask & (1u << k)) c.cnt[k] = 0;
+}
+
+__global__ void k_set_counter(uint32_t *cnt, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *cnt = v;
+}
+
+__global__ void k_zero_f4(float4 *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// --------------------------------------------------------------- checks
+
+__global__ void k_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo,
+                             const float *u, float *rgb, float *pdf, float *wo_out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mat m = load_mat(mats[i]);
+    f3 a = f3{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]};
+    if (op == NORI_BSDF_EVAL || op == NORI_BSDF_PDF) {
+        f3 b = f3{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]}, f;
+        float pd;
+        material_eval_pdf(a, b, m, f, pd);
+        if (rgb) { rgb[3 * i] = f.x; rgb[3 * i + 1] = f.y; rgb[3 * i + 2] = f.z; }
+        if (pdf) pdf[i] = pd;
+    } else {
+        float uu[3] = {u[3 * i], u[3 * i + 1], u[3 * i + 2]};
+        f3 w, o;
+        float pd;
+        w = material_sample(a, uu, m, o, pd);
+        if (rgb) { rgb[3 * i] = w.x; rgb[3 * i + 1] = w.y; rgb[3 * i + 2] = w.z; }
+        if (pdf) pdf[i] = pd;
+        if (wo_out) { wo_out[3 * i] = o.x; wo_out[3 * i + 1] = o.y; wo_out[3 * i + 2] = o.z; }
+    }
+}
+
+__global__ void k_env_sample(EnvDev e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,
+                             const float *jy, float *dir, float *rgb, float *pdf, uint32_t *texel) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 d, L;
+    float p;
+    uint32_t t = env_sample(e, r[i], coin[i], jx[i], jy[i], d, L, p);
+    dir[3 * i] = d.x; dir[3 * i + 1] = d.y; dir[3 * i + 2] = d.z;
+    rgb[3 * i] = L.x; rgb[3 * i + 1] = L.y; rgb[3 * i + 2] = L.z;
+    pdf[i] = p;
+    texel[i] = t;
+}
+
+__global__ void k_env_eval(EnvDev e, uint32_t n, const float *dir, float *rgb, float *pdf) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 d = f3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+    f3 L = env_eval(e, d);
+    rgb[3 * i] = L.x; rgb[3 * i + 1] = L.y; rgb[3 * i + 2] = L.z;
+    pdf[i] = env_pdf(e, d);
+}
+
+__global__ void k_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t inc = (initseq << 1u) | 1u;
+    uint64_t st = pcg_advance(pcg_seed(initstate, initseq), advance, inc);
+    for (uint32_t i = 0; i < n; ++i) out[i] = pcg_next(st, inc);
+}
+
+inline uint32_t blocks(uint64_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+
+} // namespace
+
+void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uint32_t n, int any_hit, hipStream_t st) {
+    if (n == 0) return;
+    size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
+    if (any_hit)
+        hipLaunchKernelGGL(k_trace_batch<true>, dim3(blocks(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, rays, hits, n);
+    else
+        hipLaunchKernelGGL(k_trace_batch<false>, dim3(blocks(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, rays, hits, n);
+}
+
+void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counters c, const ViewDev &v, uint64_t seed,
+                   uint64_t first, uint32_t n, hipStream_t st) {
+    (void)c;
+    hipLaunchKernelGGL(k_raygen, dim3(blocks(n, 256)), dim3(256), 0, st, s, p, queue, v, seed, first, n);
+}
+
+void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, uint32_t cap,
+                   unsigned long long *count_stats, hipStream_t st) {
+    size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
+    if (count_stats)
+        hipLaunchKernelGGL(k_extend<true>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, queue,
+                           count, count_stats);
+    else
+        hipLaunchKernelGGL(k_extend<false>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, queue,
+                           count, count_stats);
+}
+
+void launch_shade(const SceneDev &s, const PathsDev &p, const uint32_t *q_in, const uint32_t *cnt_in, uint32_t *q_out,
+                  uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film, uint32_t bounce,
+                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_shade, dim3(blocks(cap, SHADE_BLOCK)), dim3(SHADE_BLOCK), 0, st, s, p, q_in, cnt_in, q_out,
+                       cnt_out, sh, cnt_sh, film, bounce, max_depth, rr_depth);
+}
+
+void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
+                   unsigned long long *count_stats, hipStream_t st) {
+    size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
+    if (count_stats)
+        hipLaunchKernelGGL(k_shadow<true>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh,
+                           film, count_stats);
+    else
+        hipLaunchKernelGGL(k_shadow<false>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh,
+                           film, count_stats);
+}
+
+void launch_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st) {
+    hipLaunchKernelGGL(k_counters, dim3(1), dim3(64), 0, st, c, ext_from, add_shadow, clear_mask);
+}
+
+void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_counter, dim3(1), dim3(64), 0, st, cnt, value);
+}
+
+void launch_zero_f4(float4 *p, size_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_zero_f4, dim3(blocks(n, 256)), dim3(256), 0, st, p, n);
+}
+
+void launch_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo, const float *u,
+                       float *rgb, float *pdf, float *wo_out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_bsdf_batch, dim3(blocks(n, 256)), dim3(256), 0, st, op, n, mats, wi, wo, u, rgb, pdf, wo_out);
+}
+
+void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,
+                       const float *jy, float *dir, float *rgb, float *pdf, uint32_t *texel, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_env_sample, dim3(blocks(n, 256)), dim3(256), 0, st, e, n, r, coin, jx, jy, dir, rgb, pdf, texel);
+}
+
+void launch_env_eval(const EnvDev &e, uint32_t n, const float *dir, float *rgb, float *pdf, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_env_eval, dim3(blocks(n, 256)), dim3(256), 0, st, e, n, dir, rgb, pdf);
+}
+
+void launch_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg32, dim3(1), dim3(64), 0, st, initstate, initseq, advance, n, out);
+}
+
+} // namespace ng

@@ -1,0 +1,261 @@
+// Seeded procedural scenes for the benchmark configurations, produced in the
+// flat layout of nori_scene_desc (one material record per mesh, global vertex
+// indices).  The generator stream is pcg32.seed(0x5EED0000 + config, 1) unless
+// the caller gives a seed.
+//
+//   config 1  Cornell box: floor, ceiling, back, red left and green right wall
+//             (2 triangles each), two white icospheres, a ceiling quad light with
+//             radiance (17, 12, 4); pinhole camera with a 40 degree horizontal fov.
+//   config 2  grid x grid Disney icospheres on a Lambertian ground quad, lit by a
+//             sky-gradient envmap with a 0.5 degree sun disc of radiance 5e4 in
+//             a seeded direction.
+#include "scene_gen.h"
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+namespace ng {
+
+namespace {
+
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) { s = pcg_seed(seed, 1); }
+    float next() {
+        uint64_t inc = (1ull << 1u) | 1u;
+        uint32_t u = (pcg_next(s, inc) >> 9) | 0x3f800000u;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f - 1.0f;
+    }
+};
+
+// Default record: the BSDFGPUData defaults of include/bsdf.h:46-89.
+nori_material default_material() {
+    nori_material m;
+    std::memset(&m, 0, sizeof(m));
+    m.type = NORI_MAT_DIFFUSE;
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = 0.5f;
+    m.intIOR = 1.5046f;
+    m.extIOR = 1.000277f;
+    m.alpha = 0.1f;
+    m.albedoTexIndex = m.normalTexIndex = m.roughnessTexIndex = m.metallicTexIndex = NORI_NO_TEXTURE;
+    m.specularTexIndex = m.subsurfaceTexIndex = m.alphaTexIndex = NORI_NO_TEXTURE;
+    m.roughness = 0.5f;
+    m.specular = 0.5f;
+    m.sheenTint = 0.5f;
+    m.clearcoatGloss = 1.0f;
+    m.betaN = 0.3f;
+    return m;
+}
+
+struct Builder {
+    SceneData &s;
+    uint32_t begin_mesh() { return (uint32_t)s.mesh_tri_offset.size(); }
+    void add_mesh(const std::vector<float> &pos, const std::vector<float> &nrm, const std::vector<uint32_t> &tri,
+                  const nori_material &mat) {
+        uint32_t v0 = (uint32_t)(s.pos.size() / 3), t0 = (uint32_t)(s.idx.size() / 3);
+        s.pos.insert(s.pos.end(), pos.begin(), pos.end());
+        s.nrm.insert(s.nrm.end(), nrm.begin(), nrm.end());
+        for (uint32_t i : tri) s.idx.push_back(i + v0);
+        nori_material m = mat;
+        m.vertexOffset = v0;
+        m.vertexCount = (uint32_t)(pos.size() / 3);
+        m.indexOffset = 3 * t0;
+        m.indexCount = (uint32_t)tri.size();
+        s.mesh_tri_offset.push_back(t0);
+        s.mesh_tri_count.push_back((uint32_t)(tri.size() / 3));
+        s.mats.push_back(m);
+    }
+    // Axis-aligned quad p, p+a, p+a+b, p+b; triangle normal = normalize(a x b).
+    void quad(f3 p, f3 a, f3 b, const nori_material &mat) {
+        f3 n = normalize(cross(a, b));
+        f3 q[4] = {p, p + a, p + a + b, p + b};
+        std::vector<float> pos, nrm;
+        for (auto &v : q) {
+            pos.insert(pos.end(), {v.x, v.y, v.z});
+            nrm.insert(nrm.end(), {n.x, n.y, n.z});
+        }
+        add_mesh(pos, nrm, {0, 1, 2, 0, 2, 3}, mat);
+    }
+    void icosphere(f3 c, float r, uint32_t level, const nori_material &mat) {
+        std::vector<float> pos, nrm;
+        std::vector<uint32_t> tri;
+        icosphere_mesh(level, pos, tri);
+        nrm = pos;
+        for (size_t i = 0; i < pos.size(); i += 3) {
+            pos[i] = c.x + r * pos[i];
+            pos[i + 1] = c.y + r * pos[i + 1];
+            pos[i + 2] = c.z + r * pos[i + 2];
+        }
+        add_mesh(pos, nrm, tri, mat);
+    }
+};
+
+void lookat(f3 eye, f3 target, f3 up, float *M) {
+    f3 dir = normalize(target - eye);
+    f3 left = normalize(cross(up, dir));
+    f3 nup = cross(dir, left);
+    float m[16] = {left.x, nup.x, dir.x, eye.x, left.y, nup.y, dir.y, eye.y,
+                   left.z, nup.z, dir.z, eye.z, 0, 0, 0, 1};
+    std::memcpy(M, m, sizeof(m));
+}
+
+} // namespace
+
+// Unit icosphere: the 20 faces of the icosahedron, each split into 4 per level
+// with shared, renormalised edge midpoints.  level L gives 20*4^L triangles.
+void icosphere_mesh(uint32_t level, std::vector<float> &pos, std::vector<uint32_t> &tri) {
+    const float t = (1.0f + std::sqrt(5.0f)) * 0.5f;
+    std::vector<f3> v = {mk(-1, t, 0), mk(1, t, 0),  mk(-1, -t, 0), mk(1, -t, 0), mk(0, -1, t),  mk(0, 1, t),
+                         mk(0, -1, -t), mk(0, 1, -t), mk(t, 0, -1),  mk(t, 0, 1),  mk(-t, 0, -1), mk(-t, 0, 1)};
+    for (auto &p : v) p = normalize(p);
+    std::vector<uint32_t> f = {0, 11, 5, 0, 5,  1, 0, 1, 7, 0, 7,  10, 0, 10, 11, 1, 5, 9, 5, 11,
+                               4, 11, 10, 2, 10, 7, 6, 7, 1, 8, 3,  9, 4, 3, 4,  2, 3, 2, 6, 3,
+                               6, 8, 3,  8, 9,  4, 9, 5, 2, 4, 11, 6, 2, 10, 8,  6, 7, 9, 8, 1};
+    for (uint32_t l = 0; l < level; ++l) {
+        std::map<uint64_t, uint32_t> mid;
+        auto midpoint = [&](uint32_t a, uint32_t b) {
+            uint64_t key = a < b ? ((uint64_t)a << 32) | b : ((uint64_t)b << 32) | a;
+            auto it = mid.find(key);
+            if (it != mid.end()) return it->second;
+            f3 m = normalize((v[a] + v[b]) * 0.5f);
+            v.push_back(m);
+            uint32_t id = (uint32_t)v.size() - 1;
+            mid[key] = id;
+            return id;
+        };
+        std::vector<uint32_t> nf;
+        nf.reserve(f.size() * 4);
+        for (size_t i = 0; i < f.size(); i += 3) {
+            uint32_t a = f[i], b = f[i + 1], c = f[i + 2];
+            uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+            nf.insert(nf.end(), {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca});
+        }
+        f.swap(nf);
+    }
+    pos.clear();
+    for (auto &p : v) pos.insert(pos.end(), {p.x, p.y, p.z});
+    tri = f;
+}
+
+void SceneData::finalize() {
+    // Surface areas (float sum of 0.5*|e1 x e2|, include/dpdf.h append order) and
+    // the uniform emitter selection probability (shaders/Emitter.hlsli:51,87).
+    uint32_t ne = 0;
+    for (auto &m : mats) ne += m.isEmitter ? 1 : 0;
+    for (size_t k = 0; k < mats.size(); ++k) {
+        float sum = 0.0f;
+        for (uint32_t i = 0; i < mesh_tri_count[k]; ++i) {
+            uint32_t f = mesh_tri_offset[k] + i;
+            const float *a = &pos[3 * idx[3 * f]], *b = &pos[3 * idx[3 * f + 1]], *c = &pos[3 * idx[3 * f + 2]];
+            f3 p0 = mk(a[0], a[1], a[2]), p1 = mk(b[0], b[1], b[2]), p2 = mk(c[0], c[1], c[2]);
+            sum += 0.5f * length(cross(p1 - p0, p2 - p0));
+        }
+        mats[k].surfaceArea = sum;
+        mats[k].emitterSelectionProb = mats[k].isEmitter && ne ? 1.0f / (float)ne : 0.0f;
+    }
+}
+
+SceneData generate_scene(const nori_scene_options &opt) {
+    SceneData s;
+    Builder b{s};
+    const int cfg = opt.config;
+    Rng rng(opt.seed ? opt.seed : 0x5EED0000ull + (uint64_t)cfg);
+    if (cfg == 1) {
+        uint32_t level = opt.subdiv ? opt.subdiv : 4;
+        nori_material white = default_material(), red = white, green = white, light = white;
+        white.albedo[0] = white.albedo[1] = white.albedo[2] = 0.73f;
+        red.albedo[0] = 0.65f; red.albedo[1] = 0.05f; red.albedo[2] = 0.05f;
+        green.albedo[0] = 0.12f; green.albedo[1] = 0.45f; green.albedo[2] = 0.15f;
+        b.quad(mk(-1, 0, 1), mk(2, 0, 0), mk(0, 0, -2), white);  // floor, normal +y
+        b.quad(mk(-1, 2, -1), mk(2, 0, 0), mk(0, 0, 2), white);  // ceiling, normal -y
+        b.quad(mk(-1, 0, -1), mk(2, 0, 0), mk(0, 2, 0), white);  // back wall, normal +z
+        b.quad(mk(-1, 0, 1), mk(0, 0, -2), mk(0, 2, 0), red);    // left wall, normal +x
+        b.quad(mk(1, 0, -1), mk(0, 0, 2), mk(0, 2, 0), green);   // right wall, normal -x
+        float r1 = 0.33f + 0.04f * rng.next(), r2 = 0.33f + 0.04f * rng.next();
+        b.icosphere(mk(-0.42f, r1, -0.30f + 0.1f * rng.next()), r1, level, white);
+        b.icosphere(mk(0.42f, r2, 0.25f + 0.1f * rng.next()), r2, level, white);
+        light.isEmitter = 1;
+        light.albedo[0] = light.albedo[1] = light.albedo[2] = 0.0f;
+        light.radiance[0] = 17.0f; light.radiance[1] = 12.0f; light.radiance[2] = 4.0f;
+        b.quad(mk(-0.25f, 1.999f, -0.25f), mk(0.5f, 0, 0), mk(0, 0, 0.5f), light); // normal -y
+        s.camera.fov_deg = 40.0f;
+        lookat(mk(0, 1, 3.7f), mk(0, 1, 0), mk(0, 1, 0), s.camera.to_world);
+        s.camera.width = opt.width ? opt.width : 256;
+        s.camera.height = opt.height ? opt.height : 256;
+    } else if (cfg == 2) {
+        uint32_t level = opt.subdiv ? opt.subdiv : 6, g = opt.grid ? opt.grid : 8;
+        nori_material ground = default_material();
+        ground.albedo[0] = ground.albedo[1] = ground.albedo[2] = 0.5f;
+        b.quad(mk(-20, 0, 20), mk(40, 0, 0), mk(0, 0, -40), ground);
+        for (uint32_t i = 0; i < g; ++i)
+            for (uint32_t j = 0; j < g; ++j) {
+                nori_material m = default_material();
+                m.type = NORI_MAT_DISNEY;
+                m.albedo[0] = rng.next(); m.albedo[1] = rng.next(); m.albedo[2] = rng.next();
+                m.metallic = rng.next();
+                m.specular = rng.next();
+                m.sheen = rng.next();
+                m.clearcoat = rng.next();
+                m.roughness = 0.05f + 0.95f * rng.next();
+                m.anisotropic = 0.8f * rng.next();
+                float x = ((float)i - 0.5f * (float)(g - 1)) * 1.0f, z = ((float)j - 0.5f * (float)(g - 1)) * 1.0f;
+                b.icosphere(mk(x, 0.4f, z), 0.4f, level, m);
+            }
+        uint32_t W = opt.env_width ? opt.env_width : 2048, H = opt.env_height ? opt.env_height : 1024;
+        float elev = (20.0f + 40.0f * rng.next()) * (PI / 180.0f), azim = 2.0f * PI * rng.next();
+        f3 sun = mk(std::cos(elev) * std::cos(azim), std::sin(elev), std::cos(elev) * std::sin(azim));
+        float cos_disc = std::cos(0.5f * PI / 180.0f);
+        s.env_w = W; s.env_h = H; s.env_scale = 1.0f;
+        s.env_rgb.resize((size_t)W * H * 3);
+        for (uint32_t y = 0; y < H; ++y) {
+            float theta = PI * ((float)y + 0.5f) / (float)H;
+            float ct = std::cos(theta), st = std::sin(theta);
+            f3 sky;
+            if (ct > 0.0f) sky = lerp3(mk(0.85f, 0.9f, 1.0f), mk(0.25f, 0.45f, 0.95f), std::sqrt(ct));
+            else sky = lerp3(mk(0.85f, 0.9f, 1.0f), mk(0.2f, 0.18f, 0.15f), std::sqrt(-ct));
+            for (uint32_t x = 0; x < W; ++x) {
+                float phi = 2.0f * PI * ((float)x + 0.5f) / (float)W;
+                f3 d = mk(st * std::cos(phi), ct, st * std::sin(phi));
+                f3 c = dot(d, sun) >= cos_disc ? mk(5e4f) : sky;
+                float *o = &s.env_rgb[((size_t)y * W + x) * 3];
+                o[0] = c.x; o[1] = c.y; o[2] = c.z;
+            }
+        }
+        s.camera.fov_deg = 50.0f;
+        lookat(mk(0, 4.5f, 9.5f), mk(0, 0.3f, 0), mk(0, 1, 0), s.camera.to_world);
+        s.camera.width = opt.width ? opt.width : 512;
+        s.camera.height = opt.height ? opt.height : 512;
+    } else {
+        throw std::invalid_argument("unknown scene config");
+    }
+    s.camera.near_clip = 1e-4f;
+    s.camera.far_clip = 1e4f;
+    s.camera.lens_radius = 0.0f;
+    s.camera.focal_distance = 1.0f;
+    s.finalize();
+    return s;
+}
+
+void SceneData::fill_desc(nori_scene_desc &d) const {
+    std::memset(&d, 0, sizeof(d));
+    d.num_vertices = (uint32_t)(pos.size() / 3);
+    d.num_triangles = (uint32_t)(idx.size() / 3);
+    d.num_meshes = (uint32_t)mats.size();
+    d.positions = pos.data();
+    d.normals = nrm.empty() ? nullptr : nrm.data();
+    d.uvs = uv.empty() ? nullptr : uv.data();
+    d.tangents = tan.empty() ? nullptr : tan.data();
+    d.indices = idx.data();
+    d.mesh_tri_offset = mesh_tri_offset.data();
+    d.mesh_tri_count = mesh_tri_count.data();
+    d.materials = mats.data();
+    d.env_width = env_w;
+    d.env_height = env_h;
+    d.env_rgb = env_rgb.empty() ? nullptr : env_rgb.data();
+    d.env_scale = env_scale;
+}
+
+} // namespace ng

@@ -1,0 +1,154 @@
+// Device-side traversal of the compressed BVH8 (bvh8.h), one ray per lane of a
+// wave64.  Replaces the DXR hardware traversal behind TraceClosestInline /
+// TraceShadowInline (shaders/RayQueryTrace.hlsli:36-129) and serves Nori's
+// Accel::rayIntersect semantics (src/cpu/accel.cpp:429-488):
+//
+//   * triangle test: Möller–Trumbore of src/cpu/mesh.cpp:51-89 on the stored
+//     (p0, e1, e2), evaluated without contraction so the result equals the CPU
+//     evaluation bit for bit;
+//   * closest hit: smallest t in [tmin, tmax], equal t resolved to the smaller
+//     global primitive id (independent of traversal order);
+//   * box test: planes decoded with one fma exactly as the builder verified
+//     them, tfar widened by (1 + 2*gamma_3) (Pharr, Jakob, Humphreys, PBRT 3e
+//     §3.9.2) so no box that contains an accepted hit is ever culled;
+//   * stack: the node groups (child base, hit bits | inner mask) of Ylitie et
+//     al. 2017, kept in LDS, one column per lane; its depth is the tree depth.
+#pragma once
+#include "nori_math.h"
+
+namespace ng {
+
+struct TraceOut {
+    float t, u, v;
+    uint32_t prim; // 0xFFFFFFFF: no hit
+};
+
+__device__ __forceinline__ float dot_exact(f3 a, f3 b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ f3 cross_exact(f3 a, f3 b) {
+#pragma clang fp contract(off)
+    return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 sub_exact(f3 a, f3 b) {
+#pragma clang fp contract(off)
+    return f3{a.x - b.x, a.y - b.y, a.z - b.z};
+}
+
+// Möller–Trumbore, src/cpu/mesh.cpp:51-89.
+__device__ __forceinline__ bool mt_test(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float tmin, float tmax, float &t, float &u,
+                                        float &v) {
+#pragma clang fp contract(off)
+    f3 pvec = cross_exact(d, e2);
+    float det = dot_exact(e1, pvec);
+    if (det > -1e-8f && det < 1e-8f) return false;
+    float inv_det = 1.0f / det;
+    f3 tvec = sub_exact(o, p0);
+    u = dot_exact(tvec, pvec) * inv_det;
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 qvec = cross_exact(tvec, e1);
+    v = dot_exact(d, qvec) * inv_det;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot_exact(e2, qvec) * inv_det;
+    return t >= tmin && t <= tmax;
+}
+
+__device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
+
+// Traverse; ANY = occlusion query (returns on the first accepted hit).
+// nodes: 5 uint4 per node; tris: 3 float4 per triangle; stk: this lane's LDS
+// column (entry i at stk[i * stride]).  counters (optional): nodes, triangles.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
+                                                  f3 d, float tmin, float tmax, uint2 *stk, uint32_t stride,
+                                                  uint32_t *nnodes = nullptr, uint32_t *ntris = nullptr) {
+    TraceOut res{tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
+    f3 rcp = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    const bool nx = rcp.x < 0.0f, ny = rcp.y < 0.0f, nz = rcp.z < 0.0f;
+    const uint32_t octinv = 7u - ((nx ? 4u : 0u) | (ny ? 2u : 0u) | (nz ? 1u : 0u));
+    const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
+    uint2 group = make_uint2(0u, (1u << (24u + octinv)) | 1u);
+    int sp = 0;
+    uint32_t cnt_nodes = 0, cnt_tris = 0;
+    while (true) {
+        if (group.y & 0xFF000000u) {
+            uint32_t bit = 31u - __builtin_clz(group.y & 0xFF000000u);
+            group.y &= ~(1u << bit);
+            uint32_t slot = (bit - 24u) ^ octinv;
+            uint32_t node_idx = group.x + __builtin_popcount((group.y & 0xFFu) & ((1u << slot) - 1u));
+            if (group.y & 0xFF000000u) {
+                stk[sp * stride] = group;
+                ++sp;
+            }
+            const uint4 *np = nodes + 5 * (size_t)node_idx;
+            uint4 a = np[0], b = np[1], c = np[2], dd = np[3], e = np[4];
+            if (COUNT) ++cnt_nodes;
+            f3 p = f3{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
+            f3 sc = f3{__uint_as_float(((a.w) & 0xFFu) << 23), __uint_as_float(((a.w >> 8) & 0xFFu) << 23),
+                       __uint_as_float(((a.w >> 16) & 0xFFu) << 23)};
+            uint32_t imask = a.w >> 24;
+            // near/far plane words per axis, chosen by the ray octant
+            uint32_t nxl = nx ? dd.z : c.x, nxh = nx ? dd.w : c.y, fxl = nx ? c.x : dd.z, fxh = nx ? c.y : dd.w;
+            uint32_t nyl = ny ? e.x : c.z, nyh = ny ? e.y : c.w, fyl = ny ? c.z : e.x, fyh = ny ? c.w : e.y;
+            uint32_t nzl = nz ? e.z : dd.x, nzh = nz ? e.w : dd.y, fzl = nz ? dd.x : e.z, fzh = nz ? dd.y : e.w;
+            uint32_t hitmask = 0;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                uint32_t meta = ((s < 4 ? b.z : b.w) >> (8 * (s & 3))) & 0xFFu;
+                if (meta == 0u) continue;
+                const int k = s & 3;
+                float qnx = byte_f(s < 4 ? nxl : nxh, k), qfx = byte_f(s < 4 ? fxl : fxh, k);
+                float qny = byte_f(s < 4 ? nyl : nyh, k), qfy = byte_f(s < 4 ? fyl : fyh, k);
+                float qnz = byte_f(s < 4 ? nzl : nzh, k), qfz = byte_f(s < 4 ? fzl : fzh, k);
+                float tnx = (__builtin_fmaf(qnx, sc.x, p.x) - o.x) * rcp.x;
+                float tfx = (__builtin_fmaf(qfx, sc.x, p.x) - o.x) * rcp.x;
+                float tny = (__builtin_fmaf(qny, sc.y, p.y) - o.y) * rcp.y;
+                float tfy = (__builtin_fmaf(qfy, sc.y, p.y) - o.y) * rcp.y;
+                float tnz = (__builtin_fmaf(qnz, sc.z, p.z) - o.z) * rcp.z;
+                float tfz = (__builtin_fmaf(qfz, sc.z, p.z) - o.z) * rcp.z;
+                float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                float tf = fminf(fminf(tfx, tfy), tfz) * far_scale;
+                tf = fminf(tf, res.t);
+                if (tn <= tf) {
+                    uint32_t is_inner = (meta & 0x18u) == 0x18u;
+                    uint32_t bitidx = is_inner ? ((meta ^ octinv) & 0x1Fu) : (meta & 0x1Fu);
+                    hitmask |= (meta >> 5) << bitidx;
+                }
+            }
+            group = make_uint2(b.x, (hitmask & 0xFF000000u) | imask);
+            uint32_t trimask = hitmask & 0x00FFFFFFu;
+            while (trimask) {
+                uint32_t tb = __builtin_ctz(trimask);
+                trimask &= trimask - 1u;
+                const float4 *tp = tris + 3 * (size_t)(b.y + tb);
+                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+                if (COUNT) ++cnt_tris;
+                float t, u, v;
+                if (mt_test(o, d, f3{t0.x, t0.y, t0.z}, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}, tmin, res.t, t, u,
+                            v)) {
+                    uint32_t prim = __float_as_uint(t0.w);
+                    if (t < res.t || prim < res.prim) {
+                        res.t = t;
+                        res.u = u;
+                        res.v = v;
+                        res.prim = prim;
+                        if (ANY) {
+                            if (COUNT) { *nnodes = cnt_nodes; *ntris = cnt_tris; }
+                            return res;
+                        }
+                    }
+                }
+            }
+        }
+        if (!(group.y & 0xFF000000u)) {
+            if (sp == 0) break;
+            --sp;
+            group = stk[sp * stride];
+        }
+    }
+    if (COUNT) { *nnodes = cnt_nodes; *ntris = cnt_tris; }
+    return res;
+}
+
+} // namespace ng

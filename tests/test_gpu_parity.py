@@ -1,0 +1,231 @@
+"""GPU parity: the HIP path, called through the C ABI, against the CPU oracle on
+identical inputs.  Integer/index results are compared bit for bit; floating
+point within the tolerance written in each test."""
+import sys
+
+import numpy as np
+import pytest
+
+import nori_gpu as ng
+import orc
+
+pytestmark = pytest.mark.gpu
+MISS = 0xFFFFFFFF
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_gpu_pcg32_matches_oracle(gpu):
+    _say("case: pcg32 known answer")
+    got = gpu.pcg32(42, 54, 0, 6)
+    assert [int(x) for x in got] == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+    for adv in (0, 5, 1 << 20, (7 << 20) + 3):
+        assert np.array_equal(gpu.pcg32(0x1234 ^ 77, 0xda3e39cb94b95bdb, adv, 64),
+                              orc.pcg32(0x1234 ^ 77, 0xda3e39cb94b95bdb, 64, advance=adv))
+
+
+def _random_rays(lo, hi, n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.c_[o, np.full(n, 1e-4), d, np.full(n, 1e4)].astype(np.float32)
+
+
+def _camera_rays(scene, w, h, seed):
+    rng = np.random.default_rng(seed)
+    cam = scene.camera
+    cam.width, cam.height = w, h
+    ys, xs = np.mgrid[0:h, 0:w]
+    smp = np.c_[xs.reshape(-1) + rng.random(w * h), ys.reshape(-1) + rng.random(w * h),
+                rng.random(w * h), rng.random(w * h)].astype(np.float32)
+    return orc.camera_rays(cam, smp)
+
+
+def _check_hits(gpu, oracle, rays, label):
+    _say(f"case: trace {label} n={len(rays)}")
+    gt, gp, gu, gv = gpu.trace(rays)
+    ot, op, ou, ov = oracle.trace(rays)
+    bad = np.nonzero(gp != op)[0]
+    assert len(bad) == 0, f"{label}: {len(bad)} primitive mismatches, first {bad[:5]}"
+    hit = op != MISS
+    assert np.array_equal(gt[hit].view(np.uint32), ot[hit].view(np.uint32)), label
+    assert np.array_equal(gu[hit].view(np.uint32), ou[hit].view(np.uint32)), label
+    assert np.array_equal(gv[hit].view(np.uint32), ov[hit].view(np.uint32)), label
+    _, ap, _, _ = gpu.trace(rays, any_hit=True)
+    assert np.array_equal(ap != MISS, hit), f"{label}: occlusion differs from closest hit"
+    return hit.mean()
+
+
+def test_trace_cornell_bit_exact(gpu, cornell):
+    gpu.upload(cornell)
+    o = orc.OracleScene(cornell)
+    frac = _check_hits(gpu, o, _camera_rays(cornell, 128, 128, 1), "cornell camera")
+    assert frac > 0.9
+    _check_hits(gpu, o, _random_rays([-0.95, 0.05, -0.95], [0.95, 1.95, 3.0], 1 << 17, 2), "cornell random")
+
+
+def test_trace_disney_grid_bit_exact(gpu, disney_small):
+    gpu.upload(disney_small)
+    o = orc.OracleScene(disney_small)
+    _check_hits(gpu, o, _camera_rays(disney_small, 128, 128, 3), "disney camera")
+    _check_hits(gpu, o, _random_rays([-2, 0.01, -2], [2, 1.5, 2], 1 << 17, 4), "disney random")
+
+
+@pytest.mark.slow
+def test_trace_million_triangles_bit_exact(gpu):
+    sc = ng.generate_scene(2, subdiv=5, env_width=64, env_height=32)  # 64 x 20480 triangles
+    gpu.upload(sc)
+    info = gpu.bvh_info()
+    _say(f"bvh8 nodes={info.nodes} depth={info.max_depth} build_ms={info.build_ms:.0f}")
+    o = orc.OracleScene(sc)
+    _check_hits(gpu, o, _camera_rays(sc, 256, 256, 5), "1.3M camera")
+    _check_hits(gpu, o, _random_rays([-4, 0.01, -4], [4, 1.5, 4], 1 << 18, 6), "1.3M random")
+
+
+def test_envmap_alias_draws_bit_exact(gpu, disney_small):
+    gpu.upload(disney_small)
+    o = orc.OracleScene(disney_small)
+    rng = np.random.default_rng(9)
+    n = 1 << 18
+    r = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    coin, jx, jy = (rng.random(n).astype(np.float32) for _ in range(3))
+    _say("case: envmap alias draws")
+    gd, gL, gpdf, gtex = gpu.envmap_sample(r, coin, jx, jy)
+    od, oL, opdf, otex = o.env_sample(r, coin, jx, jy, mode=0)
+    assert np.array_equal(gtex, otex)
+    assert np.allclose(gd, od, atol=2e-6)
+    assert np.allclose(gpdf, opdf, rtol=2e-5)
+    assert np.allclose(gL, oL, rtol=2e-5, atol=1e-6)
+    # pdf query of the sampled direction equals the sampling pdf
+    eL, epdf = gpu.envmap_eval(gd)
+    inner = (jx > 0.01) & (jx < 0.99) & (jy > 0.01) & (jy < 0.99)
+    assert np.allclose(epdf[inner], gpdf[inner], rtol=1e-4)
+
+
+def _bsdf_tuples(n, seed, kind):
+    rng = np.random.default_rng(seed)
+    m = ng.default_materials(n)
+    m["albedo"] = rng.random((n, 3))
+    if kind == "disney":
+        m["type"] = ng.MAT_DISNEY
+        for k in ("metallic", "specular", "sheen", "clearcoat", "specularTint", "sheenTint", "clearcoatGloss",
+                  "subsurface"):
+            m[k] = rng.random(n)
+        m["roughness"] = 0.05 + 0.95 * rng.random(n)
+        m["anisotropic"] = 0.8 * rng.random(n)
+    elif kind == "microfacet":
+        m["type"] = ng.MAT_MICROFACET
+        m["alpha"] = 0.05 + 0.5 * rng.random(n)
+        m["albedo"] *= 0.5
+    else:
+        m["type"] = ng.MAT_DIFFUSE
+
+    def hemi():
+        v = rng.normal(size=(n, 3))
+        v[:, 2] = np.abs(v[:, 2])
+        return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+    return m, hemi(), hemi(), rng.random((n, 3)).astype(np.float32)
+
+
+def _rel(a, b, floor):
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+@pytest.mark.parametrize("kind", ["disney", "microfacet", "diffuse"])
+def test_bsdf_eval_pdf_vs_fp64(gpu, kind):
+    # FP32 GPU against the FP64 oracle on the same FP32 inputs.  Tolerance: 1e-5
+    # relative for 99.9% of the tuples, and for all of them the error is within
+    # 1e-5 relative plus 64 float ulps of the largest lobe value (the
+    # conditioning floor of the half-vector normalisation at grazing angles).
+    n = 1 << 20
+    m, wi, wo, _ = _bsdf_tuples(n, 21, kind)
+    _say(f"case: bsdf eval/pdf {kind} n={n}")
+    g_rgb, g_pdf, _ = gpu.bsdf_batch(0, m, wi, wo)  # op 0 writes f and pdf
+    o_rgb, _, _ = orc.bsdf_batch(0, m, wi, wo)
+    _, o_pdf, _ = orc.bsdf_batch(1, m, wi, wo)
+    for g, o, what in ((g_rgb, o_rgb, "f"), (g_pdf, o_pdf, "pdf")):
+        scale = np.abs(o).max(axis=-1, keepdims=True) if o.ndim > 1 else np.abs(o)
+        rel = _rel(g, o, 1e-30)
+        ok = rel <= 1e-5
+        frac = ok.mean()
+        _say(f"  {kind} {what}: frac<=1e-5 {frac:.6f} max rel {rel[np.abs(o) > 1e-6].max():.3g}")
+        assert frac >= 0.999, (kind, what, frac)
+        assert np.all(np.abs(g - o) <= 1e-5 * np.abs(o) + 64 * 1.2e-7 * scale + 1e-30), (kind, what)
+
+
+@pytest.mark.parametrize("kind", ["disney", "microfacet", "diffuse"])
+def test_bsdf_sample_vs_fp64(gpu, kind):
+    n = 1 << 19
+    m, wi, _, u = _bsdf_tuples(n, 22, kind)
+    _say(f"case: bsdf sample {kind} n={n}")
+    g_w, g_pdf, g_wo = gpu.bsdf_batch(2, m, wi, u=u)
+    o_w, o_pdf, o_wo = orc.bsdf_batch(2, m, wi, u=u)
+    both = (g_pdf > 0) & (o_pdf > 0)
+    assert np.mean((g_pdf > 0) == (o_pdf > 0)) > 0.9999
+    assert np.allclose(g_wo[both], o_wo[both], atol=2e-5)
+    rel = _rel(g_pdf[both], o_pdf[both], 1e-30)
+    assert np.quantile(rel, 0.99) < 1e-4
+    relw = _rel(g_w[both], o_w[both], 1e-6)
+    assert np.quantile(relw, 0.99) < 1e-4
+
+
+def _render_pair(gpu, scene, w, h, spp, depth, seed=7):
+    gpu.upload(scene)
+    cam = scene.camera
+    cam.width, cam.height = w, h
+    gpu.reset_film()
+    g = gpu.render(ng.make_view(cam, spp, depth), seed=seed)
+    o, _ = orc.OracleScene(scene).render(cam, spp, depth, seed=seed)
+    return g, o
+
+
+def test_render_cornell_same_samples(gpu, cornell):
+    # Same pcg32 streams on both sides: the images agree sample for sample except
+    # for the rare path whose float rounding flips a branch.
+    _say("case: render cornell 64x64 32spp")
+    g, o = _render_pair(gpu, cornell, 64, 64, 32, 8)
+    assert np.array_equal(g[:, 3], o[:, 3]) and np.all(g[:, 3] == 32)
+    gi, oi = g[:, :3] / 32, o[:, :3] / 32
+    assert orc.rel_mse(gi, oi) < 1e-5
+    close = np.all(np.abs(gi - oi) <= 1e-3 * np.abs(oi) + 1e-5, axis=1)
+    assert close.mean() > 0.97
+
+
+def test_render_disney_same_samples(gpu, disney_small):
+    _say("case: render disney 64x64 16spp")
+    g, o = _render_pair(gpu, disney_small, 64, 64, 16, 12)
+    assert np.all(g[:, 3] == 16)
+    gi, oi = g[:, :3] / 16, o[:, :3] / 16
+    assert orc.rel_mse(gi, oi) < 1e-3
+    close = np.all(np.abs(gi - oi) <= 1e-3 * np.abs(oi) + 1e-5, axis=1)
+    assert close.mean() > 0.95
+
+
+def test_render_cornell_converges_to_oracle(gpu, cornell):
+    # Independent seeds, 1024 spp each: the GPU image converges to the oracle's.
+    _say("case: render cornell 48x48 1024spp independent seeds")
+    gpu.upload(cornell)
+    cam = cornell.camera
+    cam.width, cam.height = 48, 48
+    gpu.reset_film()
+    g = gpu.render(ng.make_view(cam, 1024, 8), seed=101)
+    o, _ = orc.OracleScene(cornell).render(cam, 1024, 8, seed=202)
+    assert orc.rel_mse(g[:, :3] / 1024, o[:, :3] / 1024) < 1e-3
+
+
+def test_render_progressive_offsets(gpu, cornell):
+    # Two renders of 8 spp at sample offsets 0 and 8 equal one render of 16 spp.
+    gpu.upload(cornell)
+    cam = cornell.camera
+    cam.width, cam.height = 32, 32
+    gpu.reset_film()
+    one = gpu.render(ng.make_view(cam, 16, 8), seed=3)
+    gpu.reset_film()
+    gpu.render(ng.make_view(cam, 8, 8), seed=3)
+    two = gpu.render(ng.make_view(cam, 8, 8, sample_offset=8), seed=3)
+    assert np.allclose(one, two, rtol=1e-5, atol=1e-6)
