@@ -161,3 +161,102 @@ def rel_mse(img, ref, eps=1e-2):
     """relMSE of validation_tests/compare_exr.py:101-103: mean((img-ref)^2 / (ref^2 + eps))."""
     img, ref = np.asarray(img, np.float64), np.asarray(ref, np.float64)
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + eps)))
+
+
+# ------------------------------------------------------------------ guiding
+def _guide_lib():
+    L = lib()
+    if not getattr(L, "_guide_init", False):
+        vp, i32 = ctypes.c_void_p, ctypes.c_int
+        L.orc_guide_nparams.restype = i32
+        L.orc_guide_features.argtypes = [i32, vp, vp, vp, vp, vp, vp]
+        L.orc_guide_forward.argtypes = [vp, i32, vp, i32, vp]
+        L.orc_guide_pdf.argtypes = [i32, vp, vp, vp, vp, vp, vp]
+        L.orc_guide_loss_dout.argtypes = [i32, vp, vp, vp, vp, vp, vp]
+        L.orc_guide_loss_grad.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+        L.orc_guide_loss_grad.restype = ctypes.c_double
+        L.orc_guide_adam.argtypes = [vp, vp, vp, vp, i32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float]
+        L.orc_vmf_sample.argtypes = [i32, vp, vp, vp, vp, vp]
+        L.orc_bf16_round.argtypes, L.orc_bf16_round.restype = [ctypes.c_float], ctypes.c_float
+        L._guide_init = True
+    return L
+
+
+GUIDE_PARAMS = 4 * (64 * 64 + 64)
+
+
+def guide_features(pos01, nrm, wi, rough, metal):
+    n = len(pos01)
+    f = np.zeros((n, 64), np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (pos01, nrm, wi, rough, metal)]
+    _guide_lib().orc_guide_features(n, *[_p(x) for x in a], _p(f))
+    return f
+
+
+def guide_forward(params, feat, bf16=False):
+    feat = np.ascontiguousarray(feat, np.float32)
+    out = np.zeros((len(feat), 64))
+    _guide_lib().orc_guide_forward(_p(np.ascontiguousarray(params, np.float32)), len(feat), _p(feat), int(bf16),
+                                   _p(out))
+    return out
+
+
+def guide_pdf(out, w3, b):
+    out = np.ascontiguousarray(out, np.float64)
+    n = len(out)
+    P, q, c = np.zeros(n), np.zeros(n), np.zeros(n)
+    _guide_lib().orc_guide_pdf(n, _p(out), _p(np.ascontiguousarray(w3, np.float32)),
+                               _p(np.ascontiguousarray(b, np.float32)), _p(P), _p(q), _p(c))
+    return P, q, c
+
+
+def guide_loss_dout(out, w3, b, wt):
+    out = np.ascontiguousarray(out, np.float64)
+    n = len(out)
+    g, loss = np.zeros((n, 64)), np.zeros(n)
+    _guide_lib().orc_guide_loss_dout(n, _p(out), _p(np.ascontiguousarray(w3, np.float32)),
+                                     _p(np.ascontiguousarray(b, np.float32)), _p(np.ascontiguousarray(wt, np.float32)),
+                                     _p(g), _p(loss))
+    return loss, g
+
+
+def guide_loss_grad(params, feat, w3, b, wt):
+    grad = np.zeros(GUIDE_PARAMS)
+    loss = _guide_lib().orc_guide_loss_grad(
+        _p(np.ascontiguousarray(params, np.float32)), len(feat), _p(np.ascontiguousarray(feat, np.float32)),
+        _p(np.ascontiguousarray(w3, np.float32)), _p(np.ascontiguousarray(b, np.float32)),
+        _p(np.ascontiguousarray(wt, np.float32)), _p(grad))
+    return loss, grad
+
+
+def guide_adam(params, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    p, m, v = (np.ascontiguousarray(x, np.float32).copy() for x in (params, m, v))
+    _guide_lib().orc_guide_adam(_p(p), _p(m), _p(v), _p(np.ascontiguousarray(grad, np.float64)), step, lr, b1, b2,
+                                eps)
+    return p, m, v
+
+
+def vmf_sample(mu, kappa, u1, u2):
+    mu = np.ascontiguousarray(mu, np.float64)
+    n = len(mu)
+    w = np.zeros((n, 3))
+    _guide_lib().orc_vmf_sample(n, _p(mu), _p(np.ascontiguousarray(kappa, np.float64)),
+                                _p(np.ascontiguousarray(u1, np.float64)), _p(np.ascontiguousarray(u2, np.float64)),
+                                _p(w))
+    return w
+
+
+def bf16_round(x):
+    return _guide_lib().orc_bf16_round(x)
+
+
+def guide_init_params(seed):
+    """He-uniform initialisation (the same recipe the GPU context uses by default)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(GUIDE_PARAMS, np.float32)
+    for l in range(4):
+        base = l * (64 * 64 + 64)
+        lim = np.sqrt(6.0 / 64) * (0.1 if l == 3 else 1.0)
+        p[base:base + 4096] = rng.uniform(-lim, lim, 4096)
+    return p

@@ -59,7 +59,8 @@ uint32_t orc_scene_bvh_nodes(void *hp) {
     return n;
 }
 
-// rays: n x {ox,oy,oz,mint, dx,dy,dz,maxt}.  brute=1 tests every triangle.
+// rays: n x {ox,oy,oz,mint, dx,dy,dz,maxt}.  brute=1 tests every triangle;
+// brute=2 walks the BVH2 with Nori's exact slab comparisons.
 void orc_trace(void *hp, const float *rays, uint32_t n, int any_hit, int brute, float *t, uint32_t *prim, float *u,
                float *v, int nthreads, uint64_t *counters /* nodes, tris; may be null */) {
     const SceneData &s = ((Handle *)hp)->s;
@@ -68,8 +69,8 @@ void orc_trace(void *hp, const float *rays, uint32_t n, int any_hit, int brute, 
     auto work = [&](int tid) {
         for (uint32_t i = tid; i < n; i += nth) {
             Ray r = make_ray(rays + 8 * (size_t)i);
-            Hit hh = brute ? brute_trace(s.bvh, r, any_hit != 0)
-                           : s.bvh.trace(r, any_hit != 0, &cn[2 * tid], &cn[2 * tid + 1]);
+            Hit hh = brute == 1 ? brute_trace(s.bvh, r, any_hit != 0)
+                                : s.bvh.trace(r, any_hit != 0, &cn[2 * tid], &cn[2 * tid + 1], brute != 2);
             t[i] = hh.t; prim[i] = hh.prim; u[i] = hh.u; v[i] = hh.v;
         }
     };

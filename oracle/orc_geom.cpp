@@ -160,7 +160,7 @@ static inline void take_hit(Hit &best, float t, float u, float v, uint32_t prim)
 }
 
 // src/cpu/accel.cpp:429-488 (left child first, 64-entry stack)
-Hit Bvh2::trace(const Ray &ray_in, bool any_hit, uint64_t *nv, uint64_t *nt) const {
+Hit Bvh2::trace(const Ray &ray_in, bool any_hit, uint64_t *nv, uint64_t *nt, bool robust) const {
     Hit best{INFINITY, 0, 0, 0xFFFFFFFFu};
     if (nodes.empty() || index.empty() || ray_in.maxt < ray_in.mint) return best;
     Ray ray = ray_in;
@@ -168,7 +168,7 @@ Hit Bvh2::trace(const Ray &ray_in, bool any_hit, uint64_t *nv, uint64_t *nt) con
     while (true) {
         const Node &n = nodes[ni];
         if (nv) ++*nv;
-        if (!n.box.hit(ray)) {
+        if (!n.box.hit(ray, robust)) {
             if (sp == 0) break;
             ni = stack[--sp];
             continue;

@@ -71,8 +71,13 @@ struct BBox {
         if (e.y >= e.x && e.y >= e.z) return 1;
         return 2;
     }
-    // include/bbox.h:360-392
-    bool hit(const Ray &r) const {
+    // include/bbox.h:360-392.  With robust = true the far bound is widened by
+    // (1 + 2*gamma_3) (PBRT 3e §3.9.2): Nori's exact comparisons can cull a box
+    // whose entry point equals the current hit distance to the last ulp, and then
+    // miss an equal-t hit with a smaller primitive id.  The widening only affects
+    // which boxes are visited, never which triangles are accepted.
+    bool hit(const Ray &r, bool robust = true) const {
+        const float g = robust ? 1.0f + 2.0f * 3.5762791e-07f : 1.0f;
         float nearT = -INFINITY, farT = INFINITY;
         for (int i = 0; i < 3; ++i) {
             float o = r.o[i], lo = mn[i], hi = mx[i];
@@ -83,10 +88,10 @@ struct BBox {
                 if (t1 > t2) std::swap(t1, t2);
                 nearT = std::max(t1, nearT);
                 farT = std::min(t2, farT);
-                if (!(nearT <= farT)) return false;
+                if (!(nearT <= farT * g)) return false;
             }
         }
-        return r.mint <= farT && nearT <= r.maxt;
+        return r.mint <= farT * g && nearT <= r.maxt * g;
     }
 };
 
@@ -104,7 +109,9 @@ struct Bvh2 {
     std::vector<uint32_t> index;
 
     void build(const float *pos, const uint32_t *idx, uint32_t nt);
-    Hit trace(const Ray &ray, bool any_hit, uint64_t *nodes_visited = nullptr, uint64_t *tris_tested = nullptr) const;
+    // robust = false uses Nori's exact slab comparisons.
+    Hit trace(const Ray &ray, bool any_hit, uint64_t *nodes_visited = nullptr, uint64_t *tris_tested = nullptr,
+              bool robust = true) const;
 };
 
 // Exhaustive test over every triangle (ground truth for small scenes).

@@ -9,8 +9,9 @@
 //   * closest hit: smallest t in [tmin, tmax], equal t resolved to the smaller
 //     global primitive id (independent of traversal order);
 //   * box test: planes decoded with one fma exactly as the builder verified
-//     them, tfar widened by (1 + 2*gamma_3) (Pharr, Jakob, Humphreys, PBRT 3e
-//     §3.9.2) so no box that contains an accepted hit is ever culled;
+//     them, tfar (after clipping to the current hit) widened by (1 + 2*gamma_3)
+//     (Pharr, Jakob, Humphreys, PBRT 3e §3.9.2), so no box that holds an
+//     accepted hit, or an equal-t hit with a smaller id, is ever culled;
 //   * stack: the node groups (child base, hit bits | inner mask) of Ylitie et
 //     al. 2017, kept in LDS, one column per lane; its depth is the tree depth.
 #pragma once
@@ -108,8 +109,9 @@ __device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ node
                 float tnz = (__builtin_fmaf(qnz, sc.z, p.z) - o.z) * rcp.z;
                 float tfz = (__builtin_fmaf(qfz, sc.z, p.z) - o.z) * rcp.z;
                 float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
-                float tf = fminf(fminf(tfx, tfy), tfz) * far_scale;
-                tf = fminf(tf, res.t);
+                // widened after the min with the current hit: a box entered at exactly
+                // res.t may hold an equal-t hit with a smaller primitive id
+                float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * far_scale;
                 if (tn <= tf) {
                     uint32_t is_inner = (meta & 0x18u) == 0x18u;
                     uint32_t bitidx = is_inner ? ((meta ^ octinv) & 0x1Fu) : (meta & 0x1Fu);

@@ -81,6 +81,21 @@ def test_bvh2_matches_brute_force(cornell):
     assert np.array_equal(anyh, hit)
 
 
+def test_robust_bvh2_matches_brute_force_on_ties(disney_small):
+    # Camera rays over dense icospheres hit shared edges at equal t; the widened
+    # box test keeps the canonical (smallest id) hit that brute force finds.
+    s = orc.OracleScene(disney_small)
+    cam = disney_small.camera
+    cam.width, cam.height = 96, 96
+    rng = np.random.default_rng(8)
+    n = 96 * 96
+    smp = np.c_[rng.random(n) * 96, rng.random(n) * 96, rng.random(n), rng.random(n)].astype(np.float32)
+    rays = orc.camera_rays(cam, smp)
+    a, b = s.trace(rays), s.trace(rays, brute=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
 def test_closest_hit_tie_break_smallest_prim():
     # Two coincident triangles: the canonical hit is the smaller primitive id.
     mats = ng.default_materials(1)
