@@ -171,6 +171,9 @@ typedef struct nori_render_stats {
     double ms_shade;          /* summed kernel time of raygen and shading       */
     uint64_t launches_closest, launches_shadow;
     uint64_t ext_nodes, ext_tris, shadow_nodes, shadow_tris;
+    double ms_guide_infer;    /* guiding features + network inference          */
+    double ms_guide_train;    /* guiding sample finalisation + training steps   */
+    uint64_t guide_train_steps;
 } nori_render_stats;
 int nori_gpu_render_stats(const nori_ctx *ctx, nori_render_stats *out);
 
@@ -183,8 +186,31 @@ typedef struct nori_bvh_info {
 int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
 
 /* Options: "batch" (paths per wavefront batch, default 2^21), "timing_mode"
- * (0/1), "count_traversal" (0/1). */
+ * (0/1), "count_traversal" (0/1), "guiding" (0/1: guided sampling with one-sample
+ * MIS and online training after every batch), "guide_train_steps" (Adam steps
+ * per batch, default 4), "guide_train_batch" (samples per step, default 16384),
+ * "guide_lr_micro" (learning rate x 1e6, default 2000), "guide_reset" (next use
+ * re-initialises the network). */
 int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
+
+/* ------------------------------------------------------- neural guiding */
+/* The guiding network of the context (no counterpart in the reference; see
+ * DESIGN.md).  Parameters: NORI_GUIDE_PARAMS FP32 values, four layers of
+ * W[64 out][64 in] followed by b[64].  Entry points take host arrays; features
+ * are 64 floats per query, outputs 64 floats (41 used). */
+#define NORI_GUIDE_PARAMS (4 * (64 * 64 + 64))
+int nori_guide_params(nori_ctx *ctx, float *get, const float *set);
+int nori_guide_forward(nori_ctx *ctx, const float *feat, int n, float *out);
+/* Summed gradient of sum_i -weight_i log P_i(w_i) (and the loss) at the current parameters. */
+int nori_guide_gradient(nori_ctx *ctx, const float *feat, const float *w3, const float *bsdf_pdf,
+                        const float *weight, int n, float *grad, float *loss);
+/* `steps` Adam steps (learning rate lr) on the full batch; loss per step. */
+int nori_guide_train(nori_ctx *ctx, const float *feat, const float *w3, const float *bsdf_pdf, const float *weight,
+                     int n, int steps, float lr, float *loss_per_step);
+/* Decode outputs, draw a direction from the mixture (u3 = lobe, two vMF
+ * uniforms), and evaluate P, q and c at w3 with BSDF pdf bsdf_pdf. */
+int nori_guide_mixture(nori_ctx *ctx, const float *out64, int n, const float *u3, const float *w3,
+                       const float *bsdf_pdf, float *dir3, float *P, float *q, float *c);
 
 /* Host-only BVH8 build (the builder nori_gpu_upload_scene runs), for checks
  * and build timing without a GPU.  If nodes / tris are not NULL they receive

@@ -4,6 +4,7 @@
 #include "../../include/nori_gpu.h"
 #include "bvh8.h"
 #include "envmap.h"
+#include "guide.h"
 #include "pt_kernels.h"
 #include "scene_gen.h"
 #include <hip/hip_runtime.h>
@@ -70,6 +71,13 @@ struct nori_ctx {
 
     uint32_t batch = 1u << 21;
     int timing_mode = 0, count_traversal = 0;
+    // guiding options and per-batch buffers
+    int guiding = 0;
+    uint32_t g_train_steps = 4, g_train_batch = 16384, g_max_rec = 8;
+    float g_lr = 2e-3f;
+    uint32_t g_train_seed = 1;
+    DevBuf g_rec, g_nrec, g_lpath, g_samples, g_scount;
+    uint32_t g_rec_cap = 0;
     std::vector<hipEvent_t> ev_pool;
     struct Span {
         hipEvent_t a, b;
@@ -88,6 +96,87 @@ struct nori_ctx {
     nori_render_stats last{};
 
     DevBuf scratch_rays, scratch_hits;
+
+    // ---- guiding network: parameters, Adam state, BF16 copies, batch scratch
+    bool guide_ready = false;
+    DevBuf g_params, g_m, g_v, g_grad, g_w, g_wt, g_loss;
+    GuideDev gdev{};
+    int g_step = 0;
+    uint32_t g_cap = 0;
+    DevBuf g_act[4], g_out, g_dz[2], g_ww, g_wib;
+
+    void ensure_guide() {
+        if (guide_ready) return;
+        alloc(g_params, G_PARAMS * sizeof(float));
+        alloc(g_m, G_PARAMS * sizeof(float));
+        alloc(g_v, G_PARAMS * sizeof(float));
+        alloc(g_grad, G_PARAMS * sizeof(float));
+        alloc(g_w, G_LAYERS * 4096 * sizeof(bf16_t));
+        alloc(g_wt, G_LAYERS * 4096 * sizeof(bf16_t));
+        alloc(g_loss, 16);
+        gdev = GuideDev{(float *)g_params.p, (float *)g_m.p, (float *)g_v.p, (float *)g_grad.p,
+                        (bf16_t *)g_w.p,     (bf16_t *)g_wt.p, (float *)g_loss.p};
+        // He-uniform weights from a fixed pcg32 stream, zero biases; the
+        // concentration outputs start at exp(1) and c at 0.5.
+        std::vector<float> p(G_PARAMS, 0.0f);
+        uint64_t st = pcg_seed(0x6D1D0000ull, 2);
+        for (int l = 0; l < G_LAYERS; ++l) {
+            float lim = std::sqrt(6.0f / 64.0f) * (l == G_LAYERS - 1 ? 0.1f : 1.0f);
+            for (int i = 0; i < 4096; ++i) p[l * G_LAYER_PARAMS + i] = (2.0f * pcg_float(st) - 1.0f) * lim;
+        }
+        for (int j = 0; j < 8; ++j) p[3 * G_LAYER_PARAMS + 4096 + 24 + j] = 1.0f;
+        HIPCHK(hipMemcpy(g_params.p, p.data(), G_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(g_m.p, 0, G_PARAMS * sizeof(float)));
+        HIPCHK(hipMemset(g_v.p, 0, G_PARAMS * sizeof(float)));
+        guide_refresh(gdev, stream);
+        g_step = 0;
+        guide_ready = true;
+    }
+    void ensure_guide_records(uint32_t n) {
+        if (n <= g_rec_cap) return;
+        alloc(g_rec, (size_t)n * g_max_rec * sizeof(GuideRecord));
+        alloc(g_nrec, (size_t)n * sizeof(uint32_t));
+        alloc(g_lpath, (size_t)n * sizeof(float4));
+        alloc(g_samples, (size_t)n * 4 * sizeof(GuideSample));
+        alloc(g_scount, 16);
+        g_rec_cap = n;
+    }
+    void ensure_guide_batch(uint32_t n) {
+        if (n <= g_cap) return;
+        for (auto &b : g_act) alloc(b, (size_t)n * 64 * sizeof(bf16_t));
+        for (auto &b : g_dz) alloc(b, (size_t)n * 64 * sizeof(bf16_t));
+        alloc(g_out, (size_t)n * 64 * sizeof(float));
+        alloc(g_ww, (size_t)n * sizeof(float4));
+        alloc(g_wib, (size_t)n * sizeof(float4));
+        g_cap = n;
+    }
+    const bf16_t *layer_w(int l) const { return gdev.w + l * 4096; }
+    const bf16_t *layer_wt(int l) const { return gdev.wt + l * 4096; }
+    const float *layer_b(int l) const { return gdev.params + l * G_LAYER_PARAMS + 4096; }
+    // act[0] holds the BF16 features; writes act[1..3] and g_out (FP32).
+    void guide_forward_dev(uint32_t cap_rows, const uint32_t *count) {
+        for (int l = 0; l < 3; ++l)
+            guide_layer((bf16_t *)g_act[l].p, layer_w(l), layer_b(l), g_act[l + 1].p, nullptr, 0, cap_rows, count,
+                        stream);
+        guide_layer((bf16_t *)g_act[3].p, layer_w(3), layer_b(3), g_out.p, nullptr, 1, cap_rows, count, stream);
+    }
+    // Needs g_ww / g_wib filled; accumulates the gradient (times `scale`) into gdev.grad.
+    void guide_backward_dev(uint32_t cap_rows, const uint32_t *count, float scale) {
+        guide_zero(gdev.grad, G_PARAMS, stream);
+        guide_zero(gdev.loss, 1, stream);
+        guide_dout((const float *)g_out.p, (const float4 *)g_ww.p, (const float4 *)g_wib.p, (bf16_t *)g_dz[0].p,
+                   gdev.loss, scale, cap_rows, count, stream);
+        int cur = 0;
+        for (int l = 3; l >= 0; --l) {
+            float *gW = gdev.grad + l * G_LAYER_PARAMS;
+            guide_wgrad((bf16_t *)g_dz[cur].p, (bf16_t *)g_act[l].p, gW, gW + 4096, cap_rows, count, stream);
+            if (l > 0) {
+                guide_layer((bf16_t *)g_dz[cur].p, layer_wt(l), nullptr, g_dz[cur ^ 1].p, (bf16_t *)g_act[l].p, 2,
+                            cap_rows, count, stream);
+                cur ^= 1;
+            }
+        }
+    }
 
     void *alloc(DevBuf &b, size_t bytes) {
         if (b.bytes >= bytes && b.p) return b.p;
@@ -198,8 +287,12 @@ void nori_gpu_destroy(nori_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->free_scene();
-    for (DevBuf *b : {&c->paths, &c->queues, &c->shadow, &c->counters, &c->stats, &c->scratch_rays, &c->scratch_hits})
+    for (DevBuf *b : {&c->paths, &c->queues, &c->shadow, &c->counters, &c->stats, &c->scratch_rays, &c->scratch_hits,
+                      &c->g_params, &c->g_m, &c->g_v, &c->g_grad, &c->g_w, &c->g_wt, &c->g_loss, &c->g_out, &c->g_ww,
+                      &c->g_wib, &c->g_rec, &c->g_nrec, &c->g_lpath, &c->g_samples, &c->g_scount})
         b->release();
+    for (auto &b : c->g_act) b.release();
+    for (auto &b : c->g_dz) b.release();
     for (auto &f : c->films) f.release();
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -342,6 +435,14 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             uint64_t total = (uint64_t)npix * v.spp;
             uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, std::max<uint64_t>(total, 1));
             c->ensure_paths(cap);
+            GuideShadeDev gsd{nullptr, nullptr, nullptr, nullptr, 0, 0};
+            if (c->guiding) {
+                c->ensure_guide();
+                c->ensure_guide_batch(std::max(c->cap, c->g_train_batch));
+                c->ensure_guide_records(c->cap);
+                gsd = GuideShadeDev{(const float *)c->g_out.p, (GuideRecord *)c->g_rec.p, (uint32_t *)c->g_nrec.p,
+                                    (float4 *)c->g_lpath.p, c->cap, std::min(c->g_max_rec, v.max_depth)};
+            }
             ViewDev vd{v.camera, v.spp, v.sample_offset, v.max_depth, v.rr_depth, (uint32_t)npix,
                        (float4 *)c->films[vi].p};
             for (uint64_t first = 0; first < total; first += cap) {
@@ -350,7 +451,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                 launch_set_counter(c->cdev.cnt, n, c->stream);
                 {
                     Span t(c, 2);
-                    launch_raygen(c->sdev, c->pdev, c->q[0], c->cdev, vd, seed, first, n, c->stream);
+                    launch_raygen(c->sdev, c->pdev, c->q[0], c->cdev, vd, seed, first, n, gsd, c->stream);
                 }
                 for (uint32_t b = 0; b < v.max_depth; ++b) {
                     uint32_t qi = b & 1u, qo = qi ^ 1u;
@@ -358,20 +459,44 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                         Span t(c, 0);
                         launch_extend(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, n, cs, c->stream);
                     }
+                    if (c->guiding) {
+                        Span t(c, 3);
+                        launch_guide_feats(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, (bf16_t *)c->g_act[0].p, n,
+                                           c->stream);
+                        c->guide_forward_dev(n, c->cdev.cnt + qi);
+                    }
                     {
                         Span t(c, 2);
                         launch_shade(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, c->q[qo], c->cdev.cnt + qo,
                                      c->shdev, c->cdev.cnt + 2, (float4 *)c->films[vi].p, b, v.max_depth, v.rr_depth, n,
-                                     c->stream);
+                                     gsd, c->stream);
                     }
                     {
                         Span t(c, 1);
-                        launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, (float4 *)c->films[vi].p, 2 * n, cs,
+                        launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, (float4 *)c->films[vi].p, 2 * n, cs, gsd,
                                       c->stream);
                     }
                     st.launches_closest++;
                     st.launches_shadow++;
                     launch_counters(c->cdev, qi, 1, (1u << qi) | 4u, c->stream);
+                }
+                if (c->guiding && c->g_train_steps > 0) {
+                    // Online training on the radiance this batch's paths carried.
+                    Span t(c, 4);
+                    uint32_t *scount = (uint32_t *)c->g_scount.p;
+                    launch_set_counter(scount, 0, c->stream);
+                    launch_guide_finalize(c->sdev, gsd, n, (GuideSample *)c->g_samples.p, scount, 4 * c->cap,
+                                          c->stream);
+                    const uint32_t B = c->g_train_batch;
+                    for (uint32_t k = 0; k < c->g_train_steps; ++k) {
+                        launch_guide_gather((const GuideSample *)c->g_samples.p, scount, B, c->g_train_seed++,
+                                            (bf16_t *)c->g_act[0].p, (float4 *)c->g_ww.p, (float4 *)c->g_wib.p,
+                                            c->stream);
+                        c->guide_forward_dev(B, nullptr);
+                        c->guide_backward_dev(B, nullptr, 1.0f / (float)B);
+                        guide_adam(c->gdev, ++c->g_step, c->g_lr, 0.9f, 0.999f, 1e-8f, c->stream);
+                    }
+                    st.guide_train_steps += c->g_train_steps;
                 }
             }
             st.samples += total;
@@ -394,7 +519,9 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
         for (auto &sp : c->spans) {
             float kms = 0.0f;
             HIPCHK(hipEventElapsedTime(&kms, sp.a, sp.b));
-            (sp.cat == 0 ? st.ms_trace_closest : sp.cat == 1 ? st.ms_trace_shadow : st.ms_shade) += kms;
+            double *dst[5] = {&st.ms_trace_closest, &st.ms_trace_shadow, &st.ms_shade, &st.ms_guide_infer,
+                              &st.ms_guide_train};
+            *dst[sp.cat] += kms;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -433,6 +560,19 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
         c->timing_mode = (int)value;
     } else if (n == "count_traversal") {
         c->count_traversal = (int)value;
+    } else if (n == "guiding") {
+        c->guiding = (int)value;
+    } else if (n == "guide_train_steps") {
+        if (value < 0 || value > 1024) return fail(c, NORI_E_ARG, "guide_train_steps out of range [0, 1024]");
+        c->g_train_steps = (uint32_t)value;
+    } else if (n == "guide_train_batch") {
+        if (value < 64 || value > (1 << 22)) return fail(c, NORI_E_ARG, "guide_train_batch out of range [64, 2^22]");
+        c->g_train_batch = (uint32_t)value;
+    } else if (n == "guide_lr_micro") {
+        if (value <= 0 || value > 1000000) return fail(c, NORI_E_ARG, "guide_lr_micro out of range (0, 1e6]");
+        c->g_lr = (float)value * 1e-6f;
+    } else if (n == "guide_reset") {
+        c->guide_ready = false;
     } else {
         return fail(c, NORI_E_ARG, "unknown option " + n);
     }
@@ -553,6 +693,120 @@ int nori_gpu_pcg32(nori_ctx *c, uint64_t initstate, uint64_t initseq, uint64_t a
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(out, b.p, (size_t)n * 4, hipMemcpyDeviceToHost));
         b.release();
+    });
+}
+
+// ------------------------------------------------------------------ guiding
+
+int nori_guide_params(nori_ctx *c, float *get, const float *set) {
+    if (!c) return NORI_E_ARG;
+    return guard(c, [&] {
+        c->ensure_guide();
+        if (set) {
+            HIPCHK(hipMemcpy(c->gdev.params, set, G_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemset(c->gdev.adam_m, 0, G_PARAMS * sizeof(float)));
+            HIPCHK(hipMemset(c->gdev.adam_v, 0, G_PARAMS * sizeof(float)));
+            c->g_step = 0;
+            guide_refresh(c->gdev, c->stream);
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        if (get) HIPCHK(hipMemcpy(get, c->gdev.params, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+namespace {
+// Uploads features (FP32 -> BF16 act[0]) and, if given, samples (w, weight, bsdf pdf).
+void guide_stage(nori_ctx *c, const float *feat, const float *w3, const float *bpdf, const float *wt, int n) {
+    c->ensure_guide();
+    c->ensure_guide_batch((uint32_t)n);
+    DevBuf tmp;
+    HIPCHK(hipMalloc(&tmp.p, (size_t)n * 64 * sizeof(float)));
+    HIPCHK(hipMemcpy(tmp.p, feat, (size_t)n * 64 * sizeof(float), hipMemcpyHostToDevice));
+    guide_feats_from_f32((const float *)tmp.p, (bf16_t *)c->g_act[0].p, (uint32_t)n, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    tmp.release();
+    if (w3) {
+        std::vector<float4> ww(n), wib(n);
+        for (int i = 0; i < n; ++i) {
+            ww[i] = make_float4(w3[3 * i], w3[3 * i + 1], w3[3 * i + 2], wt[i]);
+            wib[i] = make_float4(0.0f, 0.0f, 0.0f, bpdf[i]);
+        }
+        HIPCHK(hipMemcpy(c->g_ww.p, ww.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->g_wib.p, wib.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    }
+}
+} // namespace
+
+int nori_guide_forward(nori_ctx *c, const float *feat, int n, float *out) {
+    if (!c || n < 0 || (n > 0 && (!feat || !out))) return NORI_E_ARG;
+    return guard(c, [&] {
+        if (n == 0) return;
+        guide_stage(c, feat, nullptr, nullptr, nullptr, n);
+        c->guide_forward_dev((uint32_t)n, nullptr);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(out, c->g_out.p, (size_t)n * 64 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int nori_guide_gradient(nori_ctx *c, const float *feat, const float *w3, const float *bpdf, const float *wt, int n,
+                        float *grad, float *loss) {
+    if (!c || n <= 0 || !feat || !w3 || !bpdf || !wt || !grad) return NORI_E_ARG;
+    return guard(c, [&] {
+        guide_stage(c, feat, w3, bpdf, wt, n);
+        c->guide_forward_dev((uint32_t)n, nullptr);
+        c->guide_backward_dev((uint32_t)n, nullptr, 1.0f);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(grad, c->gdev.grad, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost));
+        if (loss) HIPCHK(hipMemcpy(loss, c->gdev.loss, sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int nori_guide_train(nori_ctx *c, const float *feat, const float *w3, const float *bpdf, const float *wt, int n,
+                     int steps, float lr, float *loss_per_step) {
+    if (!c || n <= 0 || steps < 0 || !feat || !w3 || !bpdf || !wt) return NORI_E_ARG;
+    return guard(c, [&] {
+        guide_stage(c, feat, w3, bpdf, wt, n);
+        for (int k = 0; k < steps; ++k) {
+            c->guide_forward_dev((uint32_t)n, nullptr);
+            c->guide_backward_dev((uint32_t)n, nullptr, 1.0f / (float)n);
+            guide_adam(c->gdev, ++c->g_step, lr, 0.9f, 0.999f, 1e-8f, c->stream);
+            if (loss_per_step) {
+                HIPCHK(hipMemcpyAsync(loss_per_step + k, c->gdev.loss, sizeof(float), hipMemcpyDeviceToHost,
+                                      c->stream));
+            }
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        if (loss_per_step)
+            for (int k = 0; k < steps; ++k) loss_per_step[k] /= (float)n;
+    });
+}
+
+int nori_guide_mixture(nori_ctx *c, const float *out64, int n, const float *u3, const float *w3, const float *bpdf,
+                       float *dir3, float *P, float *q, float *cc) {
+    if (!c || n < 0 || (n > 0 && (!out64 || !u3 || !w3 || !bpdf || !dir3 || !P || !q || !cc))) return NORI_E_ARG;
+    return guard(c, [&] {
+        if (n == 0) return;
+        std::vector<DevBuf> b(9);
+        size_t sz[9] = {(size_t)n * 256, (size_t)n * 12, (size_t)n * 12, (size_t)n * 4, (size_t)n * 12,
+                        (size_t)n * 4,   (size_t)n * 4,  (size_t)n * 4, 0};
+        const void *src[4] = {out64, u3, w3, bpdf};
+        for (int k = 0; k < 8; ++k) {
+            HIPCHK(hipMalloc(&b[k].p, sz[k]));
+            if (k < 4) HIPCHK(hipMemcpy(b[k].p, src[k], sz[k], hipMemcpyHostToDevice));
+        }
+        guide_mixture((const float *)b[0].p, (uint32_t)n, (const float *)b[1].p, (const float *)b[2].p,
+                      (const float *)b[3].p, (float *)b[4].p, (float *)b[5].p, (float *)b[6].p, (float *)b[7].p,
+                      c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(dir3, b[4].p, sz[4], hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(P, b[5].p, sz[5], hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(q, b[6].p, sz[6], hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cc, b[7].p, sz[7], hipMemcpyDeviceToHost));
+        for (auto &x : b) x.release();
     });
 }
 

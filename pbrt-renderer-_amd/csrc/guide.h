@@ -1,0 +1,184 @@
+// Online neural path guiding (no counterpart in the reference, whose integrator
+// samples only the BSDF, shaders/Shaders.hlsl:473-511).
+//
+// Model: 64 input features -> 3 hidden layers of 64 (ReLU) -> 64 outputs, of
+// which 41 parameterise an 8-lobe von Mises-Fisher mixture q and the selection
+// probability c of one-sample MIS between q and the BSDF:
+//   P(w) = c q(w) + (1 - c) b(w).
+//   features  per axis of the scene-normalised position, 8 octaves of
+//             (sin, cos)(2^k pi p) (48); shading normal (3); direction to the
+//             viewer (3); roughness, metallic; 8 zeros
+//   outputs   mean_j = normalize(o[3j..3j+2]), kappa_j = exp(clamp(o[24+j], -4, 9)),
+//             weights = softmax(o[32..39]), c = 0.1 + 0.8 sigmoid(o[40])
+//   training  minimises -W log P(w_s) over samples the renderer records (the
+//             sample gradient of KL(target || P); Mueller et al., TOG 2019),
+//             W = luminance of the radiance that arrived along w_s divided by
+//             the throughput before the sampling vertex; Adam (Kingma & Ba).
+// Matrix products use BF16 MFMA with FP32 accumulation
+// (v_mfma_f32_32x32x16_bf16 on gfx950); parameters are FP32 masters.
+#pragma once
+#include "nori_math.h"
+
+namespace ng {
+
+constexpr int G_DIM = 64;
+constexpr int G_LAYERS = 4;
+constexpr int G_LAYER_PARAMS = G_DIM * G_DIM + G_DIM;
+constexpr int G_PARAMS = G_LAYERS * G_LAYER_PARAMS;
+constexpr int G_USED_OUT = 41;
+
+typedef __bf16 bf16_t;
+
+struct GuideDev {
+    float *params;  // FP32 masters: per layer W[64 out][64 in] then b[64]
+    float *adam_m, *adam_v;
+    float *grad;    // FP32 gradient accumulator, same layout
+    bf16_t *w;      // BF16 W[l][out][in]
+    bf16_t *wt;     // BF16 W[l][in][out]
+    float *loss;    // [0] summed loss of the last step
+};
+
+// Vertex record written by the shading kernel for training (6 float4).
+struct GuideRecord {
+    float4 pos_rough;  // world position, roughness
+    float4 n_metal;    // shading normal, metallic
+    float4 wi_b;       // direction to the viewer, BSDF pdf of the sampled direction
+    float4 w_valid;    // sampled direction (world)
+    float4 thr;        // throughput before the vertex
+    float4 lsnap;      // path radiance collected up to and including this vertex's NEE
+};
+
+// Training sample after the path has finished (4 float4).
+struct GuideSample {
+    float4 pos01_rough;
+    float4 n_metal;
+    float4 wi_b;
+    float4 w_weight;
+};
+
+NG_HD void guide_features(f3 p01, f3 n, f3 wi, float rough, float metal, float *f) {
+    const float pa[3] = {p01.x, p01.y, p01.z};
+    for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 8; ++k) {
+            float s, c;
+            sincosf((float)(1 << k) * PI * pa[a], &s, &c);
+            f[a * 16 + 2 * k] = s;
+            f[a * 16 + 2 * k + 1] = c;
+        }
+    f[48] = n.x; f[49] = n.y; f[50] = n.z;
+    f[51] = wi.x; f[52] = wi.y; f[53] = wi.z;
+    f[54] = rough;
+    f[55] = metal;
+    for (int k = 56; k < 64; ++k) f[k] = 0.0f;
+}
+
+// Decoded mixture of one query.
+struct GuideMix {
+    f3 mu[8];
+    float kappa[8], norm[8], w[8], c;
+};
+
+NG_HD GuideMix guide_decode(const float *o) {
+    GuideMix g;
+    for (int j = 0; j < 8; ++j) {
+        f3 r = mk(o[3 * j], o[3 * j + 1], o[3 * j + 2]);
+        float l = fmaxf(sqrtf(dot(r, r)), 1e-12f);
+        g.mu[j] = r * (1.0f / l);
+        float k = expf(fminf(fmaxf(o[24 + j], -4.0f), 9.0f));
+        g.kappa[j] = k;
+        g.norm[j] = k / (2.0f * PI * (1.0f - expf(-2.0f * k)));
+    }
+    float mx = o[32];
+    for (int j = 1; j < 8; ++j) mx = fmaxf(mx, o[32 + j]);
+    float s = 0.0f;
+    for (int j = 0; j < 8; ++j) {
+        g.w[j] = expf(o[32 + j] - mx);
+        s += g.w[j];
+    }
+    float inv = 1.0f / s;
+    for (int j = 0; j < 8; ++j) g.w[j] *= inv;
+    g.c = 0.1f + 0.8f / (1.0f + expf(-o[40]));
+    return g;
+}
+
+NG_HD float guide_q(const GuideMix &g, f3 w) {
+    float q = 0.0f;
+    for (int j = 0; j < 8; ++j) q += g.w[j] * g.norm[j] * expf(g.kappa[j] * (dot(g.mu[j], w) - 1.0f));
+    return q;
+}
+
+// vMF lobe sample (Jakob 2012, numerically stable form), frame of Duff et al. 2017.
+NG_HD f3 vmf_sample(f3 mu, float k, float u1, float u2) {
+    float cw = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
+    float sw = sqrtf(fmaxf(0.0f, 1.0f - cw * cw));
+    float sp, cp;
+    sincosf(2.0f * PI * u2, &sp, &cp);
+    float sgn = mu.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (sgn + mu.z), b = mu.x * mu.y * a;
+    f3 t = mk(1.0f + sgn * mu.x * mu.x * a, sgn * b, -sgn * mu.x);
+    f3 s = mk(b, sgn + mu.y * mu.y * a, -mu.y);
+    return normalize(mu * cw + (t * cp + s * sp) * sw);
+}
+
+// Pick a lobe by weight with u in [0,1), then sample it.
+NG_HD f3 guide_sample(const GuideMix &g, float ul, float u1, float u2) {
+    int j = 0;
+    float acc = g.w[0];
+    while (j < 7 && ul >= acc) acc += g.w[++j];
+    return vmf_sample(g.mu[j], g.kappa[j], u1, u2);
+}
+
+// Loss -W log P and its gradient w.r.t. the outputs (41 used, rest zero).
+NG_HD float guide_loss_dout(const float *o, f3 w, float b, float weight, float *g) {
+    for (int k = 0; k < 64; ++k) g[k] = 0.0f;
+    GuideMix m = guide_decode(o);
+    float v[8], cosm[8], q = 0.0f;
+    for (int j = 0; j < 8; ++j) {
+        cosm[j] = dot(m.mu[j], w);
+        v[j] = m.norm[j] * expf(m.kappa[j] * (cosm[j] - 1.0f));
+        q += m.w[j] * v[j];
+    }
+    float P = m.c * q + (1.0f - m.c) * b;
+    if (!(P > 0.0f) || weight == 0.0f) return 0.0f;
+    float dP = -weight / P;
+    float s = (m.c - 0.1f) / 0.8f;
+    g[40] = dP * (q - b) * 0.8f * s * (1.0f - s);
+    float dq = dP * m.c;
+    for (int j = 0; j < 8; ++j) {
+        g[32 + j] = dq * m.w[j] * (v[j] - q);
+        float k = m.kappa[j], e = expf(-2.0f * k);
+        float dlogv = 1.0f / k - 2.0f * e / (1.0f - e) + (cosm[j] - 1.0f);
+        float dv = dq * m.w[j];
+        float raw = o[24 + j];
+        if (raw > -4.0f && raw < 9.0f) g[24 + j] = dv * v[j] * dlogv * k;
+        f3 r = mk(o[3 * j], o[3 * j + 1], o[3 * j + 2]);
+        float len = fmaxf(sqrtf(dot(r, r)), 1e-12f);
+        float dm = dv * v[j] * k / len;
+        g[3 * j] = dm * (w.x - cosm[j] * m.mu[j].x);
+        g[3 * j + 1] = dm * (w.y - cosm[j] * m.mu[j].y);
+        g[3 * j + 2] = dm * (w.z - cosm[j] * m.mu[j].z);
+    }
+    return -weight * logf(P);
+}
+
+// Launchers (guide.hip).  Row counts may live on the device (count != null:
+// rows = *count, grid sized for cap rows).
+void guide_refresh(const GuideDev &g, hipStream_t st);
+// Y[rows x 64] = act(X[rows x 64] M^T (+ bias)), M[64 n][64 k] BF16.
+// mode 0: +bias, ReLU -> BF16 Y; mode 1: +bias -> FP32 Y; mode 2: * (Hmask > 0) -> BF16 Y.
+void guide_layer(const bf16_t *X, const bf16_t *M, const float *bias, void *Y, const bf16_t *Hmask, int mode,
+                 uint32_t cap, const uint32_t *count, hipStream_t st);
+// grad_W[64 o][64 i] += dZ^T H ; grad_b[o] += sum dZ (rows from count or cap).
+void guide_wgrad(const bf16_t *dZ, const bf16_t *H, float *gradW, float *gradb, uint32_t cap, const uint32_t *count,
+                 hipStream_t st);
+// dZ4 (BF16) and the summed loss from the FP32 outputs and the samples' direction, BSDF pdf, weight.
+void guide_dout(const float *out, const float4 *w_weight, const float4 *wi_b, bf16_t *dZ, float *loss, float scale,
+                uint32_t cap, const uint32_t *count, hipStream_t st);
+void guide_adam(const GuideDev &g, int step, float lr, float b1, float b2, float eps, hipStream_t st);
+void guide_zero(float *p, size_t n, hipStream_t st);
+void guide_feats_from_f32(const float *feat, bf16_t *X, uint32_t rows, hipStream_t st);
+// Decode + sample (u3: lobe choice, two vMF uniforms) + mixture density at w3 (checks).
+void guide_mixture(const float *out64, uint32_t n, const float *u3, const float *w3, const float *bpdf, float *dir,
+                   float *P, float *q, float *c, hipStream_t st);
+
+} // namespace ng

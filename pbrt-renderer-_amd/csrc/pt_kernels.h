@@ -2,6 +2,7 @@
 #pragma once
 #include "../../include/nori_gpu.h"
 #include "env_dev.h"
+#include "guide.h"
 #include "nori_math.h"
 
 namespace ng {
@@ -22,6 +23,17 @@ struct SceneDev {
     uint32_t stack_depth;
     int has_env;
     EnvDev env;
+    f3 bmin, binv; // scene bounds: position -> [0,1]^3 for the guiding features
+};
+
+// Guiding hooks of one render batch.  out: network outputs per queue position
+// (null: plain BSDF sampling).  rec/nrec/lpath: training records (null: none).
+struct GuideShadeDev {
+    const float *out;
+    GuideRecord *rec;
+    uint32_t *nrec;
+    float4 *lpath;
+    uint32_t cap, max_rec;
 };
 
 // Path state, structure of arrays indexed by path slot.
@@ -57,16 +69,25 @@ struct Counters {
 
 void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uint32_t n, int any_hit, hipStream_t st);
 void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counters c, const ViewDev &v,
-                   uint64_t seed, uint64_t first_sample, uint32_t n, hipStream_t st);
+                   uint64_t seed, uint64_t first_sample, uint32_t n, const GuideShadeDev &g, hipStream_t st);
 // count_stats (optional): stats array whose [4],[5] (extend) / [6],[7] (shadow)
 // receive the summed BVH8 nodes visited and triangles tested.
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, uint32_t cap,
                    unsigned long long *count_stats, hipStream_t st);
 void launch_shade(const SceneDev &s, const PathsDev &p, const uint32_t *q_in, const uint32_t *cnt_in, uint32_t *q_out,
                   uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film, uint32_t bounce,
-                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, hipStream_t st);
+                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
-                   unsigned long long *count_stats, hipStream_t st);
+                   unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st);
+// Guiding: BF16 features of the hits in the queue (zeros where guiding does not apply).
+void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count,
+                        bf16_t *X, uint32_t cap, hipStream_t st);
+// Turn finished paths' records into training samples (appended at *count).
+void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, uint32_t npaths, GuideSample *samples,
+                           uint32_t *count, uint32_t max_samples, hipStream_t st);
+// A training batch of `rows` samples drawn (hashed indices) from samples[0..*count).
+void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint32_t rows, uint32_t seed, bf16_t *X,
+                         float4 *w_weight, float4 *wi_b, hipStream_t st);
 // Adds the finished counts into stats and zeroes the queue counters in `clear`
 // (bit i -> cnt[i]; bit 4 zeroes all stats first).
 void launch_counters(Counters c, uint32_t add_ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st);

@@ -88,12 +88,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_batch(SceneDev s, const f
 }
 
 __global__ void k_raygen(SceneDev s, PathsDev p, uint32_t *queue, ViewDev v, uint64_t seed, uint64_t first,
-                         uint32_t n) {
+                         uint32_t n, GuideShadeDev g) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t g = first + i;
-    uint32_t pixel = (uint32_t)(g % v.npix);
-    uint32_t sample = v.sample_offset + (uint32_t)(g / v.npix);
+    const uint64_t gi = first + i;
+    uint32_t pixel = (uint32_t)(gi % v.npix);
+    uint32_t sample = v.sample_offset + (uint32_t)(gi / v.npix);
     uint64_t rng = sample_stream(seed, pixel, sample);
     float jx = pcg_float(rng), jy = pcg_float(rng);
     float ax = 0.5f, ay = 0.5f;
@@ -111,6 +111,10 @@ __global__ void k_raygen(SceneDev s, PathsDev p, uint32_t *queue, ViewDev v, uin
     p.aux[i] = make_float4(1.0f, __uint_as_float(pixel), __uint_as_float((uint32_t)rng),
                            __uint_as_float((uint32_t)(rng >> 32)));
     queue[i] = i;
+    if (g.rec) {
+        g.nrec[i] = 0;
+        g.lpath[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     unsafeAtomicAdd(reinterpret_cast<float *>(v.film + pixel) + 3, 1.0f);
 }
 
@@ -144,9 +148,28 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
     if (COUNT) count_add(stats, 4, nn, nt);
 }
 
+// NEE contribution of a vertex: into the path radiance and, when the vertex
+// wrote a training record, into that record's snapshot (it is light that did
+// not arrive along the sampled direction).
+__device__ __forceinline__ void guide_nee_add(const GuideShadeDev &g, uint32_t tag, f3 c) {
+    if (!g.rec || !finite3(c)) return;
+    const uint32_t slot = tag & 0x7FFFFFFFu;
+    float *lp = reinterpret_cast<float *>(g.lpath + slot);
+    atomicAdd(lp, c.x);
+    atomicAdd(lp + 1, c.y);
+    atomicAdd(lp + 2, c.z);
+    if (tag & 0x80000000u) {
+        const uint32_t k = g.nrec[slot] - 1u;
+        float *ls = reinterpret_cast<float *>(&g.rec[(size_t)k * g.cap + slot].lsnap);
+        atomicAdd(ls, c.x);
+        atomicAdd(ls + 1, c.y);
+        atomicAdd(ls + 2, c.z);
+    }
+}
+
 template <bool COUNT>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
-                                                        float4 *film, unsigned long long *stats) {
+                                                        float4 *film, unsigned long long *stats, GuideShadeDev g) {
     extern __shared__ uint2 stk[];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = *count;
@@ -156,26 +179,58 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
         float4 a = sh.o[i], b = sh.d[i];
         TraceOut r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), 0.0f, a.w, stk + threadIdx.x,
                                                 blockDim.x, &nn, &nt);
-        if (r.prim == MISS) film_add(film, __float_as_uint(b.w), xyz(sh.c[i]));
+        if (r.prim == MISS) {
+            const float4 c = sh.c[i];
+            film_add(film, __float_as_uint(b.w), xyz(c));
+            guide_nee_add(g, __float_as_uint(c.w), xyz(c));
+        }
     }
     if (COUNT) count_add(stats, 6, nn, nt);
 }
 
 __device__ __forceinline__ void push_shadow(ShadowDev &sh, uint32_t *cnt, bool pred, f3 o, float tmax, f3 d,
-                                            uint32_t pixel, f3 c) {
+                                            uint32_t pixel, f3 c, uint32_t tag) {
     uint32_t k = wave_append(cnt, pred);
     if (pred) {
         sh.o[k] = make_float4(o.x, o.y, o.z, tmax);
         sh.d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(pixel));
-        sh.c[k] = make_float4(c.x, c.y, c.z, 0.0f);
+        sh.c[k] = make_float4(c.x, c.y, c.z, __uint_as_float(tag));
     }
+}
+
+__device__ __forceinline__ bool guide_eligible(uint32_t type) {
+    return type == NORI_MAT_DIFFUSE || type == NORI_MAT_MICROFACET || type == NORI_MAT_DISNEY;
+}
+
+// Interpolated shading normal turned to the side of the geometric normal that
+// faces the ray (shaders/Shaders.hlsl:217-231).
+__device__ __forceinline__ void hit_frame(const SceneDev &s, uint32_t prim, float u, float v, f3 d, f3 &N, f3 &Ng,
+                                          bool &back) {
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const f3 p0 = f3{s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
+    const f3 p1 = f3{s.pos[3 * i1], s.pos[3 * i1 + 1], s.pos[3 * i1 + 2]};
+    const f3 p2 = f3{s.pos[3 * i2], s.pos[3 * i2 + 1], s.pos[3 * i2 + 2]};
+    Ng = normalize(cross_exact(sub_exact(p1, p0), sub_exact(p2, p0)));
+    N = Ng;
+    if (s.nrm) {
+        f3 n0 = f3{s.nrm[3 * i0], s.nrm[3 * i0 + 1], s.nrm[3 * i0 + 2]};
+        f3 n1 = f3{s.nrm[3 * i1], s.nrm[3 * i1 + 1], s.nrm[3 * i1 + 2]};
+        f3 n2 = f3{s.nrm[3 * i2], s.nrm[3 * i2 + 1], s.nrm[3 * i2 + 2]};
+        f3 n = n0 * (1.0f - u - v) + n1 * u + n2 * v;
+        float l2 = dot(n, n);
+        N = l2 < 1e-8f ? Ng : normalize(n);
+    }
+    back = dot(Ng, d) > 0.0f;
+    if (back) Ng = -Ng;
+    if (dot(N, Ng) < 0.0f) N = -N;
 }
 
 // One bounce of shaders/Shaders.hlsl:108-627 for every path in q_in.
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, const uint32_t *__restrict__ q_in,
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *q_out,
                                                        uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film,
-                                                       uint32_t bounce, uint32_t max_depth, uint32_t rr_depth) {
+                                                       uint32_t bounce, uint32_t max_depth, uint32_t rr_depth,
+                                                       GuideShadeDev g) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *cnt_in) return;
     const uint32_t slot = q_in[i];
@@ -188,6 +243,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
     const uint32_t prim = __float_as_uint(hit.y);
 
     bool alive = false, sh_area = false, sh_env = false;
+    uint32_t tag = slot;
+    f3 lpath = g.rec ? xyz(g.lpath[slot]) : mk(0.0f);
+    auto add_L = [&](f3 c) {
+        film_add(film, pixel, c);
+        if (g.rec && finite3(c)) lpath += c;
+    };
     f3 new_o = mk(0.0f), new_d = mk(0.0f);
     f3 so = mk(0.0f), a_dir = mk(0.0f), a_c = mk(0.0f), e_dir = mk(0.0f), e_c = mk(0.0f);
     float a_tmax = 0.0f;
@@ -196,33 +257,20 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
         if (s.has_env) {
             f3 env = env_eval(s.env, d);
             f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
-            film_add(film, pixel, c);
+            add_L(c);
         }
     } else {
         const f3 P = o + d * hit.x;
-        const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
-        const f3 p0 = f3{s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
-        const f3 p1 = f3{s.pos[3 * i1], s.pos[3 * i1 + 1], s.pos[3 * i1 + 2]};
-        const f3 p2 = f3{s.pos[3 * i2], s.pos[3 * i2 + 1], s.pos[3 * i2 + 2]};
-        f3 Ng = normalize(cross_exact(sub_exact(p1, p0), sub_exact(p2, p0)));
-        f3 N = Ng;
-        if (s.nrm) {
-            f3 n0 = f3{s.nrm[3 * i0], s.nrm[3 * i0 + 1], s.nrm[3 * i0 + 2]};
-            f3 n1 = f3{s.nrm[3 * i1], s.nrm[3 * i1 + 1], s.nrm[3 * i1 + 2]};
-            f3 n2 = f3{s.nrm[3 * i2], s.nrm[3 * i2 + 1], s.nrm[3 * i2 + 2]};
-            f3 n = n0 * (1.0f - hit.z - hit.w) + n1 * hit.z + n2 * hit.w;
-            float l2 = dot(n, n);
-            N = l2 < 1e-8f ? Ng : normalize(n);
-        }
-        const bool back = dot(Ng, d) > 0.0f;
-        if (back) Ng = -Ng;
-        if (dot(N, Ng) < 0.0f) N = -N;
+        f3 N, Ng;
+        bool back;
+        hit_frame(s, prim, hit.z, hit.w, d, N, Ng, back);
+        const f3 N_feat = N;
         const uint32_t mesh = s.tri_mesh[prim];
         const nori_material &mr = s.mats[mesh];
         if (mr.isEmitter) {
             f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
             if (last_pdf == 0.0f) {
-                film_add(film, pixel, T * Le);
+                add_L(T * Le);
             } else {
                 // shaders/Emitter.hlsli:91-103
                 f3 dv = P - o;
@@ -236,7 +284,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         pe = (1.0f / (s.emit_area[k] * (float)s.n_emit)) * d2 / cl;
                     }
                 }
-                film_add(film, pixel, T * Le * balance_heuristic(last_pdf, pe));
+                add_L(T * Le * balance_heuristic(last_pdf, pe));
             }
         } else {
             f3 Tg, B;
@@ -273,9 +321,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 last_pdf = 0.0f;
                 if (refr) eta *= etaI / etaT;
                 alive = true;
-            } else if (type == NORI_MAT_DIFFUSE || type == NORI_MAT_MICROFACET || type == NORI_MAT_DISNEY) {
+            } else if (guide_eligible(type)) {
                 const Mat m = load_mat(mr);
                 so = offset_ray_origin(P, Ng, Ng);
+                // One-sample MIS with the learned mixture: the BSDF technique's
+                // density is P(w) = c q(w) + (1-c) b(w) wherever it enters MIS.
+                const bool guided = g.out != nullptr;
+                GuideMix mix;
+                if (guided) mix = guide_decode(g.out + (size_t)i * 64);
+                auto tech_pdf = [&](f3 w_world, float b) { return guided ? mix.c * guide_q(mix, w_world) + (1.0f - mix.c) * b : b; };
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
                     uint32_t pick = min((uint32_t)(pcg_float(rng) * (float)s.n_emit), s.n_emit - 1);
@@ -308,6 +362,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         f3 f;
                         float pb;
                         material_eval_pdf(wi, to_local(ww, Tg, B, N), m, f, pb);
+                        pb = tech_pdf(ww, pb);
                         float pe = pdfA * dist * dist / cl;
                         float w = balance_heuristic(pe, pb);
                         f3 Le = f3{s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]};
@@ -329,31 +384,82 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         f3 f;
                         float pb;
                         material_eval_pdf(wi, to_local(dir, Tg, B, N), m, f, pb);
+                        pb = tech_pdf(dir, pb);
                         float w = balance_heuristic(pe, pb);
                         e_c = T * (Le * f * (ct / fmaxf(pe, 1e-20f) * w));
                         e_dir = dir;
                         sh_env = true;
                     }
                 }
-                float u[3];
-                u[0] = pcg_float(rng);
-                u[1] = pcg_float(rng);
-                u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
-                f3 wo;
-                float pdf;
-                f3 wgt = material_sample(wi, u, m, wo, pdf);
-                if (pdf > 0.0f && !is_zero(wgt)) {
-                    new_o = so;
-                    new_d = to_world(wo, Tg, B, N);
-                    T *= wgt;
-                    last_pdf = fmaxf(pdf, 1e-20f);
-                    alive = true;
+                const f3 T_before = T;
+                f3 w_world = mk(0.0f);
+                float b_pdf = 0.0f;
+                if (!guided) {
+                    float u[3];
+                    u[0] = pcg_float(rng);
+                    u[1] = pcg_float(rng);
+                    u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
+                    f3 wo;
+                    float pdf;
+                    f3 wgt = material_sample(wi, u, m, wo, pdf);
+                    if (pdf > 0.0f && !is_zero(wgt)) {
+                        new_o = so;
+                        new_d = to_world(wo, Tg, B, N);
+                        T *= wgt;
+                        last_pdf = fmaxf(pdf, 1e-20f);
+                        alive = true;
+                        w_world = new_d;
+                        b_pdf = pdf;
+                    }
+                } else {
+                    f3 wo;
+                    if (pcg_float(rng) < mix.c) {
+                        float ul = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng);
+                        w_world = guide_sample(mix, ul, u1, u2);
+                        wo = to_local(w_world, Tg, B, N);
+                    } else {
+                        float u[3];
+                        u[0] = pcg_float(rng);
+                        u[1] = pcg_float(rng);
+                        u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
+                        float pdf;
+                        material_sample(wi, u, m, wo, pdf);
+                        w_world = to_world(wo, Tg, B, N);
+                    }
+                    if (wo.z > 0.0f) {
+                        f3 f;
+                        material_eval_pdf(wi, wo, m, f, b_pdf);
+                        float Pm = tech_pdf(w_world, b_pdf);
+                        if (Pm > 0.0f && !is_zero(f)) {
+                            new_o = so;
+                            new_d = w_world;
+                            T *= f * (wo.z / Pm);
+                            last_pdf = fmaxf(Pm, 1e-20f);
+                            alive = true;
+                        }
+                    }
+                }
+                if (alive && g.rec) {
+                    uint32_t k = g.nrec[slot];
+                    if (k < g.max_rec) {
+                        GuideRecord &r = g.rec[(size_t)k * g.cap + slot];
+                        f3 vv = -d;
+                        r.pos_rough = make_float4(P.x, P.y, P.z, m.roughness);
+                        r.n_metal = make_float4(N_feat.x, N_feat.y, N_feat.z, m.metallic);
+                        r.wi_b = make_float4(vv.x, vv.y, vv.z, b_pdf);
+                        r.w_valid = make_float4(w_world.x, w_world.y, w_world.z, 1.0f);
+                        r.thr = make_float4(T_before.x, T_before.y, T_before.z, 0.0f);
+                        r.lsnap = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
+                        g.nrec[slot] = k + 1;
+                        tag = slot | 0x80000000u;
+                    }
                 }
             }
         }
     }
-    push_shadow(sh, cnt_sh, sh_area, so, a_tmax, a_dir, pixel, a_c);
-    push_shadow(sh, cnt_sh, sh_env, so, 1e20f, e_dir, pixel, e_c);
+    push_shadow(sh, cnt_sh, sh_area, so, a_tmax, a_dir, pixel, a_c, tag);
+    push_shadow(sh, cnt_sh, sh_env, so, 1e20f, e_dir, pixel, e_c, tag);
+    if (g.rec) g.lpath[slot] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
 
     // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not
     // need it: the path ends either way.
@@ -372,6 +478,80 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
         p.aux[slot] = make_float4(eta, ax.y, __uint_as_float((uint32_t)rng), __uint_as_float((uint32_t)(rng >> 32)));
         q_out[k] = slot;
     }
+}
+
+// BF16 features of the surface hits in the queue, one row per queue position.
+__global__ void k_guide_feats(SceneDev s, PathsDev p, const uint32_t *__restrict__ queue,
+                              const uint32_t *__restrict__ count, bf16_t *X) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const uint32_t slot = queue[i];
+    const float4 hit = p.hit[slot];
+    const uint32_t prim = __float_as_uint(hit.y);
+    float f[64];
+    bool ok = false;
+    if (prim != MISS) {
+        const nori_material &mr = s.mats[s.tri_mesh[prim]];
+        if (!mr.isEmitter && guide_eligible(mr.type)) {
+            const f3 o = xyz(p.ray_o[slot]), d = xyz(p.ray_d[slot]);
+            const f3 P = o + d * hit.x;
+            f3 N, Ng;
+            bool back;
+            hit_frame(s, prim, hit.z, hit.w, d, N, Ng, back);
+            guide_features((P - s.bmin) * s.binv, N, -d, mr.roughness, mr.metallic, f);
+            ok = true;
+        }
+    }
+    bf16_t *row = X + (size_t)i * 64;
+    for (int k = 0; k < 64; ++k) row[k] = (bf16_t)(ok ? f[k] : 0.0f);
+}
+
+// Training samples from the records of finished paths: the weight is the
+// luminance of (final path radiance - radiance before the sampled direction),
+// divided channel-wise by the throughput before the vertex.
+__global__ void k_guide_finalize(SceneDev s, GuideShadeDev g, uint32_t npaths, GuideSample *out, uint32_t *count,
+                                 uint32_t max_samples) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= npaths) return;
+    const uint32_t nr = min(g.nrec[slot], g.max_rec);
+    const f3 Lf = xyz(g.lpath[slot]);
+    for (uint32_t k = 0; k < nr; ++k) {
+        const GuideRecord r = g.rec[(size_t)k * g.cap + slot];
+        const f3 La = Lf - xyz(r.lsnap), Tb = xyz(r.thr);
+        const f3 li = mk(Tb.x > 0.0f ? La.x / Tb.x : 0.0f, Tb.y > 0.0f ? La.y / Tb.y : 0.0f,
+                         Tb.z > 0.0f ? La.z / Tb.z : 0.0f);
+        const float W = luminance(li);
+        if (!(W > 0.0f) || !isfinite(W)) continue;
+        const uint32_t j = atomicAdd(count, 1u);
+        if (j >= max_samples) continue;
+        const f3 p01 = (xyz(r.pos_rough) - s.bmin) * s.binv;
+        out[j].pos01_rough = make_float4(p01.x, p01.y, p01.z, r.pos_rough.w);
+        out[j].n_metal = r.n_metal;
+        out[j].wi_b = r.wi_b;
+        out[j].w_weight = make_float4(r.w_valid.x, r.w_valid.y, r.w_valid.z, W);
+    }
+}
+
+__global__ void k_guide_gather(const GuideSample *__restrict__ samples, const uint32_t *__restrict__ count,
+                               uint32_t rows, uint32_t seed, bf16_t *X, float4 *w_weight, float4 *wi_b) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    const uint32_t n = *count;
+    float f[64];
+    float4 ww = make_float4(0.0f, 0.0f, 1.0f, 0.0f), wb = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
+    if (n > 0) {
+        uint64_t h = pcg_seed(((uint64_t)seed << 32) | i, 3);
+        const GuideSample smp = samples[pcg_next(h) % n];
+        guide_features(xyz(smp.pos01_rough), xyz(smp.n_metal), xyz(smp.wi_b), smp.pos01_rough.w, smp.n_metal.w, f);
+        ww = smp.w_weight;
+        wb = smp.wi_b;
+    } else {
+        for (int k = 0; k < 64; ++k) f[k] = 0.0f;
+    }
+    bf16_t *row = X + (size_t)i * 64;
+    for (int k = 0; k < 64; ++k) row[k] = (bf16_t)f[k];
+    w_weight[i] = ww;
+    wi_b[i] = wb;
 }
 
 __global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask) {
@@ -464,9 +644,28 @@ void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uin
 }
 
 void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counters c, const ViewDev &v, uint64_t seed,
-                   uint64_t first, uint32_t n, hipStream_t st) {
+                   uint64_t first, uint32_t n, const GuideShadeDev &g, hipStream_t st) {
     (void)c;
-    hipLaunchKernelGGL(k_raygen, dim3(blocks(n, 256)), dim3(256), 0, st, s, p, queue, v, seed, first, n);
+    hipLaunchKernelGGL(k_raygen, dim3(blocks(n, 256)), dim3(256), 0, st, s, p, queue, v, seed, first, n, g);
+}
+
+void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, bf16_t *X,
+                        uint32_t cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_feats, dim3(blocks(cap, 128)), dim3(128), 0, st, s, p, queue, count, X);
+}
+
+void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, uint32_t npaths, GuideSample *samples,
+                           uint32_t *count, uint32_t max_samples, hipStream_t st) {
+    if (npaths)
+        hipLaunchKernelGGL(k_guide_finalize, dim3(blocks(npaths, 256)), dim3(256), 0, st, s, g, npaths, samples, count,
+                           max_samples);
+}
+
+void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint32_t rows, uint32_t seed, bf16_t *X,
+                         float4 *w_weight, float4 *wi_b, hipStream_t st) {
+    if (rows)
+        hipLaunchKernelGGL(k_guide_gather, dim3(blocks(rows, 128)), dim3(128), 0, st, samples, count, rows, seed, X,
+                           w_weight, wi_b);
 }
 
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, uint32_t cap,
@@ -482,20 +681,20 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, 
 
 void launch_shade(const SceneDev &s, const PathsDev &p, const uint32_t *q_in, const uint32_t *cnt_in, uint32_t *q_out,
                   uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film, uint32_t bounce,
-                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, hipStream_t st) {
+                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
     hipLaunchKernelGGL(k_shade, dim3(blocks(cap, SHADE_BLOCK)), dim3(SHADE_BLOCK), 0, st, s, p, q_in, cnt_in, q_out,
-                       cnt_out, sh, cnt_sh, film, bounce, max_depth, rr_depth);
+                       cnt_out, sh, cnt_sh, film, bounce, max_depth, rr_depth, g);
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
-                   unsigned long long *count_stats, hipStream_t st) {
+                   unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
     if (count_stats)
         hipLaunchKernelGGL(k_shadow<true>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh,
-                           film, count_stats);
+                           film, count_stats, g);
     else
         hipLaunchKernelGGL(k_shadow<false>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh,
-                           film, count_stats);
+                           film, count_stats, g);
 }
 
 void launch_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st) {

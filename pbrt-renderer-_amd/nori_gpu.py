@@ -75,7 +75,8 @@ class RenderStats(ctypes.Structure):
                 ("ms_total", ctypes.c_double), ("ms_trace_closest", ctypes.c_double),
                 ("ms_trace_shadow", ctypes.c_double), ("ms_shade", ctypes.c_double),
                 ("launches_closest", _u64), ("launches_shadow", _u64), ("ext_nodes", _u64), ("ext_tris", _u64),
-                ("shadow_nodes", _u64), ("shadow_tris", _u64)]
+                ("shadow_nodes", _u64), ("shadow_tris", _u64), ("ms_guide_infer", ctypes.c_double),
+                ("ms_guide_train", ctypes.c_double), ("guide_train_steps", _u64)]
 
 
 class BvhInfo(ctypes.Structure):
@@ -358,3 +359,71 @@ class NoriGPU:
         out = np.zeros(n, np.uint32)
         self._check(self.lib.nori_gpu_pcg32(self.h, initstate, initseq, advance, n, _ptr(out)), "pcg32")
         return out
+
+
+# ------------------------------------------------------------------ guiding
+GUIDE_PARAMS = 4 * (64 * 64 + 64)
+
+
+def _guide_sigs(lib):
+    if getattr(lib, "_guide_sigs", False):
+        return
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    lib.nori_guide_params.argtypes, lib.nori_guide_params.restype = [vp, vp, vp], i32
+    lib.nori_guide_forward.argtypes, lib.nori_guide_forward.restype = [vp, vp, i32, vp], i32
+    lib.nori_guide_gradient.argtypes, lib.nori_guide_gradient.restype = [vp, vp, vp, vp, vp, i32, vp, vp], i32
+    lib.nori_guide_train.argtypes = [vp, vp, vp, vp, vp, i32, i32, ctypes.c_float, vp]
+    lib.nori_guide_train.restype = i32
+    lib.nori_guide_mixture.argtypes, lib.nori_guide_mixture.restype = [vp, vp, i32, vp, vp, vp, vp, vp, vp, vp], i32
+    lib._guide_sigs = True
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, np.float32)
+
+
+class _GuideMixin:
+    def guide_params(self, set_params=None):
+        _guide_sigs(self.lib)
+        out = np.zeros(GUIDE_PARAMS, np.float32)
+        self._check(self.lib.nori_guide_params(self.h, _ptr(out), _ptr(_f32(set_params))), "guide_params")
+        return out
+
+    def guide_forward(self, feat):
+        _guide_sigs(self.lib)
+        feat = _f32(feat)
+        out = np.zeros((len(feat), 64), np.float32)
+        self._check(self.lib.nori_guide_forward(self.h, _ptr(feat), len(feat), _ptr(out)), "guide_forward")
+        return out
+
+    def guide_gradient(self, feat, w3, bpdf, weight):
+        _guide_sigs(self.lib)
+        feat, w3, bpdf, weight = (_f32(x) for x in (feat, w3, bpdf, weight))
+        grad = np.zeros(GUIDE_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._check(self.lib.nori_guide_gradient(self.h, _ptr(feat), _ptr(w3), _ptr(bpdf), _ptr(weight), len(feat),
+                                                 _ptr(grad), _ptr(loss)), "guide_gradient")
+        return grad, float(loss[0])
+
+    def guide_train(self, feat, w3, bpdf, weight, steps, lr):
+        _guide_sigs(self.lib)
+        feat, w3, bpdf, weight = (_f32(x) for x in (feat, w3, bpdf, weight))
+        losses = np.zeros(steps, np.float32)
+        self._check(self.lib.nori_guide_train(self.h, _ptr(feat), _ptr(w3), _ptr(bpdf), _ptr(weight), len(feat),
+                                              steps, lr, _ptr(losses)), "guide_train")
+        return losses
+
+    def guide_mixture(self, out64, u3, w3, bpdf):
+        _guide_sigs(self.lib)
+        out64, u3, w3, bpdf = (_f32(x) for x in (out64, u3, w3, bpdf))
+        n = len(out64)
+        d, P, q, c = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self._check(self.lib.nori_guide_mixture(self.h, _ptr(out64), n, _ptr(u3), _ptr(w3), _ptr(bpdf), _ptr(d),
+                                                _ptr(P), _ptr(q), _ptr(c)), "guide_mixture")
+        return d, P, q, c
+
+
+for _name in ("guide_params", "guide_forward", "guide_gradient", "guide_train", "guide_mixture"):
+    setattr(NoriGPU, _name, getattr(_GuideMixin, _name))
+EXPORTS += ["nori_guide_params", "nori_guide_forward", "nori_guide_gradient", "nori_guide_train",
+            "nori_guide_mixture"]

@@ -102,7 +102,11 @@ def test_envmap_alias_draws_bit_exact(gpu, disney_small):
     assert np.allclose(gL, oL, rtol=2e-5, atol=1e-6)
     # pdf query of the sampled direction equals the sampling pdf
     eL, epdf = gpu.envmap_eval(gd)
-    inner = (jx > 0.01) & (jx < 0.99) & (jy > 0.01) & (jy < 0.99)
+    # away from texel borders and the two pole rows, where acos(y) of the
+    # reference's pdf query is ill-conditioned in FP32
+    H = disney_small.env_rgb.shape[0]
+    row = gtex // disney_small.env_rgb.shape[1]
+    inner = (jx > 0.01) & (jx < 0.99) & (jy > 0.01) & (jy < 0.99) & (row > 0) & (row < H - 1)
     assert np.allclose(epdf[inner], gpdf[inner], rtol=1e-4)
 
 
@@ -229,3 +233,33 @@ def test_render_progressive_offsets(gpu, cornell):
     gpu.render(ng.make_view(cam, 8, 8), seed=3)
     two = gpu.render(ng.make_view(cam, 8, 8, sample_offset=8), seed=3)
     assert np.allclose(one, two, rtol=1e-5, atol=1e-6)
+
+
+def test_guided_render_is_unbiased(gpu, disney_small):
+    # One-sample MIS between the learned mixture and the BSDF keeps the estimator
+    # unbiased: the guided image (network trained online from random init)
+    # converges to the plain one.
+    _say("case: guided vs plain, disney 48x48")
+    gpu.upload(disney_small)
+    cam = disney_small.camera
+    cam.width, cam.height = 48, 48
+    gpu.set_option("batch", 48 * 48 * 64)
+    gpu.reset_film()
+    ref = gpu.render(ng.make_view(cam, 2048, 12), seed=11)[:, :3] / 2048
+    gpu.reset_film()
+    plain = gpu.render(ng.make_view(cam, 256, 12), seed=12)[:, :3] / 256
+    gpu.set_option("guiding", 1)
+    gpu.set_option("guide_reset", 1)
+    try:
+        gpu.reset_film()
+        g = gpu.render(ng.make_view(cam, 256, 12), seed=13)
+        st = gpu.stats()
+    finally:
+        gpu.set_option("guiding", 0)
+        gpu.set_option("batch", 1 << 21)
+    assert np.all(g[:, 3] == 256) and st.guide_train_steps > 0
+    gi = g[:, :3] / 256
+    e_plain, e_guided = orc.rel_mse(plain, ref), orc.rel_mse(gi, ref)
+    _say(f"  relMSE plain {e_plain:.3e} guided {e_guided:.3e}; mean {plain.mean():.4f} {gi.mean():.4f}")
+    assert abs(gi.mean() - ref.mean()) < 0.02 * ref.mean()
+    assert e_guided < 4 * e_plain + 1e-4

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared_functions():
     text = open(os.path.join(ROOT, "include", "nori_gpu.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(nori_(?:gpu|scene|bvh8)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(nori_(?:gpu|scene|bvh8|guide)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
