@@ -137,15 +137,18 @@ int nori_gpu_trace(nori_ctx *ctx, const nori_ray *rays, int n, nori_hit *hits, i
 int nori_gpu_trace_device(nori_ctx *ctx, const nori_ray *d_rays, int n, nori_hit *d_hits, int any_hit);
 
 /* BSDF batch evaluation (BSDF::eval/pdf/sample, include/bsdf.h:111-139, with the
- * GPU material semantics of shaders/Material.hlsli:18-194).  One material per
- * query, local frame.  op 0: eval -> rgb; op 1: pdf -> pdf; op 2: sample with
- * u[3] -> rgb = f*cos/pdf, wo, pdf (ops 0 and 1 both write rgb and pdf).
+ * GPU material semantics of shaders/Material.hlsli:18-194 and the hair BCSDF of
+ * shaders/Hair.hlsli:248-432).  One material per query, local frame (hair:
+ * x = fiber tangent, z = normal).  op 0: eval -> rgb; op 1: pdf -> pdf; op 2:
+ * sample with u[4] (4 uniforms per query; hair uses all four, the others the
+ * first two or three) -> rgb = f*|cos|/pdf, wo, pdf (ops 0 and 1 both write rgb
+ * and pdf).  h: hair offsets across the fiber in [-1, 1] (NULL: 0).
  * Host arrays; NULL outputs are skipped. */
 #define NORI_BSDF_EVAL 0
 #define NORI_BSDF_PDF 1
 #define NORI_BSDF_SAMPLE 2
 int nori_gpu_bsdf_batch(nori_ctx *ctx, int op, int n, const nori_material *mats, const float *wi3, const float *wo3,
-                        const float *u3, float *rgb3, float *pdf, float *wo3_out);
+                        const float *u4, const float *h, float *rgb3, float *pdf, float *wo3_out);
 
 /* Envmap alias sampling of the uploaded scene: for each i, a 32-bit integer r,
  * a coin and two jitters -> direction, radiance, solid-angle pdf and texel id.
@@ -222,14 +225,20 @@ int nori_bvh8_build_host(const float *positions, uint32_t num_vertices, const ui
 /* Seeded procedural scenes of the benchmark configurations, built on the host.
  * config 1: Cornell box (5 quads, 2 icospheres, ceiling light).
  * config 2: grid of Disney icospheres on a ground quad under a sky+sun envmap.
+ * config 3: hair strands on a hemisphere scalp as camera-facing ribbons
+ *           (Chiang hair BCSDF); variant 0: melanin-coloured strands, a
+ *           diffuse head and a 3-quad-light rig; variant 1: white furnace
+ *           (colour 1 = no absorption, hair only, uniform unit envmap).
  * Options left 0 take the configuration's defaults. */
 typedef struct nori_scene_options {
     int config;
     uint64_t seed;
-    uint32_t subdiv;        /* icosphere subdivision level                  */
+    uint32_t subdiv;        /* icosphere subdivision level; config 3: segments per strand */
     uint32_t grid;          /* spheres per grid side (config 2)            */
     uint32_t env_width, env_height;
     uint32_t width, height; /* image size of the default camera            */
+    uint32_t strands;       /* config 3: number of strands (default 50000) */
+    uint32_t variant;       /* config 3: 0 lit, 1 white furnace           */
 } nori_scene_options;
 
 typedef struct nori_scene nori_scene;

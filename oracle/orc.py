@@ -31,7 +31,8 @@ def lib():
         L.orc_pcg32.argtypes = [u64, u64, u64, u32, vp]
         L.orc_pcg32_sample_floats.argtypes = [u64, u32, u32, u32, vp]
         L.orc_pcg_hash.argtypes, L.orc_pcg_hash.restype = [u32], u32
-        L.orc_scene_create.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, u32, u32, vp, ctypes.c_float]
+        L.orc_scene_create.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, u32, u32, vp, ctypes.c_float, vp,
+                                       vp]
         L.orc_scene_create.restype = vp
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_bvh_nodes.argtypes, L.orc_scene_bvh_nodes.restype = [vp], u32
@@ -42,7 +43,8 @@ def lib():
         L.orc_env_tables.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_env_sample.argtypes = [vp, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_env_eval_pdf.argtypes = [vp, u32, vp, vp, vp]
-        L.orc_bsdf_batch.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_bsdf_batch.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_hair_fn.argtypes = [i32, u32, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -74,15 +76,32 @@ def alias_build(w):
     return prob, alias
 
 
-def bsdf_batch(op, mats, wi, wo=None, u=None, precision=64):
-    """op 0 eval, 1 pdf, 2 sample -> (rgb, pdf, wo) as float64 arrays."""
+def hair_fn(fn, a, b=None, c=None, d=None, e=None):
+    """FP64 hair pieces: 0 I0(a), 1 logI0(a), 2 Mp(cos_i, cos_o, sin_i, sin_o, v),
+    3 trimmed logistic(x, s), 4 its sample(u, s), 5 logistic CDF(x, s)."""
+    a = np.ascontiguousarray(np.atleast_1d(a), np.float64)
+    n = len(a)
+    args = [np.ascontiguousarray(np.broadcast_to(x, n), np.float64) if x is not None else None for x in (b, c, d, e)]
+    out = np.zeros(n)
+    lib().orc_hair_fn(fn, n, _p(a), *(_p(x) for x in args), _p(out))
+    return out
+
+
+def bsdf_batch(op, mats, wi, wo=None, u=None, precision=64, h=None):
+    """op 0 eval, 1 pdf, 2 sample -> (rgb, pdf, wo) as float64 arrays.  u: n x 3 or
+    n x 4 uniforms (hair takes 4); h: hair offsets."""
     n = len(mats)
     mats = np.ascontiguousarray(mats)
     wi = np.ascontiguousarray(wi, np.float32)
     wo = None if wo is None else np.ascontiguousarray(wo, np.float32)
-    u = None if u is None else np.ascontiguousarray(u, np.float32)
+    if u is not None:
+        u = np.asarray(u, np.float32)
+        if u.shape[1] == 3:
+            u = np.c_[u, np.zeros(n, np.float32)]
+        u = np.ascontiguousarray(u, np.float32)
+    h = None if h is None else np.ascontiguousarray(h, np.float32)
     rgb, pdf, woo = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3))
-    lib().orc_bsdf_batch(op, precision, n, _p(mats), _p(wi), _p(wo), _p(u), _p(rgb), _p(pdf), _p(woo))
+    lib().orc_bsdf_batch(op, precision, n, _p(mats), _p(wi), _p(wo), _p(u), _p(h), _p(rgb), _p(pdf), _p(woo))
     return rgb, pdf, woo
 
 
@@ -99,7 +118,7 @@ class OracleScene:
             len(scene.positions), _p(scene.positions), _p(scene.normals), len(scene.indices), _p(scene.indices),
             _p(tri_mesh), len(scene.materials), _p(scene.materials), _p(scene.mesh_tri_offset),
             _p(scene.mesh_tri_count), 0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
-            _p(env), scene.env_scale)
+            _p(env), scene.env_scale, _p(getattr(scene, "uvs", None)), _p(getattr(scene, "tangents", None)))
 
     def __del__(self):
         if getattr(self, "h", None):

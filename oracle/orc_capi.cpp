@@ -34,12 +34,14 @@ uint32_t orc_pcg_hash(uint32_t v) { return pcg_hash(v); }
 void *orc_scene_create(uint32_t nv, const float *pos, const float *nrm, uint32_t nt, const uint32_t *idx,
                        const uint32_t *tri_mesh, uint32_t nmesh, const void *mats, const uint32_t *mesh_tri_offset,
                        const uint32_t *mesh_tri_count, uint32_t env_w, uint32_t env_h, const float *env_rgb,
-                       float env_scale) {
+                       float env_scale, const float *uv, const float *tan) {
     Handle *h = new Handle();
     SceneData &s = h->s;
     s.nv = nv; s.nt = nt; s.nmesh = nmesh;
     s.pos.assign(pos, pos + 3 * (size_t)nv);
     if (nrm) s.nrm.assign(nrm, nrm + 3 * (size_t)nv);
+    if (uv) s.uv.assign(uv, uv + 2 * (size_t)nv);
+    if (tan) s.tan.assign(tan, tan + 3 * (size_t)nv);
     s.idx.assign(idx, idx + 3 * (size_t)nt);
     s.tri_mesh.assign(tri_mesh, tri_mesh + nt);
     s.mats.resize(nmesh);
@@ -145,29 +147,31 @@ void orc_env_eval_pdf(void *hp, uint32_t n, const float *dir3, float *L3, float 
 }
 
 // ---------------------------------------------------------------- BSDFs
-// op 0 eval (f rgb), 1 pdf, 2 sample (weight rgb, wo, pdf).  One material record
-// per query.  precision 64 evaluates in double from the float inputs.
+// op 0 eval (f rgb), 1 pdf, 2 sample (weight rgb, wo, pdf; u4: 4 uniforms per
+// query).  One material record per query; hv: hair offsets h (may be null).
+// precision 64 evaluates in double from the float inputs.
 void orc_bsdf_batch(int op, int precision, uint32_t n, const void *mats, const float *wi3, const float *wo3,
-                    const float *u3, double *out_rgb, double *out_pdf, double *out_wo) {
+                    const float *u4, const float *hv, double *out_rgb, double *out_pdf, double *out_wo) {
     const MatRec *mr = (const MatRec *)mats;
     auto run = [&](auto zero) {
         using T = decltype(zero);
         for (uint32_t i = 0; i < n; ++i) {
             MatT<T> m = MatT<T>::from(mr[i]);
             V3<T> wi{T(wi3[3 * i]), T(wi3[3 * i + 1]), T(wi3[3 * i + 2])};
+            T h = hv ? T(hv[i]) : T(0);
             if (op == 0 || op == 1) {
                 V3<T> wo{T(wo3[3 * i]), T(wo3[3 * i + 1]), T(wo3[3 * i + 2])};
                 if (op == 0) {
-                    V3<T> f = material_eval(wi, wo, m);
+                    V3<T> f = material_eval_h(wi, wo, m, h);
                     out_rgb[3 * i] = f.x; out_rgb[3 * i + 1] = f.y; out_rgb[3 * i + 2] = f.z;
                 } else {
-                    out_pdf[i] = material_pdf(wi, wo, m);
+                    out_pdf[i] = material_pdf_h(wi, wo, m, h);
                 }
             } else {
-                T u[3] = {T(u3[3 * i]), T(u3[3 * i + 1]), T(u3[3 * i + 2])};
+                T u[4] = {T(u4[4 * i]), T(u4[4 * i + 1]), T(u4[4 * i + 2]), T(u4[4 * i + 3])};
                 V3<T> wo;
                 T pdf;
-                V3<T> w = material_sample(wi, u, m, wo, pdf);
+                V3<T> w = material_sample_h(wi, u, m, h, wo, pdf);
                 out_rgb[3 * i] = w.x; out_rgb[3 * i + 1] = w.y; out_rgb[3 * i + 2] = w.z;
                 out_pdf[i] = pdf;
                 out_wo[3 * i] = wo.x; out_wo[3 * i + 1] = wo.y; out_wo[3 * i + 2] = wo.z;
@@ -176,6 +180,25 @@ void orc_bsdf_batch(int op, int precision, uint32_t n, const void *mats, const f
     };
     if (precision == 64) run(0.0);
     else run(0.0f);
+}
+
+// Scalar pieces of the hair BCSDF in FP64 (closed-form checks):
+// fn 0 I0(a), 1 logI0(a), 2 Mp(cos_i=a, cos_o=b, sin_i=c, sin_o=d, v=e),
+// 3 logistic trimmed to [-pi, pi] (x=a, s=b), 4 its inverse-CDF sample (u=a, s=b),
+// 5 logistic CDF (x=a, s=b).
+void orc_hair_fn(int fn, uint32_t n, const double *a, const double *b, const double *c, const double *d,
+                 const double *e, double *out) {
+    const double PI = Consts<double>::PI;
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (fn) {
+        case 0: out[i] = hair_i0(a[i]); break;
+        case 1: out[i] = hair_log_i0(a[i]); break;
+        case 2: out[i] = hair_mp(a[i], b[i], c[i], d[i], e[i]); break;
+        case 3: out[i] = hair_trimmed_logistic(a[i], b[i], -PI, PI); break;
+        case 4: out[i] = hair_sample_trimmed_logistic(a[i], b[i], -PI, PI); break;
+        default: out[i] = hair_logistic_cdf(a[i], b[i]); break;
+        }
+    }
 }
 
 } // extern "C"

@@ -145,7 +145,46 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
         }
 
         V3f Tg, B;
-        build_onb(N, Tg, B);
+        float hh = 0.0f;
+        if (m.type == MAT_HAIR) {
+            // shaders/Shaders.hlsl:340-387: fiber tangent frame, h from UV
+            uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+            float w0 = 1.0f - h.u - h.v;
+            V3f ht(0.0f);
+            if (!s.tan.empty()) {
+                V3f t0(s.tan[3 * i0], s.tan[3 * i0 + 1], s.tan[3 * i0 + 2]);
+                V3f t1(s.tan[3 * i1], s.tan[3 * i1 + 1], s.tan[3 * i1 + 2]);
+                V3f t2(s.tan[3 * i2], s.tan[3 * i2 + 1], s.tan[3 * i2 + 2]);
+                V3f t = t0 * w0 + t1 * h.u + t2 * h.v;
+                float l2 = dot(t, t);
+                ht = l2 > 1e-8f ? t / std::sqrt(l2) : V3f(0.0f);
+            }
+            float uvx = 0.0f, uvy = 0.0f;
+            if (!s.uv.empty()) {
+                uvx = s.uv[2 * i0] * w0 + s.uv[2 * i1] * h.u + s.uv[2 * i2] * h.v;
+                uvy = s.uv[2 * i0 + 1] * w0 + s.uv[2 * i1 + 1] * h.u + s.uv[2 * i2 + 1] * h.v;
+            }
+            if (dot(ht, ht) > 1e-8f) {
+                ht = normalize(ht);
+                hh = uvy * 2.0f - 1.0f;
+            } else {
+                // hair card: fiber axis from dP/dV, h across the card width
+                V3f p0 = s.vtx(i0), p1 = s.vtx(i1), p2 = s.vtx(i2);
+                float du1 = 0, dv1 = 0, du2 = 0, dv2 = 0;
+                if (!s.uv.empty()) {
+                    du1 = s.uv[2 * i1] - s.uv[2 * i0]; dv1 = s.uv[2 * i1 + 1] - s.uv[2 * i0 + 1];
+                    du2 = s.uv[2 * i2] - s.uv[2 * i0]; dv2 = s.uv[2 * i2 + 1] - s.uv[2 * i0 + 1];
+                }
+                float det = du1 * dv2 - du2 * dv1;
+                if (std::fabs(det) > 1e-8f) ht = normalize((-du1 * (p1 - p0) + du2 * (p2 - p0)) * (1.0f / det));
+                else build_onb(N, ht, B);
+                hh = uvx * 2.0f - 1.0f;
+            }
+            Tg = normalize(ht - N * dot(ht, N));
+            B = cross(N, Tg);
+        } else {
+            build_onb(N, Tg, B);
+        }
         {
             V3f V = -ray.d;
             float ndv = dot(N, V);
@@ -176,7 +215,9 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
             ray = mk_ray(P + on * 0.001f, nd, 0.0f, 1e20f);
             last_pdf = 0.0f;
             if (refr) eta *= etaI / etaT;
-        } else if (m.type == MAT_DIFFUSE || m.type == MAT_MICROFACET || m.type == MAT_DISNEY) {
+        } else if (m.type == MAT_DIFFUSE || m.type == MAT_MICROFACET || m.type == MAT_DISNEY || m.type == MAT_HAIR) {
+            const bool hair = m.type == MAT_HAIR;
+            auto side = [&](V3f w) { return hair && dot(w, Ng) < 0.0f ? -Ng : Ng; }; // shaders/Emitter.hlsli:122-126
             V3f so = offset_ray_origin(P, Ng, Ng);
             // Area light NEE, shaders/Emitter.hlsli:43-89, 105-153
             if (E > 0) {
@@ -195,14 +236,15 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
                 float dist = length(tl);
                 V3f ww = tl / dist;
                 float ct = dot(N, ww), cl = dot(en, -ww);
-                if (ct > 0.0f && cl > 0.0f && !occluded(s, so, ww, dist - 0.001f, st)) {
+                V3f sso = hair ? offset_ray_origin(P, side(ww), side(ww)) : so;
+                if ((hair || ct > 0.0f) && cl > 0.0f && !occluded(s, sso, ww, dist - 0.001f, st)) {
                     V3f wol = to_local(ww, Tg, B, N);
-                    V3f fv = material_eval(wi, wol, m);
-                    float pb = material_pdf(wi, wol, m);
+                    V3f fv = material_eval_h(wi, wol, m, hh);
+                    float pb = material_pdf_h(wi, wol, m, hh);
                     float pe = pdfA * dist * dist / cl;
                     float w = balance_heuristic(pe, pb);
                     V3f Le(s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]);
-                    L += T * (Le * fv * ct / std::max(pe, 1e-20f) * w);
+                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w);
                 }
             }
             // Envmap NEE, shaders/Emitter.hlsli:275-326
@@ -218,23 +260,24 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
                     s.env.sample_alias(r, coin, jx, jy, dir, Le, pe);
                 }
                 float ct = dot(N, dir);
-                if (pe > 0.0f && ct > 0.0f && !occluded(s, so, dir, 1e20f, st)) {
+                V3f sso = hair ? offset_ray_origin(P, side(dir), side(dir)) : so;
+                if (pe > 0.0f && (hair || ct > 0.0f) && !occluded(s, sso, dir, 1e20f, st)) {
                     V3f wol = to_local(dir, Tg, B, N);
-                    V3f fv = material_eval(wi, wol, m);
-                    float pb = material_pdf(wi, wol, m);
+                    V3f fv = material_eval_h(wi, wol, m, hh);
+                    float pb = material_pdf_h(wi, wol, m, hh);
                     float w = balance_heuristic(pe, pb);
-                    L += T * (Le * fv * ct / std::max(pe, 1e-20f) * w);
+                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w);
                 }
             }
-            float u[3];
-            int nd = material_sample_dims(m.type);
+            float u[4];
+            int nd = material_sample_dims_h(m.type);
             for (int i = 0; i < nd; ++i) u[i] = rng.next_float();
             V3f wo;
             float pdf;
-            V3f wgt = material_sample(wi, u, m, wo, pdf);
+            V3f wgt = material_sample_h(wi, u, m, hh, wo, pdf);
             if (pdf <= 0.0f || (wgt.x == 0.0f && wgt.y == 0.0f && wgt.z == 0.0f)) break;
             V3f ww = to_world(wo, Tg, B, N);
-            ray = mk_ray(offset_ray_origin(P, Ng, Ng), ww, 0.0f, 1e20f);
+            ray = mk_ray(offset_ray_origin(P, side(ww), side(ww)), ww, 0.0f, 1e20f);
             T *= wgt;
             last_pdf = std::max(pdf, 1e-20f);
         } else {

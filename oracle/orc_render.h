@@ -18,6 +18,7 @@
 #include "orc_bsdf.h"
 #include "orc_env.h"
 #include "orc_geom.h"
+#include "orc_hair.h"
 #include "orc_pcg32.h"
 
 namespace orc {
@@ -32,6 +33,7 @@ struct Camera {
 struct SceneData {
     uint32_t nv = 0, nt = 0, nmesh = 0;
     std::vector<float> pos, nrm;      // nv*3 (nrm may be empty)
+    std::vector<float> uv, tan;       // nv*2, nv*3 (may be empty)
     std::vector<uint32_t> idx;        // nt*3, global vertex indices
     std::vector<uint32_t> tri_mesh;   // nt
     std::vector<MatRec> mats;         // nmesh

@@ -187,6 +187,206 @@ NG_HD void disney_eval_pdf(f3 wi, f3 wo, const Mat &m, const DisneyPre &q, f3 &f
     pdf = q.pd * (wo.z * INV_PI) + q.ps * pspec + q.pc * pcc;
 }
 
+// ------------------------------------------------------------- hair
+// Chiang et al. 2016 hair BCSDF (PBRT 4e §9.9) as shaders/Hair.hlsli:26-432
+// evaluates it.  Frame: x = fiber tangent, z = surface normal; sin(theta) = w.x,
+// phi = atan2(w.z, w.y); h in [-1, 1] is the offset across the fiber.  Material
+// fields: roughness = beta_m, betaN = beta_n, alpha = scale tilt in degrees,
+// intIOR = eta, albedo = fiber colour (mapped to sigma_a by Chiang eq. 9).
+NG_HD float hair_i0(float x) { // shaders/Hair.hlsli:26-40 (10-term series)
+    float val = 0.0f, x2i = 1.0f, ifact = 1.0f, i4 = 1.0f;
+    for (int i = 0; i < 10; ++i) {
+        if (i > 0) {
+            x2i *= x * x;
+            ifact *= (float)i;
+            i4 *= 4.0f;
+        }
+        val += x2i / (i4 * ifact * ifact);
+    }
+    return val;
+}
+NG_HD float hair_log_i0(float x) {
+    if (x > 12.0f) return x + 0.5f * (-logf(2.0f * PI) + logf(1.0f / x) + 1.0f / (8.0f * x));
+    return logf(hair_i0(x));
+}
+NG_HD float hair_mp(float cos_i, float cos_o, float sin_i, float sin_o, float v) { // :55-66
+    float a = cos_i * cos_o / v, b = sin_i * sin_o / v;
+    if (v <= 0.1f) return expf(hair_log_i0(a) - b - 1.0f / v + 0.6931f + logf(1.0f / (2.0f * v)));
+    return (expf(-b) * hair_i0(a)) / (sinhf(1.0f / v) * 2.0f * v);
+}
+NG_HD float hair_logistic(float x, float s) {
+    float ex = expf(-fabsf(x) / s);
+    return ex / (s * (1.0f + ex) * (1.0f + ex));
+}
+NG_HD float hair_logistic_cdf(float x, float s) { return 1.0f / (1.0f + expf(-x / s)); }
+// Logistic trimmed to [-pi, pi].
+NG_HD float hair_trimmed_logistic(float x, float s) {
+    float den = hair_logistic_cdf(PI, s) - hair_logistic_cdf(-PI, s);
+    return den > 1e-10f ? hair_logistic(x, s) / den : 0.0f;
+}
+NG_HD float hair_sample_trimmed_logistic(float u, float s) { // :90-97
+    float ca = hair_logistic_cdf(-PI, s), cb = hair_logistic_cdf(PI, s);
+    float t = fminf(fmaxf(ca + u * (cb - ca), 1e-6f), 1.0f - 1e-6f);
+    return -s * logf(1.0f / t - 1.0f);
+}
+NG_HD float hair_phi(int p, float go, float gt) { return 2.0f * (float)p * gt - 2.0f * go + (float)p * PI; }
+NG_HD float hair_np(float phi, int p, float s, float go, float gt) { // :108-117
+    float dphi = phi - hair_phi(p, go, gt);
+    while (dphi > PI) dphi -= 2.0f * PI;
+    while (dphi < -PI) dphi += 2.0f * PI;
+    return hair_trimmed_logistic(dphi, s);
+}
+NG_HD float hair_sa1(float c, float den) { // :125-133
+    if (c <= 1e-4f) return 100.0f;
+    float l = logf(c) / den;
+    return l * l;
+}
+struct HairPre {
+    float v[4], s, sin2k[3], cos2k[3];
+    f3 sa;
+};
+NG_HD HairPre hair_pre(const Mat &m) { // :145-177
+    HairPre p;
+    float bm = m.roughness;
+    float v0 = 0.726f * bm + 0.812f * bm * bm + 3.7f * powf(bm, 20.0f);
+    v0 = v0 * v0;
+    p.v[0] = v0;
+    p.v[1] = 0.25f * v0;
+    p.v[2] = 4.0f * v0;
+    p.v[3] = p.v[2];
+    float bn = m.betaN;
+    p.s = 0.626657069f * (0.265f * bn + 1.194f * bn * bn + 5.372f * powf(bn, 22.0f));
+    float ar = m.alpha * (PI / 180.0f);
+    p.sin2k[0] = sinf(ar);
+    p.cos2k[0] = sqrtf(fmaxf(0.0f, 1.0f - p.sin2k[0] * p.sin2k[0]));
+    for (int i = 1; i < 3; ++i) {
+        p.sin2k[i] = 2.0f * p.cos2k[i - 1] * p.sin2k[i - 1];
+        p.cos2k[i] = p.cos2k[i - 1] * p.cos2k[i - 1] - p.sin2k[i - 1] * p.sin2k[i - 1];
+    }
+    float b2 = bn * bn, b3 = b2 * bn, b4 = b3 * bn, b5 = b4 * bn;
+    float den = 5.969f - 0.215f * bn + 2.532f * b2 - 10.73f * b3 + 5.574f * b4 + 0.245f * b5;
+    p.sa = mk(hair_sa1(m.albedo.x, den), hair_sa1(m.albedo.y, den), hair_sa1(m.albedo.z, den));
+    return p;
+}
+// Attenuation per lobe (:182-208).
+NG_HD void hair_ap(float cos_o, float sin_o, float h, float eta, f3 sa, f3 ap[4]) {
+    float etap = sqrtf(fmaxf(0.0f, eta * eta - sin_o * sin_o)) / fmaxf(cos_o, 1e-6f);
+    float sgt = fminf(fmaxf(h / etap, -1.0f), 1.0f);
+    float cgt = sqrtf(fmaxf(0.0f, 1.0f - sgt * sgt));
+    float stt = sin_o / eta;
+    float ctt = sqrtf(fmaxf(0.0f, 1.0f - stt * stt));
+    float len = 2.0f * cgt / fmaxf(ctt, 1e-6f);
+    f3 Tr = mk(expf(-sa.x * len), expf(-sa.y * len), expf(-sa.z * len));
+    float cgo = sqrtf(fmaxf(0.0f, 1.0f - h * h));
+    float f = fresnel_dielectric(cos_o * cgo, 1.0f, eta);
+    ap[0] = mk(f);
+    ap[1] = Tr * ((1.0f - f) * (1.0f - f));
+    ap[2] = ap[1] * Tr * f;
+    f3 tf = Tr * f;
+    f3 den = mk(fmaxf(1.0f - tf.x, 1e-10f), fmaxf(1.0f - tf.y, 1e-10f), fmaxf(1.0f - tf.z, 1e-10f));
+    f3 a3 = ap[2] * f * Tr;
+    ap[3] = mk(a3.x / den.x, a3.y / den.y, a3.z / den.z);
+}
+// Scale-tilted outgoing angle of lobe p (:214-239).
+NG_HD void hair_tilt(int p, float sin_o, float cos_o, const HairPre &hp, float &sp, float &cp) {
+    if (p == 0) {
+        sp = sin_o * hp.cos2k[1] - cos_o * hp.sin2k[1];
+        cp = cos_o * hp.cos2k[1] + sin_o * hp.sin2k[1];
+    } else if (p == 1) {
+        sp = sin_o * hp.cos2k[0] + cos_o * hp.sin2k[0];
+        cp = cos_o * hp.cos2k[0] - sin_o * hp.sin2k[0];
+    } else if (p == 2) {
+        sp = sin_o * hp.cos2k[2] + cos_o * hp.sin2k[2];
+        cp = cos_o * hp.cos2k[2] - sin_o * hp.sin2k[2];
+    } else {
+        sp = sin_o;
+        cp = cos_o;
+    }
+    cp = fabsf(cp);
+}
+struct HairGeom {
+    float sin_o, cos_o, phi_o, go, gt;
+};
+NG_HD HairGeom hair_geom(f3 wi, float h, float eta) {
+    HairGeom g;
+    g.sin_o = wi.x;
+    g.cos_o = sqrtf(fmaxf(0.0f, 1.0f - g.sin_o * g.sin_o));
+    g.phi_o = atan2f(wi.z, wi.y);
+    g.go = asinf(fminf(fmaxf(h, -1.0f), 1.0f));
+    float etap = sqrtf(fmaxf(0.0f, eta * eta - g.sin_o * g.sin_o)) / fmaxf(g.cos_o, 1e-6f);
+    g.gt = asinf(fminf(fmaxf(h / etap, -1.0f), 1.0f));
+    return g;
+}
+// f (divided by |wo.z|, :248-300) and the lobe-mixture pdf (:303-348) together.
+NG_HD void hair_eval_pdf(f3 wi, f3 wo, const Mat &m, float h, const HairGeom &g, const HairPre &hp, const f3 ap[4],
+                         f3 &f, float &pdf) {
+    float sin_i = wo.x, cos_i = sqrtf(fmaxf(0.0f, 1.0f - sin_i * sin_i));
+    float phi = atan2f(wo.z, wo.y) - g.phi_o;
+    float lum[4], sum = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        lum[i] = luminance(ap[i]);
+        sum += lum[i];
+    }
+    float inv = sum > 0.0f ? 1.0f / sum : 0.0f;
+    f = mk(0.0f);
+    pdf = 0.0f;
+    for (int p = 0; p < 3; ++p) {
+        float sp, cp;
+        hair_tilt(p, g.sin_o, g.cos_o, hp, sp, cp);
+        float mn = hair_mp(cos_i, cp, sin_i, sp, hp.v[p]) * hair_np(phi, p, hp.s, g.go, g.gt);
+        f += ap[p] * mn;
+        pdf += mn * (lum[p] * inv);
+    }
+    float m3 = hair_mp(cos_i, g.cos_o, sin_i, g.sin_o, hp.v[3]) * (1.0f / (2.0f * PI));
+    f += ap[3] * m3;
+    pdf += m3 * (lum[3] * inv);
+    float ac = fabsf(wo.z);
+    if (ac > 1e-6f) f = f * (1.0f / ac);
+}
+NG_HD void hair_eval_pdf(f3 wi, f3 wo, const Mat &m, float h, f3 &f, float &pdf) {
+    HairGeom g = hair_geom(wi, h, m.intIOR);
+    HairPre hp = hair_pre(m);
+    f3 ap[4];
+    hair_ap(g.cos_o, g.sin_o, h, m.intIOR, hp.sa, ap);
+    hair_eval_pdf(wi, wo, m, h, g, hp, ap, f, pdf);
+}
+// :351-432; u[0] picks the lobe, u[1], u[2] the longitudinal angle, u[3] the azimuth.
+NG_HD f3 hair_sample(f3 wi, const float *u, const Mat &m, float h, f3 &wo, float &pdf) {
+    HairGeom g = hair_geom(wi, h, m.intIOR);
+    HairPre hp = hair_pre(m);
+    f3 ap[4];
+    hair_ap(g.cos_o, g.sin_o, h, m.intIOR, hp.sa, ap);
+    float apdf[4], sum = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        apdf[i] = luminance(ap[i]);
+        sum += apdf[i];
+    }
+    float inv = sum > 0.0f ? 1.0f / sum : 0.0f;
+    int p = 0;
+    float cdf = apdf[0] * inv;
+    while (p < 3 && u[0] > cdf) {
+        ++p;
+        cdf += apdf[p] * inv;
+    }
+    float sp, cp;
+    hair_tilt(p, g.sin_o, g.cos_o, hp, sp, cp);
+    float vp = hp.v[p];
+    float ct = 1.0f + vp * logf(fmaxf(u[1], 1e-5f) + (1.0f - u[1]) * expf(-2.0f / vp));
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    float cphi = cosf(2.0f * PI * u[2]);
+    float sin_i = -ct * sp + st * cphi * cp;
+    float cos_i = sqrtf(fmaxf(0.0f, 1.0f - sin_i * sin_i));
+    float dphi = p < 3 ? hair_phi(p, g.go, g.gt) + hair_sample_trimmed_logistic(u[3], hp.s) : 2.0f * PI * u[3];
+    float phi_i = g.phi_o + dphi;
+    float sphi, cphi_i;
+    sincosf(phi_i, &sphi, &cphi_i);
+    wo = mk(sin_i, cos_i * cphi_i, cos_i * sphi);
+    f3 f;
+    hair_eval_pdf(wi, wo, m, h, g, hp, ap, f, pdf);
+    if (pdf <= 0.0f) return mk(0.0f);
+    return f * (fabsf(wo.z) / pdf);
+}
+
 // ------------------------------------------------------------- dispatch
 NG_HD void material_eval_pdf(f3 wi, f3 wo, const Mat &m, f3 &f, float &pdf) {
     if (m.type == NORI_MAT_DISNEY) {
@@ -204,12 +404,18 @@ NG_HD void material_eval_pdf(f3 wi, f3 wo, const Mat &m, f3 &f, float &pdf) {
         pdf = 0.0f;
     }
 }
-NG_HD int material_sample_dims(uint32_t t) { return t == NORI_MAT_DISNEY ? 3 : 2; }
+// Dispatch including hair (h is ignored by the other materials).
+NG_HD void material_eval_pdf(f3 wi, f3 wo, const Mat &m, float h, f3 &f, float &pdf) {
+    if (m.type == NORI_MAT_HAIR) hair_eval_pdf(wi, wo, m, h, f, pdf);
+    else material_eval_pdf(wi, wo, m, f, pdf);
+}
+NG_HD int material_sample_dims(uint32_t t) { return t == NORI_MAT_HAIR ? 4 : t == NORI_MAT_DISNEY ? 3 : 2; }
 
-// Returns f*cos/pdf; u holds material_sample_dims(type) uniforms.
-NG_HD f3 material_sample(f3 wi, const float *u, const Mat &m, f3 &wo, float &pdf) {
+// Returns f*|cos|/pdf; u holds material_sample_dims(type) uniforms.
+NG_HD f3 material_sample(f3 wi, const float *u, const Mat &m, float h, f3 &wo, float &pdf) {
     wo = mk(0, 0, 1);
     pdf = 0.0f;
+    if (m.type == NORI_MAT_HAIR) return hair_sample(wi, u, m, h, wo, pdf);
     if (wi.z <= 0.0f) return mk(0.0f);
     if (m.type == NORI_MAT_DIFFUSE) {
         wo = cosine_sample_hemisphere(u[0], u[1]);

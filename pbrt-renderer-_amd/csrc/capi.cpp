@@ -41,6 +41,16 @@ struct ArgError : std::exception {
     const char *what() const noexcept override { return msg.c_str(); }
 };
 
+// Temporary device buffers of one call, released however the call ends.
+struct ScopedBufs {
+    std::vector<DevBuf> v;
+    explicit ScopedBufs(size_t n) : v(n) {}
+    ~ScopedBufs() {
+        for (auto &x : v) x.release();
+    }
+    DevBuf &operator[](size_t i) { return v[i]; }
+};
+
 #define HIPCHK(x)                                                                                                      \
     do {                                                                                                               \
         hipError_t e_ = (x);                                                                                           \
@@ -359,6 +369,8 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.tris = (const float4 *)c->upload(bvh.tris.data(), bvh.tris.size());
         s.pos = c->upload(d->positions, 3 * (size_t)d->num_vertices);
         s.nrm = d->normals ? c->upload(d->normals, 3 * (size_t)d->num_vertices) : nullptr;
+        s.uv = d->uvs ? c->upload(d->uvs, 2 * (size_t)d->num_vertices) : nullptr;
+        s.tan = d->tangents ? c->upload(d->tangents, 3 * (size_t)d->num_vertices) : nullptr;
         s.idx = c->upload(d->indices, 3 * (size_t)d->num_triangles);
         s.tri_mesh = c->upload(tri_mesh.data(), tri_mesh.size());
         s.mats = c->upload(mats.data(), mats.size());
@@ -368,6 +380,15 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.emit_cdf = c->upload(cdf.data(), cdf.size());
         s.emit_area = c->upload(area.data(), area.size());
         s.stack_depth = bvh.max_depth + 1;
+        // Scene bounds map hit positions to [0,1]^3 for the guiding features.
+        f3 lo = mk(INFINITY, INFINITY, INFINITY), hi = mk(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t i = 0; i < d->num_vertices; ++i) {
+            f3 p = vtx(i);
+            lo = mk(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+            hi = mk(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+        }
+        s.bmin = lo;
+        s.binv = mk(1.0f / fmaxf(hi.x - lo.x, 1e-6f), 1.0f / fmaxf(hi.y - lo.y, 1e-6f), 1.0f / fmaxf(hi.z - lo.z, 1e-6f));
         s.has_env = 0;
         if (d->env_width && d->env_height && d->env_rgb) {
             EnvTables et;
@@ -604,34 +625,35 @@ int nori_gpu_trace(nori_ctx *c, const nori_ray *rays, int n, nori_hit *hits, int
 }
 
 int nori_gpu_bsdf_batch(nori_ctx *c, int op, int n, const nori_material *mats, const float *wi3, const float *wo3,
-                        const float *u3, float *rgb3, float *pdf, float *wo3_out) {
+                        const float *u4, const float *h, float *rgb3, float *pdf, float *wo3_out) {
     if (!c || n < 0 || op < 0 || op > 2) return NORI_E_ARG;
-    if (n > 0 && (!mats || !wi3 || (op != 2 && !wo3) || (op == 2 && !u3))) return NORI_E_ARG;
+    if (n > 0 && (!mats || !wi3 || (op != 2 && !wo3) || (op == 2 && !u4))) return NORI_E_ARG;
     return guard(c, [&] {
         if (n == 0) return;
-        std::vector<DevBuf> b(7);
+        ScopedBufs b(8);
         auto up = [&](DevBuf &d, const void *src, size_t bytes) {
             HIPCHK(hipMalloc(&d.p, bytes));
             d.bytes = bytes;
             if (src) HIPCHK(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
             return d.p;
         };
-        size_t v3 = (size_t)n * 3 * sizeof(float);
+        size_t v1 = (size_t)n * sizeof(float), v3 = 3 * v1;
         void *dm = up(b[0], mats, (size_t)n * sizeof(nori_material));
         void *dwi = up(b[1], wi3, v3);
         void *dwo = op != 2 ? up(b[2], wo3, v3) : nullptr;
-        void *du = op == 2 ? up(b[3], u3, v3) : nullptr;
+        void *du = op == 2 ? up(b[3], u4, 4 * v1) : nullptr;
+        void *dh = h ? up(b[7], h, v1) : nullptr;
         void *drgb = up(b[4], nullptr, v3);
-        void *dpdf = up(b[5], nullptr, (size_t)n * sizeof(float));
+        void *dpdf = up(b[5], nullptr, v1);
         void *dwoo = up(b[6], nullptr, v3);
         launch_bsdf_batch(op, (uint32_t)n, (const nori_material *)dm, (const float *)dwi, (const float *)dwo,
-                          (const float *)du, (float *)drgb, (float *)dpdf, (float *)dwoo, c->stream);
+                          (const float *)du, (const float *)dh, (float *)drgb, (float *)dpdf, (float *)dwoo,
+                          c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipGetLastError());
         if (rgb3) HIPCHK(hipMemcpy(rgb3, drgb, v3, hipMemcpyDeviceToHost));
-        if (pdf) HIPCHK(hipMemcpy(pdf, dpdf, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+        if (pdf) HIPCHK(hipMemcpy(pdf, dpdf, v1, hipMemcpyDeviceToHost));
         if (wo3_out && op == 2) HIPCHK(hipMemcpy(wo3_out, dwoo, v3, hipMemcpyDeviceToHost));
-        for (auto &x : b) x.release();
     });
 }
 

@@ -12,6 +12,8 @@ struct SceneDev {
     const float4 *tris;  // 3 float4 per triangle (leaf order)
     const float *pos;    // 3 per vertex
     const float *nrm;    // 3 per vertex or null
+    const float *uv;     // 2 per vertex or null
+    const float *tan;    // 3 per vertex or null (hair strand tangents)
     const uint32_t *idx; // 3 per triangle (global vertex ids)
     const uint32_t *tri_mesh;
     const nori_material *mats;
@@ -95,7 +97,7 @@ void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st);
 void launch_zero_f4(float4 *p, size_t n, hipStream_t st);
 
 void launch_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo,
-                       const float *u, float *rgb, float *pdf, float *wo_out, hipStream_t st);
+                       const float *u, const float *hv, float *rgb, float *pdf, float *wo_out, hipStream_t st);
 void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,
                        const float *jy, float *dir, float *rgb, float *pdf, uint32_t *texel, hipStream_t st);
 void launch_env_eval(const EnvDev &e, uint32_t n, const float *dir, float *rgb, float *pdf, hipStream_t st);

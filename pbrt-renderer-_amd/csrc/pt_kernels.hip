@@ -225,6 +225,49 @@ __device__ __forceinline__ void hit_frame(const SceneDev &s, uint32_t prim, floa
     if (dot(N, Ng) < 0.0f) N = -N;
 }
 
+// Fiber frame of a hair hit (shaders/Shaders.hlsl:340-387): x = the strand
+// tangent (interpolated per vertex, or dP/dV of a hair card), z = N; h, the
+// offset across the fiber, from the UV (v for strands, u for cards).
+__device__ void hair_frame(const SceneDev &s, uint32_t prim, float u, float v, f3 N, f3 &Tg, f3 &B, float &h) {
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const float w0 = 1.0f - u - v;
+    f3 ht = mk(0.0f);
+    if (s.tan) {
+        f3 t0 = f3{s.tan[3 * i0], s.tan[3 * i0 + 1], s.tan[3 * i0 + 2]};
+        f3 t1 = f3{s.tan[3 * i1], s.tan[3 * i1 + 1], s.tan[3 * i1 + 2]};
+        f3 t2 = f3{s.tan[3 * i2], s.tan[3 * i2 + 1], s.tan[3 * i2 + 2]};
+        f3 t = t0 * w0 + t1 * u + t2 * v;
+        float l2 = dot(t, t);
+        if (l2 > 1e-8f) ht = t / sqrtf(l2);
+    }
+    float uvx = 0.0f, uvy = 0.0f;
+    if (s.uv) {
+        uvx = s.uv[2 * i0] * w0 + s.uv[2 * i1] * u + s.uv[2 * i2] * v;
+        uvy = s.uv[2 * i0 + 1] * w0 + s.uv[2 * i1 + 1] * u + s.uv[2 * i2 + 1] * v;
+    }
+    if (dot(ht, ht) > 1e-8f) {
+        ht = normalize(ht);
+        h = uvy * 2.0f - 1.0f;
+    } else {
+        const f3 p0 = f3{s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
+        const f3 p1 = f3{s.pos[3 * i1], s.pos[3 * i1 + 1], s.pos[3 * i1 + 2]};
+        const f3 p2 = f3{s.pos[3 * i2], s.pos[3 * i2 + 1], s.pos[3 * i2 + 2]};
+        float du1 = 0.0f, dv1 = 0.0f, du2 = 0.0f, dv2 = 0.0f;
+        if (s.uv) {
+            du1 = s.uv[2 * i1] - s.uv[2 * i0];
+            dv1 = s.uv[2 * i1 + 1] - s.uv[2 * i0 + 1];
+            du2 = s.uv[2 * i2] - s.uv[2 * i0];
+            dv2 = s.uv[2 * i2 + 1] - s.uv[2 * i0 + 1];
+        }
+        float det = du1 * dv2 - du2 * dv1;
+        if (fabsf(det) > 1e-8f) ht = normalize((-du1 * (p1 - p0) + du2 * (p2 - p0)) * (1.0f / det));
+        else build_onb(N, ht, B);
+        h = uvx * 2.0f - 1.0f;
+    }
+    Tg = normalize(ht - N * dot(ht, N));
+    B = cross(N, Tg);
+}
+
 // One bounce of shaders/Shaders.hlsl:108-627 for every path in q_in.
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, const uint32_t *__restrict__ q_in,
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *q_out,
@@ -250,7 +293,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
         if (g.rec && finite3(c)) lpath += c;
     };
     f3 new_o = mk(0.0f), new_d = mk(0.0f);
-    f3 so = mk(0.0f), a_dir = mk(0.0f), a_c = mk(0.0f), e_dir = mk(0.0f), e_c = mk(0.0f);
+    f3 so = mk(0.0f), a_o = mk(0.0f), a_dir = mk(0.0f), a_c = mk(0.0f), e_o = mk(0.0f), e_dir = mk(0.0f),
+       e_c = mk(0.0f);
     float a_tmax = 0.0f;
 
     if (prim == MISS) {
@@ -287,8 +331,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 add_L(T * Le * balance_heuristic(last_pdf, pe));
             }
         } else {
+            const uint32_t type = mr.type;
+            const bool hair = type == NORI_MAT_HAIR;
             f3 Tg, B;
-            build_onb(N, Tg, B);
+            float hh = 0.0f;
+            if (hair) hair_frame(s, prim, hit.z, hit.w, N, Tg, B, hh);
+            else build_onb(N, Tg, B);
             {
                 f3 V = -d;
                 float ndv = dot(N, V);
@@ -298,7 +346,6 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 }
             }
             const f3 wi = to_local(-d, Tg, B, N);
-            const uint32_t type = mr.type;
             if (type == NORI_MAT_MIRROR) {
                 new_o = P + Ng * 0.001f;
                 new_d = reflect(d, N);
@@ -321,12 +368,19 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 last_pdf = 0.0f;
                 if (refr) eta *= etaI / etaT;
                 alive = true;
-            } else if (guide_eligible(type)) {
+            } else if (guide_eligible(type) || hair) {
                 const Mat m = load_mat(mr);
                 so = offset_ray_origin(P, Ng, Ng);
+                // Hair transmits: rays leave from the side of the fiber surface
+                // they head to (shaders/Emitter.hlsli:122-126).
+                auto origin = [&](f3 w) {
+                    if (!hair) return so;
+                    f3 n = dot(w, Ng) < 0.0f ? -Ng : Ng;
+                    return offset_ray_origin(P, n, n);
+                };
                 // One-sample MIS with the learned mixture: the BSDF technique's
                 // density is P(w) = c q(w) + (1-c) b(w) wherever it enters MIS.
-                const bool guided = g.out != nullptr;
+                const bool guided = g.out != nullptr && !hair;
                 GuideMix mix;
                 if (guided) mix = guide_decode(g.out + (size_t)i * 64);
                 auto tech_pdf = [&](f3 w_world, float b) { return guided ? mix.c * guide_q(mix, w_world) + (1.0f - mix.c) * b : b; };
@@ -358,15 +412,16 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                     float dist = length(tl);
                     f3 ww = tl / dist;
                     float ct = dot(N, ww), cl = dot(en, -ww);
-                    if (ct > 0.0f && cl > 0.0f) {
+                    if ((hair || ct > 0.0f) && cl > 0.0f) {
                         f3 f;
                         float pb;
-                        material_eval_pdf(wi, to_local(ww, Tg, B, N), m, f, pb);
+                        material_eval_pdf(wi, to_local(ww, Tg, B, N), m, hh, f, pb);
                         pb = tech_pdf(ww, pb);
                         float pe = pdfA * dist * dist / cl;
                         float w = balance_heuristic(pe, pb);
                         f3 Le = f3{s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]};
-                        a_c = T * (Le * f * (ct / fmaxf(pe, 1e-20f) * w));
+                        a_c = T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w));
+                        a_o = origin(ww);
                         a_dir = ww;
                         a_tmax = dist - 0.001f;
                         sh_area = true;
@@ -380,13 +435,14 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                     float pe;
                     env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
                     float ct = dot(N, dir);
-                    if (pe > 0.0f && ct > 0.0f) {
+                    if (pe > 0.0f && (hair || ct > 0.0f)) {
                         f3 f;
                         float pb;
-                        material_eval_pdf(wi, to_local(dir, Tg, B, N), m, f, pb);
+                        material_eval_pdf(wi, to_local(dir, Tg, B, N), m, hh, f, pb);
                         pb = tech_pdf(dir, pb);
                         float w = balance_heuristic(pe, pb);
-                        e_c = T * (Le * f * (ct / fmaxf(pe, 1e-20f) * w));
+                        e_c = T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w));
+                        e_o = origin(dir);
                         e_dir = dir;
                         sh_env = true;
                     }
@@ -395,16 +451,17 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 f3 w_world = mk(0.0f);
                 float b_pdf = 0.0f;
                 if (!guided) {
-                    float u[3];
+                    float u[4];
                     u[0] = pcg_float(rng);
                     u[1] = pcg_float(rng);
-                    u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
+                    u[2] = type == NORI_MAT_DISNEY || hair ? pcg_float(rng) : 0.0f;
+                    u[3] = hair ? pcg_float(rng) : 0.0f;
                     f3 wo;
                     float pdf;
-                    f3 wgt = material_sample(wi, u, m, wo, pdf);
+                    f3 wgt = material_sample(wi, u, m, hh, wo, pdf);
                     if (pdf > 0.0f && !is_zero(wgt)) {
-                        new_o = so;
                         new_d = to_world(wo, Tg, B, N);
+                        new_o = origin(new_d);
                         T *= wgt;
                         last_pdf = fmaxf(pdf, 1e-20f);
                         alive = true;
@@ -418,12 +475,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         w_world = guide_sample(mix, ul, u1, u2);
                         wo = to_local(w_world, Tg, B, N);
                     } else {
-                        float u[3];
+                        float u[4];
                         u[0] = pcg_float(rng);
                         u[1] = pcg_float(rng);
                         u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
+                        u[3] = 0.0f;
                         float pdf;
-                        material_sample(wi, u, m, wo, pdf);
+                        material_sample(wi, u, m, 0.0f, wo, pdf);
                         w_world = to_world(wo, Tg, B, N);
                     }
                     if (wo.z > 0.0f) {
@@ -439,7 +497,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         }
                     }
                 }
-                if (alive && g.rec) {
+                if (alive && g.rec && !hair) {
                     uint32_t k = g.nrec[slot];
                     if (k < g.max_rec) {
                         GuideRecord &r = g.rec[(size_t)k * g.cap + slot];
@@ -457,8 +515,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
             }
         }
     }
-    push_shadow(sh, cnt_sh, sh_area, so, a_tmax, a_dir, pixel, a_c, tag);
-    push_shadow(sh, cnt_sh, sh_env, so, 1e20f, e_dir, pixel, e_c, tag);
+    push_shadow(sh, cnt_sh, sh_area, a_o, a_tmax, a_dir, pixel, a_c, tag);
+    push_shadow(sh, cnt_sh, sh_env, e_o, 1e20f, e_dir, pixel, e_c, tag);
     if (g.rec) g.lpath[slot] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
 
     // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not
@@ -579,22 +637,23 @@ __global__ void k_zero_f4(float4 *p, size_t n) {
 // --------------------------------------------------------------- checks
 
 __global__ void k_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo,
-                             const float *u, float *rgb, float *pdf, float *wo_out) {
+                             const float *u, const float *hv, float *rgb, float *pdf, float *wo_out) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Mat m = load_mat(mats[i]);
     f3 a = f3{wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]};
+    const float h = hv ? hv[i] : 0.0f;
     if (op == NORI_BSDF_EVAL || op == NORI_BSDF_PDF) {
         f3 b = f3{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]}, f;
         float pd;
-        material_eval_pdf(a, b, m, f, pd);
+        material_eval_pdf(a, b, m, h, f, pd);
         if (rgb) { rgb[3 * i] = f.x; rgb[3 * i + 1] = f.y; rgb[3 * i + 2] = f.z; }
         if (pdf) pdf[i] = pd;
     } else {
-        float uu[3] = {u[3 * i], u[3 * i + 1], u[3 * i + 2]};
+        float uu[4] = {u[4 * i], u[4 * i + 1], u[4 * i + 2], u[4 * i + 3]};
         f3 w, o;
         float pd;
-        w = material_sample(a, uu, m, o, pd);
+        w = material_sample(a, uu, m, h, o, pd);
         if (rgb) { rgb[3 * i] = w.x; rgb[3 * i + 1] = w.y; rgb[3 * i + 2] = w.z; }
         if (pdf) pdf[i] = pd;
         if (wo_out) { wo_out[3 * i] = o.x; wo_out[3 * i + 1] = o.y; wo_out[3 * i + 2] = o.z; }
@@ -710,8 +769,8 @@ void launch_zero_f4(float4 *p, size_t n, hipStream_t st) {
 }
 
 void launch_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo, const float *u,
-                       float *rgb, float *pdf, float *wo_out, hipStream_t st) {
-    if (n) hipLaunchKernelGGL(k_bsdf_batch, dim3(blocks(n, 256)), dim3(256), 0, st, op, n, mats, wi, wo, u, rgb, pdf, wo_out);
+                       const float *hv, float *rgb, float *pdf, float *wo_out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_bsdf_batch, dim3(blocks(n, 256)), dim3(256), 0, st, op, n, mats, wi, wo, u, hv, rgb, pdf, wo_out);
 }
 
 void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,

@@ -9,6 +9,15 @@
 //   config 2  grid x grid Disney icospheres on a Lambertian ground quad, lit by a
 //             sky-gradient envmap with a 0.5 degree sun disc of radiance 5e4 in
 //             a seeded direction.
+//   config 3  hair: strands rooted uniformly on the upper hemisphere of a
+//             radius-10 scalp, each a polyline whose direction random-walks
+//             (Gaussian perturbation plus a pull downwards), tessellated as
+//             ribbons of half-width 0.04 that face the camera (2 triangles per
+//             segment).  Per strand one hair material (beta_m = beta_n = 0.3,
+//             alpha = 2 degrees, eta = 1.55).  Variant 0: eumelanin U[0.3, 8]
+//             and pheomelanin U[0, 0.5] per strand, a diffuse head and three
+//             quad lights; variant 1 (white furnace): colour 1 (sigma_a = 0),
+//             hair only, under a uniform unit envmap.
 #include "scene_gen.h"
 #include <cmath>
 #include <cstring>
@@ -28,6 +37,13 @@ struct Rng {
         float f;
         std::memcpy(&f, &u, 4);
         return f - 1.0f;
+    }
+    // Standard normal pair (Box-Muller).
+    void gauss2(float &a, float &b) {
+        float u1 = 1.0f - next(), u2 = next();
+        float r = std::sqrt(-2.0f * std::log(u1));
+        a = r * std::cos(2.0f * PI * u2);
+        b = r * std::sin(2.0f * PI * u2);
     }
 };
 
@@ -52,12 +68,20 @@ nori_material default_material() {
 
 struct Builder {
     SceneData &s;
+    bool attrs = false; // scene carries per-vertex UVs and tangents (zero where a mesh has none)
     uint32_t begin_mesh() { return (uint32_t)s.mesh_tri_offset.size(); }
     void add_mesh(const std::vector<float> &pos, const std::vector<float> &nrm, const std::vector<uint32_t> &tri,
-                  const nori_material &mat) {
+                  const nori_material &mat, const std::vector<float> *uv = nullptr,
+                  const std::vector<float> *tan = nullptr) {
         uint32_t v0 = (uint32_t)(s.pos.size() / 3), t0 = (uint32_t)(s.idx.size() / 3);
         s.pos.insert(s.pos.end(), pos.begin(), pos.end());
         s.nrm.insert(s.nrm.end(), nrm.begin(), nrm.end());
+        if (attrs) {
+            if (uv) s.uv.insert(s.uv.end(), uv->begin(), uv->end());
+            else s.uv.resize(s.uv.size() + pos.size() / 3 * 2, 0.0f);
+            if (tan) s.tan.insert(s.tan.end(), tan->begin(), tan->end());
+            else s.tan.resize(s.tan.size() + pos.size(), 0.0f);
+        }
         for (uint32_t i : tri) s.idx.push_back(i + v0);
         nori_material m = mat;
         m.vertexOffset = v0;
@@ -92,6 +116,100 @@ struct Builder {
         add_mesh(pos, nrm, tri, mat);
     }
 };
+
+// Quad light of side `size` centred at c, emitting towards `target`.
+void quad_light(Builder &b, f3 c, f3 target, float size, f3 radiance) {
+    nori_material m = default_material();
+    m.isEmitter = 1;
+    m.albedo[0] = m.albedo[1] = m.albedo[2] = 0.0f;
+    m.radiance[0] = radiance.x; m.radiance[1] = radiance.y; m.radiance[2] = radiance.z;
+    f3 n = normalize(target - c);
+    f3 up = std::fabs(n.y) < 0.9f ? mk(0, 1, 0) : mk(1, 0, 0);
+    f3 a = normalize(cross(up, n)), bb = cross(n, a); // a x bb = n
+    b.quad(c - (a + bb) * (0.5f * size), a * size, bb * size, m);
+}
+
+// Chiang et al. 2016 eq. 9 inverted: the colour whose sigma_a the colour-to-sigma_a
+// fit returns is exp(-g(beta_n) sqrt(sigma_a)).
+float hair_color_from_sigma_a(float sa, float bn) {
+    float g = 5.969f - 0.215f * bn + 2.532f * bn * bn - 10.73f * bn * bn * bn + 5.574f * bn * bn * bn * bn +
+              0.245f * bn * bn * bn * bn * bn;
+    return std::exp(-g * std::sqrt(sa));
+}
+
+void hair_scene(Builder &b, Rng &rng, const nori_scene_options &opt, f3 eye) {
+    const uint32_t ns = opt.strands ? opt.strands : 50000, nseg = opt.subdiv ? opt.subdiv : 16;
+    const bool furnace = opt.variant == 1;
+    const float R = 10.0f, len = 14.0f, half_w = 0.04f, seg = len / (float)nseg;
+    b.attrs = true;
+    std::vector<f3> P(nseg + 1), T(nseg + 1);
+    std::vector<float> pos, nrm, uv, tan;
+    std::vector<uint32_t> tri;
+    for (uint32_t i = 0; i < nseg; ++i) {
+        uint32_t a0 = 2 * i, b0 = a0 + 1, a1 = a0 + 2, b1 = a0 + 3;
+        tri.insert(tri.end(), {a0, b0, a1, b0, b1, a1});
+    }
+    for (uint32_t k = 0; k < ns; ++k) {
+        // root uniform on the upper hemisphere (y uniform gives uniform area)
+        float y = rng.next(), phi = 2.0f * PI * rng.next();
+        float r = std::sqrt(std::max(0.0f, 1.0f - y * y));
+        f3 n = mk(r * std::cos(phi), y, r * std::sin(phi));
+        P[0] = n * (R * 1.001f);
+        f3 d = n;
+        for (uint32_t i = 1; i <= nseg; ++i) {
+            float g0, g1, g2, g3;
+            rng.gauss2(g0, g1);
+            rng.gauss2(g2, g3);
+            d = normalize(d + mk(0.0f, -0.12f, 0.0f) + mk(g0, g1, g2) * 0.1f);
+            f3 p = P[i - 1] + d * seg;
+            float l = length(p);
+            if (l < R * 1.02f) p = p * (R * 1.02f / l); // stay outside the head
+            P[i] = p;
+        }
+        for (uint32_t i = 0; i <= nseg; ++i) {
+            f3 t = P[std::min(i + 1, nseg)] - P[i > 0 ? i - 1 : 0];
+            T[i] = normalize(t);
+        }
+        pos.clear(); nrm.clear(); uv.clear(); tan.clear();
+        for (uint32_t i = 0; i <= nseg; ++i) {
+            f3 v = eye - P[i];
+            f3 side = cross(T[i], v);
+            float sl = length(side);
+            side = sl > 1e-12f ? side * (1.0f / sl) : normalize(cross(T[i], mk(0.3f, 0.5f, 0.81f)));
+            f3 nn = normalize(cross(side, T[i]));
+            if (dot(nn, v) < 0.0f) nn = nn * -1.0f;
+            f3 pa = P[i] - side * half_w, pb = P[i] + side * half_w;
+            float u = (float)i / (float)nseg;
+            pos.insert(pos.end(), {pa.x, pa.y, pa.z, pb.x, pb.y, pb.z});
+            nrm.insert(nrm.end(), {nn.x, nn.y, nn.z, nn.x, nn.y, nn.z});
+            uv.insert(uv.end(), {u, 0.0f, u, 1.0f}); // h = 2 v - 1 across the ribbon
+            tan.insert(tan.end(), {T[i].x, T[i].y, T[i].z, T[i].x, T[i].y, T[i].z});
+        }
+        nori_material m = default_material();
+        m.type = NORI_MAT_HAIR;
+        m.roughness = 0.3f; // beta_m
+        m.betaN = 0.3f;
+        m.alpha = 2.0f;
+        m.intIOR = 1.55f;
+        if (furnace) {
+            m.albedo[0] = m.albedo[1] = m.albedo[2] = 1.0f;
+        } else {
+            // melanin concentrations -> sigma_a (PBRT 3e §9.9.7) -> colour
+            float ce = 0.3f + 7.7f * rng.next(), cp = 0.5f * rng.next();
+            const float eu[3] = {0.419f, 0.697f, 1.37f}, ph[3] = {0.187f, 0.4f, 1.05f};
+            for (int c = 0; c < 3; ++c) m.albedo[c] = hair_color_from_sigma_a(ce * eu[c] + cp * ph[c], m.betaN);
+        }
+        b.add_mesh(pos, nrm, tri, m, &uv, &tan);
+    }
+    if (!furnace) {
+        nori_material skin = default_material();
+        skin.albedo[0] = 0.55f; skin.albedo[1] = 0.42f; skin.albedo[2] = 0.36f;
+        b.icosphere(mk(0, 0, 0), R, 5, skin);
+        quad_light(b, mk(18, 22, 26), mk(0, 0, 0), 8.0f, mk(6.0f, 5.6f, 5.0f));   // key
+        quad_light(b, mk(-28, 4, 16), mk(0, 0, 0), 10.0f, mk(1.5f, 1.7f, 2.0f));  // fill
+        quad_light(b, mk(0, 14, -30), mk(0, 0, 0), 8.0f, mk(9.0f, 9.0f, 9.0f));   // rim
+    }
+}
 
 void lookat(f3 eye, f3 target, f3 up, float *M) {
     f3 dir = normalize(target - eye);
@@ -226,6 +344,18 @@ SceneData generate_scene(const nori_scene_options &opt) {
         }
         s.camera.fov_deg = 50.0f;
         lookat(mk(0, 4.5f, 9.5f), mk(0, 0.3f, 0), mk(0, 1, 0), s.camera.to_world);
+        s.camera.width = opt.width ? opt.width : 512;
+        s.camera.height = opt.height ? opt.height : 512;
+    } else if (cfg == 3) {
+        const f3 eye = mk(0, 0, 45), target = mk(0, -1, 0);
+        hair_scene(b, rng, opt, eye);
+        if (opt.variant == 1) {
+            uint32_t W = opt.env_width ? opt.env_width : 64, H = opt.env_height ? opt.env_height : 32;
+            s.env_w = W; s.env_h = H; s.env_scale = 1.0f;
+            s.env_rgb.assign((size_t)W * H * 3, 1.0f);
+        }
+        s.camera.fov_deg = 45.0f;
+        lookat(eye, target, mk(0, 1, 0), s.camera.to_world);
         s.camera.width = opt.width ? opt.width : 512;
         s.camera.height = opt.height ? opt.height : 512;
     } else {

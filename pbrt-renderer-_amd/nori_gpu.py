@@ -86,7 +86,7 @@ class BvhInfo(ctypes.Structure):
 
 class SceneOptions(ctypes.Structure):
     _fields_ = [("config", ctypes.c_int), ("seed", _u64), ("subdiv", _u32), ("grid", _u32), ("env_width", _u32),
-                ("env_height", _u32), ("width", _u32), ("height", _u32)]
+                ("env_height", _u32), ("width", _u32), ("height", _u32), ("strands", _u32), ("variant", _u32)]
 
 
 EXPORTS = [
@@ -117,7 +117,7 @@ def load_library(path=LIB_PATH):
         "nori_gpu_reset_film": (i32, [vp]),
         "nori_gpu_trace": (i32, [vp, vp, i32, vp, i32]),
         "nori_gpu_trace_device": (i32, [vp, vp, i32, vp, i32]),
-        "nori_gpu_bsdf_batch": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp]),
+        "nori_gpu_bsdf_batch": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_envmap_sample": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
         "nori_gpu_pcg32": (i32, [vp, _u64, _u64, _u64, i32, vp]),
@@ -145,7 +145,7 @@ class HostScene:
     """Flat scene arrays (numpy, owned by Python) in the nori_scene_desc layout."""
 
     def __init__(self, positions, indices, mesh_tri_offset, mesh_tri_count, materials, normals=None,
-                 env_rgb=None, env_scale=1.0, camera=None):
+                 env_rgb=None, env_scale=1.0, camera=None, uvs=None, tangents=None):
         self.positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
@@ -155,6 +155,8 @@ class HostScene:
         self.env_rgb = None if env_rgb is None else np.ascontiguousarray(env_rgb, np.float32)
         self.env_scale = float(env_scale)
         self.camera = camera
+        self.uvs = None if uvs is None else np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+        self.tangents = None if tangents is None else np.ascontiguousarray(tangents, np.float32).reshape(-1, 3)
 
     @property
     def tri_mesh(self):
@@ -167,6 +169,7 @@ class HostScene:
         d = SceneDesc()
         d.num_vertices, d.num_triangles, d.num_meshes = len(self.positions), len(self.indices), len(self.materials)
         d.positions, d.normals = _ptr(self.positions), _ptr(self.normals)
+        d.uvs, d.tangents = _ptr(self.uvs), _ptr(self.tangents)
         d.indices = _ptr(self.indices)
         d.mesh_tri_offset, d.mesh_tri_count = _ptr(self.mesh_tri_offset), _ptr(self.mesh_tri_count)
         d.materials = _ptr(self.materials)
@@ -178,10 +181,11 @@ class HostScene:
         return d
 
 
-def generate_scene(config, seed=0, subdiv=0, grid=0, env_width=0, env_height=0, width=0, height=0):
+def generate_scene(config, seed=0, subdiv=0, grid=0, env_width=0, env_height=0, width=0, height=0, strands=0,
+                   variant=0):
     """Procedural benchmark scene built by the library (nori_scene_generate), copied into numpy."""
     lib = load_library()
-    opt = SceneOptions(config, seed, subdiv, grid, env_width, env_height, width, height)
+    opt = SceneOptions(config, seed, subdiv, grid, env_width, env_height, width, height, strands, variant)
     h = ctypes.c_void_p()
     rc = lib.nori_scene_generate(ctypes.byref(opt), ctypes.byref(h))
     if rc != NORI_OK:
@@ -200,6 +204,8 @@ def generate_scene(config, seed=0, subdiv=0, grid=0, env_width=0, env_height=0, 
 
         pos = arr(d.positions, 3 * d.num_vertices, np.float32)
         nrm = arr(d.normals, 3 * d.num_vertices, np.float32)
+        uvs = arr(d.uvs, 2 * d.num_vertices, np.float32)
+        tans = arr(d.tangents, 3 * d.num_vertices, np.float32)
         idx = arr(d.indices, 3 * d.num_triangles, np.uint32)
         mto = arr(d.mesh_tri_offset, d.num_meshes, np.uint32)
         mtc = arr(d.mesh_tri_count, d.num_meshes, np.uint32)
@@ -207,7 +213,8 @@ def generate_scene(config, seed=0, subdiv=0, grid=0, env_width=0, env_height=0, 
         env = None
         if d.env_width:
             env = arr(d.env_rgb, 3 * d.env_width * d.env_height, np.float32).reshape(d.env_height, d.env_width, 3)
-        return HostScene(pos, idx, mto, mtc, mats, normals=nrm, env_rgb=env, env_scale=d.env_scale, camera=cam)
+        return HostScene(pos, idx, mto, mtc, mats, normals=nrm, env_rgb=env, env_scale=d.env_scale, camera=cam,
+                         uvs=uvs, tangents=tans)
     finally:
         lib.nori_scene_free(h)
 
@@ -325,16 +332,23 @@ class NoriGPU:
         prim = hits[:, 1].copy().view(np.uint32)
         return t, prim, hits[:, 2].copy(), hits[:, 3].copy()
 
-    def bsdf_batch(self, op, mats, wi, wo=None, u=None):
+    def bsdf_batch(self, op, mats, wi, wo=None, u=None, h=None):
+        """op 0/1: (rgb, pdf); op 2: (f*|cos|/pdf, pdf, wo).  u: n x 3 or n x 4
+        uniforms; h: hair offsets across the fiber."""
         n = len(mats)
         mats = np.ascontiguousarray(mats, MATERIAL_DTYPE)
         wi = np.ascontiguousarray(wi, np.float32)
         wo = None if wo is None else np.ascontiguousarray(wo, np.float32)
-        u = None if u is None else np.ascontiguousarray(u, np.float32)
+        if u is not None:
+            u = np.asarray(u, np.float32)
+            if u.shape[1] == 3:
+                u = np.c_[u, np.zeros(n, np.float32)]
+            u = np.ascontiguousarray(u, np.float32)
+        h = None if h is None else np.ascontiguousarray(h, np.float32)
         rgb = np.zeros((n, 3), np.float32)
         pdf = np.zeros(n, np.float32)
         woo = np.zeros((n, 3), np.float32)
-        self._check(self.lib.nori_gpu_bsdf_batch(self.h, op, n, _ptr(mats), _ptr(wi), _ptr(wo), _ptr(u),
+        self._check(self.lib.nori_gpu_bsdf_batch(self.h, op, n, _ptr(mats), _ptr(wi), _ptr(wo), _ptr(u), _ptr(h),
                                                  _ptr(rgb), _ptr(pdf), _ptr(woo)), "bsdf_batch")
         return rgb, pdf, woo
 

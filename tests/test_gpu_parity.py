@@ -102,12 +102,21 @@ def test_envmap_alias_draws_bit_exact(gpu, disney_small):
     assert np.allclose(gL, oL, rtol=2e-5, atol=1e-6)
     # pdf query of the sampled direction equals the sampling pdf
     eL, epdf = gpu.envmap_eval(gd)
-    # away from texel borders and the two pole rows, where acos(y) of the
-    # reference's pdf query is ill-conditioned in FP32
+    # away from texel borders and the two pole rows.  The reference's pdf query
+    # takes acos(y), whose FP32 error grows like eps/sin^2(theta) near the poles:
+    # the tolerance follows that conditioning.
     H = disney_small.env_rgb.shape[0]
     row = gtex // disney_small.env_rgb.shape[1]
     inner = (jx > 0.01) & (jx < 0.99) & (jy > 0.01) & (jy < 0.99) & (row > 0) & (row < H - 1)
-    assert np.allclose(epdf[inner], gpdf[inner], rtol=1e-4)
+    st = np.sqrt(np.maximum(0.0, 1.0 - gd[:, 1].astype(np.float64) ** 2))
+    tol = 2e-5 + 16 * 2.0 ** -24 / np.maximum(st, 1e-3) ** 2
+    rel = np.abs(epdf.astype(np.float64) / gpdf - 1.0)
+    bad = inner & (rel > tol)
+    if bad.any():
+        k = np.flatnonzero(bad)[:8]
+        _say(f"  {bad.sum()} of {inner.sum()}: rows {row[k]} jx {jx[k]} jy {jy[k]} rel {rel[k]} "
+             f"pdf {gpdf[k]} query {epdf[k]}")
+    assert not bad.any()
 
 
 def _bsdf_tuples(n, seed, kind):
@@ -125,15 +134,26 @@ def _bsdf_tuples(n, seed, kind):
         m["type"] = ng.MAT_MICROFACET
         m["alpha"] = 0.05 + 0.5 * rng.random(n)
         m["albedo"] *= 0.5
+    elif kind == "hair":
+        m["type"] = ng.MAT_HAIR
+        m["roughness"] = rng.uniform(0.15, 0.9, n)
+        m["betaN"] = rng.uniform(0.15, 0.9, n)
+        m["alpha"] = rng.uniform(-4.0, 4.0, n)
+        m["intIOR"] = rng.uniform(1.4, 1.7, n)
+        m["albedo"] = rng.uniform(0.05, 1.0, (n, 3))
     else:
         m["type"] = ng.MAT_DIFFUSE
 
     def hemi():
         v = rng.normal(size=(n, 3))
-        v[:, 2] = np.abs(v[:, 2])
+        if kind != "hair":  # hair scatters over the whole sphere
+            v[:, 2] = np.abs(v[:, 2])
         return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
 
-    return m, hemi(), hemi(), rng.random((n, 3)).astype(np.float32)
+    wi, wo = hemi(), hemi()
+    u = rng.random((n, 4)).astype(np.float32)
+    h = rng.uniform(-0.98, 0.98, n).astype(np.float32) if kind == "hair" else None
+    return m, wi, wo, u, h
 
 
 def _rel(a, b, floor):
@@ -147,7 +167,7 @@ def test_bsdf_eval_pdf_vs_fp64(gpu, kind):
     # 1e-5 relative plus 64 float ulps of the largest lobe value (the
     # conditioning floor of the half-vector normalisation at grazing angles).
     n = 1 << 20
-    m, wi, wo, _ = _bsdf_tuples(n, 21, kind)
+    m, wi, wo, _, _ = _bsdf_tuples(n, 21, kind)
     _say(f"case: bsdf eval/pdf {kind} n={n}")
     g_rgb, g_pdf, _ = gpu.bsdf_batch(0, m, wi, wo)  # op 0 writes f and pdf
     o_rgb, _, _ = orc.bsdf_batch(0, m, wi, wo)
@@ -162,20 +182,66 @@ def test_bsdf_eval_pdf_vs_fp64(gpu, kind):
         assert np.all(np.abs(g - o) <= 1e-5 * np.abs(o) + 64 * 1.2e-7 * scale + 1e-30), (kind, what)
 
 
-@pytest.mark.parametrize("kind", ["disney", "microfacet", "diffuse"])
+def test_hair_eval_pdf_vs_fp64(gpu):
+    # The hair lobes are exponentials of differences of O(1/v) terms
+    # (log I0(a) - b - 1/v, shaders/Hair.hlsli:55-66): FP32 loses about
+    # |a| ulps of relative accuracy there, so the bound is 1e-4 relative for
+    # 99% of the tuples and 2e-3 relative plus 1e-6 of the largest channel for all.
+    n = 1 << 20
+    m, wi, wo, _, h = _bsdf_tuples(n, 23, "hair")
+    _say(f"case: bsdf eval/pdf hair n={n}")
+    g_rgb, g_pdf, _ = gpu.bsdf_batch(0, m, wi, wo, h=h)
+    o_rgb, _, _ = orc.bsdf_batch(0, m, wi, wo, h=h)
+    _, o_pdf, _ = orc.bsdf_batch(1, m, wi, wo, h=h)
+    for g, o, what in ((g_rgb, o_rgb, "f"), (g_pdf, o_pdf, "pdf")):
+        scale = np.abs(o).max(axis=-1, keepdims=True) if o.ndim > 1 else np.abs(o)
+        rel = _rel(g, o, 1e-30)
+        big = np.abs(o) > 1e-6 * scale
+        q = np.quantile(rel[big], [0.5, 0.99, 0.999])
+        _say(f"  hair {what}: rel quantiles 50/99/99.9% {q} max {rel[big].max():.3g}")
+        assert q[1] <= 1e-4, (what, q)
+        assert np.all(np.abs(g - o) <= 2e-3 * np.abs(o) + 1e-6 * scale + 1e-30), what
+
+
+@pytest.mark.parametrize("kind", ["disney", "microfacet", "diffuse", "hair"])
 def test_bsdf_sample_vs_fp64(gpu, kind):
     n = 1 << 19
-    m, wi, _, u = _bsdf_tuples(n, 22, kind)
+    m, wi, _, u, h = _bsdf_tuples(n, 22, kind)
     _say(f"case: bsdf sample {kind} n={n}")
-    g_w, g_pdf, g_wo = gpu.bsdf_batch(2, m, wi, u=u)
-    o_w, o_pdf, o_wo = orc.bsdf_batch(2, m, wi, u=u)
+    g_w, g_pdf, g_wo = gpu.bsdf_batch(2, m, wi, u=u, h=h)
+    o_w, o_pdf, o_wo = orc.bsdf_batch(2, m, wi, u=u, h=h)
     both = (g_pdf > 0) & (o_pdf > 0)
     assert np.mean((g_pdf > 0) == (o_pdf > 0)) > 0.9999
-    assert np.allclose(g_wo[both], o_wo[both], atol=2e-5)
+    # the sampled direction: a lobe choice (u[0] against the lobe CDF) may flip
+    # by rounding for a handful of tuples
+    dw = np.abs(g_wo[both] - o_wo[both]).max(1)
+    _say(f"  {kind}: direction diff quantiles 99/99.99% {np.quantile(dw, [0.99, 0.9999])}")
+    q, tol = (0.999, 2e-4) if kind == "hair" else (0.9999, 2e-5)
+    assert np.quantile(dw, q) < tol, (kind, np.quantile(dw, q))
     rel = _rel(g_pdf[both], o_pdf[both], 1e-30)
     assert np.quantile(rel, 0.99) < 1e-4
     relw = _rel(g_w[both], o_w[both], 1e-6)
     assert np.quantile(relw, 0.99) < 1e-4
+
+
+def test_hair_white_furnace_gpu(gpu):
+    # GPU quadrature of f |cos| for sigma_a = 0 against the same quadrature of the
+    # FP64 oracle (which the CPU suite pins to within the truncated-I0 loss of 1)
+    from test_hair_oracle import hair_dir, hair_mats, sphere_quadrature
+    _say("case: hair furnace quadrature")
+    dirs, w = sphere_quadrature(256, 512)
+    n = len(dirs)
+    for beta, to, hh in ((0.3, 0.0, 0.0), (0.5, 0.4, -0.5), (0.8, -0.9, 0.7)):
+        mats = np.repeat(hair_mats(1, beta_m=beta, beta_n=beta), n)
+        wi = np.tile(hair_dir(to, 0.3).astype(np.float32), (n, 1))
+        hv = np.full(n, hh, np.float32)
+        g, gp, _ = gpu.bsdf_batch(0, mats, wi, dirs.astype(np.float32), h=hv)
+        o, _, _ = orc.bsdf_batch(0, mats, wi, dirs.astype(np.float32), h=hv)
+        cz = np.abs(dirs[:, 2:3].astype(np.float32))
+        ge, oe = ((g * cz) * w[:, None]).sum(0), ((o * cz) * w[:, None]).sum(0)
+        _say(f"  beta {beta}: gpu {ge} oracle {oe} pdf mass {(gp * w).sum():.6f}")
+        assert np.allclose(ge, oe, rtol=1e-5)
+        assert 0.99 < ge[0] < 1.0 + 1e-4
 
 
 def _render_pair(gpu, scene, w, h, spp, depth, seed=7):
@@ -263,3 +329,32 @@ def test_guided_render_is_unbiased(gpu, disney_small):
     _say(f"  relMSE plain {e_plain:.3e} guided {e_guided:.3e}; mean {plain.mean():.4f} {gi.mean():.4f}")
     assert abs(gi.mean() - ref.mean()) < 0.02 * ref.mean()
     assert e_guided < 4 * e_plain + 1e-4
+
+
+def test_render_hair_furnace(gpu):
+    # Config-3 white furnace on both sides with the same samples: the image means
+    # over hair pixels agree within 1e-3, and with 1 within the truncated-I0 loss.
+    from test_hair_oracle import _first_hits
+    sc = ng.generate_scene(3, strands=2000, variant=1, width=64, height=64)
+    _say("case: render hair furnace 64x64 128spp depth 32")
+    g, o = _render_pair(gpu, sc, 64, 64, 128, 32, seed=3)
+    assert np.all(g[:, 3] == 128)
+    gi, oi = g[:, :3] / 128, o[:, :3] / 128
+    hit = _first_hits(orc.OracleScene(sc), sc.camera)
+    gm, om = gi[hit].mean(), oi[hit].mean()
+    _say(f"  hair pixels {hit.sum()}: gpu mean {gm:.6f} oracle mean {om:.6f} relMSE {orc.rel_mse(gi, oi):.3e}")
+    assert abs(gm - om) < 1e-3
+    assert abs(gm - 1.0) < 3e-3
+    assert orc.rel_mse(gi, oi) < 1e-4
+
+
+def test_render_hair_lit_same_samples(gpu):
+    sc = ng.generate_scene(3, strands=2000, variant=0, width=64, height=64)
+    _say("case: render hair lit 64x64 16spp depth 32")
+    g, o = _render_pair(gpu, sc, 64, 64, 16, 32, seed=5)
+    gi, oi = g[:, :3] / 16, o[:, :3] / 16
+    e = orc.rel_mse(gi, oi)
+    close = np.all(np.abs(gi - oi) <= 1e-3 * np.abs(oi) + 1e-5, axis=1)
+    _say(f"  relMSE {e:.3e} close {close.mean():.4f} mean {gi.mean(0)} {oi.mean(0)}")
+    assert e < 1e-3
+    assert close.mean() > 0.9

@@ -1,0 +1,315 @@
+"""CPU checks that pin the oracle's hair BCSDF (Chiang et al. 2016 as
+shaders/Hair.hlsli evaluates it) before it checks the GPU path.
+
+Pins: library routines (scipy's I0 and logistic distribution), closed forms of
+the published method (the 10-term I0 series is I0 minus its known tail, the
+asymptotic log I0), normalisation integrals computed by quadrature, the white
+furnace (sigma_a = 0 loses only what the truncated I0 loses), the lobe
+attenuations in closed form, the sampler's density (chi-square), and the
+symmetries and lobe shifts the model fixes.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special, stats
+
+import nori_gpu as ng
+import orc
+
+PI = math.pi
+
+
+def hair_mats(n, beta_m=0.3, beta_n=0.3, alpha=2.0, eta=1.55, color=(1.0, 1.0, 1.0)):
+    m = ng.default_materials(n)
+    m["type"] = ng.MAT_HAIR
+    m["roughness"], m["betaN"], m["alpha"], m["intIOR"] = beta_m, beta_n, alpha, eta
+    m["albedo"] = color
+    return m
+
+
+def hair_dir(theta, phi):
+    """Hair frame: x = fiber tangent, sin(theta) = w.x, phi = atan2(w.z, w.y)."""
+    theta, phi = np.broadcast_arrays(np.asarray(theta, np.float64), np.asarray(phi, np.float64))
+    return np.stack([np.sin(theta), np.cos(theta) * np.cos(phi), np.cos(theta) * np.sin(phi)], -1)
+
+
+def sphere_quadrature(nt, nphi):
+    """Gauss-Legendre nodes over theta in [-pi/2, pi/2] x phi in [-pi, pi]; weights include cos(theta)."""
+    xt, wt = np.polynomial.legendre.leggauss(nt)
+    xp, wp = np.polynomial.legendre.leggauss(nphi)
+    th, ph = 0.5 * PI * xt, PI * xp
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    W = np.outer(wt * 0.5 * PI * np.cos(th), wp * PI)
+    return hair_dir(T, P).reshape(-1, 3), W.reshape(-1)
+
+
+# ------------------------------------------------------------------ pieces
+def test_i0_series_is_i0_minus_its_tail():
+    # shaders/Hair.hlsli:26-40 sums the first 10 terms of I0(x) = sum (x/2)^2k / (k!)^2
+    x = np.linspace(0.0, 12.0, 97)
+    got = orc.hair_fn(0, x)
+    tail = np.array([sum(math.exp(2 * k * math.log(v / 2) - 2 * math.lgamma(k + 1)) for k in range(10, 80))
+                     if v > 0 else 0.0 for v in x])
+    assert np.allclose(got, special.i0(x) - tail, rtol=1e-12, atol=0)
+    small = x <= 2.0
+    assert np.allclose(got[small], special.i0(x[small]), rtol=1e-10)
+
+
+def test_log_i0():
+    x = np.linspace(0.1, 12.0, 50)
+    assert np.allclose(orc.hair_fn(1, x), np.log(orc.hair_fn(0, x)), rtol=1e-14)
+    # Asymptotic branch above 12 (shaders/Hair.hlsli:43-48, as PBRT writes it):
+    # x + (-log(2 pi) + log(1/x) + 1/(8x)) / 2.  The true expansion is
+    # log I0(x) = x - log(2 pi x)/2 + 1/(8x) + 1/(16x^2) + O(x^-3), so the branch
+    # falls short by 1/(16x) + 1/(16x^2) + O(x^-3).
+    x = np.linspace(12.01, 400.0, 200)
+    short = (np.log(special.i0e(x)) + x) - orc.hair_fn(1, x)
+    assert np.abs(short - 1 / (16 * x) - 1 / (16 * x * x)).max() < 1 / 12.0 ** 3
+
+
+def mp_textbook(ti, to, v):
+    """Longitudinal lobe of d'Eon et al. 2011:
+    csch(1/v) / (2v) * exp(-sin ti sin to / v) * I0(cos ti cos to / v)."""
+    a, b = np.cos(ti) * np.cos(to) / v, np.sin(ti) * np.sin(to) / v
+    log_sinh = 1 / v + np.log1p(-np.exp(-2 / v)) - math.log(2)
+    return np.exp(-b + a - np.log(2 * v) - log_sinh) * special.i0e(a)
+
+
+def mp_mass(to, v):
+    """Integral over theta_i of the oracle's Mp cos(theta_i)."""
+    so, co = math.sin(to), math.cos(to)
+    f = lambda t: orc.hair_fn(2, math.cos(t), co, math.sin(t), so, v)[0] * math.cos(t)
+    return integrate.quad(f, -PI / 2, PI / 2, limit=400, epsabs=1e-13)[0]
+
+
+@pytest.mark.parametrize("v", [0.03, 0.08, 0.2, 0.8])
+def test_mp_is_textbook_lobe_with_truncated_i0(v):
+    # The textbook lobe has unit mass.  The oracle's Mp (shaders/Hair.hlsli:55-66)
+    # is that lobe with I0 replaced by the 10-term series (v > 0.1), or, for
+    # v <= 0.1, by the log I0 of test_log_i0 and log 2 rounded to 0.6931.
+    to = 0.4
+    val = integrate.quad(lambda t: mp_textbook(t, to, v) * math.cos(t), -PI / 2, PI / 2, limit=400,
+                         epsabs=1e-13)[0]
+    assert abs(val - 1) < 1e-8
+    rng = np.random.default_rng(11)
+    ti, to = rng.uniform(-1.5, 1.5, 400), rng.uniform(-1.5, 1.5, 400)
+    got = orc.hair_fn(2, np.cos(ti), np.cos(to), np.sin(ti), np.sin(to), v)
+    a = np.cos(ti) * np.cos(to) / v
+    if v <= 0.1:
+        corr = orc.hair_fn(1, a) - (np.log(special.i0e(a)) + a) + (0.6931 - math.log(2))
+    else:
+        corr = np.log(orc.hair_fn(0, a)) - (np.log(special.i0e(a)) + a)
+    ok = got > 1e-300
+    assert np.allclose(np.log(got[ok]) - np.log(mp_textbook(ti, to, v)[ok]), corr[ok], atol=1e-9)
+
+
+@pytest.mark.parametrize("v", [0.03, 0.08, 0.2, 0.8])
+@pytest.mark.parametrize("theta_o", [0.0, 0.5, -1.0])
+def test_mp_mass(v, theta_o):
+    # what the truncations lose: under 1% of the lobe, never a gain
+    m = mp_mass(theta_o, v)
+    assert 0.99 < m < 1.0 + 1e-6, m
+
+
+@pytest.mark.parametrize("s", [0.05, 0.2, 0.6, 1.5])
+def test_trimmed_logistic(s):
+    x = np.linspace(-PI, PI, 41)
+    F = stats.logistic.cdf(PI, scale=s) - stats.logistic.cdf(-PI, scale=s)
+    assert np.allclose(orc.hair_fn(3, x, s), stats.logistic.pdf(x, scale=s) / F, rtol=1e-12)
+    assert np.allclose(orc.hair_fn(5, x, s), stats.logistic.cdf(x, scale=s), rtol=1e-12)
+    # the sampler inverts the trimmed CDF
+    u = np.linspace(0.001, 0.999, 37)
+    xs = orc.hair_fn(4, u, s)
+    cdf = (stats.logistic.cdf(xs, scale=s) - stats.logistic.cdf(-PI, scale=s)) / F
+    assert np.allclose(cdf, u, atol=1e-12)
+    assert xs.min() >= -PI and xs.max() <= PI
+
+
+# ------------------------------------------------------------------ BCSDF
+def _furnace(mat, wi, h, nt=256, nphi=512):
+    dirs, w = sphere_quadrature(nt, nphi)
+    n = len(dirs)
+    mats = np.repeat(mat, n)
+    wi_ = np.tile(np.asarray(wi, np.float32), (n, 1))
+    f, _, _ = orc.bsdf_batch(0, mats, wi_, dirs, h=np.full(n, h, np.float32))
+    _, pdf, _ = orc.bsdf_batch(1, mats, wi_, dirs, h=np.full(n, h, np.float32))
+    fc = f * np.abs(dirs[:, 2:3].astype(np.float32))
+    return (fc * w[:, None]).sum(0), (pdf * w).sum()
+
+
+@pytest.mark.parametrize("beta", [0.3, 0.5, 0.8])
+@pytest.mark.parametrize("theta_o,h", [(0.0, 0.0), (0.4, -0.5), (-0.9, 0.7), (1.2, 0.3)])
+def test_white_furnace(beta, theta_o, h):
+    # colour 1 maps to sigma_a = 0 (shaders/Hair.hlsli:129-131): the four lobes'
+    # attenuations then sum to one and every azimuthal lobe has unit mass, so
+    # the integral of f |cos| over the sphere is a convex combination of the
+    # longitudinal masses: one, less the truncation losses of test_mp_mass.
+    # The same holds for the integral of the pdf.
+    m = hair_mats(1, beta_m=beta, beta_n=beta)
+    wi = hair_dir(theta_o, 0.3)
+    e, p = _furnace(m, wi, h)
+    assert np.allclose(e, e[0], rtol=1e-12)
+    assert 0.99 < e[0] < 1.0 + 1e-4, e
+    assert 0.99 < p < 1.0 + 1e-4, p
+
+
+def test_pdf_mass_with_absorption():
+    m = hair_mats(1, color=(0.6, 0.35, 0.2))
+    e, p = _furnace(m, hair_dir(0.3, -1.0), 0.4)
+    assert 0.99 < p < 1.0 + 1e-4
+    assert (e < p).all() and (e > 0.0).all() and e[0] > e[1] > e[2]
+
+
+def test_sample_weight_is_one_without_absorption():
+    # sigma_a = 0: attenuations are grey and sum to one, so the lobe pdf is
+    # proportional to f |cos| and every sample weighs exactly 1
+    rng = np.random.default_rng(3)
+    n = 4096
+    m = hair_mats(n)
+    m["roughness"] = rng.uniform(0.2, 0.9, n)
+    m["betaN"] = rng.uniform(0.2, 0.9, n)
+    wi = hair_dir(rng.uniform(-1.4, 1.4, n), rng.uniform(-PI, PI, n)).astype(np.float32)
+    u = rng.random((n, 4)).astype(np.float32)
+    h = rng.uniform(-0.95, 0.95, n).astype(np.float32)
+    w, pdf, wo = orc.bsdf_batch(2, m, wi, u=u, h=h)
+    ok = (pdf > 0) & (np.abs(wo[:, 2]) > 1e-5)
+    assert ok.mean() > 0.99
+    assert np.allclose(w[ok], 1.0, atol=1e-9)
+
+
+def test_azimuthal_symmetry_at_h0():
+    # h = 0: gamma_o = gamma_t = 0, every Np is symmetric about its lobe centre
+    rng = np.random.default_rng(4)
+    n = 512
+    m = hair_mats(n, color=(0.7, 0.5, 0.3))
+    to, po = rng.uniform(-1.3, 1.3, n), rng.uniform(-PI, PI, n)
+    ti, dp = rng.uniform(-1.3, 1.3, n), rng.uniform(0.05, 3.0, n)
+    wi = hair_dir(to, po)
+    wa, wb = hair_dir(ti, po + dp), hair_dir(ti, po - dp)
+    a, pa, _ = orc.bsdf_batch(0, m, wi, wa, h=np.zeros(n, np.float32))
+    b, pb, _ = orc.bsdf_batch(0, m, wi, wb, h=np.zeros(n, np.float32))
+    _, pa, _ = orc.bsdf_batch(1, m, wi, wa, h=np.zeros(n, np.float32))
+    _, pb, _ = orc.bsdf_batch(1, m, wi, wb, h=np.zeros(n, np.float32))
+    # eval divides by |wo.z| (shaders/Hair.hlsli:295-297), which is not symmetric
+    ca = np.abs(wa[:, 2:3].astype(np.float32))
+    cb = np.abs(wb[:, 2:3].astype(np.float32))
+    assert np.allclose(a * ca, b * cb, rtol=2e-5, atol=1e-9)
+    assert np.allclose(pa, pb, rtol=2e-5, atol=1e-9)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 5.0, -3.0])
+def test_scale_tilt_shifts_lobes(alpha):
+    # Black fibre (sigma_a large): only R survives, and R is tilted by -2 alpha,
+    # so the longitudinal peak of the pdf sits near theta_i = 2 alpha - theta_o
+    # (d'Eon et al. 2011; PBRT 4e eq. 9.51).  The pdf carries no 1/|cos|.
+    m = hair_mats(1, beta_m=0.15, alpha=alpha, color=(1e-5, 1e-5, 1e-5))
+    to = 0.3
+    th = np.arange(-1.2, 1.2, 0.001)
+    n = len(th)
+    wi = np.tile(hair_dir(to, 0.0), (n, 1)).astype(np.float32)
+    wo = hair_dir(th, np.zeros(n)).astype(np.float32)
+    _, pdf, _ = orc.bsdf_batch(1, np.repeat(m, n), wi, wo, h=np.zeros(n, np.float32))
+    peak = th[np.argmax(pdf)]
+    assert abs(peak - (2 * math.radians(alpha) - to)) < 0.02, peak
+
+
+def test_sampler_density_matches_pdf():
+    # chi-square of sampled directions against the pdf integrated per bin
+    rng = np.random.default_rng(5)
+    N = 200_000
+    mat = hair_mats(1, color=(0.6, 0.4, 0.3))
+    wi = hair_dir(0.35, 0.8).astype(np.float32)
+    h = 0.25
+    u = rng.random((N, 4)).astype(np.float32)
+    _, pdf, wo = orc.bsdf_batch(2, np.repeat(mat, N), np.tile(wi, (N, 1)), u=u, h=np.full(N, h, np.float32))
+    assert (pdf > 0).mean() > 0.999
+    nt, nph = 16, 32
+    st = np.clip(wo[:, 0], -1, 1)
+    ph = np.arctan2(wo[:, 2], wo[:, 1])
+    it = np.minimum(((st + 1) * 0.5 * nt).astype(int), nt - 1)
+    ip = np.minimum(((ph + PI) / (2 * PI) * nph).astype(int), nph - 1)
+    obs = np.bincount(it * nph + ip, minlength=nt * nph)
+    # expected mass: integrate pdf over each bin in (sin theta, phi), d omega = d(sin theta) d phi
+    g = 6
+    xg, wg = np.polynomial.legendre.leggauss(g)
+    s_edges = np.linspace(-1, 1, nt + 1)
+    p_edges = np.linspace(-PI, PI, nph + 1)
+    S = (s_edges[:-1, None] + (xg[None, :] + 1) * 0.5 * (2 / nt))
+    P = (p_edges[:-1, None] + (xg[None, :] + 1) * 0.5 * (2 * PI / nph))
+    SS = np.broadcast_to(S[:, None, :, None], (nt, nph, g, g))
+    PP = np.broadcast_to(P[None, :, None, :], (nt, nph, g, g))
+    WW = np.broadcast_to((wg[:, None] * wg[None, :])[None, None], (nt, nph, g, g)) * (1 / nt) * (PI / nph)
+    dirs = hair_dir(np.arcsin(SS.reshape(-1)), PP.reshape(-1)).astype(np.float32)
+    M = len(dirs)
+    _, pq, _ = orc.bsdf_batch(1, np.repeat(mat, M), np.tile(wi, (M, 1)), dirs, h=np.full(M, h, np.float32))
+    exp = (pq.reshape(nt, nph, g, g) * WW).sum((2, 3)).reshape(-1) * N
+    assert abs(exp.sum() / N - 1) < 5e-3
+    keep = exp >= 10
+    chi2 = ((obs[keep] - exp[keep]) ** 2 / exp[keep]).sum()
+    dof = keep.sum() - 1
+    assert stats.chi2.sf(chi2, dof) > 1e-4, (chi2, dof)
+    assert obs[~keep].sum() < 0.01 * N
+
+
+@pytest.mark.parametrize("color", [(1e-5, 1e-5, 1e-5), (0.6, 0.4, 0.3), (0.95, 0.9, 0.8)])
+def test_attenuation_closed_form(color):
+    # At theta_o = 0, h = 0 the path through the fibre has length 2, so
+    # T = exp(-2 sigma_a) with sigma_a = (ln c / g(beta_n))^2 (Chiang et al. 2016
+    # eq. 9), the Fresnel factor is f0 = ((eta-1)/(eta+1))^2 and the lobe
+    # attenuations are f0, (1-f0)^2 T, (1-f0)^2 T^2 f0, (1-f0)^2 T^3 f0^2/(1-T f0).
+    # Azimuthal lobes have unit mass, so the furnace integral is the sum of
+    # attenuation x longitudinal mass, each lobe at its tilted angle
+    # (-2a, a, 4a, 0) and variance (v0, v0/4, 4v0, 4v0),
+    # v0 = (0.726 b + 0.812 b^2 + 3.7 b^20)^2 (PBRT 4e §9.9).
+    bn, bm, eta, alpha = 0.3, 0.5, 1.55, math.radians(2.0)
+    g = 5.969 - 0.215 * bn + 2.532 * bn ** 2 - 10.73 * bn ** 3 + 5.574 * bn ** 4 + 0.245 * bn ** 5
+    c = np.asarray(color)
+    sa = np.where(c > 1e-4, (np.log(np.maximum(c, 1e-30)) / g) ** 2, 100.0)
+    T = np.exp(-2 * sa)
+    f0 = ((eta - 1) / (eta + 1)) ** 2
+    A = [np.full(3, f0), (1 - f0) ** 2 * T, (1 - f0) ** 2 * T ** 2 * f0, (1 - f0) ** 2 * T ** 3 * f0 ** 2 / (1 - T * f0)]
+    v0 = (0.726 * bm + 0.812 * bm ** 2 + 3.7 * bm ** 20) ** 2
+    masses = [mp_mass(-2 * alpha, v0), mp_mass(alpha, v0 / 4), mp_mass(4 * alpha, 4 * v0), mp_mass(0.0, 4 * v0)]
+    expect = sum(a * mm for a, mm in zip(A, masses))
+    m = hair_mats(1, beta_m=bm, beta_n=bn, eta=eta, color=color)
+    # fine theta grid: the log-I0 branch jumps by 0.6% at a = 12 (test_log_i0),
+    # which Gauss-Legendre only resolves with many nodes
+    e, _ = _furnace(m, hair_dir(0.0, 0.0), 0.0, 1024, 384)
+    assert np.allclose(e, expect, rtol=2e-5), (e, expect)
+
+
+# ------------------------------------------------------------------ furnace scene
+def _first_hits(o, cam):
+    w, h = cam.width, cam.height
+    ys, xs = np.mgrid[0:h, 0:w]
+    hit = np.zeros((h, w), bool)
+    for dx, dy in ((x, y) for x in (0.02, 0.35, 0.65, 0.98) for y in (0.02, 0.35, 0.65, 0.98)):
+        smp = np.c_[xs.reshape(-1) + dx, ys.reshape(-1) + dy, np.full(w * h, 0.5), np.full(w * h, 0.5)]
+        _, prim, _, _ = o.trace(orc.camera_rays(cam, smp.astype(np.float32)))
+        hit |= (prim != 0xFFFFFFFF).reshape(h, w)
+    return hit.reshape(-1)
+
+
+def test_hair_furnace_render():
+    # Config-3 white furnace (colour 1 = sigma_a 0, hair only, uniform unit
+    # envmap): pixels that see no hair are exactly 1, and hair pixels converge to
+    # 1 less the truncated-I0 loss of test_mp_mass per bounce.
+    sc = ng.generate_scene(3, strands=2000, variant=1, width=48, height=48)
+    o = orc.OracleScene(sc)
+    img, _ = o.render(sc.camera, 128, 32, seed=3)
+    L = img[:, :3] / img[:, 3:4]
+    hit = _first_hits(o, sc.camera)
+    assert 200 < hit.sum() < len(hit) - 200
+    # a pixel whose footprint misses the hair sees the unit envmap only (the
+    # 4x4 probes of a footprint can miss a ribbon edge in a few pixels)
+    footprint = hit.reshape(48, 48)
+    near = footprint.copy()
+    near[1:] |= footprint[:-1]
+    near[:-1] |= footprint[1:]
+    near[:, 1:] |= footprint[:, :-1]
+    near[:, :-1] |= footprint[:, 1:]
+    far = ~near.reshape(-1)
+    assert far.sum() > 200 and np.mean(L[far] == 1.0) > 0.99
+    m = L[hit].mean()
+    assert abs(m - 1.0) < 3e-3, m
