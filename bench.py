@@ -5,7 +5,9 @@ configuration (8x8 level-6 icospheres = 5,242,882 triangles on a ground quad,
 A step renders spp_per_step samples per pixel of the view (sample offsets
 advance from step to step, so K steps of 32 spp form the full 256-spp render).
 Inputs are resident in HBM before the timed region; the timed region is K
-render calls, each ending in a stream synchronisation.
+render calls, each ending in a stream synchronisation.  After it, a second leg
+measures the GPU time to relMSE <= 1e-3 (validation_tests/compare_exr.py:102)
+of plain and guided rendering against a high-spp plain reference.
 
 Prints one JSON line (see the keys below).  --gpus N > 1 is not supported: this
 build runs on one GPU.
@@ -51,12 +53,54 @@ def cpu_baseline(scene, cam, spp, depth):
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch of the closest-hit kernel from the committed rocprof counter summary."""
-    path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_traffic.json")
+    """HBM bytes per launch of the closest-hit kernel from the committed rocprof counter summary
+    (its line `bytes_per_launch: N`)."""
+    path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        for line in f:
+            if line.startswith("bytes_per_launch:"):
+                return int(float(line.split(":")[1]))
+    return None
+
+
+def rel_mse(img, ref):
+    """mean over pixels and channels of (img - ref)^2 / (ref^2 + 1e-2)
+    (validation_tests/compare_exr.py:102)."""
+    return float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2)))
+
+
+def calibrate_copy(nbytes):
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
+
+
+def time_to_rel_mse(gpu, ng, cam, depth, ref, chunk, max_spp, thresh, guided, seed):
+    """Progressive render in chunks of `chunk` spp until relMSE <= thresh against `ref`.
+    Returns GPU milliseconds (all kernels of the renders, guiding included), spp and the curve."""
+    gpu.set_option("guiding", 1 if guided else 0)
+    if guided:
+        gpu.set_option("guide_reset", 1)
+    gpu.reset_film()
+    ms, spp, curve = 0.0, 0, []
+    while spp < max_spp:
+        img = gpu.render(ng.make_view(cam, chunk, depth, sample_offset=spp), seed=seed)
+        ms += gpu.stats().ms_total
+        spp += chunk
+        e = rel_mse(img[:, :3] / img[:, 3:4], ref)
+        curve.append((spp, round(ms, 2), e))
+        if e <= thresh:
+            break
+    gpu.set_option("guiding", 0)
+    return {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": e <= thresh,
+            "curve": [(s, m, float(f"{x:.4g}")) for s, m, x in curve[:: max(1, len(curve) // 12)]]}
 
 
 def main():
@@ -69,8 +113,18 @@ def main():
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--batch", type=int, default=1 << 21)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-spp", type=int, default=32)
+    ap.add_argument("--cpu-spp", type=int, default=64)
+    ap.add_argument("--no-time-to-mse", action="store_true", help="skip the time-to-relMSE leg")
+    ap.add_argument("--mse-ref-spp", type=int, default=8192)
+    ap.add_argument("--mse-chunk", type=int, default=8)
+    ap.add_argument("--mse-max-spp", type=int, default=4096)
+    ap.add_argument("--mse-threshold", type=float, default=1e-3)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES bytes (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (use --gpus 1)",
               file=sys.stderr)
@@ -139,6 +193,28 @@ def main():
     wall = time.perf_counter() - t0
 
     value = acc["samples"] / wall / 1e6
+
+    ttm = None
+    if not args.no_time_to_mse:
+        # Reference: plain path_mis at mse_ref_spp with an independent seed; then
+        # plain and guided (network trained online from its initialisation)
+        # progressive renders until relMSE <= threshold.
+        gpu.set_option("timing_mode", 0)
+        gpu.reset_film()
+        t0 = time.time()
+        refimg = gpu.render(ng.make_view(cam, args.mse_ref_spp, args.depth, sample_offset=1 << 24), seed=1234)
+        ref = refimg[:, :3] / refimg[:, 3:4]
+        log(f"[bench] relMSE reference: {args.mse_ref_spp} spp in {time.time() - t0:.1f}s")
+        plain = time_to_rel_mse(gpu, ng, cam, args.depth, ref, args.mse_chunk, args.mse_max_spp,
+                                args.mse_threshold, False, 77)
+        log(f"[bench] plain: relMSE {plain['rel_mse']:.3g} at {plain['spp']} spp after {plain['ms']:.0f} ms")
+        guided = time_to_rel_mse(gpu, ng, cam, args.depth, ref, args.mse_chunk, args.mse_max_spp,
+                                 args.mse_threshold, True, 78)
+        log(f"[bench] guided: relMSE {guided['rel_mse']:.3g} at {guided['spp']} spp after {guided['ms']:.0f} ms")
+        ttm = {"threshold": args.mse_threshold, "ref_spp": args.mse_ref_spp, "chunk_spp": args.mse_chunk,
+               "plain": plain, "guided": guided,
+               "speedup_guided_over_plain": (round(plain["ms"] / guided["ms"], 3)
+                                             if plain["reached"] and guided["reached"] else None)}
     ext_bytes = acc["ext"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
     achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
     traffic = traffic_from_profiles()
@@ -179,6 +255,7 @@ def main():
             "avg_launch_ms": round(acc["ms_ext"] / max(acc["launch_ext"], 1), 4),
         },
         "cpu_baseline": cpu,
+        "time_to_rel_mse": ttm,
     }
     if cpu:
         result["gpu_over_cpu"] = round(value / cpu["value"], 1)
