@@ -82,25 +82,29 @@ def calibrate_copy(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
 
 
-def time_to_rel_mse(gpu, ng, cam, depth, ref, chunk, max_spp, thresh, guided, seed):
-    """Progressive render in chunks of `chunk` spp until relMSE <= thresh against `ref`.
-    Returns GPU milliseconds (all kernels of the renders, guiding included), spp and the curve."""
+def time_to_rel_mse(gpu, ng, cam, depth, ref, start_spp, growth, max_spp, thresh, guided, seed):
+    """Progressive render, checking relMSE against `ref` after 1, growth, growth^2, ...
+    times start_spp samples per pixel, until relMSE <= thresh.  Returns the GPU
+    milliseconds of the renders (every kernel, guiding inference and training
+    included), the spp reached and the convergence curve."""
     gpu.set_option("guiding", 1 if guided else 0)
     if guided:
         gpu.set_option("guide_reset", 1)
     gpu.reset_film()
-    ms, spp, curve = 0.0, 0, []
+    ms, spp, curve, e = 0.0, 0, [], float("inf")
+    target = start_spp
     while spp < max_spp:
+        chunk = min(max(target - spp, 1), max_spp - spp)
         img = gpu.render(ng.make_view(cam, chunk, depth, sample_offset=spp), seed=seed)
         ms += gpu.stats().ms_total
         spp += chunk
         e = rel_mse(img[:, :3] / img[:, 3:4], ref)
-        curve.append((spp, round(ms, 2), e))
+        curve.append((spp, round(ms, 2), float(f"{e:.4g}")))
         if e <= thresh:
             break
+        target = int(target * growth)
     gpu.set_option("guiding", 0)
-    return {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": e <= thresh,
-            "curve": [(s, m, float(f"{x:.4g}")) for s, m, x in curve[:: max(1, len(curve) // 12)]]}
+    return {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": e <= thresh, "curve": curve}
 
 
 def main():
@@ -115,9 +119,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--no-time-to-mse", action="store_true", help="skip the time-to-relMSE leg")
-    ap.add_argument("--mse-ref-spp", type=int, default=8192)
-    ap.add_argument("--mse-chunk", type=int, default=8)
-    ap.add_argument("--mse-max-spp", type=int, default=4096)
+    ap.add_argument("--mse-res", type=int, default=64, help="image side of the time-to-relMSE leg")
+    ap.add_argument("--mse-ref-spp", type=int, default=1 << 20)
+    ap.add_argument("--mse-start-spp", type=int, default=256)
+    ap.add_argument("--mse-growth", type=float, default=1.25)
+    ap.add_argument("--mse-max-spp", type=int, default=1 << 18)
     ap.add_argument("--mse-threshold", type=float, default=1e-3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES bytes (counter calibration) and exit")
@@ -196,28 +202,31 @@ def main():
 
     ttm = None
     if not args.no_time_to_mse:
-        # Reference: plain path_mis at mse_ref_spp with an independent seed; then
-        # plain and guided (network trained online from its initialisation)
-        # progressive renders until relMSE <= threshold.
+        # The same view at mse_res x mse_res: a plain reference at mse_ref_spp
+        # (independent seed), then plain and guided (network trained online from
+        # its initialisation) progressive renders until relMSE <= threshold.
+        # Plain path tracing of this scene is dominated by rare high-energy paths
+        # (sun caustics off glossy spheres), so 1e-3 takes ~1e5 spp; the leg runs
+        # at reduced resolution to keep the reference converged and the run short.
         gpu.set_option("timing_mode", 0)
+        mcam = ng.lookat_camera_like(cam, args.mse_res, args.mse_res)
         gpu.reset_film()
         t0 = time.time()
-        refimg = gpu.render(ng.make_view(cam, args.mse_ref_spp, args.depth, sample_offset=1 << 24), seed=1234)
+        refimg = gpu.render(ng.make_view(mcam, args.mse_ref_spp, args.depth, sample_offset=1 << 24), seed=1234)
         ref = refimg[:, :3] / refimg[:, 3:4]
-        log(f"[bench] relMSE reference: {args.mse_ref_spp} spp in {time.time() - t0:.1f}s")
-        plain = time_to_rel_mse(gpu, ng, cam, args.depth, ref, args.mse_chunk, args.mse_max_spp,
-                                args.mse_threshold, False, 77)
-        log(f"[bench] plain: relMSE {plain['rel_mse']:.3g} at {plain['spp']} spp after {plain['ms']:.0f} ms")
-        guided = time_to_rel_mse(gpu, ng, cam, args.depth, ref, args.mse_chunk, args.mse_max_spp,
-                                 args.mse_threshold, True, 78)
-        log(f"[bench] guided: relMSE {guided['rel_mse']:.3g} at {guided['spp']} spp after {guided['ms']:.0f} ms")
-        ttm = {"threshold": args.mse_threshold, "ref_spp": args.mse_ref_spp, "chunk_spp": args.mse_chunk,
-               "plain": plain, "guided": guided,
-               "speedup_guided_over_plain": (round(plain["ms"] / guided["ms"], 3)
-                                             if plain["reached"] and guided["reached"] else None)}
-    ext_bytes = acc["ext"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
-    achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
-    traffic = traffic_from_profiles()
+        log(f"[bench] relMSE reference: {args.mse_res}x{args.mse_res} at {args.mse_ref_spp} spp in "
+            f"{time.time() - t0:.1f}s")
+        legs = {}
+        for name, guided, seed in (("plain", False, 77), ("guided", True, 78)):
+            legs[name] = time_to_rel_mse(gpu, ng, mcam, args.depth, ref, args.mse_start_spp, args.mse_growth,
+                                         args.mse_max_spp, args.mse_threshold, guided, seed)
+            r = legs[name]
+            log(f"[bench] {name}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
+        ttm = {"threshold": args.mse_threshold, "image": f"{args.mse_res}x{args.mse_res}", "ref_spp": args.mse_ref_spp,
+               "plain": legs["plain"], "guided": legs["guided"],
+               "speedup_guided_over_plain": (round(legs["plain"]["ms"] / legs["guided"]["ms"], 3)
+                                             if legs["plain"]["reached"] and legs["guided"]["reached"] else None)}
+
     result = {
         "metric": METRIC,
         "value": round(value, 3),

@@ -252,6 +252,14 @@ def make_view(camera, spp, max_depth, rr_depth=3, sample_offset=0):
     return v
 
 
+def lookat_camera_like(cam, width, height):
+    """A copy of `cam` with another image size (same pose, fov and lens)."""
+    c = Camera()
+    ctypes.memmove(ctypes.byref(c), ctypes.byref(cam), ctypes.sizeof(Camera))
+    c.width, c.height = width, height
+    return c
+
+
 def lookat_camera(eye, target, up, fov_deg, width, height, lens_radius=0.0, focal_distance=1.0):
     """Nori lookat (src/cpu/parser.cpp transform tags) as a row-major camera-to-world matrix."""
     eye, target, up = (np.asarray(x, np.float32) for x in (eye, target, up))
