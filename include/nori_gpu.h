@@ -128,6 +128,14 @@ int nori_gpu_render(nori_ctx *ctx, const nori_view *views, int nviews, uint64_t 
 /* Zero the device films. */
 int nori_gpu_reset_film(nori_ctx *ctx);
 
+/* Integrator::Li (include/integrator.h:42) for a batch of caller rays: the
+ * radiance arriving along each ray, one path per ray, with the estimator of
+ * nori_gpu_render (NEE + BSDF sampling with MIS, Russian roulette from bounce
+ * rr_depth, at most max_depth bounces).  Ray i draws from the pcg32 stream
+ * seed(seeds[i], PCG32_DEFAULT_STREAM).  rgb3: n x 3 floats (host). */
+int nori_gpu_li(nori_ctx *ctx, const nori_ray *rays, const uint64_t *seeds, int n, uint32_t max_depth,
+                uint32_t rr_depth, float *rgb3);
+
 /* Batch closest-hit (any_hit = 0) or occlusion (any_hit = 1) queries; rays and
  * hits are host arrays.  For any_hit, prim is 0 on a hit and 0xFFFFFFFF else.
  * The batch form of Accel::rayIntersect (src/cpu/accel.cpp:429-544). */

@@ -45,6 +45,7 @@ def lib():
         L.orc_env_eval_pdf.argtypes = [vp, u32, vp, vp, vp]
         L.orc_bsdf_batch.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_hair_fn.argtypes = [i32, u32, vp, vp, vp, vp, vp, vp]
+        L.orc_li.argtypes = [vp, u32, vp, vp, u32, u32, vp]
         _lib = L
     return _lib
 
@@ -144,6 +145,14 @@ class OracleScene:
         lib().orc_render(self.h, ctypes.byref(camera), spp, sample_offset, max_depth, rr_depth, seed, int(env_cdf),
                          nthreads, _p(acc), _p(stats))
         return acc, stats
+
+    def li(self, rays, seeds, max_depth, rr_depth=3):
+        """Integrator::Li per ray (rays: n x 8 float32 o, mint, d, maxt)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        out = np.zeros((len(rays), 3), np.float32)
+        lib().orc_li(self.h, len(rays), _p(rays), _p(seeds), max_depth, rr_depth, _p(out))
+        return out
 
     def env_tables(self):
         H, W = self.scene.env_rgb.shape[:2]

@@ -93,6 +93,31 @@ void orc_render(void *hp, const Camera *cam, uint32_t spp, uint32_t sample_offse
     if (stats3) { stats3[0] = st.samples; stats3[1] = st.extension_rays; stats3[2] = st.shadow_rays; }
 }
 
+// Integrator::Li per caller ray: ray i runs path_mis with the pcg32 stream
+// seed(seeds[i], PCG32_DEFAULT_STREAM).
+void orc_li(void *hp, uint32_t n, const float *rays8, const uint64_t *seeds, uint32_t max_depth, uint32_t rr_depth,
+            float *rgb3) {
+    const SceneData &s = ((Handle *)hp)->s;
+    Camera cam{};
+    RenderParams p{1, 0, max_depth, rr_depth, 0, 0, 1};
+    RenderStats st;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *r8 = rays8 + 8 * (size_t)i;
+        Ray r;
+        r.o = V3f(r8[0], r8[1], r8[2]);
+        r.d = V3f(r8[4], r8[5], r8[6]);
+        r.mint = r8[3];
+        r.maxt = r8[7];
+        r.drcp = V3f(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
+        Pcg32 rng;
+        rng.seed(seeds[i], PCG32_DEFAULT_STREAM);
+        V3f L = path_mis_li(s, cam, r, rng, p, st);
+        rgb3[3 * i] = L.x;
+        rgb3[3 * i + 1] = L.y;
+        rgb3[3 * i + 2] = L.z;
+    }
+}
+
 void orc_camera_rays(const Camera *cam, uint32_t n, const float *samples4, float *rays8) {
     for (uint32_t i = 0; i < n; ++i) {
         Ray r;

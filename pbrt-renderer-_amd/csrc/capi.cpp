@@ -264,9 +264,81 @@ struct Span {
     }
 };
 
+// The bounce loop of one batch of n paths whose state k_raygen (or
+// k_raygen_rays) wrote: extend, [guide inference], shade, shadow per bounce.
+void run_bounces(nori_ctx *c, float4 *film, uint32_t n, uint32_t max_depth, uint32_t rr_depth,
+                 const GuideShadeDev &gsd, unsigned long long *cs, nori_render_stats &st) {
+    for (uint32_t b = 0; b < max_depth; ++b) {
+        uint32_t qi = b & 1u, qo = qi ^ 1u;
+        {
+            Span t(c, 0);
+            launch_extend(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, n, cs, c->stream);
+        }
+        if (gsd.out) {
+            Span t(c, 3);
+            launch_guide_feats(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, (bf16_t *)c->g_act[0].p, n, c->stream);
+            c->guide_forward_dev(n, c->cdev.cnt + qi);
+        }
+        {
+            Span t(c, 2);
+            launch_shade(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, c->q[qo], c->cdev.cnt + qo, c->shdev,
+                         c->cdev.cnt + 2, film, b, max_depth, rr_depth, n, gsd, c->stream);
+        }
+        {
+            Span t(c, 1);
+            launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, film, 2 * n, cs, gsd, c->stream);
+        }
+        st.launches_closest++;
+        st.launches_shadow++;
+        launch_counters(c->cdev, qi, 1, (1u << qi) | 4u, c->stream);
+    }
+}
+
 } // namespace
 
 extern "C" {
+
+int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n, uint32_t max_depth, uint32_t rr_depth,
+                float *rgb3) {
+    if (!c || n < 0 || (n > 0 && (!rays || !seeds || !rgb3)) || max_depth == 0) return NORI_E_ARG;
+    return guard(c, [&] {
+        if (!c->has_scene) throw ArgError("li: no scene uploaded", NORI_E_STATE);
+        if (n == 0) return;
+        nori_render_stats st{};
+        c->spans.clear();
+        c->ev_used = 0;
+        launch_counters(c->cdev, 3, 0, 0x10u, c->stream);
+        ScopedBufs b(3);
+        size_t nr = (size_t)n;
+        HIPCHK(hipMalloc(&b[0].p, nr * sizeof(nori_ray)));
+        HIPCHK(hipMalloc(&b[1].p, nr * sizeof(uint64_t)));
+        HIPCHK(hipMalloc(&b[2].p, nr * sizeof(float4)));
+        HIPCHK(hipMemcpyAsync(b[0].p, rays, nr * sizeof(nori_ray), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b[1].p, seeds, nr * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+        float4 *film = (float4 *)b[2].p;
+        launch_zero_f4(film, nr, c->stream);
+        uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, nr);
+        c->ensure_paths(cap);
+        GuideShadeDev gsd{nullptr, nullptr, nullptr, nullptr, 0, 0};
+        for (uint64_t first = 0; first < nr; first += cap) {
+            uint32_t m = (uint32_t)std::min<uint64_t>(cap, nr - first);
+            launch_counters(c->cdev, 3, 0, 0x7u, c->stream);
+            launch_set_counter(c->cdev.cnt, m, c->stream);
+            launch_raygen_rays(c->pdev, c->q[0], (const float4 *)b[0].p, (const uint64_t *)b[1].p, first, m, film,
+                               c->stream);
+            run_bounces(c, film, m, max_depth, rr_depth, gsd, nullptr, st);
+        }
+        std::vector<float4> out(nr);
+        HIPCHK(hipMemcpyAsync(out.data(), film, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        for (size_t i = 0; i < nr; ++i) {
+            rgb3[3 * i] = out[i].x;
+            rgb3[3 * i + 1] = out[i].y;
+            rgb3[3 * i + 2] = out[i].z;
+        }
+    });
+}
 
 int nori_gpu_create(nori_ctx **out, int device) {
     if (!out) return NORI_E_ARG;
@@ -474,33 +546,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                     Span t(c, 2);
                     launch_raygen(c->sdev, c->pdev, c->q[0], c->cdev, vd, seed, first, n, gsd, c->stream);
                 }
-                for (uint32_t b = 0; b < v.max_depth; ++b) {
-                    uint32_t qi = b & 1u, qo = qi ^ 1u;
-                    {
-                        Span t(c, 0);
-                        launch_extend(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, n, cs, c->stream);
-                    }
-                    if (c->guiding) {
-                        Span t(c, 3);
-                        launch_guide_feats(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, (bf16_t *)c->g_act[0].p, n,
-                                           c->stream);
-                        c->guide_forward_dev(n, c->cdev.cnt + qi);
-                    }
-                    {
-                        Span t(c, 2);
-                        launch_shade(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, c->q[qo], c->cdev.cnt + qo,
-                                     c->shdev, c->cdev.cnt + 2, (float4 *)c->films[vi].p, b, v.max_depth, v.rr_depth, n,
-                                     gsd, c->stream);
-                    }
-                    {
-                        Span t(c, 1);
-                        launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, (float4 *)c->films[vi].p, 2 * n, cs, gsd,
-                                      c->stream);
-                    }
-                    st.launches_closest++;
-                    st.launches_shadow++;
-                    launch_counters(c->cdev, qi, 1, (1u << qi) | 4u, c->stream);
-                }
+                run_bounces(c, (float4 *)c->films[vi].p, n, v.max_depth, v.rr_depth, gsd, cs, st);
                 if (c->guiding && c->g_train_steps > 0) {
                     // Online training on the radiance this batch's paths carried.
                     Span t(c, 4);

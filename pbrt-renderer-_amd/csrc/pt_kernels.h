@@ -72,6 +72,10 @@ struct Counters {
 void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uint32_t n, int any_hit, hipStream_t st);
 void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counters c, const ViewDev &v,
                    uint64_t seed, uint64_t first_sample, uint32_t n, const GuideShadeDev &g, hipStream_t st);
+// Paths from caller rays (2 float4 per ray: o.xyz mint, d.xyz maxt) with
+// per-ray pcg32 seeds; film entry first + i collects ray i's radiance.
+void launch_raygen_rays(const PathsDev &p, uint32_t *queue, const float4 *rays, const uint64_t *seeds, uint64_t first,
+                        uint32_t n, float4 *film, hipStream_t st);
 // count_stats (optional): stats array whose [4],[5] (extend) / [6],[7] (shadow)
 // receive the summed BVH8 nodes visited and triangles tested.
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, uint32_t cap,

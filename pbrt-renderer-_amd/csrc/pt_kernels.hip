@@ -118,6 +118,24 @@ __global__ void k_raygen(SceneDev s, PathsDev p, uint32_t *queue, ViewDev v, uin
     unsafeAtomicAdd(reinterpret_cast<float *>(v.film + pixel) + 3, 1.0f);
 }
 
+// Paths from caller rays (Integrator::Li): ray i -> slot i, film entry first + i,
+// pcg32 stream seed(seeds[first + i], PCG32_DEFAULT_STREAM).
+__global__ void k_raygen_rays(PathsDev p, uint32_t *queue, const float4 *__restrict__ rays,
+                              const uint64_t *__restrict__ seeds, uint64_t first, uint32_t n, float4 *film) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t gi = first + i;
+    const float4 a = rays[2 * gi], b = rays[2 * gi + 1];
+    uint64_t rng = pcg_seed(seeds[gi], PCG_DEFAULT_STREAM);
+    p.ray_o[i] = a;
+    p.ray_d[i] = b;
+    p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p.aux[i] = make_float4(1.0f, __uint_as_float((uint32_t)gi), __uint_as_float((uint32_t)rng),
+                           __uint_as_float((uint32_t)(rng >> 32)));
+    queue[i] = i;
+    film[gi].w += 1.0f;
+}
+
 // Adds this lane's traversal counts into stats[k], stats[k+1] (one atomic pair per wave).
 __device__ __forceinline__ void count_add(unsigned long long *stats, int k, uint32_t nodes, uint32_t tris) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -706,6 +724,11 @@ void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counte
                    uint64_t first, uint32_t n, const GuideShadeDev &g, hipStream_t st) {
     (void)c;
     hipLaunchKernelGGL(k_raygen, dim3(blocks(n, 256)), dim3(256), 0, st, s, p, queue, v, seed, first, n, g);
+}
+
+void launch_raygen_rays(const PathsDev &p, uint32_t *queue, const float4 *rays, const uint64_t *seeds, uint64_t first,
+                        uint32_t n, float4 *film, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_raygen_rays, dim3(blocks(n, 256)), dim3(256), 0, st, p, queue, rays, seeds, first, n, film);
 }
 
 void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, bf16_t *X,

@@ -91,7 +91,7 @@ class SceneOptions(ctypes.Structure):
 
 EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
-    "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch",
+    "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
     "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_render_stats",
     "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_scene_generate", "nori_scene_get_desc",
     "nori_scene_get_camera", "nori_scene_free", "nori_bvh8_build_host",
@@ -118,6 +118,7 @@ def load_library(path=LIB_PATH):
         "nori_gpu_trace": (i32, [vp, vp, i32, vp, i32]),
         "nori_gpu_trace_device": (i32, [vp, vp, i32, vp, i32]),
         "nori_gpu_bsdf_batch": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "nori_gpu_li": (i32, [vp, vp, vp, i32, ctypes.c_uint32, ctypes.c_uint32, vp]),
         "nori_gpu_envmap_sample": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
         "nori_gpu_pcg32": (i32, [vp, _u64, _u64, _u64, i32, vp]),
@@ -195,28 +196,37 @@ def generate_scene(config, seed=0, subdiv=0, grid=0, env_width=0, env_height=0, 
         lib.nori_scene_get_desc(h, ctypes.byref(d))
         cam = Camera()
         lib.nori_scene_get_camera(h, ctypes.byref(cam))
-
-        def arr(p, n, dt):
-            if not p or n == 0:
-                return None
-            buf = (ctypes.c_char * (n * np.dtype(dt).itemsize)).from_address(p)
-            return np.frombuffer(buf, dt).copy()
-
-        pos = arr(d.positions, 3 * d.num_vertices, np.float32)
-        nrm = arr(d.normals, 3 * d.num_vertices, np.float32)
-        uvs = arr(d.uvs, 2 * d.num_vertices, np.float32)
-        tans = arr(d.tangents, 3 * d.num_vertices, np.float32)
-        idx = arr(d.indices, 3 * d.num_triangles, np.uint32)
-        mto = arr(d.mesh_tri_offset, d.num_meshes, np.uint32)
-        mtc = arr(d.mesh_tri_count, d.num_meshes, np.uint32)
-        mats = arr(d.materials, d.num_meshes, MATERIAL_DTYPE)
-        env = None
-        if d.env_width:
-            env = arr(d.env_rgb, 3 * d.env_width * d.env_height, np.float32).reshape(d.env_height, d.env_width, 3)
-        return HostScene(pos, idx, mto, mtc, mats, normals=nrm, env_rgb=env, env_scale=d.env_scale, camera=cam,
-                         uvs=uvs, tangents=tans)
+        return _host_scene_from_desc(d, cam)
     finally:
         lib.nori_scene_free(h)
+
+
+def _host_scene_from_desc(d, cam):
+    """Copy a nori_scene_desc (and camera) into a HostScene."""
+
+    def arr(p, n, dt):
+        if not p or n == 0:
+            return None
+        buf = (ctypes.c_char * (n * np.dtype(dt).itemsize)).from_address(p)
+        return np.frombuffer(buf, dt).copy()
+
+    def req(p, n, dt):
+        a = arr(p, n, dt)
+        return np.zeros(0, dt) if a is None else a
+
+    pos = req(d.positions, 3 * d.num_vertices, np.float32)
+    nrm = arr(d.normals, 3 * d.num_vertices, np.float32)
+    uvs = arr(d.uvs, 2 * d.num_vertices, np.float32)
+    tans = arr(d.tangents, 3 * d.num_vertices, np.float32)
+    idx = req(d.indices, 3 * d.num_triangles, np.uint32)
+    mto = req(d.mesh_tri_offset, d.num_meshes, np.uint32)
+    mtc = req(d.mesh_tri_count, d.num_meshes, np.uint32)
+    mats = req(d.materials, d.num_meshes, MATERIAL_DTYPE)
+    env = None
+    if d.env_width:
+        env = arr(d.env_rgb, 3 * d.env_width * d.env_height, np.float32).reshape(d.env_height, d.env_width, 3)
+    return HostScene(pos, idx, mto, mtc, mats, normals=nrm, env_rgb=env, env_scale=d.env_scale, camera=cam,
+                     uvs=uvs, tangents=tans)
 
 
 BVH8_NODE_DTYPE = np.dtype([("p", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),
@@ -317,6 +327,15 @@ class NoriGPU:
         npx = sum(v.camera.width * v.camera.height for v in views)
         out = np.zeros((npx, 4), np.float32) if readback else None
         self._check(self.lib.nori_gpu_render(self.h, arr, len(views), seed, _ptr(out)), "render")
+        return out
+
+    def li(self, rays, seeds, max_depth, rr_depth=3):
+        """Integrator::Li on the GPU for caller rays (n x 8 float32: o, mint, d, maxt)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        seeds = np.ascontiguousarray(seeds, np.uint64)
+        out = np.zeros((len(rays), 3), np.float32)
+        self._check(self.lib.nori_gpu_li(self.h, _ptr(rays), _ptr(seeds), len(rays), max_depth, rr_depth, _ptr(out)),
+                    "li")
         return out
 
     def reset_film(self):
@@ -449,3 +468,132 @@ for _name in ("guide_params", "guide_forward", "guide_gradient", "guide_train", 
     setattr(NoriGPU, _name, getattr(_GuideMixin, _name))
 EXPORTS += ["nori_guide_params", "nori_guide_forward", "nori_guide_gradient", "nori_guide_train",
             "nori_guide_mixture"]
+
+
+# ------------------------------------------------------------------ Nori XML scenes (libnori_host.so)
+HOST_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnori_host.so")
+HOST_EXPORTS = ["nori_xml_load", "nori_xml_load_string", "nori_xml_free", "nori_xml_info_get", "nori_xml_desc",
+                "nori_xml_render", "nori_xml_li", "nori_image_write", "nori_image_read", "nori_image_free",
+                "nori_plugin_names"]
+_host = None
+
+
+class XmlInfo(ctypes.Structure):
+    _fields_ = [("width", _u32), ("height", _u32), ("spp", _u32), ("meshes", _u32), ("triangles", _u32),
+                ("vertices", _u32), ("emitters", _u32), ("has_medium", _u32), ("envmap_loaded", _u32),
+                ("integrator", ctypes.c_char * 64)]
+
+
+def load_host_library(path=HOST_LIB_PATH):
+    global _host
+    if _host is not None:
+        return _host
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (make -C pbrt-renderer-_amd)")
+    lib = ctypes.CDLL(path)
+    vp, i32, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+    sig = {
+        "nori_xml_load": (i32, [cp, ctypes.POINTER(vp), cp, i32]),
+        "nori_xml_load_string": (i32, [cp, cp, ctypes.POINTER(vp), cp, i32]),
+        "nori_xml_free": (None, [vp]),
+        "nori_xml_info_get": (i32, [vp, ctypes.POINTER(XmlInfo)]),
+        "nori_xml_desc": (i32, [vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Camera)]),
+        "nori_xml_render": (i32, [vp, vp, cp, i32]),
+        "nori_xml_li": (i32, [vp, vp, i32, _u64, vp, cp, i32]),
+        "nori_image_write": (i32, [cp, vp, _u32, _u32, cp, i32]),
+        "nori_image_read": (i32, [cp, ctypes.POINTER(vp), ctypes.POINTER(_u32), ctypes.POINTER(_u32), cp, i32]),
+        "nori_image_free": (None, [vp]),
+        "nori_plugin_names": (i32, [cp, i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = res, args
+    _host = lib
+    return lib
+
+
+class NoriError(RuntimeError):
+    pass
+
+
+def _err():
+    return ctypes.create_string_buffer(4096)
+
+
+def plugin_names():
+    buf = ctypes.create_string_buffer(8192)
+    load_host_library().nori_plugin_names(buf, 8192)
+    return buf.value.decode().split(",")
+
+
+class XmlScene:
+    """A Nori XML scene loaded and activated by libnori_host.so (loadFromXML)."""
+
+    def __init__(self, filename=None, text=None, directory="."):
+        lib = load_host_library()
+        self.lib, self.h = lib, ctypes.c_void_p()
+        e = _err()
+        if text is not None:
+            rc = lib.nori_xml_load_string(text.encode(), directory.encode(), ctypes.byref(self.h), e, len(e))
+        else:
+            rc = lib.nori_xml_load(str(filename).encode(), ctypes.byref(self.h), e, len(e))
+        if rc != NORI_OK:
+            self.h = None
+            raise NoriError(e.value.decode())
+
+    def close(self):
+        if self.h:
+            self.lib.nori_xml_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = XmlInfo()
+        self.lib.nori_xml_info_get(self.h, ctypes.byref(i))
+        return i
+
+    def host_scene(self):
+        """The packed flat scene and camera as a HostScene (copies)."""
+        d, cam = SceneDesc(), Camera()
+        self.lib.nori_xml_desc(self.h, ctypes.byref(d), ctypes.byref(cam))
+        return _host_scene_from_desc(d, cam)
+
+    def render(self):
+        i = self.info()
+        out = np.zeros((i.height * i.width, 3), np.float32)
+        e = _err()
+        if self.lib.nori_xml_render(self.h, _ptr(out), e, len(e)) != NORI_OK:
+            raise NoriError(e.value.decode())
+        return out
+
+    def li(self, rays, seed=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(rays), 3), np.float32)
+        e = _err()
+        if self.lib.nori_xml_li(self.h, _ptr(rays), len(rays), seed, _ptr(out), e, len(e)) != NORI_OK:
+            raise NoriError(e.value.decode())
+        return out
+
+
+def write_image(filename, rgb, width, height):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    e = _err()
+    if load_host_library().nori_image_write(str(filename).encode(), _ptr(rgb), width, height, e, len(e)) != NORI_OK:
+        raise NoriError(e.value.decode())
+
+
+def read_image(filename):
+    lib = load_host_library()
+    p, w, h, e = ctypes.c_void_p(), _u32(), _u32(), _err()
+    if lib.nori_image_read(str(filename).encode(), ctypes.byref(p), ctypes.byref(w), ctypes.byref(h), e,
+                           len(e)) != NORI_OK:
+        raise NoriError(e.value.decode())
+    n = w.value * h.value * 3
+    arr = np.frombuffer((ctypes.c_float * n).from_address(p.value), np.float32).copy()
+    lib.nori_image_free(p)
+    return arr.reshape(h.value, w.value, 3)

@@ -13,10 +13,10 @@ import nori_gpu as ng
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared_functions():
-    text = open(os.path.join(ROOT, "include", "nori_gpu.h")).read()
+def _declared_functions(header="nori_gpu.h", prefixes="gpu|scene|bvh8|guide"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(nori_(?:gpu|scene|bvh8|guide)_\w+)\s*\(", text)))
+    return sorted(set(re.findall(rf"\b(nori_(?:{prefixes})_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -26,6 +26,15 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(ng.EXPORTS) == names
+
+
+def test_host_library_exports_every_declared_symbol():
+    lib = ng.load_host_library()
+    names = _declared_functions("nori_host.h", "xml|image|plugin")
+    assert len(names) >= 11
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(ng.HOST_EXPORTS) == names
 
 
 def test_create_without_device_fails_cleanly():
