@@ -19,6 +19,10 @@
 
 NORI_NAMESPACE_BEGIN
 
+void Integrator::LiBatch(const Scene *scene, Sampler *sampler, const Ray3f *rays, size_t n, Color3f *out) const {
+    for (size_t i = 0; i < n; ++i) out[i] = Li(scene, sampler, rays[i]);
+}
+
 void Integrator::render(const Scene *, std::vector<float> &) const {
     throw NoriException("Integrator \"%s\" has no batch render", toString());
 }
@@ -43,13 +47,28 @@ public:
     }
 
     Color3f Li(const Scene *scene, Sampler *sampler, const Ray3f &ray) const override {
+        Color3f out;
+        LiBatch(scene, sampler, &ray, 1, &out);
+        return out;
+    }
+
+    // One GPU path per ray; ray i's pcg32 stream is seeded from two sampler draws.
+    void LiBatch(const Scene *scene, Sampler *sampler, const Ray3f *rays, size_t n, Color3f *out) const override {
+        if (n == 0) return;
         nori_ctx *ctx = scene->getAccel()->context();
-        float a = sampler->next1D(), b = sampler->next1D();
-        uint64_t seed = ((uint64_t)(a * 4294967296.0) << 32) ^ (uint64_t)(b * 4294967296.0);
-        nori_ray r{{ray.o[0], ray.o[1], ray.o[2]}, ray.mint, {ray.d[0], ray.d[1], ray.d[2]}, ray.maxt};
-        float rgb[3] = {0, 0, 0};
-        gpuCheck(nori_gpu_li(ctx, &r, &seed, 1, (uint32_t)m_maxDepth, (uint32_t)m_rrDepth, rgb), "nori_gpu_li", ctx);
-        return Color3f(rgb[0], rgb[1], rgb[2]);
+        std::vector<nori_ray> rr(n);
+        std::vector<uint64_t> seeds(n);
+        for (size_t i = 0; i < n; ++i) {
+            const Ray3f &ray = rays[i];
+            float a = sampler->next1D(), b = sampler->next1D();
+            seeds[i] = ((uint64_t)(a * 4294967296.0) << 32) ^ (uint64_t)(b * 4294967296.0);
+            rr[i] = nori_ray{{ray.o[0], ray.o[1], ray.o[2]}, ray.mint, {ray.d[0], ray.d[1], ray.d[2]}, ray.maxt};
+        }
+        std::vector<float> rgb(3 * n);
+        gpuCheck(nori_gpu_li(ctx, rr.data(), seeds.data(), (int)n, (uint32_t)m_maxDepth, (uint32_t)m_rrDepth,
+                             rgb.data()),
+                 "nori_gpu_li", ctx);
+        for (size_t i = 0; i < n; ++i) out[i] = Color3f(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
     }
 
     void render(const Scene *scene, std::vector<float> &rgb) const override {

@@ -118,12 +118,15 @@ int nori_xml_li(nori_host_scene *s, const nori_ray *rays, int n, uint64_t seed, 
         std::unique_ptr<Sampler> sampler = sc.getSampler()->clone();
         ImageBlock block(Vector2i((int)(seed & 0x7fffffff), (int)(seed >> 32)), Vector2i(1, 1));
         sampler->prepare(block);
+        std::vector<Ray3f> rs((size_t)n);
         for (int i = 0; i < n; ++i) {
             const nori_ray &r = rays[i];
-            Ray3f ray(Point3f(r.o[0], r.o[1], r.o[2]), Vector3f(r.d[0], r.d[1], r.d[2]), r.mint, r.maxt);
-            Color3f L = sc.getIntegrator()->Li(&sc, sampler.get(), ray);
-            for (int c = 0; c < 3; ++c) rgb[3 * i + c] = L[c];
+            rs[(size_t)i] = Ray3f(Point3f(r.o[0], r.o[1], r.o[2]), Vector3f(r.d[0], r.d[1], r.d[2]), r.mint, r.maxt);
         }
+        std::vector<Color3f> L((size_t)n);
+        sc.getIntegrator()->LiBatch(&sc, sampler.get(), rs.data(), rs.size(), L.data());
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) rgb[3 * i + c] = L[(size_t)i][c];
     });
 }
 

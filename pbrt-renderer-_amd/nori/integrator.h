@@ -16,6 +16,8 @@ public:
     virtual void preprocess(const Scene *scene) { (void)scene; }
     /// Radiance arriving along `ray`.
     virtual Color3f Li(const Scene *scene, Sampler *sampler, const Ray3f &ray) const = 0;
+    /// Li for n rays at once (the default calls Li per ray).
+    virtual void LiBatch(const Scene *scene, Sampler *sampler, const Ray3f *rays, size_t n, Color3f *out) const;
     /// Renders the scene's camera view at the sampler's sample count into
     /// `rgb` (width x height x 3, row 0 at the top).  The default throws.
     virtual void render(const Scene *scene, std::vector<float> &rgb) const;
