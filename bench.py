@@ -200,6 +200,10 @@ def main():
 
     value = acc["samples"] / wall / 1e6
 
+    ext_bytes = acc["ext"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
+    achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
+    traffic = traffic_from_profiles()
+
     ttm = None
     if not args.no_time_to_mse:
         # The same view at mse_res x mse_res: a plain reference at mse_ref_spp
@@ -222,10 +226,15 @@ def main():
                                          args.mse_max_spp, args.mse_threshold, guided, seed)
             r = legs[name]
             log(f"[bench] {name}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
+        # Where the threshold is not reached within mse_max_spp, the time to it
+        # is extrapolated from the last point with relMSE ~ 1/N (labelled).
+        for r in legs.values():
+            r["ms_to_threshold"] = r["ms"] if r["reached"] else round(r["ms"] * r["rel_mse"] / args.mse_threshold, 1)
+            r["extrapolated"] = not r["reached"]
         ttm = {"threshold": args.mse_threshold, "image": f"{args.mse_res}x{args.mse_res}", "ref_spp": args.mse_ref_spp,
                "plain": legs["plain"], "guided": legs["guided"],
-               "speedup_guided_over_plain": (round(legs["plain"]["ms"] / legs["guided"]["ms"], 3)
-                                             if legs["plain"]["reached"] and legs["guided"]["reached"] else None)}
+               "speedup_guided_over_plain": round(legs["plain"]["ms_to_threshold"] / legs["guided"]["ms_to_threshold"],
+                                                  3)}
 
     result = {
         "metric": METRIC,

@@ -78,10 +78,84 @@ static uint32_t cdf_sample(const std::vector<float> &c, uint32_t n, float u) {
     return (uint32_t)std::max(0, (int)lo - 1);
 }
 
-static bool occluded(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st) {
+static bool occluded(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st, float mint = 0.0f) {
     ++st.shadow_rays;
-    Hit h = s.bvh.trace(mk_ray(o, d, 0.0f, maxt), true);
+    Hit h = s.bvh.trace(mk_ray(o, d, mint, maxt), true);
     return h.prim != 0xFFFFFFFFu;
+}
+
+// Transmittance of an unoccluded shadow ray through the scene's volumes, with
+// the ray's own stream seed(seed, PCG32_DEFAULT_STREAM).
+static V3f shadow_transmittance(const SceneData &s, V3f o, V3f d, float tmax, uint32_t seed) {
+    if (s.vols.empty()) return V3f(1.0f);
+    Pcg32 r;
+    r.seed(seed, PCG32_DEFAULT_STREAM);
+    return multi_volume_transmittance(s.vols, o, d, tmax, r);
+}
+
+// Area-light point sample (shaders/Emitter.hlsli:43-89): emitter, triangle,
+// barycentrics (1 - sqrt r1, sqrt r1 (1 - r2), sqrt r1 r2).
+struct LightSample {
+    V3f p, n, Le;
+    float pdfA;
+};
+static LightSample sample_area_light(const SceneData &s, Pcg32 &rng) {
+    const uint32_t E = (uint32_t)s.emitters.size();
+    uint32_t pick = std::min((uint32_t)(rng.next_float() * (float)E), E - 1);
+    uint32_t em = s.emitters[pick];
+    uint32_t ntri = s.mesh_tri_count[em];
+    uint32_t k = std::min(cdf_sample(s.emit_cdf[pick], ntri + 1, rng.next_float()), ntri - 1);
+    uint32_t ef = s.mesh_tri_offset[em] + k;
+    V3f p0 = s.vtx(s.idx[3 * ef]), p1 = s.vtx(s.idx[3 * ef + 1]), p2 = s.vtx(s.idx[3 * ef + 2]);
+    float r1 = rng.next_float(), r2 = rng.next_float();
+    float sr1 = std::sqrt(r1);
+    LightSample ls;
+    ls.p = (1.0f - sr1) * p0 + sr1 * (1.0f - r2) * p1 + sr1 * r2 * p2;
+    ls.n = normalize(cross(p1 - p0, p2 - p0));
+    ls.pdfA = 1.0f / (s.emit_area[pick] * (float)E);
+    ls.Le = V3f(s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]);
+    return ls;
+}
+
+// Volume NEE toward the area lights at a scatter point (shaders/Emitter.hlsli:334-385).
+static V3f volume_nee_area(const SceneData &s, V3f P, V3f wo, const Volume &v, Pcg32 &rng, RenderStats &st) {
+    if (volume_nee_skip(v, P) || s.emitters.empty()) return V3f(0.0f);
+    LightSample ls = sample_area_light(s, rng);
+    uint32_t tr_seed = rng.next_uint();
+    V3f tl = ls.p - P;
+    float dist = length(tl);
+    V3f wi = tl / dist;
+    float cl = dot(ls.n, -wi);
+    if (cl <= 0.0f) return V3f(0.0f);
+    if (occluded(s, P, wi, dist - 0.001f, st, 0.001f)) return V3f(0.0f);
+    float fp = hg_eval(dot(wo, wi), v.g);
+    float pe = ls.pdfA * dist * dist / cl;
+    float w = balance_heuristic(pe, fp);
+    V3f tr = shadow_transmittance(s, P, wi, dist, tr_seed);
+    return ls.Le * fp / std::max(pe, 1e-20f) * w * tr;
+}
+
+// Volume NEE toward the envmap (shaders/Emitter.hlsli:387-434).
+static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Pcg32 &rng, const RenderParams &p,
+                          RenderStats &st) {
+    if (volume_nee_skip(v, P) || !s.env.valid()) return V3f(0.0f);
+    V3f dir, Le;
+    float pe;
+    if (p.env_cdf) {
+        float u1 = rng.next_float(), u2 = rng.next_float();
+        s.env.sample_cdf(u1, u2, dir, Le, pe);
+    } else {
+        uint32_t r = rng.next_uint();
+        float coin = rng.next_float(), jx = rng.next_float(), jy = rng.next_float();
+        s.env.sample_alias(r, coin, jx, jy, dir, Le, pe);
+    }
+    uint32_t tr_seed = rng.next_uint();
+    if (pe <= 0.0f) return V3f(0.0f);
+    if (occluded(s, P, dir, 1e20f, st, 0.001f)) return V3f(0.0f);
+    float fp = hg_eval(dot(wo, dir), v.g);
+    float w = balance_heuristic(pe, fp);
+    V3f tr = shadow_transmittance(s, P, dir, 1e20f, tr_seed);
+    return Le * fp / std::max(pe, 1e-20f) * w * tr;
 }
 
 // RayGen body, shaders/Shaders.hlsl:55-627, for the surface materials of the
@@ -94,7 +168,34 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
     for (uint32_t bounce = 0; bounce < p.max_depth; ++bounce) {
         ++st.extension_rays;
         Hit h = s.bvh.trace(ray, false);
-        if (h.prim == 0xFFFFFFFFu) {
+        // Participating media between the ray origin and the surface
+        // (shaders/Shaders.hlsl:133-180).
+        bool vol_scattered = false;
+        if (!s.vols.empty()) {
+            float tsurf = h.prim != 0xFFFFFFFFu ? h.t : 1e20f;
+            V3f vw(1.0f);
+            float ts;
+            uint32_t vi;
+            bool sc = sample_volume_free_flight(s.vols, ray.o, ray.d, tsurf, rng, vw, ts, vi);
+            T = T * vw;
+            if (sc) {
+                vol_scattered = true;
+                const Volume &v = s.vols[vi];
+                V3f sig = v.sigma_t();
+                T = T * V3f(v.sigma_s.x / std::max(sig.x, 1e-20f), v.sigma_s.y / std::max(sig.y, 1e-20f),
+                            v.sigma_s.z / std::max(sig.z, 1e-20f));
+                V3f P = ray.o + ray.d * ts;
+                L += T * volume_nee_area(s, P, ray.d, v, rng, st);
+                L += T * volume_nee_env(s, P, ray.d, v, rng, p, st);
+                float xi1 = rng.next_float(), xi2 = rng.next_float(), ppdf;
+                V3f nd = hg_sample(ray.d, v.g, xi1, xi2, ppdf);
+                ray = mk_ray(P, nd, 0.0f, 1e20f);
+                last_pdf = std::max(ppdf, 1e-20f);
+            }
+        }
+        if (vol_scattered) {
+            // scatter event: on to Russian roulette
+        } else if (h.prim == 0xFFFFFFFFu) {
             if (s.env.valid()) {
                 V3f env = s.env.eval(ray.d);
                 if (last_pdf == 0.0f) L += T * env;
@@ -102,6 +203,7 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
             }
             break;
         }
+        if (!vol_scattered) { // surface interaction
         V3f P = ray.o + ray.d * h.t;
         uint32_t f = h.prim, mesh = s.tri_mesh[f];
         const MatRec &mr = s.mats[mesh];
@@ -221,30 +323,21 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
             V3f so = offset_ray_origin(P, Ng, Ng);
             // Area light NEE, shaders/Emitter.hlsli:43-89, 105-153
             if (E > 0) {
-                uint32_t pick = std::min((uint32_t)(rng.next_float() * (float)E), E - 1);
-                uint32_t em = s.emitters[pick];
-                uint32_t ntri = s.mesh_tri_count[em];
-                uint32_t k = std::min(cdf_sample(s.emit_cdf[pick], ntri + 1, rng.next_float()), ntri - 1);
-                uint32_t ef = s.mesh_tri_offset[em] + k;
-                V3f p0 = s.vtx(s.idx[3 * ef]), p1 = s.vtx(s.idx[3 * ef + 1]), p2 = s.vtx(s.idx[3 * ef + 2]);
-                float r1 = rng.next_float(), r2 = rng.next_float();
-                float sr1 = std::sqrt(r1);
-                V3f ep = (1.0f - sr1) * p0 + sr1 * (1.0f - r2) * p1 + sr1 * r2 * p2;
-                V3f en = normalize(cross(p1 - p0, p2 - p0));
-                float pdfA = 1.0f / (s.emit_area[pick] * (float)E);
-                V3f tl = ep - P;
+                LightSample ls = sample_area_light(s, rng);
+                uint32_t tr_seed = s.vols.empty() ? 0u : rng.next_uint();
+                V3f tl = ls.p - P;
                 float dist = length(tl);
                 V3f ww = tl / dist;
-                float ct = dot(N, ww), cl = dot(en, -ww);
+                float ct = dot(N, ww), cl = dot(ls.n, -ww);
                 V3f sso = hair ? offset_ray_origin(P, side(ww), side(ww)) : so;
                 if ((hair || ct > 0.0f) && cl > 0.0f && !occluded(s, sso, ww, dist - 0.001f, st)) {
                     V3f wol = to_local(ww, Tg, B, N);
                     V3f fv = material_eval_h(wi, wol, m, hh);
                     float pb = material_pdf_h(wi, wol, m, hh);
-                    float pe = pdfA * dist * dist / cl;
+                    float pe = ls.pdfA * dist * dist / cl;
                     float w = balance_heuristic(pe, pb);
-                    V3f Le(s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]);
-                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w);
+                    V3f tr = shadow_transmittance(s, sso, ww, dist, tr_seed);
+                    L += T * (ls.Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w) * tr;
                 }
             }
             // Envmap NEE, shaders/Emitter.hlsli:275-326
@@ -259,6 +352,7 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
                     float coin = rng.next_float(), jx = rng.next_float(), jy = rng.next_float();
                     s.env.sample_alias(r, coin, jx, jy, dir, Le, pe);
                 }
+                uint32_t tr_seed = s.vols.empty() ? 0u : rng.next_uint();
                 float ct = dot(N, dir);
                 V3f sso = hair ? offset_ray_origin(P, side(dir), side(dir)) : so;
                 if (pe > 0.0f && (hair || ct > 0.0f) && !occluded(s, sso, dir, 1e20f, st)) {
@@ -266,7 +360,8 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
                     V3f fv = material_eval_h(wi, wol, m, hh);
                     float pb = material_pdf_h(wi, wol, m, hh);
                     float w = balance_heuristic(pe, pb);
-                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w);
+                    V3f tr = shadow_transmittance(s, sso, dir, 1e20f, tr_seed);
+                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w) * tr;
                 }
             }
             float u[4];
@@ -283,6 +378,7 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
         } else {
             break;
         }
+        } // surface interaction
         // Russian roulette, shaders/Shaders.hlsl:620-626
         if (bounce >= p.rr_depth) {
             float q = std::min(maxc(T) * eta * eta, 0.95f);

@@ -20,6 +20,7 @@
 #include "orc_geom.h"
 #include "orc_hair.h"
 #include "orc_pcg32.h"
+#include "orc_volume.h"
 
 namespace orc {
 
@@ -43,6 +44,7 @@ struct SceneData {
     std::vector<uint32_t> emitters;            // mesh ids with isEmitter
     std::vector<std::vector<float>> emit_cdf;  // per emitter: Nori DiscretePDF cdf (tris+1)
     std::vector<float> emit_area;              // per emitter: total area
+    std::vector<Volume> vols;                  // participating media
 
     void finalize(); // BVH + emitter tables from the raw arrays
     V3f vtx(uint32_t i) const { return V3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]); }
