@@ -56,6 +56,22 @@ typedef struct nori_material {
     float emitterSelectionProb;
 } nori_material;
 
+/* Participating medium: an axis-aligned box with absorption / scattering
+ * coefficients and a Henyey-Greenstein phase function (the GPUVolume of
+ * src/gpu/DXRApp.h:49-67 / shaders/Volume.hlsl).  sigma_t(x) = (sigma_a +
+ * sigma_s) * density(x); density is 1 for homogeneous media, else a trilinear
+ * lookup (clamped) of a dense width x height x depth grid (x fastest) over the
+ * box.  Free flight uses per-brick majorants over bricks of `brick` voxels. */
+typedef struct nori_volume {
+    float vmin[3], vmax[3];
+    float sigma_a[3], sigma_s[3];
+    float g;
+    uint32_t heterogeneous;         /* 0: homogeneous (density 1)              */
+    uint32_t width, height, depth;  /* density grid; 0: none (density 1)       */
+    const float *density;           /* width * height * depth floats, or NULL  */
+    uint32_t brick;                 /* majorant brick edge in voxels, 0: 16    */
+} nori_volume;
+
 /* Flat scene, as src/gpu/DXRApp_Scene.cpp:172-451 packs it (SoA, one material
  * per mesh).  All arrays are copied by nori_gpu_upload_scene. */
 typedef struct nori_scene_desc {
@@ -71,7 +87,11 @@ typedef struct nori_scene_desc {
     uint32_t env_width, env_height; /* 0 = no environment map                  */
     const float *env_rgb;           /* env_height * env_width * 3, row 0 = +y  */
     float env_scale;
+    uint32_t num_volumes;           /* at most NORI_MAX_VOLUMES                */
+    const nori_volume *volumes;
 } nori_scene_desc;
+
+#define NORI_MAX_VOLUMES 8
 
 /* Nori's perspective camera (src/cpu/perspective.cpp:21-123): horizontal fov,
  * optional thin lens. */

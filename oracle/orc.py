@@ -32,7 +32,9 @@ def lib():
         L.orc_pcg32_sample_floats.argtypes = [u64, u32, u32, u32, vp]
         L.orc_pcg_hash.argtypes, L.orc_pcg_hash.restype = [u32], u32
         L.orc_scene_create.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, u32, u32, vp, ctypes.c_float, vp,
-                                       vp]
+                                       vp, u32, vp]
+        L.orc_volume_query.argtypes = [vp, i32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.orc_hg.argtypes = [i32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_scene_create.restype = vp
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_bvh_nodes.argtypes, L.orc_scene_bvh_nodes.restype = [vp], u32
@@ -88,6 +90,24 @@ def hair_fn(fn, a, b=None, c=None, d=None, e=None):
     return out
 
 
+def hg_eval(cos_theta, g):
+    c = np.ascontiguousarray(np.atleast_1d(cos_theta), np.float32)
+    gg = np.ascontiguousarray(np.broadcast_to(g, len(c)), np.float32)
+    out = np.zeros(len(c), np.float32)
+    lib().orc_hg(0, len(c), _p(c), _p(gg), None, None, None, _p(out))
+    return out
+
+
+def hg_sample(wo, g, xi1, xi2):
+    wo = np.ascontiguousarray(wo, np.float32)
+    n = len(wo)
+    gg = np.ascontiguousarray(np.broadcast_to(g, n), np.float32)
+    x1, x2 = (np.ascontiguousarray(x, np.float32) for x in (xi1, xi2))
+    d, pdf = np.zeros((n, 3), np.float32), np.zeros(n, np.float32)
+    lib().orc_hg(1, n, _p(wo), _p(gg), _p(x1), _p(x2), _p(d), _p(pdf))
+    return d, pdf
+
+
 def bsdf_batch(op, mats, wi, wo=None, u=None, precision=64, h=None):
     """op 0 eval, 1 pdf, 2 sample -> (rgb, pdf, wo) as float64 arrays.  u: n x 3 or
     n x 4 uniforms (hair takes 4); h: hair offsets."""
@@ -114,12 +134,19 @@ class OracleScene:
         L = lib()
         env = scene.env_rgb
         tri_mesh = scene.tri_mesh
-        self._keep = (scene, tri_mesh)
+        media = getattr(scene, "media", None) or []
+        vol_structs = None
+        if media:
+            import ctypes as _c
+            vol_structs = (type(media[0].struct()) * len(media))(*[m.struct() for m in media])
+        desc_vols = (len(media), None if vol_structs is None else ctypes.cast(vol_structs, ctypes.c_void_p))
+        self._keep = (scene, tri_mesh, vol_structs)
         self.h = L.orc_scene_create(
             len(scene.positions), _p(scene.positions), _p(scene.normals), len(scene.indices), _p(scene.indices),
             _p(tri_mesh), len(scene.materials), _p(scene.materials), _p(scene.mesh_tri_offset),
             _p(scene.mesh_tri_count), 0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
-            _p(env), scene.env_scale, _p(getattr(scene, "uvs", None)), _p(getattr(scene, "tangents", None)))
+            _p(env), scene.env_scale, _p(getattr(scene, "uvs", None)), _p(getattr(scene, "tangents", None)),
+            desc_vols[0], desc_vols[1])
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -153,6 +180,19 @@ class OracleScene:
         out = np.zeros((len(rays), 3), np.float32)
         lib().orc_li(self.h, len(rays), _p(rays), _p(seeds), max_depth, rr_depth, _p(out))
         return out
+
+    def volume_query(self, op, vi, o, d=None, tn=None, tf=None, seeds=None):
+        """Volume pieces (see orc_volume_query): op 0 free flight, 1 ratio
+        tracking, 2 density, 3 local majorant."""
+        o = np.ascontiguousarray(o, np.float32)
+        n = len(o)
+        d = None if d is None else np.ascontiguousarray(d, np.float32)
+        tn = None if tn is None else np.ascontiguousarray(np.broadcast_to(tn, n), np.float32)
+        tf = None if tf is None else np.ascontiguousarray(np.broadcast_to(tf, n), np.float32)
+        seeds = None if seeds is None else np.ascontiguousarray(seeds, np.uint64)
+        out3, t, flag = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32)
+        lib().orc_volume_query(self.h, op, vi, n, _p(o), _p(d), _p(tn), _p(tf), _p(seeds), _p(out3), _p(t), _p(flag))
+        return out3, t, flag
 
     def env_tables(self):
         H, W = self.scene.env_rgb.shape[:2]

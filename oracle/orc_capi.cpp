@@ -31,10 +31,18 @@ void orc_pcg32_sample_floats(uint64_t seed, uint32_t pixel, uint32_t sample, uin
 
 uint32_t orc_pcg_hash(uint32_t v) { return pcg_hash(v); }
 
+// Participating medium record (the layout of the scene description's volumes).
+struct VolDesc {
+    float vmin[3], vmax[3], sigma_a[3], sigma_s[3], g;
+    uint32_t heterogeneous, width, height, depth;
+    const float *density;
+    uint32_t brick;
+};
+
 void *orc_scene_create(uint32_t nv, const float *pos, const float *nrm, uint32_t nt, const uint32_t *idx,
                        const uint32_t *tri_mesh, uint32_t nmesh, const void *mats, const uint32_t *mesh_tri_offset,
                        const uint32_t *mesh_tri_count, uint32_t env_w, uint32_t env_h, const float *env_rgb,
-                       float env_scale, const float *uv, const float *tan) {
+                       float env_scale, const float *uv, const float *tan, uint32_t nvol, const void *vols) {
     Handle *h = new Handle();
     SceneData &s = h->s;
     s.nv = nv; s.nt = nt; s.nmesh = nmesh;
@@ -49,6 +57,23 @@ void *orc_scene_create(uint32_t nv, const float *pos, const float *nrm, uint32_t
     s.mesh_tri_offset.assign(mesh_tri_offset, mesh_tri_offset + nmesh);
     s.mesh_tri_count.assign(mesh_tri_count, mesh_tri_count + nmesh);
     if (env_w && env_h && env_rgb) s.env.build(env_w, env_h, env_rgb, env_scale);
+    for (uint32_t i = 0; i < nvol; ++i) {
+        const VolDesc &d = ((const VolDesc *)vols)[i];
+        Volume v;
+        v.vmin = V3f(d.vmin[0], d.vmin[1], d.vmin[2]);
+        v.vmax = V3f(d.vmax[0], d.vmax[1], d.vmax[2]);
+        v.sigma_a = V3f(d.sigma_a[0], d.sigma_a[1], d.sigma_a[2]);
+        v.sigma_s = V3f(d.sigma_s[0], d.sigma_s[1], d.sigma_s[2]);
+        v.g = d.g;
+        v.hetero = d.heterogeneous != 0;
+        v.brick = d.brick ? d.brick : 16;
+        if (v.hetero && d.density && d.width && d.height && d.depth) {
+            v.W = d.width; v.H = d.height; v.D = d.depth;
+            v.density.assign(d.density, d.density + (size_t)d.width * d.height * d.depth);
+            v.build_majorant();
+        }
+        s.vols.push_back(std::move(v));
+    }
     s.finalize();
     return h;
 }
@@ -91,6 +116,53 @@ void orc_render(void *hp, const Camera *cam, uint32_t spp, uint32_t sample_offse
     RenderStats st;
     render(((Handle *)hp)->s, *cam, p, accum, &st);
     if (stats3) { stats3[0] = st.samples; stats3[1] = st.extension_rays; stats3[2] = st.shadow_rays; }
+}
+
+// Volume pieces on scene volume `vi` for n rays (o3, d3, tmin/tmax):
+// op 0 free flight -> weight3, t, scattered; op 1 transmittance -> weight3;
+// op 2 density at o3; op 3 local majorant at o3 along d3 -> (mu, boundary).
+void orc_volume_query(void *hp, int op, uint32_t vi, uint32_t n, const float *o3, const float *d3, const float *tn,
+                      const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag) {
+    const SceneData &s = ((Handle *)hp)->s;
+    const Volume &v = s.vols[vi];
+    for (uint32_t i = 0; i < n; ++i) {
+        V3f o(o3[3 * i], o3[3 * i + 1], o3[3 * i + 2]), d(d3 ? d3[3 * i] : 0.f, d3 ? d3[3 * i + 1] : 0.f,
+                                                          d3 ? d3[3 * i + 2] : 0.f);
+        Pcg32 r;
+        r.seed(seeds ? seeds[i] : 0u, PCG32_DEFAULT_STREAM);
+        if (op == 0) {
+            V3f w(1.0f);
+            float ts;
+            bool sc = null_collision_free_flight(v, o, d, tn[i], tf[i], r, w, ts);
+            out3[3 * i] = w.x; out3[3 * i + 1] = w.y; out3[3 * i + 2] = w.z;
+            t_out[i] = ts;
+            flag[i] = sc ? 1 : 0;
+        } else if (op == 1) {
+            V3f w = v.hetero ? ratio_tracking(v, o, d, tn[i], tf[i], r) : V3f(0.0f);
+            out3[3 * i] = w.x; out3[3 * i + 1] = w.y; out3[3 * i + 2] = w.z;
+        } else if (op == 2) {
+            t_out[i] = v.density_at(o);
+        } else {
+            float mu, bd;
+            v.local_majorant(o, d, mu, bd);
+            out3[3 * i] = mu; out3[3 * i + 1] = bd; out3[3 * i + 2] = 0.0f;
+        }
+    }
+}
+
+// Henyey-Greenstein: op 0 eval(cos, g) -> out; op 1 sample(wo3, g, xi1, xi2) -> dir3, pdf.
+void orc_hg(int op, uint32_t n, const float *a, const float *g, const float *xi1, const float *xi2, float *dir3,
+            float *out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (op == 0) {
+            out[i] = hg_eval(a[i], g[i]);
+        } else {
+            float pdf;
+            V3f w = hg_sample(V3f(a[3 * i], a[3 * i + 1], a[3 * i + 2]), g[i], xi1[i], xi2[i], pdf);
+            dir3[3 * i] = w.x; dir3[3 * i + 1] = w.y; dir3[3 * i + 2] = w.z;
+            out[i] = pdf;
+        }
+    }
 }
 
 // Integrator::Li per caller ray: ray i runs path_mis with the pcg32 stream

@@ -52,12 +52,44 @@ def default_materials(n):
     return m
 
 
+class Volume(ctypes.Structure):
+    _fields_ = [("vmin", _f * 3), ("vmax", _f * 3), ("sigma_a", _f * 3), ("sigma_s", _f * 3), ("g", _f),
+                ("heterogeneous", _u32), ("width", _u32), ("height", _u32), ("depth", _u32),
+                ("density", ctypes.c_void_p), ("brick", _u32)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("num_vertices", _u32), ("num_triangles", _u32), ("num_meshes", _u32),
                 ("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("uvs", ctypes.c_void_p),
                 ("tangents", ctypes.c_void_p), ("indices", ctypes.c_void_p), ("mesh_tri_offset", ctypes.c_void_p),
                 ("mesh_tri_count", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("env_width", _u32),
-                ("env_height", _u32), ("env_rgb", ctypes.c_void_p), ("env_scale", _f)]
+                ("env_height", _u32), ("env_rgb", ctypes.c_void_p), ("env_scale", _f), ("num_volumes", _u32),
+                ("volumes", ctypes.c_void_p)]
+
+
+class MediumSpec:
+    """A participating medium for HostScene: box, coefficients, phase g and an
+    optional density grid (depth x height x width array, x fastest)."""
+
+    def __init__(self, vmin, vmax, sigma_a, sigma_s, g=0.0, density=None, brick=16):
+        self.vmin, self.vmax = [float(x) for x in vmin], [float(x) for x in vmax]
+        self.sigma_a = [float(x) for x in np.broadcast_to(sigma_a, 3)]
+        self.sigma_s = [float(x) for x in np.broadcast_to(sigma_s, 3)]
+        self.g = float(g)
+        self.density = None if density is None else np.ascontiguousarray(density, np.float32)
+        self.brick = int(brick)
+
+    def struct(self):
+        v = Volume()
+        v.vmin[:], v.vmax[:] = self.vmin, self.vmax
+        v.sigma_a[:], v.sigma_s[:] = self.sigma_a, self.sigma_s
+        v.g = self.g
+        v.heterogeneous = 1 if self.density is not None else 0
+        if self.density is not None:
+            v.depth, v.height, v.width = self.density.shape
+            v.density = self.density.ctypes.data
+        v.brick = self.brick
+        return v
 
 
 class Camera(ctypes.Structure):
@@ -146,7 +178,7 @@ class HostScene:
     """Flat scene arrays (numpy, owned by Python) in the nori_scene_desc layout."""
 
     def __init__(self, positions, indices, mesh_tri_offset, mesh_tri_count, materials, normals=None,
-                 env_rgb=None, env_scale=1.0, camera=None, uvs=None, tangents=None):
+                 env_rgb=None, env_scale=1.0, camera=None, uvs=None, tangents=None, media=None):
         self.positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
@@ -156,6 +188,7 @@ class HostScene:
         self.env_rgb = None if env_rgb is None else np.ascontiguousarray(env_rgb, np.float32)
         self.env_scale = float(env_scale)
         self.camera = camera
+        self.media = list(media or [])
         self.uvs = None if uvs is None else np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
         self.tangents = None if tangents is None else np.ascontiguousarray(tangents, np.float32).reshape(-1, 3)
 
@@ -171,6 +204,9 @@ class HostScene:
         d.num_vertices, d.num_triangles, d.num_meshes = len(self.positions), len(self.indices), len(self.materials)
         d.positions, d.normals = _ptr(self.positions), _ptr(self.normals)
         d.uvs, d.tangents = _ptr(self.uvs), _ptr(self.tangents)
+        self._vols = (Volume * max(1, len(self.media)))(*[m.struct() for m in self.media])
+        d.num_volumes = len(self.media)
+        d.volumes = ctypes.cast(self._vols, ctypes.c_void_p) if self.media else None
         d.indices = _ptr(self.indices)
         d.mesh_tri_offset, d.mesh_tri_count = _ptr(self.mesh_tri_offset), _ptr(self.mesh_tri_count)
         d.materials = _ptr(self.materials)
