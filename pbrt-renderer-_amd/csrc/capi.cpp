@@ -7,6 +7,7 @@
 #include "guide.h"
 #include "pt_kernels.h"
 #include "scene_gen.h"
+#include "volume.h"
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdio>
@@ -213,13 +214,13 @@ struct nori_ctx {
         paths.release(); queues.release(); shadow.release();
         alloc(paths, (size_t)n * 5 * sizeof(float4));
         alloc(queues, (size_t)n * 2 * sizeof(uint32_t));
-        alloc(shadow, (size_t)n * 2 * 3 * sizeof(float4));
+        alloc(shadow, (size_t)n * 2 * 4 * sizeof(float4));
         float4 *pp = (float4 *)paths.p;
         pdev = PathsDev{pp, pp + n, pp + 2 * (size_t)n, pp + 3 * (size_t)n, pp + 4 * (size_t)n};
         q[0] = (uint32_t *)queues.p;
         q[1] = q[0] + n;
         float4 *sp = (float4 *)shadow.p;
-        shdev = ShadowDev{sp, sp + 2 * (size_t)n, sp + 4 * (size_t)n};
+        shdev = ShadowDev{sp, sp + 2 * (size_t)n, sp + 4 * (size_t)n, sp + 6 * (size_t)n};
         cap = n;
     }
 };
@@ -473,6 +474,41 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
             s.env.scale = d->env_scale;
             s.has_env = 1;
         }
+        // Participating media: copies of the density grids and their majorants.
+        if (d->num_volumes > NORI_MAX_VOLUMES) throw ArgError("scene: too many volumes", NORI_E_LIMIT);
+        if (d->num_volumes && !d->volumes) throw ArgError("scene: null volume array");
+        std::vector<VolumeDev> vd(d->num_volumes);
+        for (uint32_t i = 0; i < d->num_volumes; ++i) {
+            const nori_volume &v = d->volumes[i];
+            VolumeDev &o = vd[i];
+            o = VolumeDev{};
+            o.vmin = mk(v.vmin[0], v.vmin[1], v.vmin[2]);
+            o.vmax = mk(v.vmax[0], v.vmax[1], v.vmax[2]);
+            o.sigma_a = mk(v.sigma_a[0], v.sigma_a[1], v.sigma_a[2]);
+            o.sigma_s = mk(v.sigma_s[0], v.sigma_s[1], v.sigma_s[2]);
+            o.g = v.g;
+            o.hetero = v.heterogeneous ? 1u : 0u;
+            for (int k = 0; k < 3; ++k)
+                if (!(v.vmax[k] > v.vmin[k])) throw ArgError("scene: empty volume box");
+            for (float c : {v.sigma_a[0], v.sigma_a[1], v.sigma_a[2], v.sigma_s[0], v.sigma_s[1], v.sigma_s[2]})
+                if (!(c >= 0.0f) || !std::isfinite(c)) throw ArgError("scene: volume coefficients must be >= 0");
+            if (!(v.g > -1.0f && v.g < 1.0f)) throw ArgError("scene: phase g must be in (-1, 1)");
+            if (v.heterogeneous && v.density && v.width && v.height && v.depth) {
+                size_t nvox = (size_t)v.width * v.height * v.depth;
+                for (size_t k = 0; k < nvox; ++k)
+                    if (!(v.density[k] >= 0.0f) || !std::isfinite(v.density[k]))
+                        throw ArgError("scene: volume densities must be finite and >= 0");
+                o.density = c->upload(v.density, nvox);
+                o.W = v.width;
+                o.H = v.height;
+                o.D = v.depth;
+                std::vector<float> maj;
+                build_majorant(v.width, v.height, v.depth, v.density, v.brick, o.mW, o.mH, o.mD, maj);
+                o.majorant = c->upload(maj.data(), maj.size());
+            }
+        }
+        s.n_vol = d->num_volumes;
+        s.vols = d->num_volumes ? c->upload(vd.data(), vd.size()) : nullptr;
         if ((size_t)s.stack_depth * 64 * sizeof(uint2) > 64 * 1024)
             throw ArgError("BVH8 deeper than the LDS stack allows", NORI_E_LIMIT);
         c->sdev = s;

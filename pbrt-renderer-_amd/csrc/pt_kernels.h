@@ -4,6 +4,7 @@
 #include "env_dev.h"
 #include "guide.h"
 #include "nori_math.h"
+#include "vol_dev.h"
 
 namespace ng {
 
@@ -26,6 +27,8 @@ struct SceneDev {
     int has_env;
     EnvDev env;
     f3 bmin, binv; // scene bounds: position -> [0,1]^3 for the guiding features
+    const VolumeDev *vols; // participating media (n_vol entries)
+    uint32_t n_vol;
 };
 
 // Guiding hooks of one render batch.  out: network outputs per queue position
@@ -47,11 +50,14 @@ struct PathsDev {
     float4 *aux;   // eta, pixel bits, rng state low/high bits
 };
 
-// Shadow-ray queue: origin+tmax, direction+pixel, contribution.
+// Shadow-ray queue: origin + occlusion tmax, direction + pixel, contribution +
+// guiding tag, and {tmin, transmittance range, transmittance seed bits, flags}
+// (flag bit 0: estimate the transmittance through the scene's media).
 struct ShadowDev {
     float4 *o;
     float4 *d;
     float4 *c;
+    float4 *e;
 };
 
 struct ViewDev {

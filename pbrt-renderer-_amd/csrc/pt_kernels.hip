@@ -194,25 +194,40 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
     if (COUNT ? (i - threadIdx.x % 64u) >= n : i >= n) return;
     uint32_t nn = 0, nt = 0;
     if (i < n) {
-        float4 a = sh.o[i], b = sh.d[i];
-        TraceOut r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), 0.0f, a.w, stk + threadIdx.x,
+        float4 a = sh.o[i], b = sh.d[i], e = sh.e[i];
+        TraceOut r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), e.x, a.w, stk + threadIdx.x,
                                                 blockDim.x, &nn, &nt);
         if (r.prim == MISS) {
             const float4 c = sh.c[i];
-            film_add(film, __float_as_uint(b.w), xyz(c));
-            guide_nee_add(g, __float_as_uint(c.w), xyz(c));
+            f3 col = xyz(c);
+            if (__float_as_uint(e.w) & 1u) {
+                // transmittance through the media, with the ray's own stream
+                uint64_t rr = pcg_seed((uint64_t)__float_as_uint(e.z), PCG_DEFAULT_STREAM);
+                col = col * vol_transmittance(s.vols, s.n_vol, xyz(a), xyz(b), e.y, rr);
+            }
+            film_add(film, __float_as_uint(b.w), col);
+            guide_nee_add(g, __float_as_uint(c.w), col);
         }
     }
     if (COUNT) count_add(stats, 6, nn, nt);
 }
 
-__device__ __forceinline__ void push_shadow(ShadowDev &sh, uint32_t *cnt, bool pred, f3 o, float tmax, f3 d,
-                                            uint32_t pixel, f3 c, uint32_t tag) {
+// One next-event shadow ray: occlusion over [tmin, tmax]; with media, the
+// transmittance over [0, tr_tmax] from stream seed(tr_seed).
+struct ShadowRec {
+    f3 o, d, c;
+    float tmin, tmax, tr_tmax;
+    uint32_t tr_seed, flags;
+};
+
+__device__ __forceinline__ void push_shadow(ShadowDev &sh, uint32_t *cnt, bool pred, const ShadowRec &r,
+                                            uint32_t pixel, uint32_t tag) {
     uint32_t k = wave_append(cnt, pred);
     if (pred) {
-        sh.o[k] = make_float4(o.x, o.y, o.z, tmax);
-        sh.d[k] = make_float4(d.x, d.y, d.z, __uint_as_float(pixel));
-        sh.c[k] = make_float4(c.x, c.y, c.z, __uint_as_float(tag));
+        sh.o[k] = make_float4(r.o.x, r.o.y, r.o.z, r.tmax);
+        sh.d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(pixel));
+        sh.c[k] = make_float4(r.c.x, r.c.y, r.c.z, __uint_as_float(tag));
+        sh.e[k] = make_float4(r.tmin, r.tr_tmax, __uint_as_float(r.tr_seed), __uint_as_float(r.flags));
     }
 }
 
@@ -286,6 +301,41 @@ __device__ void hair_frame(const SceneDev &s, uint32_t prim, float u, float v, f
     B = cross(N, Tg);
 }
 
+// Area-light point sample (shaders/Emitter.hlsli:43-89): uniform emitter, its
+// triangle by binary search on the area CDF, barycentrics (1 - sqrt r1,
+// sqrt r1 (1 - r2), sqrt r1 r2).  Draws 4 uniforms.
+struct LightPt {
+    f3 p, n, Le;
+    float pdfA;
+};
+__device__ __forceinline__ LightPt sample_light(const SceneDev &s, uint64_t &rng) {
+    uint32_t pick = min((uint32_t)(pcg_float(rng) * (float)s.n_emit), s.n_emit - 1);
+    uint32_t em = s.emitters[pick];
+    uint32_t ntri = s.mats[em].indexCount / 3;
+    const float *cdf = s.emit_cdf + s.emit_cdf_off[pick];
+    float uc = pcg_float(rng);
+    uint32_t lo = 0, hi = ntri;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= uc) lo = mid + 1;
+        else hi = mid;
+    }
+    uint32_t k = min((uint32_t)max(0, (int)lo - 1), ntri - 1);
+    uint32_t ef = s.mats[em].indexOffset / 3 + k;
+    uint32_t j0 = s.idx[3 * ef], j1 = s.idx[3 * ef + 1], j2 = s.idx[3 * ef + 2];
+    f3 q0 = f3{s.pos[3 * j0], s.pos[3 * j0 + 1], s.pos[3 * j0 + 2]};
+    f3 q1 = f3{s.pos[3 * j1], s.pos[3 * j1 + 1], s.pos[3 * j1 + 2]};
+    f3 q2 = f3{s.pos[3 * j2], s.pos[3 * j2 + 1], s.pos[3 * j2 + 2]};
+    float r1 = pcg_float(rng), r2 = pcg_float(rng);
+    float sr1 = sqrtf(r1);
+    LightPt lp;
+    lp.p = (1.0f - sr1) * q0 + sr1 * (1.0f - r2) * q1 + sr1 * r2 * q2;
+    lp.n = normalize(cross(q1 - q0, q2 - q0));
+    lp.pdfA = 1.0f / (s.emit_area[pick] * (float)s.n_emit);
+    lp.Le = f3{s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]};
+    return lp;
+}
+
 // One bounce of shaders/Shaders.hlsl:108-627 for every path in q_in.
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, const uint32_t *__restrict__ q_in,
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *q_out,
@@ -311,11 +361,70 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
         if (g.rec && finite3(c)) lpath += c;
     };
     f3 new_o = mk(0.0f), new_d = mk(0.0f);
-    f3 so = mk(0.0f), a_o = mk(0.0f), a_dir = mk(0.0f), a_c = mk(0.0f), e_o = mk(0.0f), e_dir = mk(0.0f),
-       e_c = mk(0.0f);
-    float a_tmax = 0.0f;
+    f3 so = mk(0.0f);
+    ShadowRec ra{}, re{}; // area-light and envmap shadow rays
+    const uint32_t tr_flag = s.n_vol ? 1u : 0u;
 
-    if (prim == MISS) {
+    // Participating media between the origin and the surface
+    // (shaders/Shaders.hlsl:133-180).
+    bool vol_scat = false;
+    if (s.n_vol) {
+        f3 vw = mk(1.0f);
+        float ts;
+        uint32_t vi;
+        bool sc = vol_sample_free_flight(s.vols, s.n_vol, o, d, prim != MISS ? hit.x : 1e20f, rng, vw, ts, vi);
+        T = T * vw;
+        if (sc) {
+            vol_scat = true;
+            const VolumeDev &v = s.vols[vi];
+            const f3 sig = vol_sigma_t(v);
+            T = T * mk(v.sigma_s.x / fmaxf(sig.x, 1e-20f), v.sigma_s.y / fmaxf(sig.y, 1e-20f),
+                       v.sigma_s.z / fmaxf(sig.z, 1e-20f));
+            const f3 P = o + d * ts;
+            const bool skip = vol_nee_skip(v, P);
+            // Volume NEE, shaders/Emitter.hlsli:334-434: phase function instead
+            // of BSDF and cosine, shadow rays from the scatter point (tmin 0.001).
+            if (!skip && s.n_emit > 0) {
+                LightPt lp = sample_light(s, rng);
+                uint32_t seed = pcg_next(rng);
+                f3 tl = lp.p - P;
+                float dist = length(tl);
+                f3 wi = tl / dist;
+                float cl = dot(lp.n, -wi);
+                if (cl > 0.0f) {
+                    float fp = hg_eval(dot(d, wi), v.g);
+                    float pe = lp.pdfA * dist * dist / cl;
+                    float w = balance_heuristic(pe, fp);
+                    ra = ShadowRec{P, wi, T * (lp.Le * (fp / fmaxf(pe, 1e-20f) * w)), 0.001f, dist - 0.001f, dist,
+                                   seed, 1u};
+                    sh_area = true;
+                }
+            }
+            if (!skip && s.has_env) {
+                uint32_t r = pcg_next(rng);
+                float coin = pcg_float(rng), jx = pcg_float(rng), jy = pcg_float(rng);
+                f3 dir, Le;
+                float pe;
+                env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
+                uint32_t seed = pcg_next(rng);
+                if (pe > 0.0f) {
+                    float fp = hg_eval(dot(d, dir), v.g);
+                    float w = balance_heuristic(pe, fp);
+                    re = ShadowRec{P, dir, T * (Le * (fp / fmaxf(pe, 1e-20f) * w)), 0.001f, 1e20f, 1e20f, seed, 1u};
+                    sh_env = true;
+                }
+            }
+            float xi1 = pcg_float(rng), xi2 = pcg_float(rng), ppdf;
+            new_d = hg_sample(d, v.g, xi1, xi2, ppdf);
+            new_o = P;
+            last_pdf = fmaxf(ppdf, 1e-20f);
+            alive = true;
+        }
+    }
+
+    if (vol_scat) {
+        // scattered in a medium: on to Russian roulette
+    } else if (prim == MISS) {
         if (s.has_env) {
             f3 env = env_eval(s.env, d);
             f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
@@ -404,44 +513,21 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 auto tech_pdf = [&](f3 w_world, float b) { return guided ? mix.c * guide_q(mix, w_world) + (1.0f - mix.c) * b : b; };
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
-                    uint32_t pick = min((uint32_t)(pcg_float(rng) * (float)s.n_emit), s.n_emit - 1);
-                    uint32_t em = s.emitters[pick];
-                    uint32_t ntri = s.mats[em].indexCount / 3;
-                    const float *cdf = s.emit_cdf + s.emit_cdf_off[pick];
-                    float uc = pcg_float(rng);
-                    uint32_t lo = 0, hi = ntri;
-                    while (lo < hi) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (cdf[mid] <= uc) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    uint32_t k = min((uint32_t)max(0, (int)lo - 1), ntri - 1);
-                    uint32_t ef = s.mats[em].indexOffset / 3 + k;
-                    uint32_t j0 = s.idx[3 * ef], j1 = s.idx[3 * ef + 1], j2 = s.idx[3 * ef + 2];
-                    f3 q0 = f3{s.pos[3 * j0], s.pos[3 * j0 + 1], s.pos[3 * j0 + 2]};
-                    f3 q1 = f3{s.pos[3 * j1], s.pos[3 * j1 + 1], s.pos[3 * j1 + 2]};
-                    f3 q2 = f3{s.pos[3 * j2], s.pos[3 * j2 + 1], s.pos[3 * j2 + 2]};
-                    float r1 = pcg_float(rng), r2 = pcg_float(rng);
-                    float sr1 = sqrtf(r1);
-                    f3 ep = (1.0f - sr1) * q0 + sr1 * (1.0f - r2) * q1 + sr1 * r2 * q2;
-                    f3 en = normalize(cross(q1 - q0, q2 - q0));
-                    float pdfA = 1.0f / (s.emit_area[pick] * (float)s.n_emit);
-                    f3 tl = ep - P;
+                    LightPt lp = sample_light(s, rng);
+                    uint32_t seed = s.n_vol ? pcg_next(rng) : 0u;
+                    f3 tl = lp.p - P;
                     float dist = length(tl);
                     f3 ww = tl / dist;
-                    float ct = dot(N, ww), cl = dot(en, -ww);
+                    float ct = dot(N, ww), cl = dot(lp.n, -ww);
                     if ((hair || ct > 0.0f) && cl > 0.0f) {
                         f3 f;
                         float pb;
                         material_eval_pdf(wi, to_local(ww, Tg, B, N), m, hh, f, pb);
                         pb = tech_pdf(ww, pb);
-                        float pe = pdfA * dist * dist / cl;
+                        float pe = lp.pdfA * dist * dist / cl;
                         float w = balance_heuristic(pe, pb);
-                        f3 Le = f3{s.mats[em].radiance[0], s.mats[em].radiance[1], s.mats[em].radiance[2]};
-                        a_c = T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w));
-                        a_o = origin(ww);
-                        a_dir = ww;
-                        a_tmax = dist - 0.001f;
+                        ra = ShadowRec{origin(ww), ww, T * (lp.Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w)), 0.0f,
+                                       dist - 0.001f, dist, seed, tr_flag};
                         sh_area = true;
                     }
                 }
@@ -452,6 +538,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                     f3 dir, Le;
                     float pe;
                     env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
+                    uint32_t seed = s.n_vol ? pcg_next(rng) : 0u;
                     float ct = dot(N, dir);
                     if (pe > 0.0f && (hair || ct > 0.0f)) {
                         f3 f;
@@ -459,9 +546,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         material_eval_pdf(wi, to_local(dir, Tg, B, N), m, hh, f, pb);
                         pb = tech_pdf(dir, pb);
                         float w = balance_heuristic(pe, pb);
-                        e_c = T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w));
-                        e_o = origin(dir);
-                        e_dir = dir;
+                        re = ShadowRec{origin(dir), dir, T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w)), 0.0f, 1e20f,
+                                       1e20f, seed, tr_flag};
                         sh_env = true;
                     }
                 }
@@ -533,8 +619,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
             }
         }
     }
-    push_shadow(sh, cnt_sh, sh_area, a_o, a_tmax, a_dir, pixel, a_c, tag);
-    push_shadow(sh, cnt_sh, sh_env, e_o, 1e20f, e_dir, pixel, e_c, tag);
+    push_shadow(sh, cnt_sh, sh_area, ra, pixel, tag);
+    push_shadow(sh, cnt_sh, sh_env, re, pixel, tag);
     if (g.rec) g.lpath[slot] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
 
     // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not

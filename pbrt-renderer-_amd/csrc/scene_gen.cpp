@@ -18,6 +18,12 @@
 //             and pheomelanin U[0, 0.5] per strand, a diffuse head and three
 //             quad lights; variant 1 (white furnace): colour 1 (sigma_a = 0),
 //             hair only, under a uniform unit envmap.
+//   config 4  heterogeneous medium: a grid^3 density (default 256) of seeded
+//             4-octave value-noise fBm, thresholded at 0.9 of its range,
+//             clamped >= 0 and scaled so that max sigma_t = 20 (albedo 0.9,
+//             sigma_s = 18, sigma_a = 2); Henyey-Greenstein g = 0, 0.3 or 0.8
+//             (variant 0, 1, 2); majorants on 16^3-voxel bricks; in an open
+//             diffuse box (floor and three walls) under the config-2 envmap.
 #include "scene_gen.h"
 #include <cmath>
 #include <cstring>
@@ -211,6 +217,112 @@ void hair_scene(Builder &b, Rng &rng, const nori_scene_options &opt, f3 eye) {
     }
 }
 
+// Sky gradient with a 0.5-degree sun disc of radiance 5e4 at a seeded
+// elevation (20-60 degrees) and azimuth; 2048 x 1024 by default.
+void sky_sun_env(SceneData &s, Rng &rng, const nori_scene_options &opt) {
+    uint32_t W = opt.env_width ? opt.env_width : 2048, H = opt.env_height ? opt.env_height : 1024;
+    float elev = (20.0f + 40.0f * rng.next()) * (PI / 180.0f), azim = 2.0f * PI * rng.next();
+    f3 sun = mk(std::cos(elev) * std::cos(azim), std::sin(elev), std::cos(elev) * std::sin(azim));
+    float cos_disc = std::cos(0.5f * PI / 180.0f);
+    s.env_w = W; s.env_h = H; s.env_scale = 1.0f;
+    s.env_rgb.resize((size_t)W * H * 3);
+    for (uint32_t y = 0; y < H; ++y) {
+        float theta = PI * ((float)y + 0.5f) / (float)H;
+        float ct = std::cos(theta), st = std::sin(theta);
+        f3 sky;
+        if (ct > 0.0f) sky = lerp3(mk(0.85f, 0.9f, 1.0f), mk(0.25f, 0.45f, 0.95f), std::sqrt(ct));
+        else sky = lerp3(mk(0.85f, 0.9f, 1.0f), mk(0.2f, 0.18f, 0.15f), std::sqrt(-ct));
+        for (uint32_t x = 0; x < W; ++x) {
+            float phi = 2.0f * PI * ((float)x + 0.5f) / (float)W;
+            f3 d = mk(st * std::cos(phi), ct, st * std::sin(phi));
+            f3 c = dot(d, sun) >= cos_disc ? mk(5e4f) : sky;
+            float *o = &s.env_rgb[((size_t)y * W + x) * 3];
+            o[0] = c.x; o[1] = c.y; o[2] = c.z;
+        }
+    }
+}
+
+// 32-bit PCG hash (Jarzynski and Olano 2020, "Hash Functions for GPU Rendering").
+uint32_t hash32(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+// Value noise: trilinear interpolation of per-lattice-point values in [0, 1)
+// from a hash of the lattice coordinates and the seed.
+float lattice(int x, int y, int z, uint32_t seed) {
+    uint32_t h = hash32((uint32_t)x * 73856093u ^ (uint32_t)y * 19349663u ^ (uint32_t)z * 83492791u ^ seed);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+float value_noise(float x, float y, float z, uint32_t seed) {
+    float fx = std::floor(x), fy = std::floor(y), fz = std::floor(z);
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    float tx = x - fx, ty = y - fy, tz = z - fz;
+    auto sm = [](float t) { return t * t * (3.0f - 2.0f * t); };
+    tx = sm(tx); ty = sm(ty); tz = sm(tz);
+    float c[2][2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 2; ++k) c[a][b][k] = lattice(ix + a, iy + b, iz + k, seed);
+    float x00 = c[0][0][0] + (c[1][0][0] - c[0][0][0]) * tx, x10 = c[0][1][0] + (c[1][1][0] - c[0][1][0]) * tx;
+    float x01 = c[0][0][1] + (c[1][0][1] - c[0][0][1]) * tx, x11 = c[0][1][1] + (c[1][1][1] - c[0][1][1]) * tx;
+    float y0 = x00 + (x10 - x00) * ty, y1 = x01 + (x11 - x01) * ty;
+    return y0 + (y1 - y0) * tz;
+}
+
+void medium_scene(SceneData &s, Builder &b, Rng &rng, const nori_scene_options &opt) {
+    const uint32_t N = opt.grid ? opt.grid : 256;
+    const uint32_t seed = hash32((uint32_t)(rng.next() * 16777216.0f));
+    std::vector<float> grid((size_t)N * N * N);
+    for (uint32_t z = 0; z < N; ++z)
+        for (uint32_t y = 0; y < N; ++y)
+            for (uint32_t x = 0; x < N; ++x) {
+                float px = ((float)x + 0.5f) / (float)N, py = ((float)y + 0.5f) / (float)N, pz = ((float)z + 0.5f) / (float)N;
+                float f = 0.0f, amp = 0.5f, freq = 4.0f;
+                for (int o = 0; o < 4; ++o) {
+                    f += amp * value_noise(px * freq, py * freq, pz * freq, seed + (uint32_t)o * 101u);
+                    amp *= 0.5f;
+                    freq *= 2.0f;
+                }
+                // soft falloff towards the box faces keeps the cloud inside
+                float e = std::min(std::min(std::min(px, 1 - px), std::min(py, 1 - py)), std::min(pz, 1 - pz));
+                grid[(size_t)x + (size_t)N * ((size_t)y + (size_t)N * z)] = f * std::min(1.0f, e * 8.0f);
+            }
+    float hi = 0.0f;
+    for (float v : grid) hi = std::max(hi, v);
+    const float thr = 0.9f * hi * 0.5f;
+    float top = 0.0f;
+    for (float &v : grid) {
+        v = std::max(0.0f, v - thr);
+        top = std::max(top, v);
+    }
+    for (float &v : grid) v = top > 0.0f ? v / top : 0.0f;
+    s.grids.push_back(std::move(grid));
+    nori_volume v;
+    std::memset(&v, 0, sizeof(v));
+    const float lo[3] = {-1, 0, -1}, up[3] = {1, 2, 1};
+    for (int k = 0; k < 3; ++k) {
+        v.vmin[k] = lo[k];
+        v.vmax[k] = up[k];
+        v.sigma_s[k] = 18.0f;
+        v.sigma_a[k] = 2.0f;
+    }
+    const float gs[3] = {0.0f, 0.3f, 0.8f};
+    v.g = gs[std::min(opt.variant, 2u)];
+    v.heterogeneous = 1;
+    v.width = v.height = v.depth = N;
+    v.density = s.grids.back().data();
+    v.brick = 16;
+    s.vols.push_back(v);
+    nori_material wall = default_material();
+    wall.albedo[0] = wall.albedo[1] = wall.albedo[2] = 0.6f;
+    b.quad(mk(-3, 0, 3), mk(6, 0, 0), mk(0, 0, -6), wall);  // floor, normal +y
+    b.quad(mk(-3, 0, -3), mk(6, 0, 0), mk(0, 3, 0), wall);  // back wall, normal +z
+    b.quad(mk(-3, 0, 3), mk(0, 0, -6), mk(0, 3, 0), wall);  // left wall, normal +x
+    b.quad(mk(3, 0, -3), mk(0, 0, 6), mk(0, 3, 0), wall);   // right wall, normal -x
+}
+
 void lookat(f3 eye, f3 target, f3 up, float *M) {
     f3 dir = normalize(target - eye);
     f3 left = normalize(cross(up, dir));
@@ -322,26 +434,7 @@ SceneData generate_scene(const nori_scene_options &opt) {
                 float x = ((float)i - 0.5f * (float)(g - 1)) * 1.0f, z = ((float)j - 0.5f * (float)(g - 1)) * 1.0f;
                 b.icosphere(mk(x, 0.4f, z), 0.4f, level, m);
             }
-        uint32_t W = opt.env_width ? opt.env_width : 2048, H = opt.env_height ? opt.env_height : 1024;
-        float elev = (20.0f + 40.0f * rng.next()) * (PI / 180.0f), azim = 2.0f * PI * rng.next();
-        f3 sun = mk(std::cos(elev) * std::cos(azim), std::sin(elev), std::cos(elev) * std::sin(azim));
-        float cos_disc = std::cos(0.5f * PI / 180.0f);
-        s.env_w = W; s.env_h = H; s.env_scale = 1.0f;
-        s.env_rgb.resize((size_t)W * H * 3);
-        for (uint32_t y = 0; y < H; ++y) {
-            float theta = PI * ((float)y + 0.5f) / (float)H;
-            float ct = std::cos(theta), st = std::sin(theta);
-            f3 sky;
-            if (ct > 0.0f) sky = lerp3(mk(0.85f, 0.9f, 1.0f), mk(0.25f, 0.45f, 0.95f), std::sqrt(ct));
-            else sky = lerp3(mk(0.85f, 0.9f, 1.0f), mk(0.2f, 0.18f, 0.15f), std::sqrt(-ct));
-            for (uint32_t x = 0; x < W; ++x) {
-                float phi = 2.0f * PI * ((float)x + 0.5f) / (float)W;
-                f3 d = mk(st * std::cos(phi), ct, st * std::sin(phi));
-                f3 c = dot(d, sun) >= cos_disc ? mk(5e4f) : sky;
-                float *o = &s.env_rgb[((size_t)y * W + x) * 3];
-                o[0] = c.x; o[1] = c.y; o[2] = c.z;
-            }
-        }
+        sky_sun_env(s, rng, opt);
         s.camera.fov_deg = 50.0f;
         lookat(mk(0, 4.5f, 9.5f), mk(0, 0.3f, 0), mk(0, 1, 0), s.camera.to_world);
         s.camera.width = opt.width ? opt.width : 512;
@@ -356,6 +449,13 @@ SceneData generate_scene(const nori_scene_options &opt) {
         }
         s.camera.fov_deg = 45.0f;
         lookat(eye, target, mk(0, 1, 0), s.camera.to_world);
+        s.camera.width = opt.width ? opt.width : 512;
+        s.camera.height = opt.height ? opt.height : 512;
+    } else if (cfg == 4) {
+        medium_scene(s, b, rng, opt);
+        sky_sun_env(s, rng, opt);
+        s.camera.fov_deg = 45.0f;
+        lookat(mk(0, 1.4f, 5.0f), mk(0, 0.9f, 0), mk(0, 1, 0), s.camera.to_world);
         s.camera.width = opt.width ? opt.width : 512;
         s.camera.height = opt.height ? opt.height : 512;
     } else {
@@ -386,6 +486,8 @@ void SceneData::fill_desc(nori_scene_desc &d) const {
     d.env_height = env_h;
     d.env_rgb = env_rgb.empty() ? nullptr : env_rgb.data();
     d.env_scale = env_scale;
+    d.num_volumes = (uint32_t)vols.size();
+    d.volumes = vols.empty() ? nullptr : vols.data();
 }
 
 } // namespace ng

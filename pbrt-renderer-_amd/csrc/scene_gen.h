@@ -13,6 +13,8 @@ struct SceneData {
     uint32_t env_w = 0, env_h = 0;
     float env_scale = 1.0f;
     std::vector<float> env_rgb;
+    std::vector<nori_volume> vols;           // density pointers into grids
+    std::vector<std::vector<float>> grids;   // one dense grid per heterogeneous volume
     nori_camera camera{};
 
     void finalize();

@@ -1,0 +1,262 @@
+// Participating media on the device (shaders/Volume.hlsl): null-collision free
+// flight with a hero channel over per-brick majorants, ratio-tracking
+// transmittance, Henyey-Greenstein phase function, and the NEE skip of
+// shaders/Emitter.hlsli:336-345.
+//
+// Layout: one VolumeDev per medium (at most NORI_MAX_VOLUMES); the density grid
+// is W x H x D floats (x fastest) looked up trilinearly with clamping (texel
+// centres at (i + 0.5) / W); the majorant grid holds, per brick, the maximum
+// density over every voxel a lookup inside the brick can touch (the brick's
+// voxels plus a one-voxel apron), so density / majorant <= 1.
+#pragma once
+#include "nori_math.h"
+
+namespace ng {
+
+struct VolumeDev {
+    f3 vmin, vmax, sigma_a, sigma_s;
+    float g;
+    uint32_t hetero;
+    const float *density; // null: density 1
+    uint32_t W, H, D;
+    const float *majorant; // null: the global majorant everywhere
+    uint32_t mW, mH, mD;
+};
+
+NG_HD f3 vol_sigma_t(const VolumeDev &v) { return v.sigma_a + v.sigma_s; }
+NG_HD float vol_sigma_t_max(const VolumeDev &v) { return maxc(vol_sigma_t(v)); }
+
+NG_HD float vol_density(const VolumeDev &v, f3 p) {
+    if (!v.hetero || !v.density) return 1.0f;
+    f3 size = mk(fmaxf(v.vmax.x - v.vmin.x, 1e-6f), fmaxf(v.vmax.y - v.vmin.y, 1e-6f), fmaxf(v.vmax.z - v.vmin.z, 1e-6f));
+    float u = (p.x - v.vmin.x) / size.x * (float)v.W - 0.5f;
+    float w1 = (p.y - v.vmin.y) / size.y * (float)v.H - 0.5f;
+    float w2 = (p.z - v.vmin.z) / size.z * (float)v.D - 0.5f;
+    float fu = floorf(u), fv = floorf(w1), fw = floorf(w2);
+    float tx = u - fu, ty = w1 - fv, tz = w2 - fw;
+    int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
+    const int W = (int)v.W, H = (int)v.H, D = (int)v.D;
+    int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
+    int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
+    int za = min(max(z0, 0), D - 1), zb = min(max(z0 + 1, 0), D - 1);
+    auto at = [&](int x, int y, int z) { return v.density[(size_t)x + (size_t)W * ((size_t)y + (size_t)H * z)]; };
+    float c00 = at(xa, ya, za) * (1 - tx) + at(xb, ya, za) * tx;
+    float c10 = at(xa, yb, za) * (1 - tx) + at(xb, yb, za) * tx;
+    float c01 = at(xa, ya, zb) * (1 - tx) + at(xb, ya, zb) * tx;
+    float c11 = at(xa, yb, zb) * (1 - tx) + at(xb, yb, zb) * tx;
+    float c0 = c00 * (1 - ty) + c10 * ty, c1 = c01 * (1 - ty) + c11 * ty;
+    return c0 * (1 - tz) + c1 * tz;
+}
+
+// shaders/Volume.hlsl:119-164
+NG_HD void vol_local_majorant(const VolumeDev &v, f3 pos, f3 dir, float &mu, float &bd) {
+    const float gmu = vol_sigma_t_max(v);
+    if (!v.hetero || !v.majorant) {
+        mu = gmu;
+        bd = 1e20f;
+        return;
+    }
+    f3 size = mk(fmaxf(v.vmax.x - v.vmin.x, 1e-6f), fmaxf(v.vmax.y - v.vmin.y, 1e-6f), fmaxf(v.vmax.z - v.vmin.z, 1e-6f));
+    const float mres[3] = {(float)v.mW, (float)v.mH, (float)v.mD};
+    const float sz[3] = {size.x, size.y, size.z}, lo[3] = {v.vmin.x, v.vmin.y, v.vmin.z};
+    const float p[3] = {pos.x, pos.y, pos.z}, d[3] = {dir.x, dir.y, dir.z};
+    int bc[3];
+    for (int i = 0; i < 3; ++i) {
+        float bf = (p[i] - lo[i]) / sz[i] * mres[i];
+        bc[i] = min(max((int)bf, 0), (int)mres[i] - 1);
+    }
+    mu = v.majorant[(size_t)bc[0] + (size_t)v.mW * ((size_t)bc[1] + (size_t)v.mH * bc[2])] * gmu;
+    float best = 1e20f;
+    for (int i = 0; i < 3; ++i) {
+        float di = 1e20f;
+        if (fabsf(d[i]) >= 1e-20f) {
+            float edge = (float)bc[i] + (d[i] > 0.0f ? 1.0f : 0.0f);
+            float world = lo[i] + edge * (sz[i] / mres[i]);
+            di = fmaxf((world - p[i]) / d[i], 1e-5f);
+        }
+        best = fminf(best, di);
+    }
+    bd = best;
+}
+
+// shaders/Volume.hlsl:14-27
+NG_HD bool vol_ray_aabb(f3 o, f3 d, f3 bmin, f3 bmax, float &tn, float &tf) {
+    f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    f3 t0 = mk((bmin.x - o.x) * inv.x, (bmin.y - o.y) * inv.y, (bmin.z - o.z) * inv.z);
+    f3 t1 = mk((bmax.x - o.x) * inv.x, (bmax.y - o.y) * inv.y, (bmax.z - o.z) * inv.z);
+    tn = fmaxf(fmaxf(fminf(t0.x, t1.x), fminf(t0.y, t1.y)), fminf(t0.z, t1.z));
+    tf = fminf(fminf(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y)), fmaxf(t0.z, t1.z));
+    return tf >= fmaxf(tn, 0.0f);
+}
+
+// shaders/Volume.hlsl:31-65
+NG_HD float hg_eval(float ct, float g) {
+    float g2 = g * g, den = 1.0f + g2 - 2.0f * g * ct;
+    return (1.0f / (4.0f * PI)) * (1.0f - g2) / (den * sqrtf(fmaxf(den, 1e-8f)));
+}
+NG_HD f3 hg_sample(f3 wo, float g, float xi1, float xi2, float &pdf) {
+    float ct;
+    if (fabsf(g) < 1e-3f) {
+        ct = 1.0f - 2.0f * xi1;
+    } else {
+        float sqr = (1.0f - g * g) / (1.0f + g - 2.0f * g * xi1);
+        ct = fminf(fmaxf((1.0f + g * g - sqr * sqr) / (2.0f * g), -1.0f), 1.0f);
+    }
+    float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct)), phi = 2.0f * PI * xi2;
+    f3 T, B;
+    build_onb(wo, T, B);
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    f3 wi = T * (st * cp) + B * (st * sp) + wo * ct;
+    pdf = hg_eval(ct, g);
+    return normalize(wi);
+}
+
+struct VolSeg {
+    uint32_t vi;
+    float tn, tf;
+};
+
+// shaders/Volume.hlsl:187-222
+NG_HD uint32_t vol_segments(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, VolSeg seg[8]) {
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < nv && n < 8; ++i) {
+        float tn, tf;
+        if (!vol_ray_aabb(o, d, vols[i].vmin, vols[i].vmax, tn, tf)) continue;
+        tn = fmaxf(tn, 0.0f);
+        tf = fminf(tf, tmax);
+        if (tf <= tn) continue;
+        uint32_t k = n;
+        for (; k > 0; --k) {
+            if (seg[k - 1].tn <= tn) break;
+            seg[k] = seg[k - 1];
+        }
+        seg[k] = VolSeg{i, tn, tf};
+        ++n;
+    }
+    return n;
+}
+
+// shaders/Volume.hlsl:236-319
+NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, uint64_t &rng, f3 &weight, float &ts) {
+    const float gmu = vol_sigma_t_max(v);
+    const f3 st = vol_sigma_t(v);
+    float t = tn;
+    for (uint32_t step = 0; step < 512; ++step) {
+        f3 pos = o + d * t;
+        float mu, bd;
+        vol_local_majorant(v, pos, d, mu, bd);
+        if (mu < 1e-8f) {
+            t += bd;
+            if (t >= tf) { ts = tf; return false; }
+            continue;
+        }
+        float xi = fmaxf(pcg_float(rng), 1e-10f);
+        float dt = -logf(xi) / mu;
+        if (dt > bd) {
+            t += bd;
+            if (t >= tf) { ts = tf; return false; }
+            continue;
+        }
+        t += dt;
+        if (t >= tf) { ts = tf; return false; }
+        float dens = vol_density(v, o + d * t);
+        float p_real = fminf(fmaxf((dens * gmu) / fmaxf(mu, 1e-20f), 0.0f), 1.0f);
+        if (p_real >= 1.0f - 1e-6f || pcg_float(rng) < p_real) {
+            float inv = 1.0f / fmaxf(gmu, 1e-20f);
+            weight = weight * mk(st.x * inv, st.y * inv, st.z * inv);
+            ts = t;
+            return true;
+        }
+        float den = fmaxf(mu * (1.0f - p_real), 1e-20f);
+        weight = weight * mk((mu - dens * st.x) / den, (mu - dens * st.y) / den, (mu - dens * st.z) / den);
+    }
+    ts = tf;
+    return false;
+}
+
+// shaders/Volume.hlsl:326-372
+NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, uint64_t &rng) {
+    const f3 st = vol_sigma_t(v);
+    f3 tr = mk(1.0f);
+    float t = tn;
+    for (uint32_t step = 0; step < 512; ++step) {
+        f3 pos = o + d * t;
+        float mu, bd;
+        vol_local_majorant(v, pos, d, mu, bd);
+        if (mu < 1e-8f) {
+            t += bd;
+            if (t >= tf) return tr;
+            continue;
+        }
+        float xi = fmaxf(pcg_float(rng), 1e-10f);
+        float dt = -logf(xi) / mu;
+        if (dt > bd) {
+            t += bd;
+            if (t >= tf) return tr;
+            continue;
+        }
+        t += dt;
+        if (t >= tf) return tr;
+        float dens = vol_density(v, o + d * t);
+        float inv = 1.0f / fmaxf(mu, 1e-20f);
+        tr = tr * mk((mu - dens * st.x) * inv, (mu - dens * st.y) * inv, (mu - dens * st.z) * inv);
+        if (maxc(tr) < 1e-6f) return mk(0.0f);
+    }
+    return tr;
+}
+
+// shaders/Volume.hlsl:381-415
+NG_HD bool vol_sample_free_flight(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, uint64_t &rng,
+                                  f3 &weight, float &ts, uint32_t &vi) {
+    ts = tmax;
+    vi = 0;
+    if (nv == 0) return false;
+    VolSeg seg[8];
+    uint32_t n = vol_segments(vols, nv, o, d, tmax, seg);
+    for (uint32_t k = 0; k < n; ++k) {
+        const VolumeDev &v = vols[seg[k].vi];
+        if (vol_sigma_t_max(v) < 1e-8f) continue;
+        float th;
+        if (vol_free_flight(v, o, d, seg[k].tn, seg[k].tf, rng, weight, th)) {
+            ts = th;
+            vi = seg[k].vi;
+            return true;
+        }
+    }
+    return false;
+}
+
+// shaders/Volume.hlsl:419-454
+NG_HD f3 vol_transmittance(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, uint64_t &rng) {
+    if (nv == 0) return mk(1.0f);
+    VolSeg seg[8];
+    uint32_t n = vol_segments(vols, nv, o, d, tmax, seg);
+    f3 tr = mk(1.0f);
+    for (uint32_t k = 0; k < n; ++k) {
+        const VolumeDev &v = vols[seg[k].vi];
+        f3 st = vol_sigma_t(v);
+        if (maxc(st) < 1e-8f) continue;
+        f3 s;
+        if (v.hetero) {
+            s = vol_ratio_tracking(v, o, d, seg[k].tn, seg[k].tf, rng);
+        } else {
+            float len = seg[k].tf - seg[k].tn;
+            s = mk(expf(-st.x * len), expf(-st.y * len), expf(-st.z * len));
+        }
+        tr = tr * s;
+        if (maxc(tr) < 1e-6f) return mk(0.0f);
+    }
+    return tr;
+}
+
+// shaders/Emitter.hlsli:336-345
+NG_HD bool vol_nee_skip(const VolumeDev &v, f3 p) {
+    f3 st = vol_sigma_t(v);
+    float stmin = fminf(st.x, fminf(st.y, st.z));
+    float md = fminf(fminf(fminf(p.x - v.vmin.x, p.y - v.vmin.y), fminf(p.z - v.vmin.z, v.vmax.x - p.x)),
+                     fminf(v.vmax.y - p.y, v.vmax.z - p.z));
+    return md * stmin > 8.0f;
+}
+
+} // namespace ng

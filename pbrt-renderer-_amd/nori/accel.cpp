@@ -7,8 +7,10 @@
 #include "bsdf.h"
 #include "emitter.h"
 #include "gpu.h"
+#include "medium.h"
 
 #include <cstring>
+#include <fstream>
 
 NORI_NAMESPACE_BEGIN
 
@@ -33,6 +35,55 @@ void Accel::setEnvironment(uint32_t width, uint32_t height, std::vector<float> r
     m_envH = height;
     m_env = std::move(rgb);
     m_envScale = scale;
+}
+
+namespace {
+// VOL1 density file (src/gpu/DXRApp_Scene.cpp:583-615): "VOL1", W, H, D
+// (uint32), a bounding box (6 floats, not used: the medium's bounds place the
+// grid), then W * H * D floats, x fastest.
+void readVol1(const std::string &path, uint32_t &W, uint32_t &H, uint32_t &D, std::vector<float> &data) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw NoriException("Unable to open volume file \"%s\"", path);
+    char magic[4];
+    f.read(magic, 4);
+    if (!f || std::memcmp(magic, "VOL1", 4) != 0) throw NoriException("\"%s\" is not a VOL1 volume", path);
+    uint32_t dims[3];
+    float box[6];
+    f.read(reinterpret_cast<char *>(dims), 12);
+    f.read(reinterpret_cast<char *>(box), 24);
+    W = dims[0];
+    H = dims[1];
+    D = dims[2];
+    if (!f || W == 0 || H == 0 || D == 0 || (uint64_t)W * H * D > (1ull << 31))
+        throw NoriException("\"%s\": bad VOL1 dimensions", path);
+    data.resize((size_t)W * H * D);
+    f.read(reinterpret_cast<char *>(data.data()), (std::streamsize)(data.size() * 4));
+    if (!f) throw NoriException("\"%s\": truncated VOL1 data", path);
+}
+} // namespace
+
+void Accel::setMedium(const Medium *medium) {
+    m_vols.clear();
+    m_volGrid.clear();
+    if (!medium) return;
+    BoundingBox3f box = medium->hasExplicitBounds() ? medium->getExplicitBounds() : m_bbox;
+    Color3f sa = medium->getSigmaARGB(), ss = medium->getSigmaSRGB();
+    nori_volume v;
+    std::memset(&v, 0, sizeof(v));
+    for (int k = 0; k < 3; ++k) {
+        v.vmin[k] = box.min[k];
+        v.vmax[k] = box.max[k];
+        v.sigma_a[k] = sa[k];
+        v.sigma_s[k] = ss[k];
+    }
+    v.g = medium->getPhaseG();
+    v.heterogeneous = medium->isHeterogeneous() ? 1u : 0u;
+    v.brick = 4;
+    if (medium->isHeterogeneous() && !medium->getVolumePath().empty()) {
+        readVol1(getFileResolver()->resolve(medium->getVolumePath()), v.width, v.height, v.depth, m_volGrid);
+        v.density = m_volGrid.data();
+    }
+    m_vols.push_back(v);
 }
 
 void Accel::build() {
@@ -102,6 +153,8 @@ void Accel::build() {
     m_desc.env_height = m_env.empty() ? 0 : m_envH;
     m_desc.env_rgb = m_env.empty() ? nullptr : m_env.data();
     m_desc.env_scale = m_envScale;
+    m_desc.num_volumes = (uint32_t)m_vols.size();
+    m_desc.volumes = m_vols.empty() ? nullptr : m_vols.data();
     m_uploaded = false;
 }
 

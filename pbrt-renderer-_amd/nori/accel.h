@@ -19,6 +19,11 @@ public:
     /// Equirectangular RGB environment (width x height x 3 floats, row 0 = +y),
     /// uploaded with the geometry; empty: none.
     void setEnvironment(uint32_t width, uint32_t height, std::vector<float> rgb, float scale);
+    /// The scene's participating medium (src/gpu/DXRApp_Scene.cpp:452-492 and
+    /// src/gpu/DXRApp_Textures.cpp:495-518): its box is the explicit bounds or
+    /// the scene's bounding box; a heterogeneous medium with a `volume` file
+    /// reads its VOL1 density grid (majorant bricks of 4 voxels).
+    void setMedium(const class Medium *medium);
     void build();
     void clear();
     const BoundingBox3f &getBoundingBox() const { return m_bbox; }
@@ -48,6 +53,8 @@ private:
     std::vector<float> m_pos, m_nrm, m_uv, m_tan, m_env;
     std::vector<uint32_t> m_idx, m_meshTriOffset, m_meshTriCount;
     std::vector<nori_material> m_mats;
+    std::vector<nori_volume> m_vols;
+    std::vector<float> m_volGrid;
     uint32_t m_envW = 0, m_envH = 0;
     float m_envScale = 1.0f;
     nori_scene_desc m_desc{};

@@ -40,11 +40,7 @@ public:
         if (m_rrDepth < 0) throw NoriException("%s: rrDepth must not be negative", name);
     }
 
-    void preprocess(const Scene *scene) override {
-        if (scene->getMedium())
-            throw NoriException("%s: participating media are not supported by this build's GPU path", m_name);
-        scene->getAccel()->context(); // upload
-    }
+    void preprocess(const Scene *scene) override { scene->getAccel()->context(); /* upload */ }
 
     Color3f Li(const Scene *scene, Sampler *sampler, const Ray3f &ray) const override {
         Color3f out;

@@ -45,6 +45,7 @@ void Scene::activate() {
         env.rgb = {0.5f, 0.5f, 0.5f};
     }
     m_accel->setEnvironment(env.width, env.height, std::move(env.rgb), m_envmapScale);
+    m_accel->setMedium(m_medium);
     m_accel->build();
     if (!m_integrator) cerr << "Warning: No integrator was specified." << endl;
 }

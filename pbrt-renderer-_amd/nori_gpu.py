@@ -261,8 +261,15 @@ def _host_scene_from_desc(d, cam):
     env = None
     if d.env_width:
         env = arr(d.env_rgb, 3 * d.env_width * d.env_height, np.float32).reshape(d.env_height, d.env_width, 3)
+    media = []
+    for i in range(d.num_volumes):
+        v = Volume.from_address(d.volumes + i * ctypes.sizeof(Volume))
+        dens = None
+        if v.heterogeneous and v.density and v.width:
+            dens = arr(v.density, v.width * v.height * v.depth, np.float32).reshape(v.depth, v.height, v.width)
+        media.append(MediumSpec(list(v.vmin), list(v.vmax), list(v.sigma_a), list(v.sigma_s), v.g, dens, v.brick))
     return HostScene(pos, idx, mto, mtc, mats, normals=nrm, env_rgb=env, env_scale=d.env_scale, camera=cam,
-                     uvs=uvs, tangents=tans)
+                     uvs=uvs, tangents=tans, media=media)
 
 
 BVH8_NODE_DTYPE = np.dtype([("p", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),

@@ -2,6 +2,7 @@
 scene generation and validation work, and compute entry points fail cleanly
 without a GPU."""
 import ctypes
+import math
 import os
 import re
 
@@ -103,3 +104,18 @@ def test_scene_generation_is_seeded():
     c = ng.generate_scene(2, seed=99, subdiv=1, grid=2, env_width=32, env_height=16)
     assert a.materials.tobytes() == b.materials.tobytes()
     assert a.materials.tobytes() != c.materials.tobytes()
+
+
+def test_medium_config():
+    sc = ng.generate_scene(4, grid=32, env_width=64, env_height=32, variant=2)
+    assert len(sc.indices) == 8 and len(sc.media) == 1
+    m = sc.media[0]
+    assert m.density.shape == (32, 32, 32)
+    assert m.density.min() == 0.0 and m.density.max() == 1.0
+    assert 0.05 < (m.density > 0).mean() < 0.95
+    assert np.allclose(np.array(m.sigma_s) + np.array(m.sigma_a), 20.0)
+    assert np.allclose(np.array(m.sigma_s) / 20.0, 0.9)
+    assert math.isclose(m.g, 0.8, rel_tol=1e-6) and m.brick == 16
+    assert (sc.camera.width, sc.camera.height) == (512, 512)
+    # seeded: the same options give the same grid
+    assert np.array_equal(ng.generate_scene(4, grid=32, env_width=64, env_height=32).media[0].density, m.density)

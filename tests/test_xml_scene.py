@@ -279,3 +279,29 @@ def test_hdr_reader_flat_and_rle(tmp_path):
     for name, data in (("flat.hdr", flat), ("rle.hdr", rle)):
         (tmp_path / name).write_bytes(data)
         assert np.array_equal(ng.read_image(tmp_path / name), expect), name
+
+
+def test_media_packing(tmp_path):
+    (tmp_path / "m.obj").write_text(QUAD)
+    xml = """<scene><camera type='perspective'/>
+      <mesh type='obj'><string name='filename' value='m.obj'/></mesh>
+      <medium type='homogeneous'><color name='sigmaA' value='0.1 0.2 0.3'/><float name='sigmaS' value='0.5'/>
+        <float name='g' value='0.4'/></medium></scene>"""
+    hs = ng.XmlScene(text=xml, directory=str(tmp_path)).host_scene()
+    m = hs.media[0]
+    # no explicit bounds: the scene's bounding box (the quad, flat in y)
+    assert m.vmin == [-1, 0, -1] and m.vmax == [1, 0, 1]
+    assert np.allclose(m.sigma_a, [0.1, 0.2, 0.3]) and np.allclose(m.sigma_s, 0.5) and math.isclose(m.g, 0.4, rel_tol=1e-6)
+    assert m.density is None
+    # heterogeneous with a VOL1 grid: "VOL1", W, H, D, bbox (6 floats), W*H*D floats
+    grid = np.random.default_rng(0).random((3, 4, 5)).astype(np.float32)
+    with open(tmp_path / "d.vol", "wb") as f:
+        f.write(b"VOL1" + struct.pack("<3I", 5, 4, 3) + struct.pack("<6f", 0, 0, 0, 1, 1, 1) + grid.tobytes())
+    xml = """<scene><camera type='perspective'/>
+      <medium type='heterogeneous'><string name='volume' value='d.vol'/>
+        <point name='boundsMin' value='-1, 0, -2'/><point name='boundsMax' value='1, 3, 2'/></medium></scene>"""
+    hs = ng.XmlScene(text=xml, directory=str(tmp_path)).host_scene()
+    m = hs.media[0]
+    assert m.vmin == [-1, 0, -2] and m.vmax == [1, 3, 2] and m.brick == 4
+    assert np.allclose(m.sigma_s, 5.0) and math.isclose(m.g, 0.85, rel_tol=1e-6)
+    assert np.array_equal(m.density, grid)
