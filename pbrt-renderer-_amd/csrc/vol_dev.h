@@ -8,6 +8,9 @@
 // centres at (i + 0.5) / W); the majorant grid holds, per brick, the maximum
 // density over every voxel a lookup inside the brick can touch (the brick's
 // voxels plus a one-voxel apron), so density / majorant <= 1.
+//
+// Floating-point contraction is off in these functions, as in the CPU
+// restatement, so both evaluate the same roundings.
 #pragma once
 #include "nori_math.h"
 
@@ -27,6 +30,7 @@ NG_HD f3 vol_sigma_t(const VolumeDev &v) { return v.sigma_a + v.sigma_s; }
 NG_HD float vol_sigma_t_max(const VolumeDev &v) { return maxc(vol_sigma_t(v)); }
 
 NG_HD float vol_density(const VolumeDev &v, f3 p) {
+#pragma clang fp contract(off)
     if (!v.hetero || !v.density) return 1.0f;
     f3 size = mk(fmaxf(v.vmax.x - v.vmin.x, 1e-6f), fmaxf(v.vmax.y - v.vmin.y, 1e-6f), fmaxf(v.vmax.z - v.vmin.z, 1e-6f));
     float u = (p.x - v.vmin.x) / size.x * (float)v.W - 0.5f;
@@ -50,6 +54,7 @@ NG_HD float vol_density(const VolumeDev &v, f3 p) {
 
 // shaders/Volume.hlsl:119-164
 NG_HD void vol_local_majorant(const VolumeDev &v, f3 pos, f3 dir, float &mu, float &bd) {
+#pragma clang fp contract(off)
     const float gmu = vol_sigma_t_max(v);
     if (!v.hetero || !v.majorant) {
         mu = gmu;
@@ -81,6 +86,7 @@ NG_HD void vol_local_majorant(const VolumeDev &v, f3 pos, f3 dir, float &mu, flo
 
 // shaders/Volume.hlsl:14-27
 NG_HD bool vol_ray_aabb(f3 o, f3 d, f3 bmin, f3 bmax, float &tn, float &tf) {
+#pragma clang fp contract(off)
     f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     f3 t0 = mk((bmin.x - o.x) * inv.x, (bmin.y - o.y) * inv.y, (bmin.z - o.z) * inv.z);
     f3 t1 = mk((bmax.x - o.x) * inv.x, (bmax.y - o.y) * inv.y, (bmax.z - o.z) * inv.z);
@@ -91,10 +97,12 @@ NG_HD bool vol_ray_aabb(f3 o, f3 d, f3 bmin, f3 bmax, float &tn, float &tf) {
 
 // shaders/Volume.hlsl:31-65
 NG_HD float hg_eval(float ct, float g) {
+#pragma clang fp contract(off)
     float g2 = g * g, den = 1.0f + g2 - 2.0f * g * ct;
     return (1.0f / (4.0f * PI)) * (1.0f - g2) / (den * sqrtf(fmaxf(den, 1e-8f)));
 }
 NG_HD f3 hg_sample(f3 wo, float g, float xi1, float xi2, float &pdf) {
+#pragma clang fp contract(off)
     float ct;
     if (fabsf(g) < 1e-3f) {
         ct = 1.0f - 2.0f * xi1;
@@ -139,6 +147,7 @@ NG_HD uint32_t vol_segments(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, floa
 
 // shaders/Volume.hlsl:236-319
 NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, uint64_t &rng, f3 &weight, float &ts) {
+#pragma clang fp contract(off)
     const float gmu = vol_sigma_t_max(v);
     const f3 st = vol_sigma_t(v);
     float t = tn;
@@ -177,6 +186,7 @@ NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, u
 
 // shaders/Volume.hlsl:326-372
 NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, uint64_t &rng) {
+#pragma clang fp contract(off)
     const f3 st = vol_sigma_t(v);
     f3 tr = mk(1.0f);
     float t = tn;
@@ -229,6 +239,7 @@ NG_HD bool vol_sample_free_flight(const VolumeDev *vols, uint32_t nv, f3 o, f3 d
 
 // shaders/Volume.hlsl:419-454
 NG_HD f3 vol_transmittance(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, uint64_t &rng) {
+#pragma clang fp contract(off)
     if (nv == 0) return mk(1.0f);
     VolSeg seg[8];
     uint32_t n = vol_segments(vols, nv, o, d, tmax, seg);
@@ -252,6 +263,7 @@ NG_HD f3 vol_transmittance(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float
 
 // shaders/Emitter.hlsli:336-345
 NG_HD bool vol_nee_skip(const VolumeDev &v, f3 p) {
+#pragma clang fp contract(off)
     f3 st = vol_sigma_t(v);
     float stmin = fminf(st.x, fminf(st.y, st.z));
     float md = fminf(fminf(fminf(p.x - v.vmin.x, p.y - v.vmin.y), fminf(p.z - v.vmin.z, v.vmax.x - p.x)),
