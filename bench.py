@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--depth", type=int, default=12)
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--batch", type=int, default=1 << 21)
+    ap.add_argument("--no-autotune", action="store_true", help="keep --batch and the default chunk length")
+    ap.add_argument("--tune-spp", type=int, default=4, help="spp of the autotuner's calibration renders")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--no-time-to-mse", action="store_true", help="skip the time-to-relMSE leg")
@@ -157,6 +159,14 @@ def main():
         f"{info.tri_bytes / 1e6:.1f} MB triangles), build {info.build_ms:.0f} ms, total {time.time() - t0:.1f}s")
     gpu.set_option("batch", args.batch)
     spp = args.spp_per_step
+    tune = None
+    if not args.no_autotune:
+        # Untimed: pool size and chunk length for this view, from short renders.
+        r = gpu.autotune(ng.make_view(cam, args.tune_spp, args.depth), calib_spp=args.tune_spp)
+        tune = {"pool": r.best_pool, "chunk_iters": r.best_chunk,
+                "candidates": [[r.pool[k], r.chunk[k], round(r.msamples_per_s[k], 1)] for k in range(r.count)]}
+        log(f"[bench] autotune: pool {r.best_pool}, chunk {r.best_chunk} "
+            f"({r.best_msamples_per_s:.1f} Msamples/s on the calibration renders)")
 
     # Traversal counts of this workload (instrumented kernels, untimed): average
     # BVH8 nodes and triangles per extension and per shadow ray.
@@ -172,7 +182,6 @@ def main():
     log(f"[bench] per extension ray {ext_nodes:.2f} nodes {ext_tris:.2f} tris; per shadow ray {sh_nodes:.2f} nodes "
         f"{sh_tris:.2f} tris")
 
-    gpu.set_option("timing_mode", 1)
     gpu.reset_film()
     for w in range(args.warmup):
         gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=(10 ** 6) + 64 + w * spp), seed=7,
@@ -180,6 +189,7 @@ def main():
         log(f"[bench] warmup step {w + 1}/{args.warmup} done")
     gpu.reset_film()
 
+    # Timed region: K render calls (graph-replayed bounce loop, no per-kernel events).
     acc = {"samples": 0, "ext": 0, "sh": 0, "ms_ext": 0.0, "ms_sh": 0.0, "ms_shade": 0.0, "launch_ext": 0,
            "ms_gpu": 0.0}
     t0 = time.perf_counter()
@@ -189,18 +199,30 @@ def main():
         acc["samples"] += s.samples
         acc["ext"] += s.extension_rays
         acc["sh"] += s.shadow_rays
+        acc["ms_gpu"] += s.ms_total
+        log(f"[bench] step {k + 1}/{args.steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s")
+    wall = time.perf_counter() - t0
+
+    # Kernel times: the same K steps again with HIP events around every launch
+    # on the renderer's stream (eager launches; events inside a graph are not used).
+    gpu.set_option("timing_mode", 1)
+    gpu.reset_film()
+    acc["ext_timed"] = 0
+    for k in range(args.steps):
+        gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=k * spp), seed=7, readback=False)
+        s = gpu.stats()
+        acc["ext_timed"] += s.extension_rays
         acc["ms_ext"] += s.ms_trace_closest
         acc["ms_sh"] += s.ms_trace_shadow
         acc["ms_shade"] += s.ms_shade
         acc["launch_ext"] += s.launches_closest
-        acc["ms_gpu"] += s.ms_total
-        log(f"[bench] step {k + 1}/{args.steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s "
-            f"(extend {s.ms_trace_closest:.1f} ms, shadow {s.ms_trace_shadow:.1f} ms, shade {s.ms_shade:.1f} ms)")
-    wall = time.perf_counter() - t0
+        log(f"[bench] timing pass {k + 1}/{args.steps}: extend {s.ms_trace_closest:.1f} ms, shadow "
+            f"{s.ms_trace_shadow:.1f} ms, shade {s.ms_shade:.1f} ms, total {s.ms_total:.1f} ms")
+    gpu.set_option("timing_mode", 0)
 
     value = acc["samples"] / wall / 1e6
 
-    ext_bytes = acc["ext"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
+    ext_bytes = acc["ext_timed"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
     achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
     traffic = traffic_from_profiles()
 
@@ -254,7 +276,8 @@ def main():
                         f"2048x1024 sky+sun envmap, {cam.width}x{cam.height}, {args.steps * spp} spp, "
                         f"max depth {args.depth}, plain path_mis",
             "spp_per_step": spp,
-            "batch_paths": args.batch,
+            "path_pool": tune["pool"] if tune else args.batch,
+            "autotune": tune,
         },
         "mrays_per_s": round((acc["ext"] + acc["sh"]) / wall / 1e6, 2),
         "gpu_ms": {"extend": round(acc["ms_ext"], 2), "shadow": round(acc["ms_sh"], 2),

@@ -92,6 +92,7 @@ typedef struct nori_scene_desc {
 } nori_scene_desc;
 
 #define NORI_MAX_VOLUMES 8
+#define NORI_MAX_VIEWS 4096
 
 /* Nori's perspective camera (src/cpu/perspective.cpp:21-123): horizontal fov,
  * optional thin lens. */
@@ -216,19 +217,47 @@ typedef struct nori_bvh_info {
 } nori_bvh_info;
 int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
 
-/* Options: "batch" (paths per wavefront batch, default 2^21), "timing_mode"
- * (0/1), "count_traversal" (0/1), "guiding" (0/1: guided sampling with one-sample
- * MIS and online training after every batch), "guide_train_steps" (Adam steps
- * per batch, default 4), "guide_train_batch" (samples per step, default 16384),
- * "guide_lr_micro" (learning rate x 1e6, default 2000), "guide_reset" (next use
- * re-initialises the network). */
+/* Options: "batch" (path slots of the persistent pool, default 2^21),
+ * "graph" (0/1: replay the bounce loop as a HIP graph of two iterations,
+ * default 1; timing_mode forces eager launches), "chunk_iters" (even, pool
+ * iterations between done-flag checks and between guide training rounds,
+ * default 8), "timing_mode" (0/1), "count_traversal" (0/1), "guiding" (0/1:
+ * guided sampling with one-sample MIS and online training, one round per
+ * pool's worth of samples started), "guide_train_steps" (Adam steps per
+ * training round, default 4),
+ * "guide_train_batch" (samples per step, default 16384), "guide_lr_micro"
+ * (learning rate x 1e6, default 2000), "guide_reset" (next use re-initialises
+ * the network). */
 int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
 
+/* ----------------------------------------------------------- autotuner */
+/* Picks the path-pool size (option "batch") and the iterations between
+ * done-flag checks (option "chunk_iters") for a set of views by timing short
+ * renders of them (calib_spp samples per pixel each, at sample offsets far
+ * above any the caller uses) for every candidate, and sets the fastest.  It
+ * has no counterpart in the reference (DXR sizes its own launches); it serves
+ * callers that render the same views repeatedly.  The film is cleared
+ * afterwards.  Candidates: pools 2^18 .. 2^22 (those no larger than four times
+ * the calibration's samples), then chunks 4 .. 32 at the best pool. */
+#define NORI_TUNE_MAX 16
+typedef struct nori_tune_result {
+    uint32_t best_pool, best_chunk;
+    double best_msamples_per_s;
+    uint32_t count;                           /* candidates timed            */
+    uint32_t pool[NORI_TUNE_MAX], chunk[NORI_TUNE_MAX];
+    double msamples_per_s[NORI_TUNE_MAX];
+} nori_tune_result;
+int nori_gpu_autotune(nori_ctx *ctx, const nori_view *views, int nviews, uint32_t calib_spp, nori_tune_result *out);
+
 /* ------------------------------------------------------- neural guiding */
-/* The guiding network of the context (no counterpart in the reference; see
- * DESIGN.md).  Parameters: NORI_GUIDE_PARAMS FP32 values, four layers of
- * W[64 out][64 in] followed by b[64].  Entry points take host arrays; features
- * are 64 floats per query, outputs 64 floats (41 used). */
+/* The guiding network of the context (no counterpart in the reference: an MLP
+ * 64 -> 64 -> 64 -> 64 -> 64 whose outputs parameterise an 8-lobe vMF
+ * mixture, trained online on the tracer's own radiance samples).
+ * Parameters: NORI_GUIDE_PARAMS FP32 values, four layers of W[64 out][64 in]
+ * followed by b[64].  Entry points take host arrays; features are 64 floats
+ * per query, outputs 64 floats (41 used).  nori_guide_forward runs the fused
+ * inference kernel the renderer uses; gradient and train run the per-layer
+ * forward that keeps the activations for the backward pass. */
 #define NORI_GUIDE_PARAMS (4 * (64 * 64 + 64))
 int nori_guide_params(nori_ctx *ctx, float *get, const float *set);
 int nori_guide_forward(nori_ctx *ctx, const float *feat, int n, float *out);

@@ -70,15 +70,44 @@ struct nori_ctx {
     bool has_scene = false;
     nori_bvh_info bvh_info{};
 
-    uint32_t cap = 0; // path capacity of the allocated state
+    uint32_t cap = 0;  // path capacity of the allocated state
+    uint32_t pool = 0; // path slots the current render uses (<= cap)
     DevBuf paths, queues, shadow, counters, stats;
-    PathsDev pdev{};
+    PathsDev pd[2]{};                      // the two path-state buffers (active queue by parity)
     ShadowDev shdev{};
-    uint32_t *q[2] = {nullptr, nullptr};
+    uint32_t *qf[2] = {nullptr, nullptr};  // free path-id lists
+    uint32_t *qc = nullptr;                // shading-class queues (3 x cap)
+    int sort_shading = 1;                  // option "sort": class-sorted shading launches
+    bool scene_has_hair = false;
     Counters cdev{};
 
-    std::vector<DevBuf> films;
-    std::vector<size_t> film_px;
+    // One film for all views of the last render, view after view.
+    DevBuf film;
+    std::vector<size_t> film_layout;
+    DevBuf views_buf;
+
+    // The bounce loop as a graph of two iterations (even, odd), kept while
+    // the buffers it was captured on stay the same.
+    struct GraphKey {
+        const void *paths, *film, *views, *guide_out, *guide_rec, *stats, *scene;
+        uint32_t cap;
+        bool operator==(const GraphKey &o) const {
+            return paths == o.paths && film == o.film && views == o.views && guide_out == o.guide_out &&
+                   guide_rec == o.guide_rec && stats == o.stats && scene == o.scene && cap == o.cap;
+        }
+    };
+    hipGraphExec_t graph = nullptr;
+    GraphKey graph_key{};
+    int use_graph = 1;
+    uint64_t scene_gen = 0; // bumped by every scene upload
+    uint32_t chunk_iters = 8; // iterations between done-flag checks (and training steps)
+    uint32_t *h_flags = nullptr; // pinned: done flags of the last two chunks
+    unsigned long long *h_claimed = nullptr; // pinned: samples claimed after the last two chunks
+    hipEvent_t flag_ev[2] = {nullptr, nullptr};
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
 
     uint32_t batch = 1u << 21;
     int timing_mode = 0, count_traversal = 0;
@@ -211,14 +240,20 @@ struct nori_ctx {
     }
     void ensure_paths(uint32_t n) {
         if (n <= cap) return;
+        drop_graph();
         paths.release(); queues.release(); shadow.release();
-        alloc(paths, (size_t)n * 5 * sizeof(float4));
-        alloc(queues, (size_t)n * 2 * sizeof(uint32_t));
+        const size_t per = 5 * sizeof(float4) + 2 * sizeof(uint32_t);
+        alloc(paths, (size_t)n * 2 * per);
+        alloc(queues, (size_t)n * 5 * sizeof(uint32_t));
         alloc(shadow, (size_t)n * 2 * 4 * sizeof(float4));
-        float4 *pp = (float4 *)paths.p;
-        pdev = PathsDev{pp, pp + n, pp + 2 * (size_t)n, pp + 3 * (size_t)n, pp + 4 * (size_t)n};
-        q[0] = (uint32_t *)queues.p;
-        q[1] = q[0] + n;
+        for (int b = 0; b < 2; ++b) {
+            float4 *pp = (float4 *)((char *)paths.p + (size_t)b * n * per);
+            uint32_t *pu = (uint32_t *)(pp + 5 * (size_t)n);
+            pd[b] = PathsDev{pp, pp + n, pp + 2 * (size_t)n, pp + 3 * (size_t)n, pp + 4 * (size_t)n, pu, pu + n};
+        }
+        qf[0] = (uint32_t *)queues.p;
+        qf[1] = qf[0] + n;
+        qc = qf[1] + n;
         float4 *sp = (float4 *)shadow.p;
         shdev = ShadowDev{sp, sp + 2 * (size_t)n, sp + 4 * (size_t)n, sp + 6 * (size_t)n};
         cap = n;
@@ -265,34 +300,68 @@ struct Span {
     }
 };
 
-// The bounce loop of one batch of n paths whose state k_raygen (or
-// k_raygen_rays) wrote: extend, [guide inference], shade, shadow per bounce.
-void run_bounces(nori_ctx *c, float4 *film, uint32_t n, uint32_t max_depth, uint32_t rr_depth,
-                 const GuideShadeDev &gsd, unsigned long long *cs, nori_render_stats &st) {
+// The bounce loop of one batch of n caller rays whose state k_raygen_rays
+// wrote (Integrator::Li): extend, shade, shadow per bounce; views holds the
+// one entry with the depth limits.
+void run_bounces(nori_ctx *c, float4 *film, const ViewTab *views, uint32_t n, uint32_t max_depth,
+                 nori_render_stats &st) {
+    const GuideShadeDev none{nullptr, nullptr, nullptr, nullptr, 0, 0};
     for (uint32_t b = 0; b < max_depth; ++b) {
         uint32_t qi = b & 1u, qo = qi ^ 1u;
-        {
-            Span t(c, 0);
-            launch_extend(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, n, cs, c->stream);
-        }
-        if (gsd.out) {
-            Span t(c, 3);
-            launch_guide_feats(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, (bf16_t *)c->g_act[0].p, n, c->stream);
-            c->guide_forward_dev(n, c->cdev.cnt + qi);
-        }
-        {
-            Span t(c, 2);
-            launch_shade(c->sdev, c->pdev, c->q[qi], c->cdev.cnt + qi, c->q[qo], c->cdev.cnt + qo, c->shdev,
-                         c->cdev.cnt + 2, film, b, max_depth, rr_depth, n, gsd, c->stream);
-        }
-        {
-            Span t(c, 1);
-            launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, film, 2 * n, cs, gsd, c->stream);
-        }
+        launch_extend(c->sdev, c->pd[qi], c->cdev.cnt + qi, n, nullptr, nullptr, nullptr, 0, c->stream);
+        launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, c->cdev.cnt + qi, c->cdev.cnt + qo, nullptr, nullptr,
+                     c->shdev, c->cdev.cnt + 2, film, n, none, c->stream);
+        launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, film, 2 * n, nullptr, none, c->stream);
         st.launches_closest++;
         st.launches_shadow++;
         launch_counters(c->cdev, qi, 1, (1u << qi) | 4u, c->stream);
     }
+}
+
+// One iteration of the persistent path pool over state buffer qi: refill
+// with new samples for the free ids (appended after the survivors), extend,
+// [guide inference], shade (survivors compacted into buffer qi^1, ended ids to
+// free list qi^1), shadow, [guide training samples of the ended paths],
+// counters.
+void pool_iteration(nori_ctx *c, uint32_t qi, float4 *film, const ViewTab *views, const GuideShadeDev &gsd,
+                    unsigned long long *cs) {
+    const uint32_t qo = qi ^ 1u, n = c->pool;
+    uint32_t *cnt = c->cdev.cnt;
+    {
+        Span t(c, 2);
+        launch_regen(c->sdev, c->pd[qi], views, c->cdev, c->qf[qi], cnt + 3 + qi, cnt + qi, film, n, gsd, c->stream);
+    }
+    // Scenes without media shade in class-sorted launches (a path's class is
+    // known once its hit is; with media, free flight decides first).
+    const bool sorted = c->sort_shading && c->sdev.n_vol == 0;
+    {
+        Span t(c, 0);
+        launch_extend(c->sdev, c->pd[qi], cnt + qi, n, cs, sorted ? c->qc : nullptr, cnt + 6, c->cap, c->stream);
+    }
+    if (gsd.out) {
+        Span t(c, 3);
+        launch_guide_feats(c->sdev, c->pd[qi], cnt + qi, (bf16_t *)c->g_act[0].p, n, c->stream);
+        guide_infer((const bf16_t *)c->g_act[0].p, c->gdev, (float *)c->g_out.p, n, cnt + qi, c->stream);
+    }
+    {
+        Span t(c, 2);
+        if (sorted)
+            launch_shade_sorted(c->sdev, c->pd[qi], c->pd[qo], views, c->qc, c->cap, cnt + 6, c->scene_has_hair,
+                                cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev, cnt + 2, film, n, gsd, c->stream);
+        else
+            launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, cnt + qi, cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev,
+                         cnt + 2, film, n, gsd, c->stream);
+    }
+    {
+        Span t(c, 1);
+        launch_shadow(c->sdev, c->shdev, cnt + 2, film, 2 * n, cs, gsd, c->stream);
+    }
+    if (gsd.rec) {
+        Span t(c, 4);
+        launch_guide_finalize(c->sdev, gsd, c->qf[qo], cnt + 3 + qo, n, (GuideSample *)c->g_samples.p,
+                              (uint32_t *)c->g_scount.p, 4 * c->g_rec_cap, c->stream);
+    }
+    launch_iter_end(c->cdev, qi, c->stream);
 }
 
 } // namespace
@@ -308,26 +377,30 @@ int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n,
         nori_render_stats st{};
         c->spans.clear();
         c->ev_used = 0;
-        launch_counters(c->cdev, 3, 0, 0x10u, c->stream);
-        ScopedBufs b(3);
+        launch_counters(c->cdev, 3, 0, 0x1000u, c->stream);
+        ScopedBufs b(4);
         size_t nr = (size_t)n;
         HIPCHK(hipMalloc(&b[0].p, nr * sizeof(nori_ray)));
         HIPCHK(hipMalloc(&b[1].p, nr * sizeof(uint64_t)));
         HIPCHK(hipMalloc(&b[2].p, nr * sizeof(float4)));
+        HIPCHK(hipMalloc(&b[3].p, sizeof(ViewTab)));
+        ViewTab vt{};
+        vt.max_depth = max_depth;
+        vt.rr_depth = rr_depth;
         HIPCHK(hipMemcpyAsync(b[0].p, rays, nr * sizeof(nori_ray), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(b[1].p, seeds, nr * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b[3].p, &vt, sizeof(ViewTab), hipMemcpyHostToDevice, c->stream));
         float4 *film = (float4 *)b[2].p;
         launch_zero_f4(film, nr, c->stream);
         uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, nr);
         c->ensure_paths(cap);
-        GuideShadeDev gsd{nullptr, nullptr, nullptr, nullptr, 0, 0};
         for (uint64_t first = 0; first < nr; first += cap) {
             uint32_t m = (uint32_t)std::min<uint64_t>(cap, nr - first);
-            launch_counters(c->cdev, 3, 0, 0x7u, c->stream);
+            launch_counters(c->cdev, 3, 0, 0x1FFu, c->stream);
             launch_set_counter(c->cdev.cnt, m, c->stream);
-            launch_raygen_rays(c->pdev, c->q[0], (const float4 *)b[0].p, (const uint64_t *)b[1].p, first, m, film,
+            launch_raygen_rays(c->pd[0], (const float4 *)b[0].p, (const uint64_t *)b[1].p, first, m, film,
                                c->stream);
-            run_bounces(c, film, m, max_depth, rr_depth, gsd, nullptr, st);
+            run_bounces(c, film, (const ViewTab *)b[3].p, m, max_depth, st);
         }
         std::vector<float4> out(nr);
         HIPCHK(hipMemcpyAsync(out.data(), film, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
@@ -351,11 +424,17 @@ int nori_gpu_create(nori_ctx **out, int device) {
     c->device = device;
     int rc = guard(c, [&] {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        c->alloc(c->counters, 4 * sizeof(uint32_t));
-        c->alloc(c->stats, 8 * sizeof(unsigned long long));
-        HIPCHK(hipMemset(c->counters.p, 0, 4 * sizeof(uint32_t)));
-        HIPCHK(hipMemset(c->stats.p, 0, 8 * sizeof(unsigned long long)));
-        c->cdev = Counters{(uint32_t *)c->counters.p, (unsigned long long *)c->stats.p};
+        c->alloc(c->counters, 16 * sizeof(uint32_t));
+        c->alloc(c->stats, 12 * sizeof(unsigned long long));
+        HIPCHK(hipMemset(c->counters.p, 0, 16 * sizeof(uint32_t)));
+        HIPCHK(hipMemset(c->stats.p, 0, 12 * sizeof(unsigned long long)));
+        c->cdev = Counters{(uint32_t *)c->counters.p, (unsigned long long *)c->stats.p,
+                           (unsigned long long *)c->stats.p + 8};
+        HIPCHK(hipHostMalloc((void **)&c->h_flags, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&c->h_claimed, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+        c->h_flags[0] = c->h_flags[1] = 0;
+        c->h_claimed[0] = c->h_claimed[1] = 0;
+        for (auto &e : c->flag_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     });
     if (rc != NORI_OK) {
         delete c;
@@ -376,7 +455,13 @@ void nori_gpu_destroy(nori_ctx *c) {
         b->release();
     for (auto &b : c->g_act) b.release();
     for (auto &b : c->g_dz) b.release();
-    for (auto &f : c->films) f.release();
+    c->drop_graph();
+    c->film.release();
+    c->views_buf.release();
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (c->h_claimed) (void)hipHostFree(c->h_claimed);
+    for (auto e : c->flag_ev)
+        if (e) (void)hipEventDestroy(e);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -432,6 +517,9 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         }
         for (uint32_t m = 0; m < d->num_meshes; ++m)
             mats[m].emitterSelectionProb = mats[m].isEmitter ? 1.0f / (float)emitters.size() : 0.0f;
+        c->scene_has_hair = false;
+        for (uint32_t m = 0; m < d->num_meshes; ++m)
+            if (!mats[m].isEmitter && mats[m].type == NORI_MAT_HAIR) c->scene_has_hair = true;
 
         Bvh8 bvh;
         build_bvh8(d->positions, d->indices, tri_mesh.data(), d->num_triangles, bvh);
@@ -513,6 +601,8 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
             throw ArgError("BVH8 deeper than the LDS stack allows", NORI_E_LIMIT);
         c->sdev = s;
         c->has_scene = true;
+        ++c->scene_gen;
+        c->drop_graph();
         c->bvh_info.nodes = (uint32_t)bvh.nodes.size();
         c->bvh_info.triangles = (uint32_t)bvh.tris.size();
         c->bvh_info.max_depth = bvh.max_depth;
@@ -527,7 +617,9 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
 int nori_gpu_reset_film(nori_ctx *c) {
     if (!c) return NORI_E_ARG;
     return guard(c, [&] {
-        for (size_t i = 0; i < c->films.size(); ++i) launch_zero_f4((float4 *)c->films[i].p, c->film_px[i], c->stream);
+        size_t npix = 0;
+        for (size_t x : c->film_layout) npix += x;
+        if (c->film.p) launch_zero_f4((float4 *)c->film.p, npix, c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
@@ -536,74 +628,145 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
     if (!c || !views || nviews <= 0) return NORI_E_ARG;
     return guard(c, [&] {
         if (!c->has_scene) throw ArgError("render: no scene uploaded", NORI_E_STATE);
+        if (nviews > NORI_MAX_VIEWS) throw ArgError("render: more than NORI_MAX_VIEWS views", NORI_E_LIMIT);
+        // The views' samples in one index space, their films one after another.
+        std::vector<ViewTab> vt((size_t)nviews);
+        std::vector<size_t> layout((size_t)nviews);
+        uint64_t total = 0, npix_all = 0;
+        uint32_t max_depth = 0;
+        for (int vi = 0; vi < nviews; ++vi) {
+            const nori_view &v = views[vi];
+            if (v.camera.width == 0 || v.camera.height == 0 || v.max_depth == 0)
+                throw ArgError("render: empty view");
+            if (v.max_depth > 0xFFFFu) throw ArgError("render: max_depth above 65535", NORI_E_LIMIT);
+            const uint64_t npix = (uint64_t)v.camera.width * v.camera.height;
+            layout[vi] = (size_t)npix;
+            vt[vi] = ViewTab{v.camera, v.spp, v.sample_offset, v.max_depth, v.rr_depth, (uint32_t)std::min<uint64_t>(npix, ~0u),
+                             (uint32_t)npix_all, (unsigned long long)total};
+            npix_all += npix;
+            if (npix_all > (1ull << 28)) throw ArgError("render: views larger than 2^28 pixels in all", NORI_E_LIMIT);
+            total += npix * v.spp;
+            max_depth = std::max(max_depth, v.max_depth);
+        }
+        if (layout != c->film_layout) {
+            c->film.release();
+            c->alloc(c->film, npix_all * sizeof(float4));
+            launch_zero_f4((float4 *)c->film.p, npix_all, c->stream);
+            c->film_layout = layout;
+        }
+        c->alloc(c->views_buf, std::max<size_t>((size_t)nviews, 16) * sizeof(ViewTab));
+        HIPCHK(hipMemcpyAsync(c->views_buf.p, vt.data(), vt.size() * sizeof(ViewTab), hipMemcpyHostToDevice,
+                              c->stream));
+        const unsigned long long hn[3] = {(unsigned long long)total, (unsigned long long)seed,
+                                          (unsigned long long)nviews};
+        HIPCHK(hipMemcpyAsync(c->cdev.next + 1, hn, sizeof(hn), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream)); // the host arrays above go out of scope
+
         nori_render_stats st{};
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         c->spans.clear();
         c->ev_used = 0;
-        launch_counters(c->cdev, 3, 0, 0x10u, c->stream); // zero the stats
+        launch_counters(c->cdev, 3, 0, 0x1000u, c->stream); // zero the stats
         HIPCHK(hipEventRecord(e0, c->stream));
         unsigned long long *cs = c->count_traversal ? c->cdev.stats : nullptr;
-        if ((int)c->films.size() < nviews) {
-            c->films.resize(nviews);
-            c->film_px.resize(nviews, 0);
-        }
-        for (int vi = 0; vi < nviews; ++vi) {
-            const nori_view &v = views[vi];
-            if (v.camera.width == 0 || v.camera.height == 0 || v.max_depth == 0)
-                throw ArgError("render: empty view");
-            size_t npix = (size_t)v.camera.width * v.camera.height;
-            if (npix > (1u << 28)) throw ArgError("render: view too large", NORI_E_LIMIT);
-            if (c->film_px[vi] != npix) {
-                c->films[vi].release();
-                c->alloc(c->films[vi], npix * sizeof(float4));
-                c->film_px[vi] = npix;
-                launch_zero_f4((float4 *)c->films[vi].p, npix, c->stream);
-            }
-            uint64_t total = (uint64_t)npix * v.spp;
-            uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, std::max<uint64_t>(total, 1));
+        float4 *film = (float4 *)c->film.p;
+        const ViewTab *dviews = (const ViewTab *)c->views_buf.p;
+        if (total > 0) {
+            const uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, (total + 255) / 256 * 256);
             c->ensure_paths(cap);
+            c->pool = cap;
             GuideShadeDev gsd{nullptr, nullptr, nullptr, nullptr, 0, 0};
             if (c->guiding) {
                 c->ensure_guide();
                 c->ensure_guide_batch(std::max(c->cap, c->g_train_batch));
                 c->ensure_guide_records(c->cap);
                 gsd = GuideShadeDev{(const float *)c->g_out.p, (GuideRecord *)c->g_rec.p, (uint32_t *)c->g_nrec.p,
-                                    (float4 *)c->g_lpath.p, c->cap, std::min(c->g_max_rec, v.max_depth)};
+                                    (float4 *)c->g_lpath.p, c->cap, std::min(c->g_max_rec, max_depth)};
+                launch_set_counter((uint32_t *)c->g_scount.p, 0, c->stream);
             }
-            ViewDev vd{v.camera, v.spp, v.sample_offset, v.max_depth, v.rr_depth, (uint32_t)npix,
-                       (float4 *)c->films[vi].p};
-            for (uint64_t first = 0; first < total; first += cap) {
-                uint32_t n = (uint32_t)std::min<uint64_t>(cap, total - first);
-                launch_counters(c->cdev, 3, 0, 0x7u, c->stream);
-                launch_set_counter(c->cdev.cnt, n, c->stream);
-                {
-                    Span t(c, 2);
-                    launch_raygen(c->sdev, c->pdev, c->q[0], c->cdev, vd, seed, first, n, gsd, c->stream);
+            const bool graphed = c->use_graph && !c->timing_mode;
+            if (graphed) {
+                nori_ctx::GraphKey key{c->paths.p, c->film.p, c->views_buf.p, gsd.out, gsd.rec, cs,
+                                       (const void *)(uintptr_t)c->scene_gen, c->pool};
+                if (!c->graph || !(key == c->graph_key)) {
+                    c->drop_graph();
+                    hipGraph_t g = nullptr;
+                    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+                    pool_iteration(c, 0, film, dviews, gsd, cs);
+                    pool_iteration(c, 1, film, dviews, gsd, cs);
+                    HIPCHK(hipStreamEndCapture(c->stream, &g));
+                    hipGraphExec_t ge = nullptr;
+                    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    HIPCHK(e);
+                    c->graph = ge;
+                    c->graph_key = key;
                 }
-                run_bounces(c, (float4 *)c->films[vi].p, n, v.max_depth, v.rr_depth, gsd, cs, st);
-                if (c->guiding && c->g_train_steps > 0) {
-                    // Online training on the radiance this batch's paths carried.
-                    Span t(c, 4);
-                    uint32_t *scount = (uint32_t *)c->g_scount.p;
-                    launch_set_counter(scount, 0, c->stream);
-                    launch_guide_finalize(c->sdev, gsd, n, (GuideSample *)c->g_samples.p, scount, 4 * c->cap,
-                                          c->stream);
-                    const uint32_t B = c->g_train_batch;
-                    for (uint32_t k = 0; k < c->g_train_steps; ++k) {
-                        launch_guide_gather((const GuideSample *)c->g_samples.p, scount, B, c->g_train_seed++,
-                                            (bf16_t *)c->g_act[0].p, (float4 *)c->g_ww.p, (float4 *)c->g_wib.p,
-                                            c->stream);
-                        c->guide_forward_dev(B, nullptr);
-                        c->guide_backward_dev(B, nullptr, 1.0f / (float)B);
-                        guide_adam(c->gdev, ++c->g_step, c->g_lr, 0.9f, 0.999f, 1e-8f, c->stream);
+            }
+            launch_pool_start(c->cdev, c->qf[0], c->pool, c->stream);
+            // Every slot runs at most max_depth iterations per sample and every
+            // slot frees up within max_depth iterations, so the pool drains
+            // within (samples / cap + 1) * max_depth iterations.
+            const uint64_t max_iters = 2 * ((total + c->pool - 1) / c->pool + 1) * ((uint64_t)max_depth + 1) + 64;
+            uint64_t iters = 0, trained = 0;
+            for (uint32_t k = 0;; ++k) {
+                for (uint32_t r = 0; r < c->chunk_iters; r += 2) {
+                    if (graphed) {
+                        HIPCHK(hipGraphLaunch(c->graph, c->stream));
+                    } else {
+                        pool_iteration(c, 0, film, dviews, gsd, cs);
+                        pool_iteration(c, 1, film, dviews, gsd, cs);
                     }
-                    st.guide_train_steps += c->g_train_steps;
                 }
+                iters += c->chunk_iters;
+                // Progress of the previous chunk (read while this one runs):
+                // done flag and samples claimed so far.
+                HIPCHK(hipMemcpyAsync(c->h_flags + (k & 1u), c->cdev.cnt + 5, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                      c->stream));
+                HIPCHK(hipMemcpyAsync(c->h_claimed + (k & 1u), c->cdev.next, sizeof(unsigned long long),
+                                      hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipEventRecord(c->flag_ev[k & 1u], c->stream));
+                bool done = false;
+                uint64_t claimed = 0;
+                if (k > 0) {
+                    HIPCHK(hipEventSynchronize(c->flag_ev[(k - 1) & 1u]));
+                    done = c->h_flags[(k - 1) & 1u] != 0;
+                    claimed = std::min<uint64_t>(c->h_claimed[(k - 1) & 1u], total);
+                }
+                if (c->guiding && c->g_train_steps > 0) {
+                    // Online training: one round of g_train_steps Adam steps per
+                    // pool's worth of samples started, on the samples of the
+                    // paths that ended since the last round, and a closing
+                    // round once every path has ended.
+                    uint64_t rounds = (claimed - trained) / c->pool;
+                    if (done && rounds == 0) rounds = 1; // the paths that ended since the last round
+                    trained = std::min<uint64_t>(trained + rounds * c->pool, claimed);
+                    if (rounds > 0) {
+                        Span t(c, 4);
+                        uint32_t *scount = (uint32_t *)c->g_scount.p;
+                        const uint32_t B = c->g_train_batch;
+                        const uint64_t steps = rounds * c->g_train_steps;
+                        for (uint64_t s2 = 0; s2 < steps; ++s2) {
+                            launch_guide_gather((const GuideSample *)c->g_samples.p, scount, 4 * c->g_rec_cap, B,
+                                                c->g_train_seed++,
+                                                (bf16_t *)c->g_act[0].p, (float4 *)c->g_ww.p, (float4 *)c->g_wib.p,
+                                                c->stream);
+                            c->guide_forward_dev(B, nullptr);
+                            c->guide_backward_dev(B, nullptr, 1.0f / (float)B);
+                            guide_adam(c->gdev, ++c->g_step, c->g_lr, 0.9f, 0.999f, 1e-8f, c->stream);
+                        }
+                        launch_set_counter(scount, 0, c->stream);
+                        st.guide_train_steps += steps;
+                    }
+                }
+                if (done) break;
+                if (iters > max_iters) throw HipError("render: the path pool did not drain");
             }
-            st.samples += total;
+            st.launches_closest = st.launches_shadow = iters;
         }
+        st.samples = total;
         HIPCHK(hipEventRecord(e1, c->stream));
         HIPCHK(hipEventSynchronize(e1));
         HIPCHK(hipGetLastError());
@@ -629,15 +792,67 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         c->last = st;
-        if (accum) {
-            size_t off = 0;
-            for (int vi = 0; vi < nviews; ++vi) {
-                HIPCHK(hipMemcpy(accum + 4 * off, c->films[vi].p, c->film_px[vi] * sizeof(float4),
-                                 hipMemcpyDeviceToHost));
-                off += c->film_px[vi];
-            }
-        }
+        if (accum) HIPCHK(hipMemcpy(accum, c->film.p, npix_all * sizeof(float4), hipMemcpyDeviceToHost));
     });
+}
+
+int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t calib_spp, nori_tune_result *out) {
+    if (!c || !views || nviews <= 0 || !out || calib_spp == 0) return NORI_E_ARG;
+    nori_tune_result r{};
+    const uint32_t batch0 = c->batch, chunk0 = c->chunk_iters;
+    std::vector<nori_view> cal(views, views + nviews);
+    uint64_t samples = 0;
+    for (auto &v : cal) {
+        v.spp = calib_spp;
+        v.sample_offset = 0x40000000u;
+        samples += (uint64_t)v.camera.width * v.camera.height * calib_spp;
+    }
+    // Time one candidate: a warm-up render, then the best of two.
+    auto time_it = [&](uint32_t pool, uint32_t chunk) -> double {
+        c->batch = pool;
+        c->chunk_iters = chunk;
+        double best = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            int rc = nori_gpu_render(c, cal.data(), nviews, 0x7E57u + k, nullptr);
+            if (rc != NORI_OK) throw ArgError(c->err, rc);
+            if (k > 0) best = std::max(best, (double)c->last.samples / std::max(c->last.ms_total, 1e-3) / 1e3);
+        }
+        if (r.count < NORI_TUNE_MAX) {
+            r.pool[r.count] = pool;
+            r.chunk[r.count] = chunk;
+            r.msamples_per_s[r.count] = best;
+            ++r.count;
+        }
+        if (best > r.best_msamples_per_s) {
+            r.best_msamples_per_s = best;
+            r.best_pool = pool;
+            r.best_chunk = chunk;
+        }
+        return best;
+    };
+    int rc = NORI_OK;
+    try {
+        for (uint32_t lg = 18; lg <= 22; ++lg) {
+            const uint32_t pool = 1u << lg;
+            if (lg > 18 && (uint64_t)pool > 4 * samples) break;
+            time_it(pool, chunk0);
+        }
+        const uint32_t bp = r.best_pool;
+        for (uint32_t chunk : {4u, 16u, 32u})
+            if (chunk != chunk0) time_it(bp, chunk);
+    } catch (const ArgError &e) {
+        rc = e.code;
+    }
+    if (rc != NORI_OK) {
+        c->batch = batch0;
+        c->chunk_iters = chunk0;
+        return rc;
+    }
+    c->batch = r.best_pool;
+    c->chunk_iters = r.best_chunk;
+    rc = nori_gpu_reset_film(c);
+    *out = r;
+    return rc;
 }
 
 int nori_gpu_render_stats(const nori_ctx *c, nori_render_stats *out) {
@@ -659,6 +874,15 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
     if (n == "batch") {
         if (value < 64 || value > (1ll << 26)) return fail(c, NORI_E_ARG, "batch out of range [64, 2^26]");
         c->batch = (uint32_t)value;
+    } else if (n == "graph") {
+        c->use_graph = value != 0;
+        c->drop_graph();
+    } else if (n == "sort") {
+        c->sort_shading = value != 0;
+        c->drop_graph();
+    } else if (n == "chunk_iters") {
+        if (value < 2 || value > 4096 || (value & 1)) return fail(c, NORI_E_ARG, "chunk_iters must be even, in [2, 4096]");
+        c->chunk_iters = (uint32_t)value;
     } else if (n == "timing_mode") {
         c->timing_mode = (int)value;
     } else if (n == "count_traversal") {
@@ -846,7 +1070,7 @@ int nori_guide_forward(nori_ctx *c, const float *feat, int n, float *out) {
     return guard(c, [&] {
         if (n == 0) return;
         guide_stage(c, feat, nullptr, nullptr, nullptr, n);
-        c->guide_forward_dev((uint32_t)n, nullptr);
+        guide_infer((const bf16_t *)c->g_act[0].p, c->gdev, (float *)c->g_out.p, (uint32_t)n, nullptr, c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(out, c->g_out.p, (size_t)n * 64 * sizeof(float), hipMemcpyDeviceToHost));

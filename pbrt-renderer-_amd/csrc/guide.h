@@ -164,6 +164,9 @@ NG_HD float guide_loss_dout(const float *o, f3 w, float b, float weight, float *
 // Launchers (guide.hip).  Row counts may live on the device (count != null:
 // rows = *count, grid sized for cap rows).
 void guide_refresh(const GuideDev &g, hipStream_t st);
+// Network outputs (FP32, 64 per row) of the BF16 features X through all four
+// layers in one launch (inference: no activations kept).
+void guide_infer(const bf16_t *X, const GuideDev &g, float *out, uint32_t cap, const uint32_t *count, hipStream_t st);
 // Y[rows x 64] = act(X[rows x 64] M^T (+ bias)), M[64 n][64 k] BF16.
 // mode 0: +bias, ReLU -> BF16 Y; mode 1: +bias -> FP32 Y; mode 2: * (Hmask > 0) -> BF16 Y.
 void guide_layer(const bf16_t *X, const bf16_t *M, const float *bias, void *Y, const bf16_t *Hmask, int mode,

@@ -41,13 +41,18 @@ struct GuideShadeDev {
     uint32_t cap, max_rec;
 };
 
-// Path state, structure of arrays indexed by path slot.
+// Path state, structure of arrays indexed by position in the active queue:
+// the live paths of an iteration occupy positions [0, count) of one of two
+// buffers, and shading moves the survivors, compacted, to the other.  `id`
+// is the path's stable number (its guiding records, the free list).
 struct PathsDev {
-    float4 *ray_o; // o.xyz, tmin
-    float4 *ray_d; // d.xyz, tmax
-    float4 *hit;   // t, prim bits, u, v
-    float4 *thr;   // throughput.xyz, pdf of the last BSDF sample (0 after delta / camera)
-    float4 *aux;   // eta, pixel bits, rng state low/high bits
+    float4 *ray_o;  // o.xyz, tmin
+    float4 *ray_d;  // d.xyz, tmax
+    float4 *hit;    // t, prim bits, u, v
+    float4 *thr;    // throughput.xyz, pdf of the last BSDF sample (0 after delta / camera)
+    float4 *aux;    // eta, film entry bits, rng state low/high bits
+    uint32_t *meta; // bounce (bits 0-15) | view (bits 16-31)
+    uint32_t *id;
 };
 
 // Shadow-ray queue: origin + occlusion tmax, direction + pixel, contribution +
@@ -60,48 +65,86 @@ struct ShadowDev {
     float4 *e;
 };
 
-struct ViewDev {
+// One view of a render on the device.  The views' samples form one global
+// index space: view v owns [first, first + npix * spp), its sample g maps to
+// pixel (g - first) % npix and sample index sample_offset + (g - first) / npix,
+// and its film occupies entries [film_off, film_off + npix) of the shared film.
+struct ViewTab {
     nori_camera cam;
     uint32_t spp, sample_offset, max_depth, rr_depth;
-    uint32_t npix;
-    float4 *film;
+    uint32_t npix, film_off;
+    unsigned long long first;
 };
 
-// Counters on the device: [0],[1] the two path queues, [2] shadow queue,
-// [3] unused; stats[0..7]: extension rays, shadow rays, unused, bounces,
-// extension nodes, extension triangles, shadow nodes, shadow triangles.
+// Counters on the device.
+//   cnt[0], cnt[1]: the two active path queues (ping-pong by iteration parity)
+//   cnt[2]:         shadow queue
+//   cnt[3], cnt[4]: the two free-slot queues
+//   cnt[5]:         done flag of the render (no samples left, no live paths)
+//   cnt[6..8]:      shading-class queues of the iteration (light, surface, hair)
+//   stats[0..7]:    extension rays, shadow rays, unused, bounce iterations,
+//                   extension nodes, extension triangles, shadow nodes, shadow
+//                   triangles
+//   next[0..3]:     next unclaimed global sample, total samples, render seed,
+//                   number of views
 struct Counters {
     uint32_t *cnt;
     unsigned long long *stats;
+    unsigned long long *next;
 };
 
 void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uint32_t n, int any_hit, hipStream_t st);
-void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counters c, const ViewDev &v,
-                   uint64_t seed, uint64_t first_sample, uint32_t n, const GuideShadeDev &g, hipStream_t st);
+// Path pool start: free list 0 = every path id, other queues empty, sample
+// counter at 0 (total, seed and view count are written by the host).
+void launch_pool_start(Counters c, uint32_t *q_free0, uint32_t cap, hipStream_t st);
+// Refill: every id in the free list claims the next global sample (one atomic
+// per wave) and its camera path is appended to the active buffer p.
+void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Counters c, const uint32_t *q_free,
+                  const uint32_t *cnt_free, uint32_t *cnt_act, float4 *film, uint32_t cap, const GuideShadeDev &g,
+                  hipStream_t st);
+// End of an iteration whose active buffer was qi: statistics, the counters of
+// parity qi cleared, done flag.
+void launch_iter_end(Counters c, uint32_t qi, hipStream_t st);
 // Paths from caller rays (2 float4 per ray: o.xyz mint, d.xyz maxt) with
-// per-ray pcg32 seeds; film entry first + i collects ray i's radiance.
-void launch_raygen_rays(const PathsDev &p, uint32_t *queue, const float4 *rays, const uint64_t *seeds, uint64_t first,
-                        uint32_t n, float4 *film, hipStream_t st);
+// per-ray pcg32 seeds at positions 0..n-1 of p; film entry first + i collects
+// ray i's radiance.
+void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *seeds, uint64_t first, uint32_t n,
+                        float4 *film, hipStream_t st);
 // count_stats (optional): stats array whose [4],[5] (extend) / [6],[7] (shadow)
 // receive the summed BVH8 nodes visited and triangles tested.
-void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, uint32_t cap,
-                   unsigned long long *count_stats, hipStream_t st);
-void launch_shade(const SceneDev &s, const PathsDev &p, const uint32_t *q_in, const uint32_t *cnt_in, uint32_t *q_out,
-                  uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film, uint32_t bounce,
-                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
+// q_cls (optional): sort the positions into the shading-class queues q_cls +
+// {0, 1, 2} * cls_stride with counts cls_cnt[0..2] (scenes without media).
+void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, uint32_t cap,
+                   unsigned long long *count_stats, uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cls_stride,
+                   hipStream_t st);
+// Shading of the paths at positions [0, *cnt_in) of p: survivors to p_out
+// (count cnt_out); with q_free, the ids of paths that ended to q_free.  Bounce
+// and view come from p.meta, the view's depth limits from views.
+void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
+                  const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
+                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
+// The same over the class queues k_extend sorted: one launch per class (the
+// hair launch only when `hair`), each compiled for its class alone.
+void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
+                         const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
+                         uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
+                         float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st);
-// Guiding: BF16 features of the hits in the queue (zeros where guiding does not apply).
-void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count,
-                        bf16_t *X, uint32_t cap, hipStream_t st);
-// Turn finished paths' records into training samples (appended at *count).
-void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, uint32_t npaths, GuideSample *samples,
-                           uint32_t *count, uint32_t max_samples, hipStream_t st);
-// A training batch of `rows` samples drawn (hashed indices) from samples[0..*count).
-void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint32_t rows, uint32_t seed, bf16_t *X,
-                         float4 *w_weight, float4 *wi_b, hipStream_t st);
+// Guiding: BF16 features of the hits at positions [0, *count) (zeros where
+// guiding does not apply).
+void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *count, bf16_t *X, uint32_t cap,
+                        hipStream_t st);
+// Turn the records of the paths whose ids are listed in ids[0..*nids) into
+// training samples (appended at *count).
+void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, const uint32_t *ids, const uint32_t *nids,
+                           uint32_t cap, GuideSample *samples, uint32_t *count, uint32_t max_samples, hipStream_t st);
+// A training batch of `rows` samples drawn (hashed indices) from
+// samples[0 .. min(*count, max_samples)).
+void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint32_t max_samples, uint32_t rows,
+                         uint32_t seed, bf16_t *X, float4 *w_weight, float4 *wi_b, hipStream_t st);
 // Adds the finished counts into stats and zeroes the queue counters in `clear`
-// (bit i -> cnt[i]; bit 4 zeroes all stats first).
+// (bit i -> cnt[i], i < 9; bit 12 zeroes all stats first).
 void launch_counters(Counters c, uint32_t add_ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st);
 void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st);
 void launch_zero_f4(float4 *p, size_t n, hipStream_t st);

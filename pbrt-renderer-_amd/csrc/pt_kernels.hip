@@ -16,6 +16,8 @@ namespace ng {
 namespace {
 
 constexpr uint32_t MISS = 0xFFFFFFFFu;
+// Shading classes of a hit (the queues k_extend sorts positions into).
+constexpr int CLS_ALL = 0, CLS_LIGHT = 1, CLS_SURF = 2, CLS_HAIR = 3;
 constexpr int TRACE_BLOCK = 64;
 constexpr int SHADE_BLOCK = 128;
 
@@ -87,41 +89,104 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_batch(SceneDev s, const f
     hits[i] = make_float4(r.t, __uint_as_float(prim), r.u, r.v);
 }
 
-__global__ void k_raygen(SceneDev s, PathsDev p, uint32_t *queue, ViewDev v, uint64_t seed, uint64_t first,
-                         uint32_t n, GuideShadeDev g) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t gi = first + i;
-    uint32_t pixel = (uint32_t)(gi % v.npix);
-    uint32_t sample = v.sample_offset + (uint32_t)(gi / v.npix);
-    uint64_t rng = sample_stream(seed, pixel, sample);
-    float jx = pcg_float(rng), jy = pcg_float(rng);
+// Path pool start: every slot free, queues empty, no sample claimed yet.
+__global__ void k_pool_start(Counters c, uint32_t *q_free0, uint32_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cap) q_free0[i] = i;
+    if (i == 0) {
+        for (int k = 0; k < 9; ++k) c.cnt[k] = 0;
+        c.cnt[3] = cap;
+        c.next[0] = 0;
+    }
+}
+
+// Refill from the global sample counter (one atomic per wave): every free path
+// id claims the next sample, and the new camera path is appended to the active
+// buffer.  A claimed sample g of view v starts exactly as a batch renderer
+// would start sample g: pixel (g - first) % npix, sample index sample_offset +
+// (g - first) / npix, stream sample_stream(seed, pixel, sample), so the image
+// does not depend on the position, id or iteration a sample runs in.
+__global__ __launch_bounds__(256) void k_regen(SceneDev s, PathsDev p, const ViewTab *__restrict__ views, Counters c,
+                                               const uint32_t *__restrict__ q_free, const uint32_t *__restrict__ cnt_free,
+                                               uint32_t *cnt_act, float4 *film, GuideShadeDev g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nfree = *cnt_free;
+    if (i - (threadIdx.x & 63u) >= nfree) return; // the whole wave is past the list
+    const unsigned long long total = c.next[1];
+    const bool want = i < nfree;
+    const uint64_t mask = __ballot(want); // lane 0 wants: the wave's first index is < nfree
+    const uint32_t lane = __lane_id(), leader = (uint32_t)__builtin_ctzll(mask);
+    uint32_t blo = 0, bhi = 0;
+    if (lane == leader) {
+        const unsigned long long b = atomicAdd(c.next, (unsigned long long)__popcll(mask));
+        blo = (uint32_t)b;
+        bhi = (uint32_t)(b >> 32);
+    }
+    blo = __shfl(blo, leader);
+    bhi = __shfl(bhi, leader);
+    const unsigned long long gidx =
+        (((unsigned long long)bhi << 32) | blo) + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+    const bool ok = want && gidx < total;
+    // New paths go to the end of the active buffer, after the survivors.
+    const uint32_t k = wave_append(cnt_act, ok);
+    if (!ok) return;
+    const uint32_t id = q_free[i];
+    // the view owning gidx: the last one whose first sample is <= gidx
+    uint32_t lo = 0, hi = (uint32_t)c.next[3];
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].first <= gidx) lo = mid;
+        else hi = mid;
+    }
+    const ViewTab &v = views[lo];
+    const unsigned long long l = gidx - v.first;
+    const uint32_t pixel = (uint32_t)(l % v.npix);
+    const uint32_t sample = v.sample_offset + (uint32_t)(l / v.npix);
+    uint64_t rng = sample_stream(c.next[2], pixel, sample);
+    const float jx = pcg_float(rng), jy = pcg_float(rng);
     float ax = 0.5f, ay = 0.5f;
     if (v.cam.lens_radius > 0.0f) {
         ax = pcg_float(rng);
         ay = pcg_float(rng);
     }
-    uint32_t x = pixel % v.cam.width, y = pixel / v.cam.width;
+    const uint32_t x = pixel % v.cam.width, y = pixel / v.cam.width;
     f3 o, d;
     float mint, maxt;
     camera_ray(v.cam, (float)x + jx, (float)y + jy, ax, ay, o, d, mint, maxt);
-    p.ray_o[i] = make_float4(o.x, o.y, o.z, mint);
-    p.ray_d[i] = make_float4(d.x, d.y, d.z, maxt);
-    p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p.aux[i] = make_float4(1.0f, __uint_as_float(pixel), __uint_as_float((uint32_t)rng),
+    const uint32_t fe = v.film_off + pixel;
+    p.ray_o[k] = make_float4(o.x, o.y, o.z, mint);
+    p.ray_d[k] = make_float4(d.x, d.y, d.z, maxt);
+    p.thr[k] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    p.aux[k] = make_float4(1.0f, __uint_as_float(fe), __uint_as_float((uint32_t)rng),
                            __uint_as_float((uint32_t)(rng >> 32)));
-    queue[i] = i;
+    p.meta[k] = lo << 16;
+    p.id[k] = id;
     if (g.rec) {
-        g.nrec[i] = 0;
-        g.lpath[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        g.nrec[id] = 0;
+        g.lpath[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    unsafeAtomicAdd(reinterpret_cast<float *>(v.film + pixel) + 3, 1.0f);
+    unsafeAtomicAdd(reinterpret_cast<float *>(film + fe) + 3, 1.0f);
 }
 
-// Paths from caller rays (Integrator::Li): ray i -> slot i, film entry first + i,
+// End of an iteration over active queue qi: statistics, the counters of parity
+// qi (active and free queue) and the shadow queue cleared for the next
+// iteration, done = every sample claimed and no path left for the next one.
+__global__ void k_iter_end(Counters c, uint32_t qi) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c.stats[0] += c.cnt[qi];
+    c.stats[1] += c.cnt[2];
+    c.stats[3] += 1;
+    c.cnt[qi] = 0;
+    c.cnt[2] = 0;
+    c.cnt[3 + qi] = 0;
+    c.cnt[6] = c.cnt[7] = c.cnt[8] = 0;
+    c.cnt[5] = (c.next[0] >= c.next[1] && c.cnt[qi ^ 1u] == 0) ? 1u : 0u;
+}
+
+// Paths from caller rays (Integrator::Li): ray i -> position i, film entry first + i,
 // pcg32 stream seed(seeds[first + i], PCG32_DEFAULT_STREAM).
-__global__ void k_raygen_rays(PathsDev p, uint32_t *queue, const float4 *__restrict__ rays,
-                              const uint64_t *__restrict__ seeds, uint64_t first, uint32_t n, float4 *film) {
+__global__ void k_raygen_rays(PathsDev p, const float4 *__restrict__ rays, const uint64_t *__restrict__ seeds,
+                              uint64_t first, uint32_t n, float4 *film) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t gi = first + i;
@@ -132,7 +197,8 @@ __global__ void k_raygen_rays(PathsDev p, uint32_t *queue, const float4 *__restr
     p.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     p.aux[i] = make_float4(1.0f, __uint_as_float((uint32_t)gi), __uint_as_float((uint32_t)rng),
                            __uint_as_float((uint32_t)(rng >> 32)));
-    queue[i] = i;
+    p.meta[i] = 0;
+    p.id[i] = i;
     film[gi].w += 1.0f;
 }
 
@@ -148,20 +214,37 @@ __device__ __forceinline__ void count_add(unsigned long long *stats, int k, uint
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, const uint32_t *__restrict__ queue,
-                                                        const uint32_t *__restrict__ count, unsigned long long *stats) {
+// Closest hit of the paths at positions [0, *count).  SORT: the position is
+// also appended to the queue of its shading class (light, surface, hair at
+// q_cls, q_cls + cap, q_cls + 2 cap; counts cls_cnt[0..2]).
+template <bool COUNT, bool SORT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
+                                                        unsigned long long *stats, uint32_t *q_cls,
+                                                        uint32_t *cls_cnt, uint32_t cap) {
     extern __shared__ uint2 stk[];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = *count;
-    if (COUNT ? (i - threadIdx.x % 64u) >= n : i >= n) return;
+    if ((COUNT || SORT) ? (i - threadIdx.x % 64u) >= n : i >= n) return;
     uint32_t nn = 0, nt = 0;
+    int cls = 0;
     if (i < n) {
-        uint32_t slot = queue[i];
-        float4 a = p.ray_o[slot], b = p.ray_d[slot];
+        float4 a = p.ray_o[i], b = p.ray_d[i];
         TraceOut r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x,
                                                  blockDim.x, &nn, &nt);
-        p.hit[slot] = make_float4(r.t, __uint_as_float(r.prim), r.u, r.v);
+        p.hit[i] = make_float4(r.t, __uint_as_float(r.prim), r.u, r.v);
+        if (SORT) {
+            cls = CLS_LIGHT;
+            if (r.prim != MISS) {
+                const nori_material &mr = s.mats[s.tri_mesh[r.prim]];
+                cls = mr.isEmitter ? CLS_LIGHT : (mr.type == NORI_MAT_HAIR ? CLS_HAIR : CLS_SURF);
+            }
+        }
+    }
+    if (SORT) {
+        for (int c = CLS_LIGHT; c <= CLS_HAIR; ++c) {
+            const uint32_t k = wave_append(cls_cnt + (c - 1), cls == c);
+            if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
+        }
     }
     if (COUNT) count_add(stats, 4, nn, nt);
 }
@@ -171,21 +254,21 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
 // not arrive along the sampled direction).
 __device__ __forceinline__ void guide_nee_add(const GuideShadeDev &g, uint32_t tag, f3 c) {
     if (!g.rec || !finite3(c)) return;
-    const uint32_t slot = tag & 0x7FFFFFFFu;
-    float *lp = reinterpret_cast<float *>(g.lpath + slot);
+    const uint32_t id = tag & 0x7FFFFFFFu;
+    float *lp = reinterpret_cast<float *>(g.lpath + id);
     atomicAdd(lp, c.x);
     atomicAdd(lp + 1, c.y);
     atomicAdd(lp + 2, c.z);
     if (tag & 0x80000000u) {
-        const uint32_t k = g.nrec[slot] - 1u;
-        float *ls = reinterpret_cast<float *>(&g.rec[(size_t)k * g.cap + slot].lsnap);
+        const uint32_t k = g.nrec[id] - 1u;
+        float *ls = reinterpret_cast<float *>(&g.rec[(size_t)k * g.cap + id].lsnap);
         atomicAdd(ls, c.x);
         atomicAdd(ls + 1, c.y);
         atomicAdd(ls + 2, c.z);
     }
 }
 
-template <bool COUNT>
+template <bool COUNT, bool VOL>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
                                                         float4 *film, unsigned long long *stats, GuideShadeDev g) {
     extern __shared__ uint2 stk[];
@@ -200,7 +283,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
         if (r.prim == MISS) {
             const float4 c = sh.c[i];
             f3 col = xyz(c);
-            if (__float_as_uint(e.w) & 1u) {
+            if (VOL && (__float_as_uint(e.w) & 1u)) {
                 // transmittance through the media, with the ray's own stream
                 uint64_t rr = pcg_seed((uint64_t)__float_as_uint(e.z), PCG_DEFAULT_STREAM);
                 col = col * vol_transmittance(s.vols, s.n_vol, xyz(a), xyz(b), e.y, rr);
@@ -337,15 +420,25 @@ __device__ __forceinline__ LightPt sample_light(const SceneDev &s, uint64_t &rng
 }
 
 // One bounce of shaders/Shaders.hlsl:108-627 for every path in q_in.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, const uint32_t *__restrict__ q_in,
-                                                       const uint32_t *__restrict__ cnt_in, uint32_t *q_out,
-                                                       uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film,
-                                                       uint32_t bounce, uint32_t max_depth, uint32_t rr_depth,
-                                                       GuideShadeDev g) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// VOL: the scene has media; GUIDE: guided sampling and training records.
+// CLS: which paths the launch shades.  CLS_ALL takes positions [0, *cnt_in);
+// the others take the positions k_extend sorted into their queue (q_cls), so
+// each launch carries only its class's code: misses and emitter hits
+// (CLS_LIGHT), scattering surfaces but hair (CLS_SURF), hair (CLS_HAIR).
+template <bool VOL, bool GUIDE, int CLS>
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, PathsDev p_out,
+                                                       const ViewTab *__restrict__ views,
+                                                       const uint32_t *__restrict__ q_cls,
+                                                       const uint32_t *__restrict__ cnt_in, uint32_t *cnt_out,
+                                                       uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
+                                                       uint32_t *cnt_sh, float4 *film, GuideShadeDev g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *cnt_in) return;
-    const uint32_t slot = q_in[i];
-    const float4 ro = p.ray_o[slot], rd = p.ray_d[slot], hit = p.hit[slot], th = p.thr[slot], ax = p.aux[slot];
+    const uint32_t pos = CLS == CLS_ALL ? i : q_cls[i];
+    const float4 ro = p.ray_o[pos], rd = p.ray_d[pos], hit = p.hit[pos], th = p.thr[pos], ax = p.aux[pos];
+    const uint32_t meta = p.meta[pos], id = p.id[pos];
+    const uint32_t bounce = meta & 0xFFFFu;
+    const uint32_t max_depth = views[meta >> 16].max_depth, rr_depth = views[meta >> 16].rr_depth;
     const f3 o = xyz(ro), d = xyz(rd);
     f3 T = xyz(th);
     float last_pdf = th.w, eta = ax.x;
@@ -354,21 +447,21 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
     const uint32_t prim = __float_as_uint(hit.y);
 
     bool alive = false, sh_area = false, sh_env = false;
-    uint32_t tag = slot;
-    f3 lpath = g.rec ? xyz(g.lpath[slot]) : mk(0.0f);
+    uint32_t tag = id;
+    f3 lpath = GUIDE && g.rec ? xyz(g.lpath[id]) : mk(0.0f);
     auto add_L = [&](f3 c) {
         film_add(film, pixel, c);
-        if (g.rec && finite3(c)) lpath += c;
+        if (GUIDE && g.rec && finite3(c)) lpath += c;
     };
     f3 new_o = mk(0.0f), new_d = mk(0.0f);
     f3 so = mk(0.0f);
     ShadowRec ra{}, re{}; // area-light and envmap shadow rays
-    const uint32_t tr_flag = s.n_vol ? 1u : 0u;
+    const uint32_t tr_flag = VOL && s.n_vol ? 1u : 0u;
 
     // Participating media between the origin and the surface
     // (shaders/Shaders.hlsl:133-180).
     bool vol_scat = false;
-    if (s.n_vol) {
+    if (VOL && s.n_vol) {
         f3 vw = mk(1.0f);
         float ts;
         uint32_t vi;
@@ -424,7 +517,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
 
     if (vol_scat) {
         // scattered in a medium: on to Russian roulette
-    } else if (prim == MISS) {
+    } else if ((CLS == CLS_ALL || CLS == CLS_LIGHT) && prim == MISS) {
         if (s.has_env) {
             f3 env = env_eval(s.env, d);
             f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
@@ -438,7 +531,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
         const f3 N_feat = N;
         const uint32_t mesh = s.tri_mesh[prim];
         const nori_material &mr = s.mats[mesh];
-        if (mr.isEmitter) {
+        const bool emitter = CLS == CLS_LIGHT ? true : (CLS == CLS_ALL ? mr.isEmitter != 0 : false);
+        if (emitter) {
             f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
             if (last_pdf == 0.0f) {
                 add_L(T * Le);
@@ -459,7 +553,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
             }
         } else {
             const uint32_t type = mr.type;
-            const bool hair = type == NORI_MAT_HAIR;
+            const bool hair = CLS == CLS_HAIR ? true : (CLS == CLS_ALL ? type == NORI_MAT_HAIR : false);
             f3 Tg, B;
             float hh = 0.0f;
             if (hair) hair_frame(s, prim, hit.z, hit.w, N, Tg, B, hh);
@@ -507,14 +601,14 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                 };
                 // One-sample MIS with the learned mixture: the BSDF technique's
                 // density is P(w) = c q(w) + (1-c) b(w) wherever it enters MIS.
-                const bool guided = g.out != nullptr && !hair;
+                const bool guided = GUIDE && g.out != nullptr && !hair;
                 GuideMix mix;
-                if (guided) mix = guide_decode(g.out + (size_t)i * 64);
+                if (guided) mix = guide_decode(g.out + (size_t)pos * 64);
                 auto tech_pdf = [&](f3 w_world, float b) { return guided ? mix.c * guide_q(mix, w_world) + (1.0f - mix.c) * b : b; };
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
                     LightPt lp = sample_light(s, rng);
-                    uint32_t seed = s.n_vol ? pcg_next(rng) : 0u;
+                    uint32_t seed = VOL && s.n_vol ? pcg_next(rng) : 0u;
                     f3 tl = lp.p - P;
                     float dist = length(tl);
                     f3 ww = tl / dist;
@@ -538,7 +632,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                     f3 dir, Le;
                     float pe;
                     env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
-                    uint32_t seed = s.n_vol ? pcg_next(rng) : 0u;
+                    uint32_t seed = VOL && s.n_vol ? pcg_next(rng) : 0u;
                     float ct = dot(N, dir);
                     if (pe > 0.0f && (hair || ct > 0.0f)) {
                         f3 f;
@@ -601,10 +695,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         }
                     }
                 }
-                if (alive && g.rec && !hair) {
-                    uint32_t k = g.nrec[slot];
+                if (GUIDE && alive && g.rec && !hair) {
+                    uint32_t k = g.nrec[id];
                     if (k < g.max_rec) {
-                        GuideRecord &r = g.rec[(size_t)k * g.cap + slot];
+                        GuideRecord &r = g.rec[(size_t)k * g.cap + id];
                         f3 vv = -d;
                         r.pos_rough = make_float4(P.x, P.y, P.z, m.roughness);
                         r.n_metal = make_float4(N_feat.x, N_feat.y, N_feat.z, m.metallic);
@@ -612,8 +706,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
                         r.w_valid = make_float4(w_world.x, w_world.y, w_world.z, 1.0f);
                         r.thr = make_float4(T_before.x, T_before.y, T_before.z, 0.0f);
                         r.lsnap = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
-                        g.nrec[slot] = k + 1;
-                        tag = slot | 0x80000000u;
+                        g.nrec[id] = k + 1;
+                        tag = id | 0x80000000u;
                     }
                 }
             }
@@ -621,7 +715,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
     }
     push_shadow(sh, cnt_sh, sh_area, ra, pixel, tag);
     push_shadow(sh, cnt_sh, sh_env, re, pixel, tag);
-    if (g.rec) g.lpath[slot] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
+    if (GUIDE && g.rec) g.lpath[id] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
 
     // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not
     // need it: the path ends either way.
@@ -632,30 +726,35 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, c
         else T = T * (1.0f / q);
     }
     alive = alive && more;
+    // Survivors move, with their whole state, to the next buffer in queue
+    // order (coalesced); ended paths return their id to the free list.
     uint32_t k = wave_append(cnt_out, alive);
     if (alive) {
-        p.ray_o[slot] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
-        p.ray_d[slot] = make_float4(new_d.x, new_d.y, new_d.z, 1e20f);
-        p.thr[slot] = make_float4(T.x, T.y, T.z, last_pdf);
-        p.aux[slot] = make_float4(eta, ax.y, __uint_as_float((uint32_t)rng), __uint_as_float((uint32_t)(rng >> 32)));
-        q_out[k] = slot;
+        p_out.ray_o[k] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+        p_out.ray_d[k] = make_float4(new_d.x, new_d.y, new_d.z, 1e20f);
+        p_out.thr[k] = make_float4(T.x, T.y, T.z, last_pdf);
+        p_out.aux[k] = make_float4(eta, ax.y, __uint_as_float((uint32_t)rng), __uint_as_float((uint32_t)(rng >> 32)));
+        p_out.meta[k] = meta + 1u;
+        p_out.id[k] = id;
+    }
+    if (q_free) {
+        const uint32_t kf = wave_append(cnt_free, !alive);
+        if (!alive) q_free[kf] = id;
     }
 }
 
 // BF16 features of the surface hits in the queue, one row per queue position.
-__global__ void k_guide_feats(SceneDev s, PathsDev p, const uint32_t *__restrict__ queue,
-                              const uint32_t *__restrict__ count, bf16_t *X) {
+__global__ void k_guide_feats(SceneDev s, PathsDev p, const uint32_t *__restrict__ count, bf16_t *X) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *count) return;
-    const uint32_t slot = queue[i];
-    const float4 hit = p.hit[slot];
+    const float4 hit = p.hit[i];
     const uint32_t prim = __float_as_uint(hit.y);
     float f[64];
     bool ok = false;
     if (prim != MISS) {
         const nori_material &mr = s.mats[s.tri_mesh[prim]];
         if (!mr.isEmitter && guide_eligible(mr.type)) {
-            const f3 o = xyz(p.ray_o[slot]), d = xyz(p.ray_d[slot]);
+            const f3 o = xyz(p.ray_o[i]), d = xyz(p.ray_d[i]);
             const f3 P = o + d * hit.x;
             f3 N, Ng;
             bool back;
@@ -671,14 +770,16 @@ __global__ void k_guide_feats(SceneDev s, PathsDev p, const uint32_t *__restrict
 // Training samples from the records of finished paths: the weight is the
 // luminance of (final path radiance - radiance before the sampled direction),
 // divided channel-wise by the throughput before the vertex.
-__global__ void k_guide_finalize(SceneDev s, GuideShadeDev g, uint32_t npaths, GuideSample *out, uint32_t *count,
+__global__ void k_guide_finalize(SceneDev s, GuideShadeDev g, const uint32_t *__restrict__ ids,
+                                 const uint32_t *__restrict__ nids, GuideSample *out, uint32_t *count,
                                  uint32_t max_samples) {
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= npaths) return;
-    const uint32_t nr = min(g.nrec[slot], g.max_rec);
-    const f3 Lf = xyz(g.lpath[slot]);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *nids) return;
+    const uint32_t id = ids[i];
+    const uint32_t nr = min(g.nrec[id], g.max_rec);
+    const f3 Lf = xyz(g.lpath[id]);
     for (uint32_t k = 0; k < nr; ++k) {
-        const GuideRecord r = g.rec[(size_t)k * g.cap + slot];
+        const GuideRecord r = g.rec[(size_t)k * g.cap + id];
         const f3 La = Lf - xyz(r.lsnap), Tb = xyz(r.thr);
         const f3 li = mk(Tb.x > 0.0f ? La.x / Tb.x : 0.0f, Tb.y > 0.0f ? La.y / Tb.y : 0.0f,
                          Tb.z > 0.0f ? La.z / Tb.z : 0.0f);
@@ -694,11 +795,15 @@ __global__ void k_guide_finalize(SceneDev s, GuideShadeDev g, uint32_t npaths, G
     }
 }
 
+// The finalize kernel counts every sample it produced, also those past the
+// buffer's capacity (which it drops), so draws are taken from the first
+// min(*count, max_samples) entries.
 __global__ void k_guide_gather(const GuideSample *__restrict__ samples, const uint32_t *__restrict__ count,
-                               uint32_t rows, uint32_t seed, bf16_t *X, float4 *w_weight, float4 *wi_b) {
+                               uint32_t max_samples, uint32_t rows, uint32_t seed, bf16_t *X, float4 *w_weight,
+                               float4 *wi_b) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
-    const uint32_t n = *count;
+    const uint32_t n = min(*count, max_samples);
     float f[64];
     float4 ww = make_float4(0.0f, 0.0f, 1.0f, 0.0f), wb = make_float4(0.0f, 0.0f, 1.0f, 1.0f);
     if (n > 0) {
@@ -718,14 +823,14 @@ __global__ void k_guide_gather(const GuideSample *__restrict__ samples, const ui
 
 __global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (clear_mask & 0x10u)
+    if (clear_mask & 0x1000u)
         for (int k = 0; k < 8; ++k) c.stats[k] = 0;
     if (ext_from < 2) {
         c.stats[0] += c.cnt[ext_from];
         c.stats[3] += 1;
     }
     if (add_shadow) c.stats[1] += c.cnt[2];
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 9; ++k)
         if (clear_mask & (1u << k)) c.cnt[k] = 0;
 }
 
@@ -806,63 +911,91 @@ void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uin
         hipLaunchKernelGGL(k_trace_batch<false>, dim3(blocks(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, rays, hits, n);
 }
 
-void launch_raygen(const SceneDev &s, const PathsDev &p, uint32_t *queue, Counters c, const ViewDev &v, uint64_t seed,
-                   uint64_t first, uint32_t n, const GuideShadeDev &g, hipStream_t st) {
-    (void)c;
-    hipLaunchKernelGGL(k_raygen, dim3(blocks(n, 256)), dim3(256), 0, st, s, p, queue, v, seed, first, n, g);
+void launch_pool_start(Counters c, uint32_t *q_free0, uint32_t cap, hipStream_t st) {
+    hipLaunchKernelGGL(k_pool_start, dim3(blocks(cap ? cap : 1u, 256)), dim3(256), 0, st, c, q_free0, cap);
 }
 
-void launch_raygen_rays(const PathsDev &p, uint32_t *queue, const float4 *rays, const uint64_t *seeds, uint64_t first,
-                        uint32_t n, float4 *film, hipStream_t st) {
-    if (n) hipLaunchKernelGGL(k_raygen_rays, dim3(blocks(n, 256)), dim3(256), 0, st, p, queue, rays, seeds, first, n, film);
+void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Counters c, const uint32_t *q_free,
+                  const uint32_t *cnt_free, uint32_t *cnt_act, float4 *film, uint32_t cap, const GuideShadeDev &g,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_regen, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, views, c, q_free, cnt_free, cnt_act,
+                       film, g);
 }
 
-void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, bf16_t *X,
-                        uint32_t cap, hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_feats, dim3(blocks(cap, 128)), dim3(128), 0, st, s, p, queue, count, X);
+void launch_iter_end(Counters c, uint32_t qi, hipStream_t st) {
+    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, st, c, qi);
 }
 
-void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, uint32_t npaths, GuideSample *samples,
-                           uint32_t *count, uint32_t max_samples, hipStream_t st) {
-    if (npaths)
-        hipLaunchKernelGGL(k_guide_finalize, dim3(blocks(npaths, 256)), dim3(256), 0, st, s, g, npaths, samples, count,
-                           max_samples);
+void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *seeds, uint64_t first, uint32_t n,
+                        float4 *film, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_raygen_rays, dim3(blocks(n, 256)), dim3(256), 0, st, p, rays, seeds, first, n, film);
 }
 
-void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint32_t rows, uint32_t seed, bf16_t *X,
-                         float4 *w_weight, float4 *wi_b, hipStream_t st) {
+void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *count, bf16_t *X, uint32_t cap,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_feats, dim3(blocks(cap, 128)), dim3(128), 0, st, s, p, count, X);
+}
+
+void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, const uint32_t *ids, const uint32_t *nids,
+                           uint32_t cap, GuideSample *samples, uint32_t *count, uint32_t max_samples, hipStream_t st) {
+    if (cap)
+        hipLaunchKernelGGL(k_guide_finalize, dim3(blocks(cap, 256)), dim3(256), 0, st, s, g, ids, nids, samples,
+                           count, max_samples);
+}
+
+void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint32_t max_samples, uint32_t rows,
+                         uint32_t seed, bf16_t *X, float4 *w_weight, float4 *wi_b, hipStream_t st) {
     if (rows)
-        hipLaunchKernelGGL(k_guide_gather, dim3(blocks(rows, 128)), dim3(128), 0, st, samples, count, rows, seed, X,
-                           w_weight, wi_b);
+        hipLaunchKernelGGL(k_guide_gather, dim3(blocks(rows, 128)), dim3(128), 0, st, samples, count, max_samples,
+                           rows, seed, X, w_weight, wi_b);
 }
 
-void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *queue, const uint32_t *count, uint32_t cap,
-                   unsigned long long *count_stats, hipStream_t st) {
+void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, uint32_t cap,
+                   unsigned long long *count_stats, uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cls_stride,
+                   hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
-    if (count_stats)
-        hipLaunchKernelGGL(k_extend<true>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, queue,
-                           count, count_stats);
-    else
-        hipLaunchKernelGGL(k_extend<false>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, queue,
-                           count, count_stats);
+    const bool sort = q_cls != nullptr;
+    auto k = count_stats ? (sort ? k_extend<true, true> : k_extend<true, false>)
+                         : (sort ? k_extend<false, true> : k_extend<false, false>);
+    hipLaunchKernelGGL(k, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, count, count_stats,
+                       q_cls, cls_cnt, cls_stride);
 }
 
-void launch_shade(const SceneDev &s, const PathsDev &p, const uint32_t *q_in, const uint32_t *cnt_in, uint32_t *q_out,
-                  uint32_t *cnt_out, ShadowDev sh, uint32_t *cnt_sh, float4 *film, uint32_t bounce,
-                  uint32_t max_depth, uint32_t rr_depth, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
-    hipLaunchKernelGGL(k_shade, dim3(blocks(cap, SHADE_BLOCK)), dim3(SHADE_BLOCK), 0, st, s, p, q_in, cnt_in, q_out,
-                       cnt_out, sh, cnt_sh, film, bounce, max_depth, rr_depth, g);
+void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
+                  const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
+                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
+    const bool vol = s.n_vol > 0, guide = g.out != nullptr || g.rec != nullptr;
+    auto k = vol ? (guide ? k_shade<true, true, CLS_ALL> : k_shade<true, false, CLS_ALL>)
+                 : (guide ? k_shade<false, true, CLS_ALL> : k_shade<false, false, CLS_ALL>);
+    hipLaunchKernelGGL(k, dim3(blocks(cap, SHADE_BLOCK)), dim3(SHADE_BLOCK), 0, st, s, p, p_out, views, nullptr,
+                       cnt_in, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+}
+
+void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
+                         const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
+                         uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
+                         float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
+    const bool guide = g.out != nullptr || g.rec != nullptr;
+    const dim3 grid(blocks(cap, SHADE_BLOCK)), block(SHADE_BLOCK);
+    const size_t cs = cls_stride;
+    hipLaunchKernelGGL((guide ? k_shade<false, true, CLS_LIGHT> : k_shade<false, false, CLS_LIGHT>), grid, block, 0,
+                       st, s, p, p_out, views, q_cls, cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    hipLaunchKernelGGL((guide ? k_shade<false, true, CLS_SURF> : k_shade<false, false, CLS_SURF>), grid, block, 0, st,
+                       s, p, p_out, views, q_cls + cs, cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    if (hair)
+        hipLaunchKernelGGL((guide ? k_shade<false, true, CLS_HAIR> : k_shade<false, false, CLS_HAIR>), grid, block, 0,
+                           st, s, p, p_out, views, q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free,
+                           sh, cnt_sh, film, g);
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
-    if (count_stats)
-        hipLaunchKernelGGL(k_shadow<true>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh,
-                           film, count_stats, g);
-    else
-        hipLaunchKernelGGL(k_shadow<false>, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh,
-                           film, count_stats, g);
+    const bool vol = s.n_vol > 0;
+    auto k = count_stats ? (vol ? k_shadow<true, true> : k_shadow<true, false>)
+                         : (vol ? k_shadow<false, true> : k_shadow<false, false>);
+    hipLaunchKernelGGL(k, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh, film, count_stats,
+                       g);
 }
 
 void launch_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st) {

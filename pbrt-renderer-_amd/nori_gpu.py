@@ -111,6 +111,11 @@ class RenderStats(ctypes.Structure):
                 ("ms_guide_train", ctypes.c_double), ("guide_train_steps", _u64)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("best_pool", _u32), ("best_chunk", _u32), ("best_msamples_per_s", ctypes.c_double),
+                ("count", _u32), ("pool", _u32 * 16), ("chunk", _u32 * 16), ("msamples_per_s", ctypes.c_double * 16)]
+
+
 class BvhInfo(ctypes.Structure):
     _fields_ = [("nodes", _u32), ("triangles", _u32), ("max_depth", _u32), ("leaf_tris_max", _u32),
                 ("node_bytes", _u64), ("tri_bytes", _u64), ("build_ms", ctypes.c_double)]
@@ -125,7 +130,7 @@ EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
     "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
     "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_render_stats",
-    "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_scene_generate", "nori_scene_get_desc",
+    "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_gpu_autotune", "nori_scene_generate", "nori_scene_get_desc",
     "nori_scene_get_camera", "nori_scene_free", "nori_bvh8_build_host",
 ]
 
@@ -383,6 +388,16 @@ class NoriGPU:
 
     def reset_film(self):
         self._check(self.lib.nori_gpu_reset_film(self.h), "reset_film")
+
+    def autotune(self, views, calib_spp=1):
+        """Time short renders of `views` per pool size / chunk length and keep the fastest
+        (sets the options "batch" and "chunk_iters"; clears the film)."""
+        if isinstance(views, View):
+            views = [views]
+        arr = (View * len(views))(*views)
+        r = TuneResult()
+        self._check(self.lib.nori_gpu_autotune(self.h, arr, len(views), int(calib_spp), ctypes.byref(r)), "autotune")
+        return r
 
     def stats(self):
         s = RenderStats()

@@ -1,0 +1,180 @@
+"""GPU checks of the persistent path pool behind nori_gpu_render: one call over
+many views of uneven sizes and spp equals the views rendered one at a time;
+the graph-replayed bounce loop equals eager launches; the image does not depend
+on the pool size (a sample runs on its own pcg32 stream in whatever slot and
+iteration it lands); deep paths and tiny pools drain.  Films are sums of float
+atomics, so equal sample sets agree to float rounding of the summation order."""
+import sys
+
+import numpy as np
+import pytest
+
+import nori_gpu as ng
+import orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _same_film(a, b, label):
+    # per-pixel radiance sums over the same samples in another order
+    assert np.array_equal(a[:, 3], b[:, 3]), f"{label}: sample counts differ"
+    tol = 1e-5 * np.maximum(np.abs(a[:, :3]), np.abs(b[:, :3])) + 1e-5
+    bad = np.abs(a[:, :3] - b[:, :3]) > tol
+    assert bad.mean() < 1e-4, f"{label}: {bad.sum()} channel sums differ"
+
+
+def _views(scene):
+    sizes = [(48, 32, 5), (17, 23, 3), (64, 64, 1), (8, 8, 37), (40, 24, 2)]
+    return [ng.make_view(ng.lookat_camera_like(scene.camera, w, h), spp, 8) for (w, h, spp) in sizes]
+
+
+@pytest.fixture
+def pool_defaults(gpu):
+    yield gpu
+    gpu.set_option("batch", 1 << 21)
+    gpu.set_option("graph", 1)
+    gpu.set_option("chunk_iters", 8)
+    gpu.set_option("sort", 1)
+
+
+def test_multiview_equals_separate_renders(pool_defaults, cornell):
+    gpu = pool_defaults
+    gpu.upload(cornell)
+    views = _views(cornell)
+    _say(f"case: {len(views)} views in one render vs one at a time")
+    gpu.reset_film()
+    together = gpu.render(views, seed=11)
+    st = gpu.stats()
+    assert st.samples == sum(v.camera.width * v.camera.height * v.spp for v in views)
+    off = 0
+    for v in views:
+        gpu.reset_film()
+        alone = gpu.render(v, seed=11)
+        n = v.camera.width * v.camera.height
+        _same_film(together[off:off + n], alone, f"view {v.camera.width}x{v.camera.height}")
+        assert np.all(alone[:, 3] == v.spp)
+        off += n
+
+
+def test_multiview_matches_oracle(pool_defaults, cornell):
+    gpu = pool_defaults
+    gpu.upload(cornell)
+    views = _views(cornell)[:2]
+    _say("case: two views in one render vs the oracle, same samples")
+    gpu.reset_film()
+    g = gpu.render(views, seed=5)
+    o = orc.OracleScene(cornell)
+    off = 0
+    for v in views:
+        n = v.camera.width * v.camera.height
+        ref, _ = o.render(v.camera, v.spp, v.max_depth, seed=5)
+        gi = g[off:off + n, :3] / v.spp
+        e = orc.rel_mse(gi, ref[:, :3] / v.spp)
+        _say(f"  {v.camera.width}x{v.camera.height} relMSE {e:.2e}")
+        assert e < 1e-5
+        off += n
+
+
+@pytest.mark.parametrize("batch", [4096, 1 << 16])
+def test_pool_size_does_not_change_the_image(pool_defaults, cornell, batch):
+    gpu = pool_defaults
+    gpu.upload(cornell)
+    view = ng.make_view(ng.lookat_camera_like(cornell.camera, 64, 48), 6, 8)
+    _say(f"case: pool {batch} vs default")
+    gpu.reset_film()
+    ref = gpu.render(view, seed=3)
+    gpu.set_option("batch", batch)
+    gpu.reset_film()
+    img = gpu.render(view, seed=3)
+    _same_film(img, ref, f"pool {batch}")
+
+
+def test_graph_replay_equals_eager(pool_defaults, disney_small):
+    gpu = pool_defaults
+    gpu.upload(disney_small)
+    view = ng.make_view(ng.lookat_camera_like(disney_small.camera, 64, 64), 8, 12)
+    _say("case: graph replay vs eager launches, three renders each")
+    gpu.set_option("graph", 0)
+    gpu.reset_film()
+    eager = [gpu.render(view, seed=s).copy() for s in (1, 2, 3)]
+    gpu.set_option("graph", 1)
+    gpu.reset_film()
+    graphed = [gpu.render(view, seed=s).copy() for s in (1, 2, 3)]
+    for k, (a, b) in enumerate(zip(eager, graphed)):
+        _same_film(a, b, f"render {k}")
+    for chunk in (2, 32):
+        gpu.set_option("chunk_iters", chunk)
+        gpu.reset_film()
+        _same_film(gpu.render(view, seed=1), eager[0], f"chunk {chunk}")
+
+
+@pytest.mark.parametrize("config", [2, 3])
+def test_class_sorted_shading_equals_one_launch(pool_defaults, config):
+    # Disney grid (light + surface classes) and hair strands (all three)
+    gpu = pool_defaults
+    sc = (ng.generate_scene(2, subdiv=3, grid=4, env_width=256, env_height=128) if config == 2 else
+          ng.generate_scene(3, strands=2000, env_width=64, env_height=32))
+    gpu.upload(sc)
+    view = ng.make_view(ng.lookat_camera_like(sc.camera, 48, 48), 8, 12)
+    _say(f"case: config {config}, class-sorted shading vs one shading launch")
+    gpu.set_option("sort", 0)
+    gpu.reset_film()
+    one = gpu.render(view, seed=21)
+    gpu.set_option("sort", 1)
+    gpu.reset_film()
+    _same_film(gpu.render(view, seed=21), one, f"config {config} sorted")
+
+
+def test_deep_paths_drain(pool_defaults):
+    gpu = pool_defaults
+    sc = ng.generate_scene(4, grid=16, env_width=64, env_height=32, width=16, height=16, variant=1)
+    gpu.upload(sc)
+    gpu.set_option("batch", 512)
+    _say("case: medium scene, depth 64, pool of 512 slots")
+    gpu.reset_film()
+    img = gpu.render(ng.make_view(sc.camera, 4, 64), seed=9)
+    assert np.all(img[:, 3] == 4)
+    assert np.isfinite(img).all()
+    st = gpu.stats()
+    assert st.samples == 16 * 16 * 4
+    assert st.bounces >= 2
+
+
+def test_autotune_keeps_the_fastest_timed_candidate(pool_defaults, cornell):
+    gpu = pool_defaults
+    gpu.upload(cornell)
+    views = _views(cornell)
+    _say("case: autotune over the five Cornell views, 64 spp calibration")
+    gpu.reset_film()
+    before = gpu.render(views, seed=11)
+    r = gpu.autotune(views, calib_spp=64)
+    cands = [(r.pool[k], r.chunk[k], r.msamples_per_s[k]) for k in range(r.count)]
+    _say("  " + ", ".join(f"pool {p} chunk {c}: {m:.1f} Ms/s" for p, c, m in cands))
+    assert r.count >= 4 and all(m > 0 for _, _, m in cands)
+    best = max(cands, key=lambda x: x[2])
+    assert (r.best_pool, r.best_chunk) == best[:2]
+    # the tuned options are in effect and the film was cleared: same image again
+    after = gpu.render(views, seed=11)
+    _same_film(after, before, "tuned")
+
+
+def test_guided_render_with_more_samples_than_the_sample_buffer(disney_small):
+    # A context whose pool is small: a training interval produces more
+    # training samples than its buffer (4 x pool entries) holds; the extra
+    # ones are dropped and training draws from the stored ones only.
+    g2 = ng.NoriGPU(0)
+    try:
+        g2.upload(disney_small)
+        g2.set_option("batch", 4096)
+        g2.set_option("guiding", 1)
+        _say("case: guided 32x32x64 with a 4096-slot pool (sample buffer overflow)")
+        img = g2.render(ng.make_view(ng.lookat_camera_like(disney_small.camera, 32, 32), 64, 12), seed=4)
+        assert np.all(img[:, 3] == 64) and np.isfinite(img).all()
+        assert g2.stats().guide_train_steps > 0
+    finally:
+        g2.close()
