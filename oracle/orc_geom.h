@@ -114,6 +114,43 @@ struct Bvh2 {
               bool robust = true) const;
 };
 
+// Shadow ray of the renderer (shaders/RayQueryTrace.hlsli:96-124): 0 when a
+// triangle whose mesh does not pass (ior_of(prim).x <= 0) lies in [mint, maxt];
+// otherwise the product over the pass-through triangles in the interval of
+// 1 - fresnel(|d . Ng|, ior.x, ior.y).  Every triangle in the interval is
+// visited, so each crossing counts once.
+template <class IorOf, class Fresnel>
+float shadow_pass(const Bvh2 &b, const Ray &ray, IorOf ior_of, Fresnel fresnel) {
+    if (b.nodes.empty() || b.index.empty() || ray.maxt < ray.mint) return 1.0f;
+    float tr = 1.0f;
+    uint32_t stack[64], sp = 0, ni = 0;
+    while (true) {
+        const Bvh2::Node &n = b.nodes[ni];
+        if (!n.box.hit(ray, true)) {
+            if (sp == 0) break;
+            ni = stack[--sp];
+            continue;
+        }
+        if (!n.leaf) {
+            stack[sp++] = n.right_or_start;
+            ++ni;
+            continue;
+        }
+        for (uint32_t i = n.right_or_start, e = n.right_or_start + n.size; i < e; ++i) {
+            const uint32_t f = b.index[i];
+            float u, v, t;
+            if (!intersect_tri(b.p0[f], b.e1[f], b.e2[f], ray, u, v, t)) continue;
+            const auto ior = ior_of(f);
+            if (!(ior.first > 0.0f)) return 0.0f;
+            const V3f ng = normalize(cross(b.e1[f], b.e2[f]));
+            tr *= 1.0f - fresnel(std::fabs(dot(ray.d, ng)), ior.first, ior.second);
+        }
+        if (sp == 0) break;
+        ni = stack[--sp];
+    }
+    return tr;
+}
+
 // Exhaustive test over every triangle (ground truth for small scenes).
 Hit brute_trace(const Bvh2 &tris, const Ray &ray, bool any_hit);
 

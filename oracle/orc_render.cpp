@@ -9,6 +9,8 @@ static const float PI_F = 3.14159265358979323846f;
 
 void SceneData::finalize() {
     bvh.build(pos.data(), idx.data(), nt);
+    has_pass = false;
+    for (uint32_t m = 0; m < nmesh; ++m) has_pass = has_pass || mats[m].type == 1 || mats[m].type == 2;
     emitters.clear(); emit_cdf.clear(); emit_area.clear();
     for (uint32_t m = 0; m < nmesh; ++m) {
         if (!mats[m].isEmitter) continue;
@@ -78,10 +80,20 @@ static uint32_t cdf_sample(const std::vector<float> &c, uint32_t n, float u) {
     return (uint32_t)std::max(0, (int)lo - 1);
 }
 
-static bool occluded(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st, float mint = 0.0f) {
+// Shadow ray: 0 if occluded, else the Fresnel transmission through the mirror
+// and dielectric meshes it crosses (shaders/RayQueryTrace.hlsli:96-124; those
+// meshes are the non-opaque instances, every other mesh occludes).
+static float shadow_pass_factor(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st, float mint = 0.0f) {
     ++st.shadow_rays;
-    Hit h = s.bvh.trace(mk_ray(o, d, mint, maxt), true);
-    return h.prim != 0xFFFFFFFFu;
+    if (!s.has_pass) return s.bvh.trace(mk_ray(o, d, mint, maxt), true).prim != 0xFFFFFFFFu ? 0.0f : 1.0f;
+    return shadow_pass(
+        s.bvh, mk_ray(o, d, mint, maxt),
+        [&](uint32_t f) {
+            const MatRec &m = s.mats[s.tri_mesh[f]];
+            const bool pass = m.type == 1 || m.type == 2;
+            return std::make_pair(pass ? std::max(m.extIOR, 1e-6f) : 0.0f, m.intIOR);
+        },
+        [](float c, float ei, float et) { return fresnel_dielectric(c, ei, et); });
 }
 
 // Transmittance of an unoccluded shadow ray through the scene's volumes, with
@@ -127,12 +139,13 @@ static V3f volume_nee_area(const SceneData &s, V3f P, V3f wo, const Volume &v, P
     V3f wi = tl / dist;
     float cl = dot(ls.n, -wi);
     if (cl <= 0.0f) return V3f(0.0f);
-    if (occluded(s, P, wi, dist - 0.001f, st, 0.001f)) return V3f(0.0f);
+    const float pass = shadow_pass_factor(s, P, wi, dist - 0.001f, st, 0.001f);
+    if (pass == 0.0f) return V3f(0.0f);
     float fp = hg_eval(dot(wo, wi), v.g);
     float pe = ls.pdfA * dist * dist / cl;
     float w = balance_heuristic(pe, fp);
     V3f tr = shadow_transmittance(s, P, wi, dist, tr_seed);
-    return ls.Le * fp / std::max(pe, 1e-20f) * w * tr;
+    return ls.Le * fp / std::max(pe, 1e-20f) * w * tr * pass;
 }
 
 // Volume NEE toward the envmap (shaders/Emitter.hlsli:387-434).
@@ -151,11 +164,12 @@ static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Pc
     }
     uint32_t tr_seed = rng.next_uint();
     if (pe <= 0.0f) return V3f(0.0f);
-    if (occluded(s, P, dir, 1e20f, st, 0.001f)) return V3f(0.0f);
+    const float pass = shadow_pass_factor(s, P, dir, 1e20f, st, 0.001f);
+    if (pass == 0.0f) return V3f(0.0f);
     float fp = hg_eval(dot(wo, dir), v.g);
     float w = balance_heuristic(pe, fp);
     V3f tr = shadow_transmittance(s, P, dir, 1e20f, tr_seed);
-    return Le * fp / std::max(pe, 1e-20f) * w * tr;
+    return Le * fp / std::max(pe, 1e-20f) * w * tr * pass;
 }
 
 // RayGen body, shaders/Shaders.hlsl:55-627, for the surface materials of the
@@ -330,14 +344,15 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
                 V3f ww = tl / dist;
                 float ct = dot(N, ww), cl = dot(ls.n, -ww);
                 V3f sso = hair ? offset_ray_origin(P, side(ww), side(ww)) : so;
-                if ((hair || ct > 0.0f) && cl > 0.0f && !occluded(s, sso, ww, dist - 0.001f, st)) {
+                float pass = 0.0f;
+                if ((hair || ct > 0.0f) && cl > 0.0f && (pass = shadow_pass_factor(s, sso, ww, dist - 0.001f, st)) > 0.0f) {
                     V3f wol = to_local(ww, Tg, B, N);
                     V3f fv = material_eval_h(wi, wol, m, hh);
                     float pb = material_pdf_h(wi, wol, m, hh);
                     float pe = ls.pdfA * dist * dist / cl;
                     float w = balance_heuristic(pe, pb);
                     V3f tr = shadow_transmittance(s, sso, ww, dist, tr_seed);
-                    L += T * (ls.Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w) * tr;
+                    L += T * (ls.Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w) * tr * pass;
                 }
             }
             // Envmap NEE, shaders/Emitter.hlsli:275-326
@@ -355,13 +370,14 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
                 uint32_t tr_seed = s.vols.empty() ? 0u : rng.next_uint();
                 float ct = dot(N, dir);
                 V3f sso = hair ? offset_ray_origin(P, side(dir), side(dir)) : so;
-                if (pe > 0.0f && (hair || ct > 0.0f) && !occluded(s, sso, dir, 1e20f, st)) {
+                float pass = 0.0f;
+                if (pe > 0.0f && (hair || ct > 0.0f) && (pass = shadow_pass_factor(s, sso, dir, 1e20f, st)) > 0.0f) {
                     V3f wol = to_local(dir, Tg, B, N);
                     V3f fv = material_eval_h(wi, wol, m, hh);
                     float pb = material_pdf_h(wi, wol, m, hh);
                     float w = balance_heuristic(pe, pb);
                     V3f tr = shadow_transmittance(s, sso, dir, 1e20f, tr_seed);
-                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w) * tr;
+                    L += T * (Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w) * tr * pass;
                 }
             }
             float u[4];

@@ -45,6 +45,7 @@ struct SceneData {
     std::vector<std::vector<float>> emit_cdf;  // per emitter: Nori DiscretePDF cdf (tris+1)
     std::vector<float> emit_area;              // per emitter: total area
     std::vector<Volume> vols;                  // participating media
+    bool has_pass = false;                     // mirror/dielectric meshes present (shadow rays pass them)
 
     void finalize(); // BVH + emitter tables from the raw arrays
     V3f vtx(uint32_t i) const { return V3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]); }

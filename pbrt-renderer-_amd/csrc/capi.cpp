@@ -329,7 +329,7 @@ void pool_iteration(nori_ctx *c, uint32_t qi, float4 *film, const ViewTab *views
     uint32_t *cnt = c->cdev.cnt;
     {
         Span t(c, 2);
-        launch_regen(c->sdev, c->pd[qi], views, c->cdev, c->qf[qi], cnt + 3 + qi, cnt + qi, film, n, gsd, c->stream);
+        launch_regen(c->sdev, c->pd[qi], views, c->cdev, c->qf[qi], cnt + 3 + qi, cnt + qi, n, gsd, c->stream);
     }
     // Scenes without media shade in class-sorted launches (a path's class is
     // known once its hit is; with media, free flight decides first).
@@ -520,6 +520,16 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         c->scene_has_hair = false;
         for (uint32_t m = 0; m < d->num_meshes; ++m)
             if (!mats[m].isEmitter && mats[m].type == NORI_MAT_HAIR) c->scene_has_hair = true;
+        // Shadow rays pass mirror and dielectric meshes with Fresnel attenuation
+        // (the non-opaque instances of shaders/RayQueryTrace.hlsli:96-124).
+        std::vector<float> pass_ior(2 * (size_t)d->num_meshes, 0.0f);
+        bool any_pass = false;
+        for (uint32_t m = 0; m < d->num_meshes; ++m)
+            if (mats[m].type == NORI_MAT_MIRROR || mats[m].type == NORI_MAT_DIELECTRIC) {
+                pass_ior[2 * m] = std::max(mats[m].extIOR, 1e-6f);
+                pass_ior[2 * m + 1] = mats[m].intIOR;
+                any_pass = true;
+            }
 
         Bvh8 bvh;
         build_bvh8(d->positions, d->indices, tri_mesh.data(), d->num_triangles, bvh);
@@ -597,6 +607,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         }
         s.n_vol = d->num_volumes;
         s.vols = d->num_volumes ? c->upload(vd.data(), vd.size()) : nullptr;
+        s.pass_ior = any_pass ? (const float2 *)c->upload(pass_ior.data(), pass_ior.size()) : nullptr;
         if ((size_t)s.stack_depth * 64 * sizeof(uint2) > 64 * 1024)
             throw ArgError("BVH8 deeper than the LDS stack allows", NORI_E_LIMIT);
         c->sdev = s;
@@ -706,6 +717,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                 }
             }
             launch_pool_start(c->cdev, c->qf[0], c->pool, c->stream);
+            launch_film_count(dviews, (uint32_t)nviews, (uint32_t)npix_all, film, c->stream);
             // Every slot runs at most max_depth iterations per sample and every
             // slot frees up within max_depth iterations, so the pool drains
             // within (samples / cap + 1) * max_depth iterations.

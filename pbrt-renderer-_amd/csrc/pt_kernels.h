@@ -29,6 +29,9 @@ struct SceneDev {
     f3 bmin, binv; // scene bounds: position -> [0,1]^3 for the guiding features
     const VolumeDev *vols; // participating media (n_vol entries)
     uint32_t n_vol;
+    // per mesh {extIOR, intIOR} of mirror/dielectric meshes, {0, 0} otherwise:
+    // shadow rays pass those with Fresnel attenuation (null: none in the scene)
+    const float2 *pass_ior;
 };
 
 // Guiding hooks of one render batch.  out: network outputs per queue position
@@ -97,11 +100,15 @@ void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uin
 // Path pool start: free list 0 = every path id, other queues empty, sample
 // counter at 0 (total, seed and view count are written by the host).
 void launch_pool_start(Counters c, uint32_t *q_free0, uint32_t cap, hipStream_t st);
-// Refill: every id in the free list claims the next global sample (one atomic
-// per wave) and its camera path is appended to the active buffer p.
+// Refill: the free ids claim the next global samples in list order and their
+// camera paths are appended to the active buffer p after the survivors (no
+// atomics: entry i takes sample base + i and position survivors + i); a
+// one-thread commit then advances both counts.
 void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Counters c, const uint32_t *q_free,
-                  const uint32_t *cnt_free, uint32_t *cnt_act, float4 *film, uint32_t cap, const GuideShadeDev &g,
-                  hipStream_t st);
+                  const uint32_t *cnt_free, uint32_t *cnt_act, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
+// Film sample counts of a render: spp_v added to the fourth channel of every
+// entry of view v.
+void launch_film_count(const ViewTab *views, uint32_t nviews, uint32_t npix_all, float4 *film, hipStream_t st);
 // End of an iteration whose active buffer was qi: statistics, the counters of
 // parity qi cleared, done flag.
 void launch_iter_end(Counters c, uint32_t qi, hipStream_t st);

@@ -48,6 +48,27 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
     return base + (uint32_t)__popcll(below);
 }
 
+// Wave-aggregated append of v (0..VMAX) entries per lane: one atomic per
+// wave; returns this lane's first slot.  Built from ballots only, so lanes
+// that have left the kernel do not take part.
+template <int VMAX>
+__device__ __forceinline__ uint32_t wave_append_n(uint32_t *counter, uint32_t v) {
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t total = 0, off = 0;
+#pragma unroll
+    for (uint32_t j = 1; j <= (uint32_t)VMAX; ++j) {
+        const uint64_t m = __ballot(v >= j);
+        total += (uint32_t)__popcll(m);
+        off += (uint32_t)__popcll(m & lt);
+    }
+    if (total == 0) return 0;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(true));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, total);
+    return __shfl(base, leader) + off;
+}
+
 // Nori perspective camera, src/cpu/perspective.cpp:86-123, with the
 // sample-to-camera transform in closed form.
 __device__ void camera_ray(const nori_camera &c, float px, float py, float ax, float ay, f3 &o, f3 &d, float &mint,
@@ -100,36 +121,26 @@ __global__ void k_pool_start(Counters c, uint32_t *q_free0, uint32_t cap) {
     }
 }
 
-// Refill from the global sample counter (one atomic per wave): every free path
-// id claims the next sample, and the new camera path is appended to the active
-// buffer.  A claimed sample g of view v starts exactly as a batch renderer
-// would start sample g: pixel (g - first) % npix, sample index sample_offset +
-// (g - first) / npix, stream sample_stream(seed, pixel, sample), so the image
-// does not depend on the position, id or iteration a sample runs in.
+// Refill: free-list entry i claims global sample base + i (base = samples
+// claimed so far) and its camera path goes to position survivors + i of the
+// active buffer, for i < m = min(free ids, samples left); k_regen_commit then
+// advances both counts by m.  A claimed sample g of view v starts exactly as a
+// batch renderer would start sample g: pixel (g - first) % npix, sample index
+// sample_offset + (g - first) / npix, stream sample_stream(seed, pixel,
+// sample), so the image does not depend on the position, id or iteration a
+// sample runs in.  Film sample counts are added per view by k_film_count.
+__device__ __forceinline__ uint32_t regen_count(const Counters &c, uint32_t nfree) {
+    const unsigned long long base = c.next[0], total = c.next[1];
+    return base >= total ? 0u : (uint32_t)min((unsigned long long)nfree, total - base);
+}
+
 __global__ __launch_bounds__(256) void k_regen(SceneDev s, PathsDev p, const ViewTab *__restrict__ views, Counters c,
                                                const uint32_t *__restrict__ q_free, const uint32_t *__restrict__ cnt_free,
-                                               uint32_t *cnt_act, float4 *film, GuideShadeDev g) {
+                                               const uint32_t *__restrict__ cnt_act, GuideShadeDev g) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t nfree = *cnt_free;
-    if (i - (threadIdx.x & 63u) >= nfree) return; // the whole wave is past the list
-    const unsigned long long total = c.next[1];
-    const bool want = i < nfree;
-    const uint64_t mask = __ballot(want); // lane 0 wants: the wave's first index is < nfree
-    const uint32_t lane = __lane_id(), leader = (uint32_t)__builtin_ctzll(mask);
-    uint32_t blo = 0, bhi = 0;
-    if (lane == leader) {
-        const unsigned long long b = atomicAdd(c.next, (unsigned long long)__popcll(mask));
-        blo = (uint32_t)b;
-        bhi = (uint32_t)(b >> 32);
-    }
-    blo = __shfl(blo, leader);
-    bhi = __shfl(bhi, leader);
-    const unsigned long long gidx =
-        (((unsigned long long)bhi << 32) | blo) + (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
-    const bool ok = want && gidx < total;
-    // New paths go to the end of the active buffer, after the survivors.
-    const uint32_t k = wave_append(cnt_act, ok);
-    if (!ok) return;
+    if (i >= regen_count(c, *cnt_free)) return;
+    const unsigned long long gidx = c.next[0] + i;
+    const uint32_t k = *cnt_act + i;
     const uint32_t id = q_free[i];
     // the view owning gidx: the last one whose first sample is <= gidx
     uint32_t lo = 0, hi = (uint32_t)c.next[3];
@@ -165,7 +176,27 @@ __global__ __launch_bounds__(256) void k_regen(SceneDev s, PathsDev p, const Vie
         g.nrec[id] = 0;
         g.lpath[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    unsafeAtomicAdd(reinterpret_cast<float *>(film + fe) + 3, 1.0f);
+}
+
+__global__ void k_regen_commit(Counters c, const uint32_t *__restrict__ cnt_free, uint32_t *cnt_act) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t m = regen_count(c, *cnt_free);
+    *cnt_act += m;
+    c.next[0] += m;
+}
+
+// Sample counts of a render: film entry e of view v gets spp_v (every pixel
+// of a view receives exactly spp samples).
+__global__ void k_film_count(const ViewTab *__restrict__ views, uint32_t nviews, uint32_t npix_all, float4 *film) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npix_all) return;
+    uint32_t lo = 0, hi = nviews;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].film_off <= e) lo = mid;
+        else hi = mid;
+    }
+    film[e].w += (float)views[lo].spp;
 }
 
 // End of an iteration over active queue qi: statistics, the counters of parity
@@ -268,7 +299,9 @@ __device__ __forceinline__ void guide_nee_add(const GuideShadeDev &g, uint32_t t
     }
 }
 
-template <bool COUNT, bool VOL>
+// VOL: transmittance through media; PASS: mirror/dielectric meshes attenuate
+// instead of occluding.
+template <bool COUNT, bool VOL, bool PASS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
                                                         float4 *film, unsigned long long *stats, GuideShadeDev g) {
     extern __shared__ uint2 stk[];
@@ -278,11 +311,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
     uint32_t nn = 0, nt = 0;
     if (i < n) {
         float4 a = sh.o[i], b = sh.d[i], e = sh.e[i];
-        TraceOut r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), e.x, a.w, stk + threadIdx.x,
-                                                blockDim.x, &nn, &nt);
+        float trans = 1.0f;
+        TraceOut r = traverse_bvh8<true, COUNT, PASS>(s.nodes, s.tris, xyz(a), xyz(b), e.x, a.w, stk + threadIdx.x,
+                                                      blockDim.x, &nn, &nt, s.pass_ior, &trans);
         if (r.prim == MISS) {
             const float4 c = sh.c[i];
-            f3 col = xyz(c);
+            f3 col = PASS ? xyz(c) * trans : xyz(c);
             if (VOL && (__float_as_uint(e.w) & 1u)) {
                 // transmittance through the media, with the ray's own stream
                 uint64_t rr = pcg_seed((uint64_t)__float_as_uint(e.z), PCG_DEFAULT_STREAM);
@@ -303,15 +337,12 @@ struct ShadowRec {
     uint32_t tr_seed, flags;
 };
 
-__device__ __forceinline__ void push_shadow(ShadowDev &sh, uint32_t *cnt, bool pred, const ShadowRec &r,
-                                            uint32_t pixel, uint32_t tag) {
-    uint32_t k = wave_append(cnt, pred);
-    if (pred) {
-        sh.o[k] = make_float4(r.o.x, r.o.y, r.o.z, r.tmax);
-        sh.d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(pixel));
-        sh.c[k] = make_float4(r.c.x, r.c.y, r.c.z, __uint_as_float(tag));
-        sh.e[k] = make_float4(r.tmin, r.tr_tmax, __uint_as_float(r.tr_seed), __uint_as_float(r.flags));
-    }
+__device__ __forceinline__ void write_shadow(ShadowDev &sh, uint32_t k, const ShadowRec &r, uint32_t pixel,
+                                             uint32_t tag) {
+    sh.o[k] = make_float4(r.o.x, r.o.y, r.o.z, r.tmax);
+    sh.d[k] = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(pixel));
+    sh.c[k] = make_float4(r.c.x, r.c.y, r.c.z, __uint_as_float(tag));
+    sh.e[k] = make_float4(r.tmin, r.tr_tmax, __uint_as_float(r.tr_seed), __uint_as_float(r.flags));
 }
 
 __device__ __forceinline__ bool guide_eligible(uint32_t type) {
@@ -713,8 +744,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
             }
         }
     }
-    push_shadow(sh, cnt_sh, sh_area, ra, pixel, tag);
-    push_shadow(sh, cnt_sh, sh_env, re, pixel, tag);
+    {
+        // both shadow rays of the vertex with one atomic per wave
+        uint32_t ks = wave_append_n<2>(cnt_sh, (sh_area ? 1u : 0u) + (sh_env ? 1u : 0u));
+        if (sh_area) write_shadow(sh, ks++, ra, pixel, tag);
+        if (sh_env) write_shadow(sh, ks, re, pixel, tag);
+    }
     if (GUIDE && g.rec) g.lpath[id] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
 
     // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not
@@ -774,24 +809,42 @@ __global__ void k_guide_finalize(SceneDev s, GuideShadeDev g, const uint32_t *__
                                  const uint32_t *__restrict__ nids, GuideSample *out, uint32_t *count,
                                  uint32_t max_samples) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nids) return;
-    const uint32_t id = ids[i];
-    const uint32_t nr = min(g.nrec[id], g.max_rec);
-    const f3 Lf = xyz(g.lpath[id]);
-    for (uint32_t k = 0; k < nr; ++k) {
-        const GuideRecord r = g.rec[(size_t)k * g.cap + id];
+    const uint32_t n = *nids;
+    if (i - (threadIdx.x & 63u) >= n) return; // the whole wave is past the list
+    const bool live = i < n;
+    const uint32_t id = live ? ids[i] : 0u;
+    const uint32_t nr = live ? min(g.nrec[id], min(g.max_rec, 8u)) : 0u;
+    const f3 Lf = live ? xyz(g.lpath[id]) : mk(0.0f);
+    // weights first (one slot per positive weight), then one append per wave
+    float W[8];
+    uint32_t nv = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        W[k] = 0.0f;
+        if (k >= nr) continue;
+        const GuideRecord &r = g.rec[(size_t)k * g.cap + id];
         const f3 La = Lf - xyz(r.lsnap), Tb = xyz(r.thr);
         const f3 li = mk(Tb.x > 0.0f ? La.x / Tb.x : 0.0f, Tb.y > 0.0f ? La.y / Tb.y : 0.0f,
                          Tb.z > 0.0f ? La.z / Tb.z : 0.0f);
-        const float W = luminance(li);
-        if (!(W > 0.0f) || !isfinite(W)) continue;
-        const uint32_t j = atomicAdd(count, 1u);
-        if (j >= max_samples) continue;
-        const f3 p01 = (xyz(r.pos_rough) - s.bmin) * s.binv;
-        out[j].pos01_rough = make_float4(p01.x, p01.y, p01.z, r.pos_rough.w);
-        out[j].n_metal = r.n_metal;
-        out[j].wi_b = r.wi_b;
-        out[j].w_weight = make_float4(r.w_valid.x, r.w_valid.y, r.w_valid.z, W);
+        const float w = luminance(li);
+        if (w > 0.0f && isfinite(w)) {
+            W[k] = w;
+            ++nv;
+        }
+    }
+    uint32_t j = wave_append_n<8>(count, nv);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        if (!(W[k] > 0.0f)) continue;
+        if (j < max_samples) {
+            const GuideRecord &r = g.rec[(size_t)k * g.cap + id];
+            const f3 p01 = (xyz(r.pos_rough) - s.bmin) * s.binv;
+            out[j].pos01_rough = make_float4(p01.x, p01.y, p01.z, r.pos_rough.w);
+            out[j].n_metal = r.n_metal;
+            out[j].wi_b = r.wi_b;
+            out[j].w_weight = make_float4(r.w_valid.x, r.w_valid.y, r.w_valid.z, W[k]);
+        }
+        ++j;
     }
 }
 
@@ -916,10 +969,14 @@ void launch_pool_start(Counters c, uint32_t *q_free0, uint32_t cap, hipStream_t 
 }
 
 void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Counters c, const uint32_t *q_free,
-                  const uint32_t *cnt_free, uint32_t *cnt_act, float4 *film, uint32_t cap, const GuideShadeDev &g,
-                  hipStream_t st) {
-    hipLaunchKernelGGL(k_regen, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, views, c, q_free, cnt_free, cnt_act,
-                       film, g);
+                  const uint32_t *cnt_free, uint32_t *cnt_act, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
+    hipLaunchKernelGGL(k_regen, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, views, c, q_free, cnt_free, cnt_act, g);
+    hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, st, c, cnt_free, cnt_act);
+}
+
+void launch_film_count(const ViewTab *views, uint32_t nviews, uint32_t npix_all, float4 *film, hipStream_t st) {
+    if (npix_all && nviews)
+        hipLaunchKernelGGL(k_film_count, dim3(blocks(npix_all, 256)), dim3(256), 0, st, views, nviews, npix_all, film);
 }
 
 void launch_iter_end(Counters c, uint32_t qi, hipStream_t st) {
@@ -991,11 +1048,13 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
-    const bool vol = s.n_vol > 0;
-    auto k = count_stats ? (vol ? k_shadow<true, true> : k_shadow<true, false>)
-                         : (vol ? k_shadow<false, true> : k_shadow<false, false>);
-    hipLaunchKernelGGL(k, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh, film, count_stats,
-                       g);
+    const int v = (count_stats ? 4 : 0) | (s.n_vol > 0 ? 2 : 0) | (s.pass_ior ? 1 : 0);
+    decltype(&k_shadow<false, false, false>) ks[8] = {
+        k_shadow<false, false, false>, k_shadow<false, false, true>, k_shadow<false, true, false>,
+        k_shadow<false, true, true>,   k_shadow<true, false, false>, k_shadow<true, false, true>,
+        k_shadow<true, true, false>,   k_shadow<true, true, true>};
+    hipLaunchKernelGGL(ks[v], dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh, film,
+                       count_stats, g);
 }
 
 void launch_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st) {
