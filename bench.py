@@ -53,15 +53,16 @@ def cpu_baseline(scene, cam, spp, depth):
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch of the closest-hit kernel from the committed rocprof counter summary
-    (its line `bytes_per_launch: N`)."""
+    """HBM bytes per extension ray of the closest-hit kernel from the committed rocprof counter summary
+    (its line `hbm_bytes_per_ray: N`: FETCH_SIZE + WRITE_SIZE per launch, scaled by the copy calibration,
+    divided by the rays of those launches)."""
     path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("bytes_per_launch:"):
-                return int(float(line.split(":")[1]))
+            if line.startswith("hbm_bytes_per_ray:"):
+                return float(line.split(":")[1])
     return None
 
 
@@ -117,6 +118,8 @@ def main():
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--batch", type=int, default=1 << 21)
     ap.add_argument("--no-autotune", action="store_true", help="keep --batch and the default chunk length")
+    ap.add_argument("--guided", action="store_true",
+                    help="time guided rendering (online-trained network) instead of plain path_mis")
     ap.add_argument("--tune-spp", type=int, default=4, help="spp of the autotuner's calibration renders")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=64)
@@ -182,6 +185,8 @@ def main():
     log(f"[bench] per extension ray {ext_nodes:.2f} nodes {ext_tris:.2f} tris; per shadow ray {sh_nodes:.2f} nodes "
         f"{sh_tris:.2f} tris")
 
+    if args.guided:
+        gpu.set_option("guiding", 1)
     gpu.reset_film()
     for w in range(args.warmup):
         gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=(10 ** 6) + 64 + w * spp), seed=7,
@@ -219,12 +224,15 @@ def main():
         log(f"[bench] timing pass {k + 1}/{args.steps}: extend {s.ms_trace_closest:.1f} ms, shadow "
             f"{s.ms_trace_shadow:.1f} ms, shade {s.ms_shade:.1f} ms, total {s.ms_total:.1f} ms")
     gpu.set_option("timing_mode", 0)
+    gpu.set_option("guiding", 0)
 
     value = acc["samples"] / wall / 1e6
 
     ext_bytes = acc["ext_timed"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
     achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
-    traffic = traffic_from_profiles()
+    traffic_per_ray = traffic_from_profiles()
+    rays_per_launch = acc["ext_timed"] / max(acc["launch_ext"], 1)
+    traffic = None if traffic_per_ray is None else round(traffic_per_ray * rays_per_launch)
 
     ttm = None
     if not args.no_time_to_mse:
@@ -274,7 +282,7 @@ def main():
         "config": {
             "workload": f"Disney icosphere grid 8x8, level {args.subdiv}, {len(scene.indices)} triangles, "
                         f"2048x1024 sky+sun envmap, {cam.width}x{cam.height}, {args.steps * spp} spp, "
-                        f"max depth {args.depth}, plain path_mis",
+                        f"max depth {args.depth}, {'guided' if args.guided else 'plain'} path_mis",
             "spp_per_step": spp,
             "path_pool": tune["pool"] if tune else args.batch,
             "autotune": tune,

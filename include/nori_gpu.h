@@ -190,6 +190,11 @@ int nori_gpu_envmap_eval(nori_ctx *ctx, int n, const float *dir3, float *rgb3, f
 
 /* pcg32 stream check: n outputs of seed(initstate, initseq) advanced by `advance`. */
 int nori_gpu_pcg32(nori_ctx *ctx, uint64_t initstate, uint64_t initseq, uint64_t advance, int n, uint32_t *out);
+/* Owen-scrambled Sobol' stream check (shaders/RNG.hlsli:59-99, InitRNG +
+ * NextFloat): the raw 32-bit draws of dimensions first_dim .. first_dim + n - 1
+ * for pixel index `pixel` (y * width + x) and sample `sample`; the float of a
+ * draw v is min(v * 2^-32, 0.99999994). */
+int nori_gpu_sobol(nori_ctx *ctx, uint32_t pixel, uint32_t sample, uint32_t first_dim, int n, uint32_t *out);
 
 /* Counters and timings of the last nori_gpu_render call.  The per-kernel
  * times are filled when option "timing_mode" is 1 (HIP events around every
@@ -221,7 +226,9 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * "graph" (0/1: replay the bounce loop as a HIP graph of two iterations,
  * default 1; timing_mode forces eager launches), "chunk_iters" (even, pool
  * iterations between done-flag checks and between guide training rounds,
- * default 8), "timing_mode" (0/1), "count_traversal" (0/1), "guiding" (0/1:
+ * default 8), "sampler" (0: pcg32 per pixel, the default; 1: the reference GPU's
+ * Owen-scrambled Sobol', shaders/RNG.hlsli), "sort" (0/1: class-sorted shading
+ * launches, default 1), "timing_mode" (0/1), "count_traversal" (0/1), "guiding" (0/1:
  * guided sampling with one-sample MIS and online training, one round per
  * pool's worth of samples started), "guide_train_steps" (Adam steps per
  * training round, default 4),

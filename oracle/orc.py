@@ -39,7 +39,9 @@ def lib():
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_bvh_nodes.argtypes, L.orc_scene_bvh_nodes.restype = [vp], u32
         L.orc_trace.argtypes = [vp, vp, u32, i32, i32, vp, vp, vp, vp, i32, vp]
-        L.orc_render.argtypes = [vp, vp, u32, u32, u32, u32, u64, i32, i32, vp, vp]
+        L.orc_render.argtypes = [vp, vp, u32, u32, u32, u32, u64, i32, i32, i32, vp, vp]
+        L.orc_sobol.argtypes = [u32, u32, u32, u32, vp]
+        L.orc_sobol_parts.argtypes = [u32, u32, vp, vp, vp]
         L.orc_camera_rays.argtypes = [vp, u32, vp, vp]
         L.orc_alias_build.argtypes = [vp, u32, vp, vp]
         L.orc_env_tables.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -54,6 +56,22 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def sobol(pixel, sample, first_dim, n):
+    """Raw 32-bit draws of dimensions first_dim .. first_dim + n - 1 of the Sobol' stream of (pixel, sample)."""
+    out = np.zeros(n, np.uint32)
+    lib().orc_sobol(pixel, sample, first_dim, n, _p(out))
+    return out
+
+
+def sobol_part(op, a, b=None):
+    """Pieces of the Sobol' sampler: op 0 pcg_hash(a), 1 sobol_2d(a, b), 2 laine_karras(a, b), 3 owen_scramble(a, b)."""
+    a = np.ascontiguousarray(a, np.uint32)
+    b = np.zeros_like(a) if b is None else np.ascontiguousarray(np.broadcast_to(b, a.shape), np.uint32)
+    out = np.zeros_like(a)
+    lib().orc_sobol_parts(op, len(a), _p(a), _p(b), _p(out))
+    return out
 
 
 def pcg32(initstate, initseq, n, advance=0):
@@ -165,12 +183,14 @@ class OracleScene:
             counters[:] = cnt
         return t, prim, u, v
 
-    def render(self, camera, spp, max_depth, rr_depth=3, seed=0, sample_offset=0, env_cdf=False, nthreads=0):
+    def render(self, camera, spp, max_depth, rr_depth=3, seed=0, sample_offset=0, env_cdf=False, nthreads=0,
+               sampler=0):
+        """sampler 0: pcg32 per pixel; 1: Owen-scrambled Sobol' (shaders/RNG.hlsli)."""
         w, h = camera.width, camera.height
         acc = np.zeros((w * h, 4), np.float32)
         stats = np.zeros(3, np.uint64)
         lib().orc_render(self.h, ctypes.byref(camera), spp, sample_offset, max_depth, rr_depth, seed, int(env_cdf),
-                         nthreads, _p(acc), _p(stats))
+                         nthreads, int(sampler), _p(acc), _p(stats))
         return acc, stats
 
     def li(self, rays, seeds, max_depth, rr_depth=3):

@@ -17,6 +17,24 @@ struct Handle {
 
 extern "C" {
 
+// Raw draws of dimensions first_dim .. first_dim + n - 1 of the Sobol' stream of (pixel, sample).
+void orc_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out) {
+    SobolRng r(pixel, sample);
+    r.dim = first_dim;
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next_uint();
+}
+
+void orc_sobol_parts(uint32_t op, uint32_t n, const uint32_t *a, const uint32_t *b, uint32_t *out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        switch (op) {
+        case 0: out[i] = pcg_hash(a[i]); break;
+        case 1: out[i] = sobol_2d(a[i], b[i]); break;
+        case 2: out[i] = laine_karras(a[i], b[i]); break;
+        default: out[i] = owen_scramble(a[i], b[i]); break;
+        }
+    }
+}
+
 void orc_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out) {
     Pcg32 r;
     r.seed(initstate, initseq);
@@ -111,8 +129,9 @@ void orc_trace(void *hp, const float *rays, uint32_t n, int any_hit, int brute, 
 }
 
 void orc_render(void *hp, const Camera *cam, uint32_t spp, uint32_t sample_offset, uint32_t max_depth,
-                uint32_t rr_depth, uint64_t seed, int env_cdf, int nthreads, float *accum, uint64_t *stats3) {
-    RenderParams p{spp, sample_offset, max_depth, rr_depth, seed, env_cdf, nthreads};
+                uint32_t rr_depth, uint64_t seed, int env_cdf, int nthreads, int sampler, float *accum,
+                uint64_t *stats3) {
+    RenderParams p{spp, sample_offset, max_depth, rr_depth, seed, env_cdf, nthreads, sampler};
     RenderStats st;
     render(((Handle *)hp)->s, *cam, p, accum, &st);
     if (stats3) { stats3[0] = st.samples; stats3[1] = st.extension_rays; stats3[2] = st.shadow_rays; }
@@ -171,7 +190,7 @@ void orc_li(void *hp, uint32_t n, const float *rays8, const uint64_t *seeds, uin
             float *rgb3) {
     const SceneData &s = ((Handle *)hp)->s;
     Camera cam{};
-    RenderParams p{1, 0, max_depth, rr_depth, 0, 0, 1};
+    RenderParams p{1, 0, max_depth, rr_depth, 0, 0, 1, 0};
     RenderStats st;
     for (uint32_t i = 0; i < n; ++i) {
         const float *r8 = rays8 + 8 * (size_t)i;

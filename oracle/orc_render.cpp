@@ -111,7 +111,7 @@ struct LightSample {
     V3f p, n, Le;
     float pdfA;
 };
-static LightSample sample_area_light(const SceneData &s, Pcg32 &rng) {
+template <class Rng> static LightSample sample_area_light(const SceneData &s, Rng &rng) {
     const uint32_t E = (uint32_t)s.emitters.size();
     uint32_t pick = std::min((uint32_t)(rng.next_float() * (float)E), E - 1);
     uint32_t em = s.emitters[pick];
@@ -130,7 +130,8 @@ static LightSample sample_area_light(const SceneData &s, Pcg32 &rng) {
 }
 
 // Volume NEE toward the area lights at a scatter point (shaders/Emitter.hlsli:334-385).
-static V3f volume_nee_area(const SceneData &s, V3f P, V3f wo, const Volume &v, Pcg32 &rng, RenderStats &st) {
+template <class Rng>
+static V3f volume_nee_area(const SceneData &s, V3f P, V3f wo, const Volume &v, Rng &rng, RenderStats &st) {
     if (volume_nee_skip(v, P) || s.emitters.empty()) return V3f(0.0f);
     LightSample ls = sample_area_light(s, rng);
     uint32_t tr_seed = rng.next_uint();
@@ -149,7 +150,8 @@ static V3f volume_nee_area(const SceneData &s, V3f P, V3f wo, const Volume &v, P
 }
 
 // Volume NEE toward the envmap (shaders/Emitter.hlsli:387-434).
-static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Pcg32 &rng, const RenderParams &p,
+template <class Rng>
+static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Rng &rng, const RenderParams &p,
                           RenderStats &st) {
     if (volume_nee_skip(v, P) || !s.env.valid()) return V3f(0.0f);
     V3f dir, Le;
@@ -174,7 +176,8 @@ static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Pc
 
 // RayGen body, shaders/Shaders.hlsl:55-627, for the surface materials of the
 // scene (diffuse, mirror, dielectric, microfacet, Disney).
-V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, const RenderParams &p, RenderStats &st) {
+template <class Rng>
+static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, const RenderParams &p, RenderStats &st) {
     (void)cam;
     V3f L(0.0f), T(1.0f);
     float last_pdf = 0.0f, eta = 1.0f;
@@ -406,6 +409,23 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
     return L;
 }
 
+V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, const RenderParams &p, RenderStats &st) {
+    return path_li(s, cam, ray, rng, p, st);
+}
+
+// One camera sample of pixel (x, y) with generator `rng` (pixel jitter, lens,
+// then the path).
+template <class Rng>
+static V3f camera_sample(const SceneData &s, const Camera &cam, uint32_t x, uint32_t y, Rng &rng,
+                         const RenderParams &p, RenderStats &st) {
+    float jx = rng.next_float(), jy = rng.next_float();
+    float ax = 0.5f, ay = 0.5f;
+    if (cam.lens_radius > 0.0f) { ax = rng.next_float(); ay = rng.next_float(); }
+    Ray ray;
+    camera_ray(cam, (float)x + jx, (float)y + jy, ax, ay, ray);
+    return path_li(s, cam, ray, rng, p, st);
+}
+
 void render(const SceneData &s, const Camera &cam, const RenderParams &p, float *accum, RenderStats *stats) {
     const uint32_t W = cam.width, H = cam.height, BS = 32;
     const uint32_t bx = (W + BS - 1) / BS, by = (H + BS - 1) / BS, nblocks = bx * by;
@@ -423,13 +443,14 @@ void render(const SceneData &s, const Camera &cam, const RenderParams &p, float 
                     uint32_t pix = y * W + x;
                     float *a = accum + 4 * (size_t)pix;
                     for (uint32_t si = 0; si < p.spp; ++si) {
-                        Pcg32 rng = sample_stream(p.seed, pix, p.sample_offset + si);
-                        float jx = rng.next_float(), jy = rng.next_float();
-                        float ax = 0.5f, ay = 0.5f;
-                        if (cam.lens_radius > 0.0f) { ax = rng.next_float(); ay = rng.next_float(); }
-                        Ray ray;
-                        camera_ray(cam, (float)x + jx, (float)y + jy, ax, ay, ray);
-                        V3f L = path_mis_li(s, cam, ray, rng, p, st);
+                        V3f L;
+                        if (p.sampler == 1) {
+                            SobolRng rng(pix, p.sample_offset + si);
+                            L = camera_sample(s, cam, x, y, rng, p, st);
+                        } else {
+                            Pcg32 rng = sample_stream(p.seed, pix, p.sample_offset + si);
+                            L = camera_sample(s, cam, x, y, rng, p, st);
+                        }
                         a[0] += L.x; a[1] += L.y; a[2] += L.z; a[3] += 1.0f;
                         ++st.samples;
                     }

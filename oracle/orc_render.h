@@ -21,6 +21,7 @@
 #include "orc_hair.h"
 #include "orc_pcg32.h"
 #include "orc_volume.h"
+#include "orc_sobol.h"
 
 namespace orc {
 
@@ -56,6 +57,7 @@ struct RenderParams {
     uint64_t seed;
     int env_cdf;   // 1: sample the envmap with the reference CDF instead of the alias table
     int nthreads;
+    int sampler;   // 0: pcg32 per pixel; 1: Owen-scrambled Sobol' (shaders/RNG.hlsli)
 };
 
 struct RenderStats {

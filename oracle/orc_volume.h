@@ -189,7 +189,8 @@ inline uint32_t enumerate_segments(const std::vector<Volume> &vols, V3f o, V3f d
 }
 
 // shaders/Volume.hlsl:236-319
-inline bool null_collision_free_flight(const Volume &v, V3f o, V3f d, float tn, float tf, Pcg32 &rng, V3f &weight,
+template <class Rng>
+inline bool null_collision_free_flight(const Volume &v, V3f o, V3f d, float tn, float tf, Rng &rng, V3f &weight,
                                        float &ts) {
     const float gmu = v.sigma_t_max();
     const V3f st = v.sigma_t();
@@ -228,7 +229,8 @@ inline bool null_collision_free_flight(const Volume &v, V3f o, V3f d, float tn, 
 }
 
 // shaders/Volume.hlsl:326-372
-inline V3f ratio_tracking(const Volume &v, V3f o, V3f d, float tn, float tf, Pcg32 &rng) {
+template <class Rng>
+inline V3f ratio_tracking(const Volume &v, V3f o, V3f d, float tn, float tf, Rng &rng) {
     const V3f st = v.sigma_t();
     V3f tr(1.0f);
     float t = tn;
@@ -259,7 +261,8 @@ inline V3f ratio_tracking(const Volume &v, V3f o, V3f d, float tn, float tf, Pcg
 }
 
 // shaders/Volume.hlsl:381-415
-inline bool sample_volume_free_flight(const std::vector<Volume> &vols, V3f o, V3f d, float tmax, Pcg32 &rng,
+template <class Rng>
+inline bool sample_volume_free_flight(const std::vector<Volume> &vols, V3f o, V3f d, float tmax, Rng &rng,
                                       V3f &weight, float &ts, uint32_t &vi) {
     ts = tmax;
     vi = 0;
@@ -280,7 +283,8 @@ inline bool sample_volume_free_flight(const std::vector<Volume> &vols, V3f o, V3
 }
 
 // shaders/Volume.hlsl:419-454
-inline V3f multi_volume_transmittance(const std::vector<Volume> &vols, V3f o, V3f d, float tmax, Pcg32 &rng) {
+template <class Rng>
+inline V3f multi_volume_transmittance(const std::vector<Volume> &vols, V3f o, V3f d, float tmax, Rng &rng) {
     if (vols.empty()) return V3f(1.0f);
     VolSegment seg[8];
     uint32_t n = enumerate_segments(vols, o, d, tmax, seg);

@@ -78,6 +78,7 @@ struct nori_ctx {
     uint32_t *qf[2] = {nullptr, nullptr};  // free path-id lists
     uint32_t *qc = nullptr;                // shading-class queues (3 x cap)
     int sort_shading = 1;                  // option "sort": class-sorted shading launches
+    uint32_t sampler = 0;                  // option "sampler": 0 pcg32 per pixel, 1 Owen-scrambled Sobol'
     bool scene_has_hair = false;
     Counters cdev{};
 
@@ -90,10 +91,11 @@ struct nori_ctx {
     // the buffers it was captured on stay the same.
     struct GraphKey {
         const void *paths, *film, *views, *guide_out, *guide_rec, *stats, *scene;
-        uint32_t cap;
+        uint32_t cap, sampler;
         bool operator==(const GraphKey &o) const {
             return paths == o.paths && film == o.film && views == o.views && guide_out == o.guide_out &&
-                   guide_rec == o.guide_rec && stats == o.stats && scene == o.scene && cap == o.cap;
+                   guide_rec == o.guide_rec && stats == o.stats && scene == o.scene && cap == o.cap &&
+                   sampler == o.sampler;
         }
     };
     hipGraphExec_t graph = nullptr;
@@ -394,6 +396,7 @@ int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n,
         launch_zero_f4(film, nr, c->stream);
         uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, nr);
         c->ensure_paths(cap);
+        c->sdev.sobol = 0; // caller rays come with pcg32 seeds (Nori's Independent sampler)
         for (uint64_t first = 0; first < nr; first += cap) {
             uint32_t m = (uint32_t)std::min<uint64_t>(cap, nr - first);
             launch_counters(c->cdev, 3, 0, 0x1FFu, c->stream);
@@ -697,10 +700,11 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                                     (float4 *)c->g_lpath.p, c->cap, std::min(c->g_max_rec, max_depth)};
                 launch_set_counter((uint32_t *)c->g_scount.p, 0, c->stream);
             }
+            c->sdev.sobol = c->sampler;
             const bool graphed = c->use_graph && !c->timing_mode;
             if (graphed) {
                 nori_ctx::GraphKey key{c->paths.p, c->film.p, c->views_buf.p, gsd.out, gsd.rec, cs,
-                                       (const void *)(uintptr_t)c->scene_gen, c->pool};
+                                       (const void *)(uintptr_t)c->scene_gen, c->pool, c->sampler};
                 if (!c->graph || !(key == c->graph_key)) {
                     c->drop_graph();
                     hipGraph_t g = nullptr;
@@ -889,6 +893,9 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
     } else if (n == "graph") {
         c->use_graph = value != 0;
         c->drop_graph();
+    } else if (n == "sampler") {
+        if (value != 0 && value != 1) return fail(c, NORI_E_ARG, "sampler must be 0 (pcg32) or 1 (Sobol')");
+        c->sampler = (uint32_t)value;
     } else if (n == "sort") {
         c->sort_shading = value != 0;
         c->drop_graph();
@@ -1019,6 +1026,19 @@ int nori_gpu_envmap_eval(nori_ctx *c, int n, const float *dir3, float *rgb3, flo
         HIPCHK(hipMemcpy(rgb3, b2.p, s3, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(pdf, p.p, (size_t)n * 4, hipMemcpyDeviceToHost));
         a.release(); b2.release(); p.release();
+    });
+}
+
+int nori_gpu_sobol(nori_ctx *c, uint32_t pixel, uint32_t sample, uint32_t first_dim, int n, uint32_t *out) {
+    if (!c || n < 0 || (n > 0 && !out)) return NORI_E_ARG;
+    return guard(c, [&] {
+        if (n == 0) return;
+        ScopedBufs b(1);
+        HIPCHK(hipMalloc(&b[0].p, (size_t)n * sizeof(uint32_t)));
+        launch_sobol(pixel, sample, first_dim, (uint32_t)n, (uint32_t *)b[0].p, c->stream);
+        HIPCHK(hipMemcpyAsync(out, b[0].p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
     });
 }
 

@@ -147,4 +147,72 @@ NG_HD uint64_t sample_stream(uint64_t render_seed, uint32_t pixel, uint32_t samp
     return pcg_advance(st, (uint64_t)sample << SAMPLE_WINDOW_LOG2);
 }
 
+// ------------------------------------------------- Owen-scrambled Sobol'
+// The progressive sampler of shaders/RNG.hlsli:9-99 (Burley 2020, "Practical
+// Hash-based Owen Scrambling"): per pixel a scramble seed, per sample its
+// index; dimension d draws axis d % 2 of the 2-D Sobol' point of pad d / 2, the
+// sample index shuffled per pad and the value Owen-scrambled per pixel, pad
+// and axis.  Integer arithmetic throughout, so host and device agree bit for
+// bit.
+NG_HD uint32_t pcg_hash(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+NG_HD uint32_t reverse_bits32(uint32_t x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_bitreverse32(x);
+#else
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0F0F0F0Fu) | ((x & 0x0F0F0F0Fu) << 4);
+    x = ((x >> 8) & 0x00FF00FFu) | ((x & 0x00FF00FFu) << 8);
+    return (x >> 16) | (x << 16);
+#endif
+}
+// Sobol' base 2, MSB-aligned: axis 0 is the van der Corput sequence, axis 1
+// uses the direction numbers v_{k+1} = v_k ^ (v_k >> 1), v_1 = 2^31.
+NG_HD uint32_t sobol2d(uint32_t index, uint32_t axis) {
+    if (axis == 0u) return reverse_bits32(index);
+    uint32_t r = 0u, dirn = 1u << 31;
+    for (uint32_t i = index; i != 0u; i >>= 1u) {
+        if (i & 1u) r ^= dirn;
+        dirn ^= dirn >> 1u;
+    }
+    return r;
+}
+NG_HD uint32_t laine_karras(uint32_t x, uint32_t seed) {
+    x += seed;
+    x ^= x * 0x6c50b47cu;
+    x ^= x * 0xb82f1e52u;
+    x ^= x * 0xc7afe638u;
+    x ^= x * 0x8d22f6e6u;
+    return x;
+}
+NG_HD uint32_t owen_scramble(uint32_t x, uint32_t seed) {
+    return reverse_bits32(laine_karras(reverse_bits32(x), seed));
+}
+struct SobolRng {
+    uint32_t pixel_seed, index, dim;
+};
+// pixel: y * width + x of the view; sample: the progressive sample number.
+NG_HD SobolRng sobol_init(uint32_t pixel, uint32_t sample, uint32_t dim = 0) {
+    return SobolRng{pcg_hash(pixel ^ 0x9e3779b9u), sample, dim};
+}
+NG_HD uint32_t sobol_next_u32(SobolRng &r) {
+    const uint32_t d = r.dim++;
+    const uint32_t axis = d & 1u, pad = d >> 1u;
+    const uint32_t sidx = owen_scramble(r.index, pcg_hash(pad * 0x9e3779b9u + 1u));
+    return owen_scramble(sobol2d(sidx, axis), pcg_hash(r.pixel_seed + pad * 0x6c8e9cf5u + axis * 0x68bc21ebu));
+}
+NG_HD float sobol_float(SobolRng &r) {
+    return fminf((float)sobol_next_u32(r) * 2.3283064365386963e-10f, 0.99999994f);
+}
+
+// The path stream as either generator: rng_float / rng_u32 take one dimension.
+NG_HD float rng_float(uint64_t &s) { return pcg_float(s); }
+NG_HD uint32_t rng_u32(uint64_t &s) { return pcg_next(s); }
+NG_HD float rng_float(SobolRng &s) { return sobol_float(s); }
+NG_HD uint32_t rng_u32(SobolRng &s) { return sobol_next_u32(s); }
+
 } // namespace ng

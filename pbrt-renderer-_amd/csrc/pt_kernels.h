@@ -32,6 +32,8 @@ struct SceneDev {
     // per mesh {extIOR, intIOR} of mirror/dielectric meshes, {0, 0} otherwise:
     // shadow rays pass those with Fresnel attenuation (null: none in the scene)
     const float2 *pass_ior;
+    // path sampler: 0 pcg32 per pixel, 1 Owen-scrambled Sobol' (shaders/RNG.hlsli)
+    uint32_t sobol;
 };
 
 // Guiding hooks of one render batch.  out: network outputs per queue position
@@ -162,5 +164,6 @@ void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const flo
                        const float *jy, float *dir, float *rgb, float *pdf, uint32_t *texel, hipStream_t st);
 void launch_env_eval(const EnvDev &e, uint32_t n, const float *dir, float *rgb, float *pdf, hipStream_t st);
 void launch_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out, hipStream_t st);
+void launch_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out, hipStream_t st);
 
 } // namespace ng

@@ -10,6 +10,7 @@
 #include "bsdf_dev.h"
 #include "pt_kernels.h"
 #include "trace_dev.h"
+#include <type_traits>
 
 namespace ng {
 
@@ -25,6 +26,26 @@ __device__ __forceinline__ uint64_t rng_load(float4 aux) {
     return ((uint64_t)__float_as_uint(aux.w) << 32) | (uint64_t)__float_as_uint(aux.z);
 }
 __device__ __forceinline__ f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+
+// The path stream in the last two words of PathsDev::aux: the pcg32 state, or
+// the Sobol' sample index and next dimension (the pixel's scramble seed is
+// recomputed from the pixel).
+template <bool SOBOL> using PathRng = typename std::conditional<SOBOL, SobolRng, uint64_t>::type;
+__device__ __forceinline__ float2 rng_words(uint64_t r) {
+    return make_float2(__uint_as_float((uint32_t)r), __uint_as_float((uint32_t)(r >> 32)));
+}
+__device__ __forceinline__ float2 rng_words(const SobolRng &r) {
+    return make_float2(__uint_as_float(r.index), __uint_as_float(r.dim));
+}
+template <bool SOBOL>
+__device__ __forceinline__ PathRng<SOBOL> rng_start(uint64_t seed, uint32_t pixel, uint32_t sample) {
+    if constexpr (SOBOL) return sobol_init(pixel, sample);
+    else return sample_stream(seed, pixel, sample);
+}
+template <bool SOBOL> __device__ __forceinline__ PathRng<SOBOL> rng_resume(float4 aux, uint32_t pixel) {
+    if constexpr (SOBOL) return sobol_init(pixel, __float_as_uint(aux.z), __float_as_uint(aux.w));
+    else return rng_load(aux);
+}
 __device__ __forceinline__ bool finite3(f3 c) { return isfinite(c.x) && isfinite(c.y) && isfinite(c.z); }
 
 __device__ __forceinline__ void film_add(float4 *film, uint32_t pixel, f3 c) {
@@ -134,6 +155,7 @@ __device__ __forceinline__ uint32_t regen_count(const Counters &c, uint32_t nfre
     return base >= total ? 0u : (uint32_t)min((unsigned long long)nfree, total - base);
 }
 
+template <bool SOBOL>
 __global__ __launch_bounds__(256) void k_regen(SceneDev s, PathsDev p, const ViewTab *__restrict__ views, Counters c,
                                                const uint32_t *__restrict__ q_free, const uint32_t *__restrict__ cnt_free,
                                                const uint32_t *__restrict__ cnt_act, GuideShadeDev g) {
@@ -153,12 +175,12 @@ __global__ __launch_bounds__(256) void k_regen(SceneDev s, PathsDev p, const Vie
     const unsigned long long l = gidx - v.first;
     const uint32_t pixel = (uint32_t)(l % v.npix);
     const uint32_t sample = v.sample_offset + (uint32_t)(l / v.npix);
-    uint64_t rng = sample_stream(c.next[2], pixel, sample);
-    const float jx = pcg_float(rng), jy = pcg_float(rng);
+    PathRng<SOBOL> rng = rng_start<SOBOL>(c.next[2], pixel, sample);
+    const float jx = rng_float(rng), jy = rng_float(rng);
     float ax = 0.5f, ay = 0.5f;
     if (v.cam.lens_radius > 0.0f) {
-        ax = pcg_float(rng);
-        ay = pcg_float(rng);
+        ax = rng_float(rng);
+        ay = rng_float(rng);
     }
     const uint32_t x = pixel % v.cam.width, y = pixel / v.cam.width;
     f3 o, d;
@@ -168,8 +190,8 @@ __global__ __launch_bounds__(256) void k_regen(SceneDev s, PathsDev p, const Vie
     p.ray_o[k] = make_float4(o.x, o.y, o.z, mint);
     p.ray_d[k] = make_float4(d.x, d.y, d.z, maxt);
     p.thr[k] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    p.aux[k] = make_float4(1.0f, __uint_as_float(fe), __uint_as_float((uint32_t)rng),
-                           __uint_as_float((uint32_t)(rng >> 32)));
+    const float2 rw = rng_words(rng);
+    p.aux[k] = make_float4(1.0f, __uint_as_float(fe), rw.x, rw.y);
     p.meta[k] = lo << 16;
     p.id[k] = id;
     if (g.rec) {
@@ -422,12 +444,12 @@ struct LightPt {
     f3 p, n, Le;
     float pdfA;
 };
-__device__ __forceinline__ LightPt sample_light(const SceneDev &s, uint64_t &rng) {
-    uint32_t pick = min((uint32_t)(pcg_float(rng) * (float)s.n_emit), s.n_emit - 1);
+template <class R> __device__ __forceinline__ LightPt sample_light(const SceneDev &s, R &rng) {
+    uint32_t pick = min((uint32_t)(rng_float(rng) * (float)s.n_emit), s.n_emit - 1);
     uint32_t em = s.emitters[pick];
     uint32_t ntri = s.mats[em].indexCount / 3;
     const float *cdf = s.emit_cdf + s.emit_cdf_off[pick];
-    float uc = pcg_float(rng);
+    float uc = rng_float(rng);
     uint32_t lo = 0, hi = ntri;
     while (lo < hi) {
         uint32_t mid = (lo + hi) >> 1;
@@ -440,7 +462,7 @@ __device__ __forceinline__ LightPt sample_light(const SceneDev &s, uint64_t &rng
     f3 q0 = f3{s.pos[3 * j0], s.pos[3 * j0 + 1], s.pos[3 * j0 + 2]};
     f3 q1 = f3{s.pos[3 * j1], s.pos[3 * j1 + 1], s.pos[3 * j1 + 2]};
     f3 q2 = f3{s.pos[3 * j2], s.pos[3 * j2 + 1], s.pos[3 * j2 + 2]};
-    float r1 = pcg_float(rng), r2 = pcg_float(rng);
+    float r1 = rng_float(rng), r2 = rng_float(rng);
     float sr1 = sqrtf(r1);
     LightPt lp;
     lp.p = (1.0f - sr1) * q0 + sr1 * (1.0f - r2) * q1 + sr1 * r2 * q2;
@@ -456,7 +478,7 @@ __device__ __forceinline__ LightPt sample_light(const SceneDev &s, uint64_t &rng
 // the others take the positions k_extend sorted into their queue (q_cls), so
 // each launch carries only its class's code: misses and emitter hits
 // (CLS_LIGHT), scattering surfaces but hair (CLS_SURF), hair (CLS_HAIR).
-template <bool VOL, bool GUIDE, int CLS>
+template <bool VOL, bool GUIDE, int CLS, bool SOBOL>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, PathsDev p_out,
                                                        const ViewTab *__restrict__ views,
                                                        const uint32_t *__restrict__ q_cls,
@@ -474,7 +496,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     f3 T = xyz(th);
     float last_pdf = th.w, eta = ax.x;
     const uint32_t pixel = __float_as_uint(ax.y);
-    uint64_t rng = rng_load(ax);
+    PathRng<SOBOL> rng = rng_resume<SOBOL>(ax, pixel - views[meta >> 16].film_off);
     const uint32_t prim = __float_as_uint(hit.y);
 
     bool alive = false, sh_area = false, sh_env = false;
@@ -510,7 +532,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
             // of BSDF and cosine, shadow rays from the scatter point (tmin 0.001).
             if (!skip && s.n_emit > 0) {
                 LightPt lp = sample_light(s, rng);
-                uint32_t seed = pcg_next(rng);
+                uint32_t seed = rng_u32(rng);
                 f3 tl = lp.p - P;
                 float dist = length(tl);
                 f3 wi = tl / dist;
@@ -525,12 +547,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 }
             }
             if (!skip && s.has_env) {
-                uint32_t r = pcg_next(rng);
-                float coin = pcg_float(rng), jx = pcg_float(rng), jy = pcg_float(rng);
+                uint32_t r = rng_u32(rng);
+                float coin = rng_float(rng), jx = rng_float(rng), jy = rng_float(rng);
                 f3 dir, Le;
                 float pe;
                 env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
-                uint32_t seed = pcg_next(rng);
+                uint32_t seed = rng_u32(rng);
                 if (pe > 0.0f) {
                     float fp = hg_eval(dot(d, dir), v.g);
                     float w = balance_heuristic(pe, fp);
@@ -538,7 +560,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     sh_env = true;
                 }
             }
-            float xi1 = pcg_float(rng), xi2 = pcg_float(rng), ppdf;
+            float xi1 = rng_float(rng), xi2 = rng_float(rng), ppdf;
             new_d = hg_sample(d, v.g, xi1, xi2, ppdf);
             new_o = P;
             last_pdf = fmaxf(ppdf, 1e-20f);
@@ -608,7 +630,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 float Fr = fresnel_dielectric(dot(-d, N), etaI, etaT);
                 f3 nd;
                 bool refr = false;
-                if (pcg_float(rng) < Fr) {
+                if (rng_float(rng) < Fr) {
                     nd = reflect(d, N);
                 } else {
                     nd = refract(d, N, etaI / etaT);
@@ -639,7 +661,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
                     LightPt lp = sample_light(s, rng);
-                    uint32_t seed = VOL && s.n_vol ? pcg_next(rng) : 0u;
+                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : 0u;
                     f3 tl = lp.p - P;
                     float dist = length(tl);
                     f3 ww = tl / dist;
@@ -658,12 +680,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 }
                 // Envmap NEE, shaders/Emitter.hlsli:275-326 (alias-table draw)
                 if (s.has_env) {
-                    uint32_t r = pcg_next(rng);
-                    float coin = pcg_float(rng), jx = pcg_float(rng), jy = pcg_float(rng);
+                    uint32_t r = rng_u32(rng);
+                    float coin = rng_float(rng), jx = rng_float(rng), jy = rng_float(rng);
                     f3 dir, Le;
                     float pe;
                     env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
-                    uint32_t seed = VOL && s.n_vol ? pcg_next(rng) : 0u;
+                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : 0u;
                     float ct = dot(N, dir);
                     if (pe > 0.0f && (hair || ct > 0.0f)) {
                         f3 f;
@@ -681,10 +703,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 float b_pdf = 0.0f;
                 if (!guided) {
                     float u[4];
-                    u[0] = pcg_float(rng);
-                    u[1] = pcg_float(rng);
-                    u[2] = type == NORI_MAT_DISNEY || hair ? pcg_float(rng) : 0.0f;
-                    u[3] = hair ? pcg_float(rng) : 0.0f;
+                    u[0] = rng_float(rng);
+                    u[1] = rng_float(rng);
+                    u[2] = type == NORI_MAT_DISNEY || hair ? rng_float(rng) : 0.0f;
+                    u[3] = hair ? rng_float(rng) : 0.0f;
                     f3 wo;
                     float pdf;
                     f3 wgt = material_sample(wi, u, m, hh, wo, pdf);
@@ -699,15 +721,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     }
                 } else {
                     f3 wo;
-                    if (pcg_float(rng) < mix.c) {
-                        float ul = pcg_float(rng), u1 = pcg_float(rng), u2 = pcg_float(rng);
+                    if (rng_float(rng) < mix.c) {
+                        float ul = rng_float(rng), u1 = rng_float(rng), u2 = rng_float(rng);
                         w_world = guide_sample(mix, ul, u1, u2);
                         wo = to_local(w_world, Tg, B, N);
                     } else {
                         float u[4];
-                        u[0] = pcg_float(rng);
-                        u[1] = pcg_float(rng);
-                        u[2] = type == NORI_MAT_DISNEY ? pcg_float(rng) : 0.0f;
+                        u[0] = rng_float(rng);
+                        u[1] = rng_float(rng);
+                        u[2] = type == NORI_MAT_DISNEY ? rng_float(rng) : 0.0f;
                         u[3] = 0.0f;
                         float pdf;
                         material_sample(wi, u, m, 0.0f, wo, pdf);
@@ -757,7 +779,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     const bool more = bounce + 1 < max_depth;
     if (alive && more && bounce >= rr_depth) {
         float q = fminf(maxc(T) * eta * eta, 0.95f);
-        if (pcg_float(rng) >= q) alive = false;
+        if (rng_float(rng) >= q) alive = false;
         else T = T * (1.0f / q);
     }
     alive = alive && more;
@@ -768,7 +790,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         p_out.ray_o[k] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
         p_out.ray_d[k] = make_float4(new_d.x, new_d.y, new_d.z, 1e20f);
         p_out.thr[k] = make_float4(T.x, T.y, T.z, last_pdf);
-        p_out.aux[k] = make_float4(eta, ax.y, __uint_as_float((uint32_t)rng), __uint_as_float((uint32_t)(rng >> 32)));
+        const float2 rw = rng_words(rng);
+        p_out.aux[k] = make_float4(eta, ax.y, rw.x, rw.y);
         p_out.meta[k] = meta + 1u;
         p_out.id[k] = id;
     }
@@ -951,6 +974,14 @@ __global__ void k_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, 
     for (uint32_t i = 0; i < n; ++i) out[i] = pcg_next(st, inc);
 }
 
+// Draws first_dim .. first_dim + n - 1 of the Sobol' stream of (pixel, sample).
+__global__ void k_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    SobolRng r = sobol_init(pixel, sample, first_dim + i);
+    out[i] = sobol_next_u32(r);
+}
+
 inline uint32_t blocks(uint64_t n, int b) { return (uint32_t)((n + b - 1) / b); }
 
 } // namespace
@@ -970,7 +1001,8 @@ void launch_pool_start(Counters c, uint32_t *q_free0, uint32_t cap, hipStream_t 
 
 void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Counters c, const uint32_t *q_free,
                   const uint32_t *cnt_free, uint32_t *cnt_act, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
-    hipLaunchKernelGGL(k_regen, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, views, c, q_free, cnt_free, cnt_act, g);
+    hipLaunchKernelGGL(s.sobol ? k_regen<true> : k_regen<false>, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, views,
+                       c, q_free, cnt_free, cnt_act, g);
     hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, st, c, cnt_free, cnt_act);
 }
 
@@ -1018,31 +1050,38 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
                        q_cls, cls_cnt, cls_stride);
 }
 
+template <int CLS>
+static decltype(&k_shade<false, false, CLS, false>) shade_kernel(bool vol, bool guide, bool sobol) {
+    decltype(&k_shade<false, false, CLS, false>) ks[8] = {
+        k_shade<false, false, CLS, false>, k_shade<false, false, CLS, true>, k_shade<false, true, CLS, false>,
+        k_shade<false, true, CLS, true>,   k_shade<true, false, CLS, false>, k_shade<true, false, CLS, true>,
+        k_shade<true, true, CLS, false>,   k_shade<true, true, CLS, true>};
+    return ks[(vol ? 4 : 0) | (guide ? 2 : 0) | (sobol ? 1 : 0)];
+}
+
 void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
                   uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
-    const bool vol = s.n_vol > 0, guide = g.out != nullptr || g.rec != nullptr;
-    auto k = vol ? (guide ? k_shade<true, true, CLS_ALL> : k_shade<true, false, CLS_ALL>)
-                 : (guide ? k_shade<false, true, CLS_ALL> : k_shade<false, false, CLS_ALL>);
-    hipLaunchKernelGGL(k, dim3(blocks(cap, SHADE_BLOCK)), dim3(SHADE_BLOCK), 0, st, s, p, p_out, views, nullptr,
-                       cnt_in, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    const bool guide = g.out != nullptr || g.rec != nullptr;
+    hipLaunchKernelGGL(shade_kernel<CLS_ALL>(s.n_vol > 0, guide, s.sobol != 0), dim3(blocks(cap, SHADE_BLOCK)),
+                       dim3(SHADE_BLOCK), 0, st, s, p, p_out, views, nullptr, cnt_in, cnt_out, q_free, cnt_free, sh,
+                       cnt_sh, film, g);
 }
 
 void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                          const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
                          float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
-    const bool guide = g.out != nullptr || g.rec != nullptr;
+    const bool guide = g.out != nullptr || g.rec != nullptr, sobol = s.sobol != 0;
     const dim3 grid(blocks(cap, SHADE_BLOCK)), block(SHADE_BLOCK);
     const size_t cs = cls_stride;
-    hipLaunchKernelGGL((guide ? k_shade<false, true, CLS_LIGHT> : k_shade<false, false, CLS_LIGHT>), grid, block, 0,
-                       st, s, p, p_out, views, q_cls, cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
-    hipLaunchKernelGGL((guide ? k_shade<false, true, CLS_SURF> : k_shade<false, false, CLS_SURF>), grid, block, 0, st,
-                       s, p, p_out, views, q_cls + cs, cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    hipLaunchKernelGGL(shade_kernel<CLS_LIGHT>(false, guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls,
+                       cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    hipLaunchKernelGGL(shade_kernel<CLS_SURF>(false, guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls + cs,
+                       cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
     if (hair)
-        hipLaunchKernelGGL((guide ? k_shade<false, true, CLS_HAIR> : k_shade<false, false, CLS_HAIR>), grid, block, 0,
-                           st, s, p, p_out, views, q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free,
-                           sh, cnt_sh, film, g);
+        hipLaunchKernelGGL(shade_kernel<CLS_HAIR>(false, guide, sobol), grid, block, 0, st, s, p, p_out, views,
+                           q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
@@ -1081,6 +1120,10 @@ void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const flo
 
 void launch_env_eval(const EnvDev &e, uint32_t n, const float *dir, float *rgb, float *pdf, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_env_eval, dim3(blocks(n, 256)), dim3(256), 0, st, e, n, dir, rgb, pdf);
+}
+
+void launch_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_sobol, dim3(blocks(n, 256)), dim3(256), 0, st, pixel, sample, first_dim, n, out);
 }
 
 void launch_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out, hipStream_t st) {

@@ -146,7 +146,8 @@ NG_HD uint32_t vol_segments(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, floa
 }
 
 // shaders/Volume.hlsl:236-319
-NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, uint64_t &rng, f3 &weight, float &ts) {
+template <class R>
+NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, R &rng, f3 &weight, float &ts) {
 #pragma clang fp contract(off)
     const float gmu = vol_sigma_t_max(v);
     const f3 st = vol_sigma_t(v);
@@ -160,7 +161,7 @@ NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, u
             if (t >= tf) { ts = tf; return false; }
             continue;
         }
-        float xi = fmaxf(pcg_float(rng), 1e-10f);
+        float xi = fmaxf(rng_float(rng), 1e-10f);
         float dt = -logf(xi) / mu;
         if (dt > bd) {
             t += bd;
@@ -171,7 +172,7 @@ NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, u
         if (t >= tf) { ts = tf; return false; }
         float dens = vol_density(v, o + d * t);
         float p_real = fminf(fmaxf((dens * gmu) / fmaxf(mu, 1e-20f), 0.0f), 1.0f);
-        if (p_real >= 1.0f - 1e-6f || pcg_float(rng) < p_real) {
+        if (p_real >= 1.0f - 1e-6f || rng_float(rng) < p_real) {
             float inv = 1.0f / fmaxf(gmu, 1e-20f);
             weight = weight * mk(st.x * inv, st.y * inv, st.z * inv);
             ts = t;
@@ -185,7 +186,8 @@ NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, u
 }
 
 // shaders/Volume.hlsl:326-372
-NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, uint64_t &rng) {
+template <class R>
+NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, R &rng) {
 #pragma clang fp contract(off)
     const f3 st = vol_sigma_t(v);
     f3 tr = mk(1.0f);
@@ -199,7 +201,7 @@ NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, 
             if (t >= tf) return tr;
             continue;
         }
-        float xi = fmaxf(pcg_float(rng), 1e-10f);
+        float xi = fmaxf(rng_float(rng), 1e-10f);
         float dt = -logf(xi) / mu;
         if (dt > bd) {
             t += bd;
@@ -217,7 +219,8 @@ NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, 
 }
 
 // shaders/Volume.hlsl:381-415
-NG_HD bool vol_sample_free_flight(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, uint64_t &rng,
+template <class R>
+NG_HD bool vol_sample_free_flight(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, R &rng,
                                   f3 &weight, float &ts, uint32_t &vi) {
     ts = tmax;
     vi = 0;
@@ -238,7 +241,8 @@ NG_HD bool vol_sample_free_flight(const VolumeDev *vols, uint32_t nv, f3 o, f3 d
 }
 
 // shaders/Volume.hlsl:419-454
-NG_HD f3 vol_transmittance(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, uint64_t &rng) {
+template <class R>
+NG_HD f3 vol_transmittance(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, R &rng) {
 #pragma clang fp contract(off)
     if (nv == 0) return mk(1.0f);
     VolSeg seg[8];

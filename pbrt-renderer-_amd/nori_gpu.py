@@ -129,7 +129,7 @@ class SceneOptions(ctypes.Structure):
 EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
     "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
-    "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_render_stats",
+    "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_render_stats",
     "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_gpu_autotune", "nori_scene_generate", "nori_scene_get_desc",
     "nori_scene_get_camera", "nori_scene_free", "nori_bvh8_build_host",
 ]
@@ -159,6 +159,8 @@ def load_library(path=LIB_PATH):
         "nori_gpu_envmap_sample": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
         "nori_gpu_pcg32": (i32, [vp, _u64, _u64, _u64, i32, vp]),
+        "nori_gpu_sobol": (i32, [vp, _u32, _u32, _u32, i32, vp]),
+        "nori_gpu_autotune": (i32, [vp, ctypes.POINTER(View), i32, _u32, ctypes.POINTER(TuneResult)]),
         "nori_gpu_render_stats": (i32, [vp, ctypes.POINTER(RenderStats)]),
         "nori_gpu_bvh_info": (i32, [vp, ctypes.POINTER(BvhInfo)]),
         "nori_gpu_set_option": (i32, [vp, ctypes.c_char_p, ctypes.c_int64]),
@@ -457,6 +459,12 @@ class NoriGPU:
     def pcg32(self, initstate, initseq, advance, n):
         out = np.zeros(n, np.uint32)
         self._check(self.lib.nori_gpu_pcg32(self.h, initstate, initseq, advance, n, _ptr(out)), "pcg32")
+        return out
+
+    def sobol(self, pixel, sample, first_dim, n):
+        """Raw 32-bit Owen-scrambled Sobol' draws of dimensions first_dim .. first_dim + n - 1."""
+        out = np.zeros(n, np.uint32)
+        self._check(self.lib.nori_gpu_sobol(self.h, pixel, sample, first_dim, n, _ptr(out)), "sobol")
         return out
 
 
