@@ -72,6 +72,19 @@ typedef struct nori_volume {
     uint32_t brick;                 /* majorant brick edge in voxels, 0: 16    */
 } nori_volume;
 
+/* RGBA8 texture (src/gpu/DXRApp_Textures.cpp:33-130): width x height texels,
+ * row 0 first, 4 bytes per texel.  srgb: colour data decoded from sRGB when
+ * sampled (albedo maps); otherwise linear (normal, roughness, specular,
+ * subsurface and alpha maps).  The library builds the full mip chain with the
+ * reference's 2x2 box filter on the 8-bit values; samples are trilinear with
+ * wrapped coordinates (the reference's MIN_MAG_MIP_LINEAR / WRAP sampler,
+ * src/gpu/DXRApp_Pipeline.cpp:48-51).  Materials refer to textures by index
+ * into nori_scene_desc.textures (NORI_NO_TEXTURE: none). */
+typedef struct nori_texture {
+    uint32_t width, height, srgb;
+    const uint8_t *rgba;
+} nori_texture;
+
 /* Flat scene, as src/gpu/DXRApp_Scene.cpp:172-451 packs it (SoA, one material
  * per mesh).  All arrays are copied by nori_gpu_upload_scene. */
 typedef struct nori_scene_desc {
@@ -89,10 +102,13 @@ typedef struct nori_scene_desc {
     float env_scale;
     uint32_t num_volumes;           /* at most NORI_MAX_VOLUMES                */
     const nori_volume *volumes;
+    uint32_t num_textures;          /* at most NORI_MAX_TEXTURES               */
+    const nori_texture *textures;
 } nori_scene_desc;
 
 #define NORI_MAX_VOLUMES 8
 #define NORI_MAX_VIEWS 4096
+#define NORI_MAX_TEXTURES 4096
 
 /* Nori's perspective camera (src/cpu/perspective.cpp:21-123): horizontal fov,
  * optional thin lens. */
@@ -195,6 +211,11 @@ int nori_gpu_pcg32(nori_ctx *ctx, uint64_t initstate, uint64_t initseq, uint64_t
  * for pixel index `pixel` (y * width + x) and sample `sample`; the float of a
  * draw v is min(v * 2^-32, 0.99999994). */
 int nori_gpu_sobol(nori_ctx *ctx, uint32_t pixel, uint32_t sample, uint32_t first_dim, int n, uint32_t *out);
+/* Texture lookup check (the g_textures[i].SampleLevel(g_sampler, uv, lod) of
+ * shaders/Shaders.hlsl:232-315): n trilinear samples of the uploaded scene's
+ * textures tex[i] at (uv2[2i], uv2[2i+1]) and level of detail lod[i]; rgba4
+ * receives linear RGBA floats. */
+int nori_gpu_texture_sample(nori_ctx *ctx, int n, const uint32_t *tex, const float *uv2, const float *lod, float *rgba4);
 
 /* Counters and timings of the last nori_gpu_render call.  The per-kernel
  * times are filled when option "timing_mode" is 1 (HIP events around every
@@ -301,8 +322,10 @@ typedef struct nori_scene_options {
     uint32_t grid;          /* spheres per grid side (config 2)            */
     uint32_t env_width, env_height;
     uint32_t width, height; /* image size of the default camera            */
-    uint32_t strands;       /* config 3: number of strands (default 50000) */
+    uint32_t strands;       /* config 3: number of strands (default 50000); config 5: alpha-tested hair cards (20000) */
     uint32_t variant;       /* config 3: 0 lit, 1 white furnace           */
+    uint32_t tex_size;      /* config 5: texture side (default 1024)      */
+    uint32_t meshes;        /* config 5: displaced icospheres (default 199) */
 } nori_scene_options;
 
 typedef struct nori_scene nori_scene;

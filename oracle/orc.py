@@ -32,7 +32,8 @@ def lib():
         L.orc_pcg32_sample_floats.argtypes = [u64, u32, u32, u32, vp]
         L.orc_pcg_hash.argtypes, L.orc_pcg_hash.restype = [u32], u32
         L.orc_scene_create.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, u32, u32, vp, ctypes.c_float, vp,
-                                       vp, u32, vp]
+                                       vp, u32, vp, u32, vp]
+        L.orc_texture_sample.argtypes = [vp, u32, vp, vp, vp, vp]
         L.orc_volume_query.argtypes = [vp, i32, u32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_hg.argtypes = [i32, u32, vp, vp, vp, vp, vp, vp]
         L.orc_scene_create.restype = vp
@@ -158,13 +159,29 @@ class OracleScene:
             import ctypes as _c
             vol_structs = (type(media[0].struct()) * len(media))(*[m.struct() for m in media])
         desc_vols = (len(media), None if vol_structs is None else ctypes.cast(vol_structs, ctypes.c_void_p))
-        self._keep = (scene, tri_mesh, vol_structs)
+        texs = getattr(scene, "textures", None) or []
+
+        class _Tex(ctypes.Structure):
+            _fields_ = [("w", ctypes.c_uint32), ("h", ctypes.c_uint32), ("srgb", ctypes.c_uint32),
+                        ("rgba", ctypes.c_void_p)]
+
+        tex_structs = (_Tex * max(1, len(texs)))(*[_Tex(t.shape[1], t.shape[0], int(sr), _p(t)) for t, sr in texs])
+        self._keep = (scene, tri_mesh, vol_structs, tex_structs)
         self.h = L.orc_scene_create(
             len(scene.positions), _p(scene.positions), _p(scene.normals), len(scene.indices), _p(scene.indices),
             _p(tri_mesh), len(scene.materials), _p(scene.materials), _p(scene.mesh_tri_offset),
             _p(scene.mesh_tri_count), 0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
             _p(env), scene.env_scale, _p(getattr(scene, "uvs", None)), _p(getattr(scene, "tangents", None)),
-            desc_vols[0], desc_vols[1])
+            desc_vols[0], desc_vols[1], len(texs), ctypes.cast(tex_structs, ctypes.c_void_p))
+
+    def texture_sample(self, tex, uv, lod):
+        """Trilinear lookups of the scene's textures: tex (n,), uv (n, 2), lod (n,) -> RGBA (n, 4)."""
+        tex = np.ascontiguousarray(tex, np.uint32)
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        lod = np.ascontiguousarray(np.broadcast_to(lod, tex.shape), np.float32)
+        out = np.zeros((len(tex), 4), np.float32)
+        lib().orc_texture_sample(self.h, len(tex), _p(tex), _p(uv), _p(lod), _p(out))
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -60,7 +60,8 @@ struct VolDesc {
 void *orc_scene_create(uint32_t nv, const float *pos, const float *nrm, uint32_t nt, const uint32_t *idx,
                        const uint32_t *tri_mesh, uint32_t nmesh, const void *mats, const uint32_t *mesh_tri_offset,
                        const uint32_t *mesh_tri_count, uint32_t env_w, uint32_t env_h, const float *env_rgb,
-                       float env_scale, const float *uv, const float *tan, uint32_t nvol, const void *vols) {
+                       float env_scale, const float *uv, const float *tan, uint32_t nvol, const void *vols,
+                       uint32_t ntex, const void *texs) {
     Handle *h = new Handle();
     SceneData &s = h->s;
     s.nv = nv; s.nt = nt; s.nmesh = nmesh;
@@ -92,8 +93,30 @@ void *orc_scene_create(uint32_t nv, const float *pos, const float *nrm, uint32_t
         }
         s.vols.push_back(std::move(v));
     }
+    struct TexDesc {
+        uint32_t w, h, srgb;
+        const uint8_t *rgba;
+    };
+    for (uint32_t i = 0; i < ntex; ++i) {
+        const TexDesc &d = ((const TexDesc *)texs)[i];
+        Texture t;
+        t.build(d.w, d.h, d.srgb != 0, d.rgba);
+        s.tex.push_back(std::move(t));
+    }
     s.finalize();
     return h;
+}
+
+// n trilinear lookups of the scene's textures: tex[i] at uv2[2i..], lod[i] -> rgba4.
+void orc_texture_sample(void *hp, uint32_t n, const uint32_t *tex, const float *uv2, const float *lod, float *rgba4) {
+    const SceneData &s = ((Handle *)hp)->s;
+    for (uint32_t i = 0; i < n; ++i) {
+        const Rgba c = sample(s.tex[tex[i]], uv2[2 * i], uv2[2 * i + 1], lod[i]);
+        rgba4[4 * i] = c.r;
+        rgba4[4 * i + 1] = c.g;
+        rgba4[4 * i + 2] = c.b;
+        rgba4[4 * i + 3] = c.a;
+    }
 }
 
 void orc_scene_destroy(void *h) { delete (Handle *)h; }

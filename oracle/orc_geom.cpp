@@ -153,11 +153,6 @@ void Bvh2::build(const float *pos, const uint32_t *idx, uint32_t nt) {
     bl.binned(0, index.data(), index.data() + nt, temp.data());
 }
 
-static inline void take_hit(Hit &best, float t, float u, float v, uint32_t prim) {
-    if (t < best.t || (t == best.t && prim < best.prim)) {
-        best.t = t; best.u = u; best.v = v; best.prim = prim;
-    }
-}
 
 // src/cpu/accel.cpp:429-488 (left child first, 64-entry stack)
 Hit Bvh2::trace(const Ray &ray_in, bool any_hit, uint64_t *nv, uint64_t *nt, bool robust) const {

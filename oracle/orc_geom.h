@@ -151,6 +151,82 @@ float shadow_pass(const Bvh2 &b, const Ray &ray, IorOf ior_of, Fresnel fresnel) 
     return tr;
 }
 
+// Closest hit so far: smaller t, equal t resolved to the smaller primitive id.
+inline void take_hit(Hit &best, float t, float u, float v, uint32_t prim) {
+    if (t < best.t || (t == best.t && prim < best.prim)) {
+        best.t = t; best.u = u; best.v = v; best.prim = prim;
+    }
+}
+
+// Closest or any hit where accept(prim, u, v) decides whether a candidate
+// that passes the triangle test counts (the reference's any-hit programs:
+// alpha test).  Ties in t go to the smaller primitive id, as in trace().
+template <class Accept> Hit trace_accept(const Bvh2 &b, const Ray &ray_in, bool any_hit, Accept accept) {
+    Hit best{INFINITY, 0, 0, 0xFFFFFFFFu};
+    if (b.nodes.empty() || b.index.empty() || ray_in.maxt < ray_in.mint) return best;
+    Ray ray = ray_in;
+    uint32_t stack[64], sp = 0, ni = 0;
+    while (true) {
+        const Bvh2::Node &n = b.nodes[ni];
+        if (!n.box.hit(ray, true)) {
+            if (sp == 0) break;
+            ni = stack[--sp];
+            continue;
+        }
+        if (!n.leaf) {
+            stack[sp++] = n.right_or_start;
+            ++ni;
+            continue;
+        }
+        for (uint32_t i = n.right_or_start, e = n.right_or_start + n.size; i < e; ++i) {
+            const uint32_t f = b.index[i];
+            float u, v, t;
+            if (!intersect_tri(b.p0[f], b.e1[f], b.e2[f], ray, u, v, t) || !accept(f, u, v)) continue;
+            take_hit(best, t, u, v, f);
+            if (any_hit) return best;
+            ray.maxt = best.t;
+        }
+        if (sp == 0) break;
+        ni = stack[--sp];
+    }
+    return best;
+}
+
+// Shadow ray through an any-hit filter: factor(prim, u, v, |d . Ng|) < 0 means
+// the candidate occludes (result 0); otherwise the transmission is multiplied
+// by it (1: the candidate is ignored).  Every triangle in [mint, maxt] is
+// visited once.
+template <class Factor> float shadow_filter(const Bvh2 &b, const Ray &ray, Factor factor) {
+    if (b.nodes.empty() || b.index.empty() || ray.maxt < ray.mint) return 1.0f;
+    float tr = 1.0f;
+    uint32_t stack[64], sp = 0, ni = 0;
+    while (true) {
+        const Bvh2::Node &n = b.nodes[ni];
+        if (!n.box.hit(ray, true)) {
+            if (sp == 0) break;
+            ni = stack[--sp];
+            continue;
+        }
+        if (!n.leaf) {
+            stack[sp++] = n.right_or_start;
+            ++ni;
+            continue;
+        }
+        for (uint32_t i = n.right_or_start, e = n.right_or_start + n.size; i < e; ++i) {
+            const uint32_t f = b.index[i];
+            float u, v, t;
+            if (!intersect_tri(b.p0[f], b.e1[f], b.e2[f], ray, u, v, t)) continue;
+            const V3f ng = normalize(cross(b.e1[f], b.e2[f]));
+            const float k = factor(f, u, v, std::fabs(dot(ray.d, ng)));
+            if (k < 0.0f) return 0.0f;
+            tr *= k;
+        }
+        if (sp == 0) break;
+        ni = stack[--sp];
+    }
+    return tr;
+}
+
 // Exhaustive test over every triangle (ground truth for small scenes).
 Hit brute_trace(const Bvh2 &tris, const Ray &ray, bool any_hit);
 

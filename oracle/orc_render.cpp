@@ -1,6 +1,7 @@
 // CPU oracle — test infrastructure only.  See orc_render.h.
 #include "orc_render.h"
 #include <atomic>
+#include <cstring>
 #include <thread>
 
 namespace orc {
@@ -10,7 +11,11 @@ static const float PI_F = 3.14159265358979323846f;
 void SceneData::finalize() {
     bvh.build(pos.data(), idx.data(), nt);
     has_pass = false;
-    for (uint32_t m = 0; m < nmesh; ++m) has_pass = has_pass || mats[m].type == 1 || mats[m].type == 2;
+    has_alpha = false;
+    for (uint32_t m = 0; m < nmesh; ++m) {
+        has_pass = has_pass || mats[m].type == 1 || mats[m].type == 2;
+        has_alpha = has_alpha || (mats[m].alphaTex != 0xFFFFFFFFu && !tex.empty());
+    }
     emitters.clear(); emit_cdf.clear(); emit_area.clear();
     for (uint32_t m = 0; m < nmesh; ++m) {
         if (!mats[m].isEmitter) continue;
@@ -69,6 +74,84 @@ static inline Ray mk_ray(V3f o, V3f d, float mint, float maxt) {
     return r;
 }
 
+// Alpha test of a candidate (shaders/Shaders.hlsl:674-724): a = red of the
+// texture at the candidate's UV (level 0) for radiance rays; for shadow rays
+// alpha when below 1, else red; ignored below 0.01, else accepted when
+// a^2 >= xi with xi = pcg_hash(seed ^ pcg_hash(prim)) * 2^-32 (the ray's seed
+// and the triangle only, independent of the traversal order).
+static bool alpha_accept(const SceneData &s, uint32_t prim, float u, float v, uint32_t seed, bool shadow) {
+    const MatRec &m = s.mats[s.tri_mesh[prim]];
+    if (m.alphaTex == 0xFFFFFFFFu || s.uv.empty()) return true;
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const float w = 1.0f - u - v;
+    const float tu = s.uv[2 * i0] * w + s.uv[2 * i1] * u + s.uv[2 * i2] * v;
+    const float tv = s.uv[2 * i0 + 1] * w + s.uv[2 * i1 + 1] * u + s.uv[2 * i2 + 1] * v;
+    const Rgba c = sample(s.tex[m.alphaTex], tu, tv, 0.0f);
+    const float a = shadow ? (c.a < 1.0f ? c.a : c.r) : c.r;
+    if (a < 0.01f) return false;
+    const float xi = (float)pcg_hash(seed ^ pcg_hash(prim)) * 2.3283064365386963e-10f;
+    return a * a >= xi;
+}
+// Seeds: a radiance ray's from its path stream's state words, a shadow ray's
+// from its seed word (alpha_seed_shadow).
+static uint32_t alpha_seed(const Pcg32 &r) {
+    return pcg_hash((uint32_t)r.state ^ pcg_hash((uint32_t)(r.state >> 32)));
+}
+static uint32_t alpha_seed(const SobolRng &r) { return pcg_hash(r.index ^ pcg_hash(r.dim)); }
+static uint32_t alpha_seed_shadow(uint32_t word) { return pcg_hash(word ^ 0x2545F491u); }
+// Seed word of a surface NEE shadow ray: a transmittance seed drawn from the
+// stream in scenes with media, else the stream's state hash (no draw).
+template <class Rng> static uint32_t shadow_word(const SceneData &s, Rng &rng) {
+    return s.vols.empty() ? alpha_seed(rng) : rng.next_uint();
+}
+
+// Texture maps at a hit (shaders/Shaders.hlsl:232-315) with the level of
+// detail from one view pixel's UV footprint at the segment's length
+// (shaders/GeometryUtils.hlsli:120-154; dims 0: level 0).
+static void apply_textures(const SceneData &s, uint32_t dims_max, uint32_t f, float bu, float bv, float hit_t,
+                           const MatRec &mr, V3f &N, MatT<float> &m) {
+    const uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+    const float w = 1.0f - bu - bv;
+    const float tu = s.uv[2 * i0] * w + s.uv[2 * i1] * bu + s.uv[2 * i2] * bv;
+    const float tv = s.uv[2 * i0 + 1] * w + s.uv[2 * i1 + 1] * bu + s.uv[2 * i2 + 1] * bv;
+    const V3f p0 = s.vtx(i0), p1 = s.vtx(i1), p2 = s.vtx(i2);
+    const float u0 = s.uv[2 * i0], v0 = s.uv[2 * i0 + 1], u1 = s.uv[2 * i1], v1 = s.uv[2 * i1 + 1];
+    const float u2 = s.uv[2 * i2], v2 = s.uv[2 * i2 + 1];
+    const V3f e1 = p1 - p0, e2 = p2 - p0;
+    const float world_area = 0.5f * length(cross(e1, e2));
+    const float uv_area = 0.5f * std::fabs((u1 - u0) * (v2 - v0) - (u2 - u0) * (v1 - v0));
+    float foot = 0.0f;
+    if (world_area >= 1e-12f && uv_area >= 1e-12f && dims_max > 0)
+        foot = hit_t * (1.0f / (float)dims_max) * std::sqrt(uv_area / world_area);
+    auto look = [&](uint32_t ti) { return sample(s.tex[ti], tu, tv, lod_for(s.tex[ti], foot)); };
+    if (mr.albedoTex != 0xFFFFFFFFu) {
+        const Rgba c = look(mr.albedoTex);
+        m.albedo = V3f(c.r, c.g, c.b);
+    }
+    if (mr.normalTex != 0xFFFFFFFFu) {
+        const float du1 = u1 - u0, dv1 = v1 - v0, du2 = u2 - u0, dv2 = v2 - v0;
+        const float det = du1 * dv2 - du2 * dv1;
+        V3f T, B;
+        if (std::fabs(det) > 1e-8f) {
+            const float inv = 1.0f / det;
+            T = normalize((e1 * dv2 - e2 * dv1) * inv);
+            T = normalize(T - N * dot(N, T));
+            B = cross(N, T);
+        } else {
+            build_onb(N, T, B);
+        }
+        const Rgba c = look(mr.normalTex);
+        N = normalize(T * (c.r * 2.0f - 1.0f) + B * (c.g * 2.0f - 1.0f) + N * (c.b * 2.0f - 1.0f));
+    }
+    if (mr.type != MAT_HAIR && mr.roughnessTex != 0xFFFFFFFFu) {
+        const float r = look(mr.roughnessTex).r;
+        m.alpha = r;
+        m.roughness = r;
+    }
+    if (mr.specularTex != 0xFFFFFFFFu) m.specular = look(mr.specularTex).r * 0.5f;
+    if (mr.subsurfaceTex != 0xFFFFFFFFu) m.subsurface = look(mr.subsurfaceTex).r;
+}
+
 // shaders/Emitter.hlsli:17-31
 static uint32_t cdf_sample(const std::vector<float> &c, uint32_t n, float u) {
     uint32_t lo = 0, hi = n - 1;
@@ -82,18 +165,23 @@ static uint32_t cdf_sample(const std::vector<float> &c, uint32_t n, float u) {
 
 // Shadow ray: 0 if occluded, else the Fresnel transmission through the mirror
 // and dielectric meshes it crosses (shaders/RayQueryTrace.hlsli:96-124; those
-// meshes are the non-opaque instances, every other mesh occludes).
-static float shadow_pass_factor(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st, float mint = 0.0f) {
+// meshes and the alpha-tested ones are the non-opaque instances, every other
+// mesh occludes).  `word` seeds the alpha tests: the ray's transmittance seed
+// in scenes with media, else the path stream's state hash where the ray was
+// made (shadow_word).
+static float shadow_pass_factor(const SceneData &s, V3f o, V3f d, float maxt, RenderStats &st, uint32_t word,
+                                float mint = 0.0f) {
     ++st.shadow_rays;
-    if (!s.has_pass) return s.bvh.trace(mk_ray(o, d, mint, maxt), true).prim != 0xFFFFFFFFu ? 0.0f : 1.0f;
-    return shadow_pass(
-        s.bvh, mk_ray(o, d, mint, maxt),
-        [&](uint32_t f) {
-            const MatRec &m = s.mats[s.tri_mesh[f]];
-            const bool pass = m.type == 1 || m.type == 2;
-            return std::make_pair(pass ? std::max(m.extIOR, 1e-6f) : 0.0f, m.intIOR);
-        },
-        [](float c, float ei, float et) { return fresnel_dielectric(c, ei, et); });
+    if (!s.has_pass && !s.has_alpha)
+        return s.bvh.trace(mk_ray(o, d, mint, maxt), true).prim != 0xFFFFFFFFu ? 0.0f : 1.0f;
+    const uint32_t seed = s.has_alpha ? alpha_seed_shadow(word) : 0u;
+    return shadow_filter(s.bvh, mk_ray(o, d, mint, maxt), [&](uint32_t f, float u, float v, float cosi) {
+        const MatRec &m = s.mats[s.tri_mesh[f]];
+        if (s.has_alpha && m.alphaTex != 0xFFFFFFFFu) return alpha_accept(s, f, u, v, seed, true) ? -1.0f : 1.0f;
+        if (m.type == MAT_MIRROR || m.type == MAT_DIELECTRIC)
+            return 1.0f - fresnel_dielectric(cosi, std::max(m.extIOR, 1e-6f), m.intIOR);
+        return -1.0f;
+    });
 }
 
 // Transmittance of an unoccluded shadow ray through the scene's volumes, with
@@ -140,7 +228,7 @@ static V3f volume_nee_area(const SceneData &s, V3f P, V3f wo, const Volume &v, R
     V3f wi = tl / dist;
     float cl = dot(ls.n, -wi);
     if (cl <= 0.0f) return V3f(0.0f);
-    const float pass = shadow_pass_factor(s, P, wi, dist - 0.001f, st, 0.001f);
+    const float pass = shadow_pass_factor(s, P, wi, dist - 0.001f, st, tr_seed, 0.001f);
     if (pass == 0.0f) return V3f(0.0f);
     float fp = hg_eval(dot(wo, wi), v.g);
     float pe = ls.pdfA * dist * dist / cl;
@@ -166,7 +254,7 @@ static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Rn
     }
     uint32_t tr_seed = rng.next_uint();
     if (pe <= 0.0f) return V3f(0.0f);
-    const float pass = shadow_pass_factor(s, P, dir, 1e20f, st, 0.001f);
+    const float pass = shadow_pass_factor(s, P, dir, 1e20f, st, tr_seed, 0.001f);
     if (pass == 0.0f) return V3f(0.0f);
     float fp = hg_eval(dot(wo, dir), v.g);
     float w = balance_heuristic(pe, fp);
@@ -178,13 +266,19 @@ static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Rn
 // scene (diffuse, mirror, dielectric, microfacet, Disney).
 template <class Rng>
 static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, const RenderParams &p, RenderStats &st) {
-    (void)cam;
     V3f L(0.0f), T(1.0f);
     float last_pdf = 0.0f, eta = 1.0f;
     const uint32_t E = (uint32_t)s.emitters.size();
     for (uint32_t bounce = 0; bounce < p.max_depth; ++bounce) {
         ++st.extension_rays;
-        Hit h = s.bvh.trace(ray, false);
+        Hit h;
+        if (s.has_alpha) {
+            const uint32_t seed = alpha_seed(rng);
+            h = trace_accept(s.bvh, ray, false,
+                             [&](uint32_t f, float u, float v) { return alpha_accept(s, f, u, v, seed, false); });
+        } else {
+            h = s.bvh.trace(ray, false);
+        }
         // Participating media between the ray origin and the surface
         // (shaders/Shaders.hlsl:133-180).
         bool vol_scattered = false;
@@ -240,6 +334,12 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
         bool back = dot(Ng, ray.d) > 0.0f;
         if (back) Ng = -Ng;
         if (dot(N, Ng) < 0.0f) N = -N;
+        const V3f N_untextured = N;
+        (void)N_untextured;
+        if (!s.tex.empty() && !s.uv.empty() &&
+            (mr.albedoTex != 0xFFFFFFFFu || mr.normalTex != 0xFFFFFFFFu || mr.roughnessTex != 0xFFFFFFFFu ||
+             mr.specularTex != 0xFFFFFFFFu || mr.subsurfaceTex != 0xFFFFFFFFu))
+            apply_textures(s, std::max(cam.width, cam.height), f, h.u, h.v, h.t, mr, N, m);
 
         if (mr.isEmitter) {
             V3f Le(mr.radiance[0], mr.radiance[1], mr.radiance[2]);
@@ -341,14 +441,14 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
             // Area light NEE, shaders/Emitter.hlsli:43-89, 105-153
             if (E > 0) {
                 LightSample ls = sample_area_light(s, rng);
-                uint32_t tr_seed = s.vols.empty() ? 0u : rng.next_uint();
+                uint32_t tr_seed = shadow_word(s, rng);
                 V3f tl = ls.p - P;
                 float dist = length(tl);
                 V3f ww = tl / dist;
                 float ct = dot(N, ww), cl = dot(ls.n, -ww);
                 V3f sso = hair ? offset_ray_origin(P, side(ww), side(ww)) : so;
                 float pass = 0.0f;
-                if ((hair || ct > 0.0f) && cl > 0.0f && (pass = shadow_pass_factor(s, sso, ww, dist - 0.001f, st)) > 0.0f) {
+                if ((hair || ct > 0.0f) && cl > 0.0f && (pass = shadow_pass_factor(s, sso, ww, dist - 0.001f, st, tr_seed)) > 0.0f) {
                     V3f wol = to_local(ww, Tg, B, N);
                     V3f fv = material_eval_h(wi, wol, m, hh);
                     float pb = material_pdf_h(wi, wol, m, hh);
@@ -370,11 +470,11 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
                     float coin = rng.next_float(), jx = rng.next_float(), jy = rng.next_float();
                     s.env.sample_alias(r, coin, jx, jy, dir, Le, pe);
                 }
-                uint32_t tr_seed = s.vols.empty() ? 0u : rng.next_uint();
+                uint32_t tr_seed = shadow_word(s, rng);
                 float ct = dot(N, dir);
                 V3f sso = hair ? offset_ray_origin(P, side(dir), side(dir)) : so;
                 float pass = 0.0f;
-                if (pe > 0.0f && (hair || ct > 0.0f) && (pass = shadow_pass_factor(s, sso, dir, 1e20f, st)) > 0.0f) {
+                if (pe > 0.0f && (hair || ct > 0.0f) && (pass = shadow_pass_factor(s, sso, dir, 1e20f, st, tr_seed)) > 0.0f) {
                     V3f wol = to_local(dir, Tg, B, N);
                     V3f fv = material_eval_h(wi, wol, m, hh);
                     float pb = material_pdf_h(wi, wol, m, hh);

@@ -22,6 +22,7 @@
 #include "orc_pcg32.h"
 #include "orc_volume.h"
 #include "orc_sobol.h"
+#include "orc_texture.h"
 
 namespace orc {
 
@@ -47,6 +48,8 @@ struct SceneData {
     std::vector<float> emit_area;              // per emitter: total area
     std::vector<Volume> vols;                  // participating media
     bool has_pass = false;                     // mirror/dielectric meshes present (shadow rays pass them)
+    std::vector<Texture> tex;                  // scene textures (mip chains)
+    bool has_alpha = false;                    // some mesh has an alpha texture
 
     void finalize(); // BVH + emitter tables from the raw arrays
     V3f vtx(uint32_t i) const { return V3f(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]); }

@@ -7,12 +7,14 @@
 #include "guide.h"
 #include "pt_kernels.h"
 #include "scene_gen.h"
+#include "texture.h"
 #include "volume.h"
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -611,6 +613,37 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.n_vol = d->num_volumes;
         s.vols = d->num_volumes ? c->upload(vd.data(), vd.size()) : nullptr;
         s.pass_ior = any_pass ? (const float2 *)c->upload(pass_ior.data(), pass_ior.size()) : nullptr;
+        // Textures: mip chains and lookup tables (tex_dev.h); materials refer
+        // to them by index.
+        if (d->num_textures > NORI_MAX_TEXTURES) throw ArgError("scene: more than NORI_MAX_TEXTURES textures", NORI_E_LIMIT);
+        if (d->num_textures && !d->textures) throw ArgError("scene: null texture array");
+        s.has_alpha = 0;
+        for (uint32_t m = 0; m < d->num_meshes; ++m) {
+            const nori_material &mr = mats[m];
+            for (uint32_t ti : {mr.albedoTexIndex, mr.normalTexIndex, mr.roughnessTexIndex, mr.metallicTexIndex,
+                                mr.specularTexIndex, mr.subsurfaceTexIndex, mr.alphaTexIndex})
+                if (ti != NORI_NO_TEXTURE && ti >= d->num_textures)
+                    throw ArgError("scene: material " + std::to_string(m) + " refers to texture " + std::to_string(ti) +
+                                   " of " + std::to_string(d->num_textures));
+            if (mr.alphaTexIndex != NORI_NO_TEXTURE) s.has_alpha = 1;
+        }
+        if ((s.has_alpha || d->num_textures) && !d->uvs && d->num_textures)
+            for (uint32_t m = 0; m < d->num_meshes; ++m)
+                if (mats[m].albedoTexIndex != NORI_NO_TEXTURE || mats[m].alphaTexIndex != NORI_NO_TEXTURE ||
+                    mats[m].normalTexIndex != NORI_NO_TEXTURE || mats[m].roughnessTexIndex != NORI_NO_TEXTURE ||
+                    mats[m].specularTexIndex != NORI_NO_TEXTURE || mats[m].subsurfaceTexIndex != NORI_NO_TEXTURE)
+                    throw ArgError("scene: textured material without texture coordinates");
+        s.tex = TexTables{nullptr, nullptr, nullptr, 0};
+        if (d->num_textures) {
+            TexUpload tu;
+            try {
+                build_textures(d->textures, d->num_textures, tu);
+            } catch (const std::invalid_argument &e) {
+                throw ArgError(std::string("scene: ") + e.what());
+            }
+            s.tex = TexTables{c->upload(tu.tex.data(), tu.tex.size()), c->upload(tu.texel.data(), tu.texel.size()),
+                              c->upload(tu.lut.data(), tu.lut.size()), (uint32_t)tu.tex.size()};
+        }
         if ((size_t)s.stack_depth * 64 * sizeof(uint2) > 64 * 1024)
             throw ArgError("BVH8 deeper than the LDS stack allows", NORI_E_LIMIT);
         c->sdev = s;
@@ -1026,6 +1059,30 @@ int nori_gpu_envmap_eval(nori_ctx *c, int n, const float *dir3, float *rgb3, flo
         HIPCHK(hipMemcpy(rgb3, b2.p, s3, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(pdf, p.p, (size_t)n * 4, hipMemcpyDeviceToHost));
         a.release(); b2.release(); p.release();
+    });
+}
+
+int nori_gpu_texture_sample(nori_ctx *c, int n, const uint32_t *tex, const float *uv2, const float *lod, float *rgba4) {
+    if (!c || n < 0 || (n > 0 && (!tex || !uv2 || !lod || !rgba4))) return NORI_E_ARG;
+    return guard(c, [&] {
+        if (!c->has_scene) throw ArgError("texture_sample: no scene uploaded", NORI_E_STATE);
+        if (n == 0) return;
+        for (int i = 0; i < n; ++i)
+            if (tex[i] >= c->sdev.tex.count) throw ArgError("texture_sample: texture index out of range");
+        const size_t nn = (size_t)n;
+        ScopedBufs b(4);
+        HIPCHK(hipMalloc(&b[0].p, nn * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&b[1].p, nn * 2 * sizeof(float)));
+        HIPCHK(hipMalloc(&b[2].p, nn * sizeof(float)));
+        HIPCHK(hipMalloc(&b[3].p, nn * 4 * sizeof(float)));
+        HIPCHK(hipMemcpy(b[0].p, tex, nn * sizeof(uint32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b[1].p, uv2, nn * 2 * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b[2].p, lod, nn * sizeof(float), hipMemcpyHostToDevice));
+        launch_tex_sample(c->sdev.tex, (uint32_t)n, (const uint32_t *)b[0].p, (const float *)b[1].p,
+                          (const float *)b[2].p, (float *)b[3].p, c->stream);
+        HIPCHK(hipMemcpyAsync(rgba4, b[3].p, nn * 4 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
     });
 }
 

@@ -4,6 +4,7 @@
 #include "env_dev.h"
 #include "guide.h"
 #include "nori_math.h"
+#include "tex_dev.h"
 #include "vol_dev.h"
 
 namespace ng {
@@ -34,6 +35,8 @@ struct SceneDev {
     const float2 *pass_ior;
     // path sampler: 0 pcg32 per pixel, 1 Owen-scrambled Sobol' (shaders/RNG.hlsli)
     uint32_t sobol;
+    TexTables tex;        // scene textures (count 0: none)
+    uint32_t has_alpha;   // some mesh has an alpha texture
 };
 
 // Guiding hooks of one render batch.  out: network outputs per queue position
@@ -165,5 +168,7 @@ void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const flo
 void launch_env_eval(const EnvDev &e, uint32_t n, const float *dir, float *rgb, float *pdf, hipStream_t st);
 void launch_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32_t n, uint32_t *out, hipStream_t st);
 void launch_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out, hipStream_t st);
+void launch_tex_sample(const TexTables &t, uint32_t n, const uint32_t *tex, const float *uv, const float *lod,
+                       float *rgba, hipStream_t st);
 
 } // namespace ng

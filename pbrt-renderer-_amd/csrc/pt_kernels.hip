@@ -270,7 +270,50 @@ __device__ __forceinline__ void count_add(unsigned long long *stats, int k, uint
 // Closest hit of the paths at positions [0, *count).  SORT: the position is
 // also appended to the queue of its shading class (light, surface, hair at
 // q_cls, q_cls + cap, q_cls + 2 cap; counts cls_cnt[0..2]).
-template <bool COUNT, bool SORT>
+// Interpolated texture coordinates of triangle `prim` at barycentrics (u, v)
+// (shaders/GeometryUtils.hlsli GetInterpolatedUV: w = 1 - u - v).
+__device__ __forceinline__ void hit_uv(const SceneDev &s, uint32_t prim, float u, float v, float &tu, float &tv) {
+#pragma clang fp contract(off)
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const float w = 1.0f - u - v;
+    tu = s.uv[2 * i0] * w + s.uv[2 * i1] * u + s.uv[2 * i2] * v;
+    tv = s.uv[2 * i0 + 1] * w + s.uv[2 * i1 + 1] * u + s.uv[2 * i2 + 1] * v;
+}
+
+// Stochastic alpha test of a candidate hit (shaders/Shaders.hlsl:674-697 for
+// radiance rays, a = texture red; :703-724 for shadow rays, a = alpha when it
+// is below 1, else red): ignored below 0.01, else accepted when a^2 >= xi.
+// The reference threads xi through the traversal order (a PCGHash chain); here
+// xi = pcg_hash(seed ^ pcg_hash(prim)) * 2^-32 depends only on the ray's seed
+// and the triangle, so the outcome does not depend on the traversal order.
+__device__ __forceinline__ bool alpha_accept(const SceneDev &s, uint32_t prim, uint32_t mesh, float u, float v,
+                                             uint32_t seed, bool shadow) {
+#pragma clang fp contract(off)
+    const uint32_t ti = s.mats[mesh].alphaTexIndex;
+    if (ti == NORI_NO_TEXTURE || !s.uv) return true;
+    float tu, tv, a;
+    hit_uv(s, prim, u, v, tu, tv);
+    const f3 c = tex_sample(s.tex, ti, tu, tv, 0.0f, a);
+    const float av = shadow ? (a < 1.0f ? a : c.x) : c.x;
+    if (av < 0.01f) return false;
+    const float xi = (float)pcg_hash(seed ^ pcg_hash(prim)) * 2.3283064365386963e-10f;
+    return av * av >= xi;
+}
+
+// Alpha seed of a radiance ray: the path stream's two state words.
+__device__ __forceinline__ uint32_t alpha_seed_path(float4 aux) {
+    return pcg_hash(__float_as_uint(aux.z) ^ pcg_hash(__float_as_uint(aux.w)));
+}
+// Hash of a path stream's current state words (the stream does not advance).
+__device__ __forceinline__ uint32_t rng_state_hash(uint64_t r) {
+    return pcg_hash((uint32_t)r ^ pcg_hash((uint32_t)(r >> 32)));
+}
+__device__ __forceinline__ uint32_t rng_state_hash(const SobolRng &r) { return pcg_hash(r.index ^ pcg_hash(r.dim)); }
+// Alpha seed of a shadow ray from its seed word: the transmittance seed in
+// scenes with media, else the path stream's state hash where the ray was made.
+__device__ __forceinline__ uint32_t alpha_seed_shadow(uint32_t word) { return pcg_hash(word ^ 0x2545F491u); }
+
+template <bool COUNT, bool SORT, bool ALPHA>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
                                                         unsigned long long *stats, uint32_t *q_cls,
                                                         uint32_t *cls_cnt, uint32_t cap) {
@@ -282,8 +325,18 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
     int cls = 0;
     if (i < n) {
         float4 a = p.ray_o[i], b = p.ray_d[i];
-        TraceOut r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x,
-                                                 blockDim.x, &nn, &nt);
+        TraceOut r;
+        if (ALPHA) {
+            const uint32_t seed = alpha_seed_path(p.aux[i]);
+            auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3, f3) {
+                return alpha_accept(s, prim, mesh, u, v, seed, false);
+            };
+            r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x,
+                                            &nn, &nt, acc);
+        } else {
+            r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x,
+                                            &nn, &nt);
+        }
         p.hit[i] = make_float4(r.t, __uint_as_float(r.prim), r.u, r.v);
         if (SORT) {
             cls = CLS_LIGHT;
@@ -321,8 +374,8 @@ __device__ __forceinline__ void guide_nee_add(const GuideShadeDev &g, uint32_t t
     }
 }
 
-// VOL: transmittance through media; PASS: mirror/dielectric meshes attenuate
-// instead of occluding.
+// VOL: transmittance through media; PASS: candidates go through the any-hit
+// filter (alpha test, mirror/dielectric meshes attenuate instead of occluding).
 template <bool COUNT, bool VOL, bool PASS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
                                                         float4 *film, unsigned long long *stats, GuideShadeDev g) {
@@ -334,8 +387,32 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
     if (i < n) {
         float4 a = sh.o[i], b = sh.d[i], e = sh.e[i];
         float trans = 1.0f;
-        TraceOut r = traverse_bvh8<true, COUNT, PASS>(s.nodes, s.tris, xyz(a), xyz(b), e.x, a.w, stk + threadIdx.x,
-                                                      blockDim.x, &nn, &nt, s.pass_ior, &trans);
+        TraceOut r;
+        if (PASS) {
+            // alpha-tested meshes first (they occlude when accepted), then
+            // mirror/dielectric meshes attenuate by 1 - Fresnel and let the
+            // ray through (shaders/RayQueryTrace.hlsli:96-124)
+            const f3 dir = xyz(b);
+            const uint32_t seed = s.has_alpha ? alpha_seed_shadow(__float_as_uint(e.z)) : 0u;
+            auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3 e1, f3 e2) {
+                if (s.has_alpha && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE)
+                    return alpha_accept(s, prim, mesh, u, v, seed, true);
+                if (s.pass_ior) {
+                    const float2 ior = s.pass_ior[mesh];
+                    if (ior.x > 0.0f) {
+                        const f3 ng = normalize(cross(e1, e2));
+                        trans *= 1.0f - fresnel_dielectric(fabsf(dot(dir, ng)), ior.x, ior.y);
+                        return false;
+                    }
+                }
+                return true;
+            };
+            r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), dir, e.x, a.w, stk + threadIdx.x, blockDim.x, &nn,
+                                           &nt, acc);
+        } else {
+            r = traverse_bvh8<true, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), e.x, a.w, stk + threadIdx.x, blockDim.x,
+                                           &nn, &nt);
+        }
         if (r.prim == MISS) {
             const float4 c = sh.c[i];
             f3 col = PASS ? xyz(c) * trans : xyz(c);
@@ -352,7 +429,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
 }
 
 // One next-event shadow ray: occlusion over [tmin, tmax]; with media, the
-// transmittance over [0, tr_tmax] from stream seed(tr_seed).
+// transmittance over [0, tr_tmax] from stream seed(tr_seed).  tr_seed also
+// seeds the ray's alpha tests (without media it is the state hash).
 struct ShadowRec {
     f3 o, d, c;
     float tmin, tmax, tr_tmax;
@@ -473,7 +551,64 @@ template <class R> __device__ __forceinline__ LightPt sample_light(const SceneDe
 }
 
 // One bounce of shaders/Shaders.hlsl:108-627 for every path in q_in.
-// VOL: the scene has media; GUIDE: guided sampling and training records.
+__device__ __forceinline__ bool mat_has_shading_tex(const nori_material &mr) {
+    return mr.albedoTexIndex != NORI_NO_TEXTURE || mr.normalTexIndex != NORI_NO_TEXTURE ||
+           mr.roughnessTexIndex != NORI_NO_TEXTURE || mr.specularTexIndex != NORI_NO_TEXTURE ||
+           mr.subsurfaceTexIndex != NORI_NO_TEXTURE;
+}
+
+// Texture maps at a hit (shaders/Shaders.hlsl:232-315): the level of detail
+// from the UV footprint of one pixel of the view at the segment's length
+// (shaders/GeometryUtils.hlsli:120-154), albedo, a tangent-space normal map
+// on the frame from the triangle's UV derivatives, roughness (not for hair),
+// specular (x 0.5) and subsurface.
+__device__ void apply_textures(const SceneDev &s, uint32_t dims_max, uint32_t prim, float bu, float bv, float hit_t,
+                               const nori_material &mr, f3 &N, Mat &m) {
+    float tu, tv;
+    hit_uv(s, prim, bu, bv, tu, tv);
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    const f3 p0 = f3{s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
+    const f3 p1 = f3{s.pos[3 * i1], s.pos[3 * i1 + 1], s.pos[3 * i1 + 2]};
+    const f3 p2 = f3{s.pos[3 * i2], s.pos[3 * i2 + 1], s.pos[3 * i2 + 2]};
+    const float u0 = s.uv[2 * i0], v0 = s.uv[2 * i0 + 1], u1 = s.uv[2 * i1], v1 = s.uv[2 * i1 + 1];
+    const float u2 = s.uv[2 * i2], v2 = s.uv[2 * i2 + 1];
+    const f3 e1 = p1 - p0, e2 = p2 - p0;
+    const float world_area = 0.5f * length(cross(e1, e2));
+    const float uv_area = 0.5f * fabsf((u1 - u0) * (v2 - v0) - (u2 - u0) * (v1 - v0));
+    float foot = 0.0f;
+    if (world_area >= 1e-12f && uv_area >= 1e-12f && dims_max > 0)
+        foot = hit_t * (1.0f / (float)dims_max) * sqrtf(uv_area / world_area);
+    float a;
+    if (mr.albedoTexIndex != NORI_NO_TEXTURE)
+        m.albedo = tex_sample(s.tex, mr.albedoTexIndex, tu, tv, tex_lod(s.tex, mr.albedoTexIndex, foot), a);
+    if (mr.normalTexIndex != NORI_NO_TEXTURE) {
+        const float du1 = u1 - u0, dv1 = v1 - v0, du2 = u2 - u0, dv2 = v2 - v0;
+        const float det = du1 * dv2 - du2 * dv1;
+        f3 T, B;
+        if (fabsf(det) > 1e-8f) {
+            const float inv = 1.0f / det;
+            T = normalize((e1 * dv2 - e2 * dv1) * inv);
+            T = normalize(T - N * dot(N, T));
+            B = cross(N, T);
+        } else {
+            build_onb(N, T, B);
+        }
+        const f3 tn = tex_sample(s.tex, mr.normalTexIndex, tu, tv, tex_lod(s.tex, mr.normalTexIndex, foot), a);
+        N = normalize(T * (tn.x * 2.0f - 1.0f) + B * (tn.y * 2.0f - 1.0f) + N * (tn.z * 2.0f - 1.0f));
+    }
+    if (mr.type != NORI_MAT_HAIR && mr.roughnessTexIndex != NORI_NO_TEXTURE) {
+        const float r =
+            tex_sample(s.tex, mr.roughnessTexIndex, tu, tv, tex_lod(s.tex, mr.roughnessTexIndex, foot), a).x;
+        m.alpha = r;
+        m.roughness = r;
+    }
+    if (mr.specularTexIndex != NORI_NO_TEXTURE)
+        m.specular = tex_sample(s.tex, mr.specularTexIndex, tu, tv, tex_lod(s.tex, mr.specularTexIndex, foot), a).x * 0.5f;
+    if (mr.subsurfaceTexIndex != NORI_NO_TEXTURE)
+        m.subsurface = tex_sample(s.tex, mr.subsurfaceTexIndex, tu, tv, tex_lod(s.tex, mr.subsurfaceTexIndex, foot), a).x;
+}
+
+// VOL: the scene has media; GUIDE: guided sampling and training records.// VOL: the scene has media; GUIDE: guided sampling and training records.
 // CLS: which paths the launch shades.  CLS_ALL takes positions [0, *cnt_in);
 // the others take the positions k_extend sorted into their queue (q_cls), so
 // each launch carries only its class's code: misses and emitter hits
@@ -584,6 +719,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         const f3 N_feat = N;
         const uint32_t mesh = s.tri_mesh[prim];
         const nori_material &mr = s.mats[mesh];
+        Mat m = load_mat(mr);
+        if (s.tex.count && s.uv && mat_has_shading_tex(mr)) {
+            const nori_camera &vc = views[meta >> 16].cam;
+            apply_textures(s, vc.width > vc.height ? vc.width : vc.height, prim, hit.z, hit.w, hit.x, mr, N, m);
+        }
         const bool emitter = CLS == CLS_LIGHT ? true : (CLS == CLS_ALL ? mr.isEmitter != 0 : false);
         if (emitter) {
             f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
@@ -643,7 +783,6 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 if (refr) eta *= etaI / etaT;
                 alive = true;
             } else if (guide_eligible(type) || hair) {
-                const Mat m = load_mat(mr);
                 so = offset_ray_origin(P, Ng, Ng);
                 // Hair transmits: rays leave from the side of the fiber surface
                 // they head to (shaders/Emitter.hlsli:122-126).
@@ -661,7 +800,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
                     LightPt lp = sample_light(s, rng);
-                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : 0u;
+                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : (s.has_alpha ? rng_state_hash(rng) : 0u);
                     f3 tl = lp.p - P;
                     float dist = length(tl);
                     f3 ww = tl / dist;
@@ -685,7 +824,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     f3 dir, Le;
                     float pe;
                     env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
-                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : 0u;
+                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : (s.has_alpha ? rng_state_hash(rng) : 0u);
                     float ct = dot(N, dir);
                     if (pe > 0.0f && (hair || ct > 0.0f)) {
                         f3 f;
@@ -753,8 +892,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     if (k < g.max_rec) {
                         GuideRecord &r = g.rec[(size_t)k * g.cap + id];
                         f3 vv = -d;
-                        r.pos_rough = make_float4(P.x, P.y, P.z, m.roughness);
-                        r.n_metal = make_float4(N_feat.x, N_feat.y, N_feat.z, m.metallic);
+                        r.pos_rough = make_float4(P.x, P.y, P.z, mr.roughness);
+                        r.n_metal = make_float4(N_feat.x, N_feat.y, N_feat.z, mr.metallic);
                         r.wi_b = make_float4(vv.x, vv.y, vv.z, b_pdf);
                         r.w_valid = make_float4(w_world.x, w_world.y, w_world.z, 1.0f);
                         r.thr = make_float4(T_before.x, T_before.y, T_before.z, 0.0f);
@@ -974,6 +1113,19 @@ __global__ void k_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, 
     for (uint32_t i = 0; i < n; ++i) out[i] = pcg_next(st, inc);
 }
 
+// Trilinear lookups of the scene textures (check entry point).
+__global__ void k_tex_sample(TexTables t, uint32_t n, const uint32_t *tex, const float *uv, const float *lod,
+                             float *rgba) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a;
+    const f3 c = tex_sample(t, tex[i], uv[2 * i], uv[2 * i + 1], lod[i], a);
+    rgba[4 * i] = c.x;
+    rgba[4 * i + 1] = c.y;
+    rgba[4 * i + 2] = c.z;
+    rgba[4 * i + 3] = a;
+}
+
 // Draws first_dim .. first_dim + n - 1 of the Sobol' stream of (pixel, sample).
 __global__ void k_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1043,10 +1195,12 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
                    unsigned long long *count_stats, uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cls_stride,
                    hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
-    const bool sort = q_cls != nullptr;
-    auto k = count_stats ? (sort ? k_extend<true, true> : k_extend<true, false>)
-                         : (sort ? k_extend<false, true> : k_extend<false, false>);
-    hipLaunchKernelGGL(k, dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, count, count_stats,
+    decltype(&k_extend<false, false, false>) ks[8] = {
+        k_extend<false, false, false>, k_extend<false, false, true>, k_extend<false, true, false>,
+        k_extend<false, true, true>,   k_extend<true, false, false>, k_extend<true, false, true>,
+        k_extend<true, true, false>,   k_extend<true, true, true>};
+    const int v = (count_stats ? 4 : 0) | (q_cls ? 2 : 0) | (s.has_alpha ? 1 : 0);
+    hipLaunchKernelGGL(ks[v], dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, count, count_stats,
                        q_cls, cls_cnt, cls_stride);
 }
 
@@ -1087,7 +1241,7 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
-    const int v = (count_stats ? 4 : 0) | (s.n_vol > 0 ? 2 : 0) | (s.pass_ior ? 1 : 0);
+    const int v = (count_stats ? 4 : 0) | (s.n_vol > 0 ? 2 : 0) | (s.pass_ior || s.has_alpha ? 1 : 0);
     decltype(&k_shadow<false, false, false>) ks[8] = {
         k_shadow<false, false, false>, k_shadow<false, false, true>, k_shadow<false, true, false>,
         k_shadow<false, true, true>,   k_shadow<true, false, false>, k_shadow<true, false, true>,
@@ -1120,6 +1274,11 @@ void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const flo
 
 void launch_env_eval(const EnvDev &e, uint32_t n, const float *dir, float *rgb, float *pdf, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_env_eval, dim3(blocks(n, 256)), dim3(256), 0, st, e, n, dir, rgb, pdf);
+}
+
+void launch_tex_sample(const TexTables &t, uint32_t n, const uint32_t *tex, const float *uv, const float *lod,
+                       float *rgba, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_tex_sample, dim3(blocks(n, 256)), dim3(256), 0, st, t, n, tex, uv, lod, rgba);
 }
 
 void launch_sobol(uint32_t pixel, uint32_t sample, uint32_t first_dim, uint32_t n, uint32_t *out, hipStream_t st) {

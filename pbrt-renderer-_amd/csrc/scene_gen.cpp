@@ -25,6 +25,9 @@
 //             (variant 0, 1, 2); majorants on 16^3-voxel bricks; in an open
 //             diffuse box (floor and three walls) under the config-2 envmap.
 #include "scene_gen.h"
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -386,6 +389,199 @@ void SceneData::finalize() {
         mats[k].surfaceArea = sum;
         mats[k].emitterSelectionProb = mats[k].isEmitter && ne ? 1.0f / (float)ne : 0.0f;
     }
+    texs.clear();
+    for (size_t i = 0; i < tex_data.size(); ++i) texs.push_back(nori_texture{tex_w[i], tex_h[i], tex_srgb[i], tex_data[i].data()});
+}
+
+// ------------------------------------------------------------ config 5
+// Procedural textures: an sRGB albedo pattern (two seeded colours mixed by
+// checker, stripes or value noise) and a tangent-space normal map from a sum
+// of sines height field; one alpha texture of vertical strands for the hair
+// cards.  Rows are filled in parallel.
+template <class F> void parallel_rows(uint32_t n, F f) {
+    const uint32_t nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    std::atomic<uint32_t> next(0);
+    for (uint32_t t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (uint32_t y; (y = next.fetch_add(1)) < n;) f(y);
+        });
+    for (auto &t : th) t.join();
+}
+
+void add_texture(SceneData &s, uint32_t w, uint32_t h, bool srgb, std::vector<uint8_t> &&rgba) {
+    s.tex_data.push_back(std::move(rgba));
+    s.tex_w.push_back(w);
+    s.tex_h.push_back(h);
+    s.tex_srgb.push_back(srgb ? 1u : 0u);
+}
+
+uint8_t to_byte(float v) { return (uint8_t)std::min(255.0f, std::max(0.0f, v * 255.0f + 0.5f)); }
+
+void albedo_texture(SceneData &s, uint32_t n, uint32_t seed) {
+    Rng r(0xA1BED000ull + seed);
+    const f3 ca = mk(r.next(), r.next(), r.next()), cb = mk(r.next(), r.next(), r.next());
+    const uint32_t kind = seed % 3, freq = 2 + (uint32_t)(r.next() * 14.0f);
+    std::vector<uint8_t> t((size_t)n * n * 4);
+    parallel_rows(n, [&](uint32_t y) {
+        for (uint32_t x = 0; x < n; ++x) {
+            const float u = ((float)x + 0.5f) / (float)n, v = ((float)y + 0.5f) / (float)n;
+            float m;
+            if (kind == 0) m = (((uint32_t)(u * freq) + (uint32_t)(v * freq)) & 1u) ? 1.0f : 0.0f;
+            else if (kind == 1) m = 0.5f + 0.5f * std::sin(2.0f * PI * (float)freq * (u + 0.3f * v));
+            else m = value_noise(u * freq, v * freq, 0.5f, seed);
+            const f3 c = ca * (1.0f - m) + cb * m;
+            uint8_t *o = &t[((size_t)y * n + x) * 4];
+            o[0] = to_byte(c.x); o[1] = to_byte(c.y); o[2] = to_byte(c.z); o[3] = 255;
+        }
+    });
+    add_texture(s, n, n, true, std::move(t));
+}
+
+void normal_texture(SceneData &s, uint32_t n, uint32_t seed) {
+    Rng r(0x0B0A7000ull + seed);
+    const float fu = 1.0f + std::floor(r.next() * 8.0f), fv = 1.0f + std::floor(r.next() * 8.0f), amp = 0.02f + 0.05f * r.next();
+    std::vector<uint8_t> t((size_t)n * n * 4);
+    parallel_rows(n, [&](uint32_t y) {
+        for (uint32_t x = 0; x < n; ++x) {
+            const float u = ((float)x + 0.5f) / (float)n, v = ((float)y + 0.5f) / (float)n;
+            // height h = amp (sin(2 pi fu u) + sin(2 pi fv v)); normal ~ (-dh/du, -dh/dv, 1)
+            const float du = amp * 2.0f * PI * fu * std::cos(2.0f * PI * fu * u);
+            const float dv = amp * 2.0f * PI * fv * std::cos(2.0f * PI * fv * v);
+            const f3 nn = normalize(mk(-du, -dv, 1.0f));
+            uint8_t *o = &t[((size_t)y * n + x) * 4];
+            o[0] = to_byte(nn.x * 0.5f + 0.5f); o[1] = to_byte(nn.y * 0.5f + 0.5f); o[2] = to_byte(nn.z * 0.5f + 0.5f);
+            o[3] = 255;
+        }
+    });
+    add_texture(s, n, n, false, std::move(t));
+}
+
+void strands_alpha_texture(SceneData &s, uint32_t n) {
+    std::vector<uint8_t> t((size_t)n * n * 4);
+    parallel_rows(n, [&](uint32_t y) {
+        for (uint32_t x = 0; x < n; ++x) {
+            const float u = ((float)x + 0.5f) / (float)n, v = ((float)y + 0.5f) / (float)n;
+            // eight strands across the card, thinning towards the tip (v = 1)
+            const float w = 0.5f * (1.0f - 0.7f * v);
+            const float c = std::fabs(std::fmod(u * 8.0f, 1.0f) - 0.5f) / std::max(w, 1e-3f);
+            const float a = std::max(0.0f, 1.0f - c * c);
+            uint8_t *o = &t[((size_t)y * n + x) * 4];
+            o[0] = to_byte(a); o[1] = to_byte(a); o[2] = to_byte(a); o[3] = to_byte(a);
+        }
+    });
+    add_texture(s, n, n, false, std::move(t));
+}
+
+float terrain_height(float x, float z) {
+    float h = 0.0f, amp = 2.0f, f = 0.04f;
+    for (int o = 0; o < 4; ++o, amp *= 0.5f, f *= 2.0f) h += amp * (value_noise(x * f, z * f, 3.7f, 0x7E44u + o) - 0.5f);
+    return h;
+}
+
+// Batched large scene: a heightfield terrain, displaced icospheres, alpha-tested
+// hair cards, 128 textures (64 albedo + 64 normal maps) plus the cards' alpha.
+void batched_scene(SceneData &s, Builder &b, Rng &rng, const nori_scene_options &opt) {
+    const uint32_t N = opt.grid ? opt.grid : 2048, level = opt.subdiv ? opt.subdiv : 4;
+    const uint32_t nsph = opt.meshes ? opt.meshes : 199, ncards = opt.strands ? opt.strands : 20000;
+    const uint32_t tsz = opt.tex_size ? opt.tex_size : 1024;
+    for (uint32_t i = 0; i < 64; ++i) albedo_texture(s, tsz, i);
+    for (uint32_t i = 0; i < 64; ++i) normal_texture(s, tsz, i);
+    strands_alpha_texture(s, tsz);
+    const uint32_t ALPHA_TEX = 128;
+    // terrain: N x N cells over [-50, 50]^2, uv = world / 10 (tiling textures)
+    {
+        std::vector<float> pos, nrm, uv;
+        std::vector<uint32_t> tri;
+        const float S = 100.0f, h = S / (float)N;
+        for (uint32_t j = 0; j <= N; ++j)
+            for (uint32_t i = 0; i <= N; ++i) {
+                const float x = -50.0f + h * (float)i, z = -50.0f + h * (float)j;
+                const float y = terrain_height(x, z);
+                const float e = 0.5f * h;
+                const f3 nn = normalize(mk(terrain_height(x - e, z) - terrain_height(x + e, z), 2.0f * e,
+                                           terrain_height(x, z - e) - terrain_height(x, z + e)));
+                pos.insert(pos.end(), {x, y, z});
+                nrm.insert(nrm.end(), {nn.x, nn.y, nn.z});
+                uv.insert(uv.end(), {x / 10.0f, z / 10.0f});
+            }
+        for (uint32_t j = 0; j < N; ++j)
+            for (uint32_t i = 0; i < N; ++i) {
+                const uint32_t a = j * (N + 1) + i, bb = a + 1, c = a + N + 1, d = c + 1;
+                tri.insert(tri.end(), {a, c, bb, bb, c, d});
+            }
+        nori_material m = default_material();
+        m.albedoTexIndex = 0;
+        m.normalTexIndex = 64;
+        b.add_mesh(pos, nrm, tri, m, &uv);
+    }
+    // displaced icospheres with spherical uv
+    std::vector<float> ipos;
+    std::vector<uint32_t> itri;
+    icosphere_mesh(level, ipos, itri);
+    for (uint32_t k = 0; k < nsph; ++k) {
+        const float r = 0.5f + 1.5f * rng.next();
+        const float x = -40.0f + 80.0f * rng.next(), z = -40.0f + 80.0f * rng.next();
+        const f3 c = mk(x, terrain_height(x, z) + r * 0.8f, z);
+        const uint32_t ns = 0x51u + k;
+        std::vector<float> pos(ipos.size()), nrm(ipos.size()), uv(ipos.size() / 3 * 2);
+        for (size_t v = 0; v < ipos.size(); v += 3) {
+            const f3 d = mk(ipos[v], ipos[v + 1], ipos[v + 2]);
+            const float rr = r * (0.9f + 0.2f * value_noise(d.x * 3.0f + 7.0f, d.y * 3.0f, d.z * 3.0f, ns));
+            const f3 p = c + d * rr;
+            pos[v] = p.x; pos[v + 1] = p.y; pos[v + 2] = p.z;
+            nrm[v] = d.x; nrm[v + 1] = d.y; nrm[v + 2] = d.z;
+            uv[v / 3 * 2] = 0.5f + std::atan2(d.z, d.x) / (2.0f * PI);
+            uv[v / 3 * 2 + 1] = std::acos(std::max(-1.0f, std::min(1.0f, d.y))) / PI;
+        }
+        nori_material m = default_material();
+        const float pick = rng.next();
+        const uint32_t t = 1 + k % 63;
+        if (pick < 0.4f) {
+            m.type = NORI_MAT_DISNEY;
+            m.metallic = rng.next();
+            m.roughness = 0.1f + 0.9f * rng.next();
+            m.clearcoat = rng.next();
+            m.sheen = rng.next();
+            m.albedoTexIndex = t;
+            m.normalTexIndex = 64 + t;
+        } else if (pick < 0.8f) {
+            m.albedoTexIndex = t;
+            m.normalTexIndex = 64 + t;
+        } else {
+            m.type = NORI_MAT_DIELECTRIC;
+            m.intIOR = 1.5f;
+            m.extIOR = 1.0f;
+        }
+        b.add_mesh(pos, nrm, itri, m, &uv);
+    }
+    // hair cards: 0.08 x 0.8 quads standing on the terrain near the camera's
+    // focus distance, alpha-tested with the strands texture
+    {
+        std::vector<float> pos, nrm, uv;
+        std::vector<uint32_t> tri;
+        for (uint32_t k = 0; k < ncards; ++k) {
+            const float x = -6.0f + 12.0f * rng.next(), z = -22.0f + 6.0f * rng.next(), yaw = 2.0f * PI * rng.next();
+            const f3 base = mk(x, terrain_height(x, z), z), a = mk(std::cos(yaw), 0.0f, std::sin(yaw)) * 0.08f;
+            const f3 up = mk(0.15f * (rng.next() - 0.5f), 0.8f, 0.15f * (rng.next() - 0.5f));
+            const f3 n = normalize(cross(a, up));
+            const uint32_t v0 = (uint32_t)(pos.size() / 3);
+            const f3 q[4] = {base - a * 0.5f, base + a * 0.5f, base + a * 0.5f + up, base - a * 0.5f + up};
+            const float quv[8] = {0, 0, 1, 0, 1, 1, 0, 1};
+            for (int i = 0; i < 4; ++i) {
+                pos.insert(pos.end(), {q[i].x, q[i].y, q[i].z});
+                nrm.insert(nrm.end(), {n.x, n.y, n.z});
+                uv.insert(uv.end(), {quv[2 * i], quv[2 * i + 1]});
+            }
+            tri.insert(tri.end(), {v0, v0 + 1, v0 + 2, v0, v0 + 2, v0 + 3});
+        }
+        nori_material m = default_material();
+        m.type = NORI_MAT_DISNEY;
+        m.albedo[0] = 0.35f; m.albedo[1] = 0.22f; m.albedo[2] = 0.12f;
+        m.roughness = 0.5f;
+        m.alphaTexIndex = ALPHA_TEX;
+        b.add_mesh(pos, nrm, tri, m, &uv);
+    }
 }
 
 SceneData generate_scene(const nori_scene_options &opt) {
@@ -458,13 +654,22 @@ SceneData generate_scene(const nori_scene_options &opt) {
         lookat(mk(0, 1.4f, 5.0f), mk(0, 0.9f, 0), mk(0, 1, 0), s.camera.to_world);
         s.camera.width = opt.width ? opt.width : 512;
         s.camera.height = opt.height ? opt.height : 512;
+    } else if (cfg == 5) {
+        b.attrs = true;
+        batched_scene(s, b, rng, opt);
+        sky_sun_env(s, rng, opt);
+        s.camera.fov_deg = 40.0f;
+        lookat(mk(0, 2.0f, -24.5f), mk(0, 0.8f, -19.5f), mk(0, 1, 0), s.camera.to_world);
+        s.camera.width = opt.width ? opt.width : 512;
+        s.camera.height = opt.height ? opt.height : 512;
     } else {
         throw std::invalid_argument("unknown scene config");
     }
     s.camera.near_clip = 1e-4f;
     s.camera.far_clip = 1e4f;
-    s.camera.lens_radius = 0.0f;
-    s.camera.focal_distance = 1.0f;
+    // config 5: thin lens, aperture radius 0.05 focused at 5
+    s.camera.lens_radius = cfg == 5 ? 0.05f : 0.0f;
+    s.camera.focal_distance = cfg == 5 ? 5.0f : 1.0f;
     s.finalize();
     return s;
 }
@@ -488,6 +693,8 @@ void SceneData::fill_desc(nori_scene_desc &d) const {
     d.env_scale = env_scale;
     d.num_volumes = (uint32_t)vols.size();
     d.volumes = vols.empty() ? nullptr : vols.data();
+    d.num_textures = (uint32_t)texs.size();
+    d.textures = texs.empty() ? nullptr : texs.data();
 }
 
 } // namespace ng

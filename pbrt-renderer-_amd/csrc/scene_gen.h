@@ -15,6 +15,9 @@ struct SceneData {
     std::vector<float> env_rgb;
     std::vector<nori_volume> vols;           // density pointers into grids
     std::vector<std::vector<float>> grids;   // one dense grid per heterogeneous volume
+    std::vector<std::vector<uint8_t>> tex_data; // RGBA8 texels of each texture
+    std::vector<nori_texture> texs;             // descriptors into tex_data (set by finalize)
+    std::vector<uint32_t> tex_w, tex_h, tex_srgb;
     nori_camera camera{};
 
     void finalize();

@@ -57,20 +57,24 @@ __device__ __forceinline__ bool mt_test(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float t
 
 __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xFFu); }
 
+// Candidate filter that accepts every triangle (opaque geometry).
+struct AcceptAll {
+    __device__ __forceinline__ bool operator()(uint32_t, uint32_t, float, float, f3, f3) const { return true; }
+};
+
 // Traverse; ANY = occlusion query (returns on the first accepted hit).
 // nodes: 5 uint4 per node; tris: 3 float4 per triangle; stk: this lane's LDS
 // column (entry i at stk[i * stride]).  counters (optional): nodes, triangles.
-// PASS (with ANY): shadow rays of the renderer (shaders/RayQueryTrace.hlsli:
-// 96-124): a hit on a triangle whose mesh has pass_ior[mesh].x > 0 (mirror and
-// dielectric meshes, {extIOR, intIOR}) does not occlude; it multiplies *trans
-// by 1 - Fresnel(|d . Ng|, extIOR, intIOR) and the search goes on.  Every
-// triangle is tested at most once, so the product covers each crossing once.
-template <bool ANY, bool COUNT, bool PASS = false>
+// accept(prim, mesh, u, v, e1, e2) decides whether a candidate that passes the
+// triangle test counts as a hit (the any-hit programs of shaders/Shaders.hlsl:
+// 674-735 and shaders/RayQueryTrace.hlsli:43-124: alpha test, glass and
+// mirrors on shadow rays); a rejected candidate leaves the search as it was.
+// Every triangle is tested at most once per ray.
+template <bool ANY, bool COUNT, class Accept = AcceptAll>
 __device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
                                                   f3 d, float tmin, float tmax, uint2 *stk, uint32_t stride,
                                                   uint32_t *nnodes = nullptr, uint32_t *ntris = nullptr,
-                                                  const float2 *__restrict__ pass_ior = nullptr,
-                                                  float *trans = nullptr) {
+                                                  Accept accept = Accept()) {
     TraceOut res{tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
     f3 rcp = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
     const bool nx = rcp.x < 0.0f, ny = rcp.y < 0.0f, nz = rcp.z < 0.0f;
@@ -136,15 +140,9 @@ __device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ node
                 float t, u, v;
                 if (mt_test(o, d, f3{t0.x, t0.y, t0.z}, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}, tmin, res.t, t, u,
                             v)) {
-                    if (PASS) {
-                        const float2 ior = pass_ior[__float_as_uint(t1.w)];
-                        if (ior.x > 0.0f) {
-                            const f3 ng = normalize(cross(f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}));
-                            *trans *= 1.0f - fresnel_dielectric(fabsf(dot(d, ng)), ior.x, ior.y);
-                            continue;
-                        }
-                    }
                     uint32_t prim = __float_as_uint(t0.w);
+                    if (!accept(prim, __float_as_uint(t1.w), u, v, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}))
+                        continue;
                     if (t < res.t || prim < res.prim) {
                         res.t = t;
                         res.u = u;

@@ -64,7 +64,11 @@ class SceneDesc(ctypes.Structure):
                 ("tangents", ctypes.c_void_p), ("indices", ctypes.c_void_p), ("mesh_tri_offset", ctypes.c_void_p),
                 ("mesh_tri_count", ctypes.c_void_p), ("materials", ctypes.c_void_p), ("env_width", _u32),
                 ("env_height", _u32), ("env_rgb", ctypes.c_void_p), ("env_scale", _f), ("num_volumes", _u32),
-                ("volumes", ctypes.c_void_p)]
+                ("volumes", ctypes.c_void_p), ("num_textures", _u32), ("textures", ctypes.c_void_p)]
+
+
+class Texture(ctypes.Structure):
+    _fields_ = [("width", _u32), ("height", _u32), ("srgb", _u32), ("rgba", ctypes.c_void_p)]
 
 
 class MediumSpec:
@@ -123,13 +127,15 @@ class BvhInfo(ctypes.Structure):
 
 class SceneOptions(ctypes.Structure):
     _fields_ = [("config", ctypes.c_int), ("seed", _u64), ("subdiv", _u32), ("grid", _u32), ("env_width", _u32),
-                ("env_height", _u32), ("width", _u32), ("height", _u32), ("strands", _u32), ("variant", _u32)]
+                ("env_height", _u32), ("width", _u32), ("height", _u32), ("strands", _u32), ("variant", _u32),
+                ("tex_size", _u32), ("meshes", _u32)]
 
 
 EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
     "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
-    "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_render_stats",
+    "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_texture_sample",
+    "nori_gpu_render_stats",
     "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_gpu_autotune", "nori_scene_generate", "nori_scene_get_desc",
     "nori_scene_get_camera", "nori_scene_free", "nori_bvh8_build_host",
 ]
@@ -160,6 +166,7 @@ def load_library(path=LIB_PATH):
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
         "nori_gpu_pcg32": (i32, [vp, _u64, _u64, _u64, i32, vp]),
         "nori_gpu_sobol": (i32, [vp, _u32, _u32, _u32, i32, vp]),
+        "nori_gpu_texture_sample": (i32, [vp, i32, vp, vp, vp, vp]),
         "nori_gpu_autotune": (i32, [vp, ctypes.POINTER(View), i32, _u32, ctypes.POINTER(TuneResult)]),
         "nori_gpu_render_stats": (i32, [vp, ctypes.POINTER(RenderStats)]),
         "nori_gpu_bvh_info": (i32, [vp, ctypes.POINTER(BvhInfo)]),
@@ -185,7 +192,7 @@ class HostScene:
     """Flat scene arrays (numpy, owned by Python) in the nori_scene_desc layout."""
 
     def __init__(self, positions, indices, mesh_tri_offset, mesh_tri_count, materials, normals=None,
-                 env_rgb=None, env_scale=1.0, camera=None, uvs=None, tangents=None, media=None):
+                 env_rgb=None, env_scale=1.0, camera=None, uvs=None, tangents=None, media=None, textures=None):
         self.positions = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.normals = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
@@ -196,6 +203,8 @@ class HostScene:
         self.env_scale = float(env_scale)
         self.camera = camera
         self.media = list(media or [])
+        # textures: list of (rgba uint8 array of shape (height, width, 4), srgb flag)
+        self.textures = [(np.ascontiguousarray(t, np.uint8), bool(srgb)) for t, srgb in (textures or [])]
         self.uvs = None if uvs is None else np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
         self.tangents = None if tangents is None else np.ascontiguousarray(tangents, np.float32).reshape(-1, 3)
 
@@ -221,15 +230,20 @@ class HostScene:
             d.env_height, d.env_width = self.env_rgb.shape[:2]
             d.env_rgb = _ptr(self.env_rgb)
         d.env_scale = self.env_scale
+        self._tex = (Texture * max(1, len(self.textures)))(
+            *[Texture(t.shape[1], t.shape[0], int(srgb), _ptr(t)) for t, srgb in self.textures])
+        d.num_textures = len(self.textures)
+        d.textures = ctypes.cast(self._tex, ctypes.c_void_p) if self.textures else None
         d._keep = self  # keep arrays alive while the descriptor is in use
         return d
 
 
 def generate_scene(config, seed=0, subdiv=0, grid=0, env_width=0, env_height=0, width=0, height=0, strands=0,
-                   variant=0):
+                   variant=0, tex_size=0, meshes=0):
     """Procedural benchmark scene built by the library (nori_scene_generate), copied into numpy."""
     lib = load_library()
-    opt = SceneOptions(config, seed, subdiv, grid, env_width, env_height, width, height, strands, variant)
+    opt = SceneOptions(config, seed, subdiv, grid, env_width, env_height, width, height, strands, variant, tex_size,
+                       meshes)
     h = ctypes.c_void_p()
     rc = lib.nori_scene_generate(ctypes.byref(opt), ctypes.byref(h))
     if rc != NORI_OK:
@@ -275,8 +289,12 @@ def _host_scene_from_desc(d, cam):
         if v.heterogeneous and v.density and v.width:
             dens = arr(v.density, v.width * v.height * v.depth, np.float32).reshape(v.depth, v.height, v.width)
         media.append(MediumSpec(list(v.vmin), list(v.vmax), list(v.sigma_a), list(v.sigma_s), v.g, dens, v.brick))
+    textures = []
+    for i in range(d.num_textures):
+        t = Texture.from_address(d.textures + i * ctypes.sizeof(Texture))
+        textures.append((arr(t.rgba, 4 * t.width * t.height, np.uint8).reshape(t.height, t.width, 4), bool(t.srgb)))
     return HostScene(pos, idx, mto, mtc, mats, normals=nrm, env_rgb=env, env_scale=d.env_scale, camera=cam,
-                     uvs=uvs, tangents=tans, media=media)
+                     uvs=uvs, tangents=tans, media=media, textures=textures)
 
 
 BVH8_NODE_DTYPE = np.dtype([("p", np.float32, 3), ("e", np.uint8, 3), ("imask", np.uint8),
@@ -459,6 +477,16 @@ class NoriGPU:
     def pcg32(self, initstate, initseq, advance, n):
         out = np.zeros(n, np.uint32)
         self._check(self.lib.nori_gpu_pcg32(self.h, initstate, initseq, advance, n, _ptr(out)), "pcg32")
+        return out
+
+    def texture_sample(self, tex, uv, lod):
+        """Trilinear lookups of the uploaded textures: tex (n,), uv (n, 2), lod (n,) -> RGBA (n, 4)."""
+        tex = np.ascontiguousarray(tex, np.uint32)
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        lod = np.ascontiguousarray(np.broadcast_to(lod, tex.shape), np.float32)
+        out = np.zeros((len(tex), 4), np.float32)
+        self._check(self.lib.nori_gpu_texture_sample(self.h, len(tex), _ptr(tex), _ptr(uv), _ptr(lod), _ptr(out)),
+                    "texture_sample")
         return out
 
     def sobol(self, pixel, sample, first_dim, n):
