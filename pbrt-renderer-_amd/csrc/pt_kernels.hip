@@ -608,7 +608,7 @@ __device__ void apply_textures(const SceneDev &s, uint32_t dims_max, uint32_t pr
         m.subsurface = tex_sample(s.tex, mr.subsurfaceTexIndex, tu, tv, tex_lod(s.tex, mr.subsurfaceTexIndex, foot), a).x;
 }
 
-// VOL: the scene has media; GUIDE: guided sampling and training records.// VOL: the scene has media; GUIDE: guided sampling and training records.
+// VOL: the scene has media; GUIDE: guided sampling and training records.
 // CLS: which paths the launch shades.  CLS_ALL takes positions [0, *cnt_in);
 // the others take the positions k_extend sorted into their queue (q_cls), so
 // each launch carries only its class's code: misses and emitter hits
