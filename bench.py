@@ -120,7 +120,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep --batch and the default chunk length")
     ap.add_argument("--guided", action="store_true",
                     help="time guided rendering (online-trained network) instead of plain path_mis")
-    ap.add_argument("--tune-spp", type=int, default=4, help="spp of the autotuner's calibration renders")
+    ap.add_argument("--tune-spp", type=int, default=0,
+                    help="spp of the autotuner's calibration renders (0: one step's spp, so the candidates run "
+                         "the timed steps' workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-spp", type=int, default=64)
     ap.add_argument("--no-time-to-mse", action="store_true", help="skip the time-to-relMSE leg")
@@ -165,7 +167,8 @@ def main():
     tune = None
     if not args.no_autotune:
         # Untimed: pool size and chunk length for this view, from short renders.
-        r = gpu.autotune(ng.make_view(cam, args.tune_spp, args.depth), calib_spp=args.tune_spp)
+        tspp = args.tune_spp or spp
+        r = gpu.autotune(ng.make_view(cam, tspp, args.depth), calib_spp=tspp)
         tune = {"pool": r.best_pool, "chunk_iters": r.best_chunk,
                 "candidates": [[r.pool[k], r.chunk[k], round(r.msamples_per_s[k], 1)] for k in range(r.count)]}
         log(f"[bench] autotune: pool {r.best_pool}, chunk {r.best_chunk} "

@@ -5,7 +5,11 @@
 //                  rrDepth (default 3), seed (default 0);
 //   "path_guided"  the same with online neural path guiding (one-sample MIS
 //                  between the learned vMF mixture and the BSDF);
-//   "direct"       path_mis limited to one bounce (maxDepth default 1).
+//   "direct"       the type the reference's scene files name for its direct-
+//                  lighting tests; the reference's renderer path traces every
+//                  scene with the same RayGen whatever the integrator type, so
+//                  "direct" is path_mis with the same defaults (maxDepth 1
+//                  keeps the first vertex's emission and next-event estimate).
 // preprocess() uploads the scene; Li() traces one ray through nori_gpu_li;
 // render() renders the camera view at the sampler's sample count through
 // nori_gpu_render.
@@ -111,7 +115,7 @@ public:
 };
 class DirectIntegrator : public GpuPathIntegrator {
 public:
-    explicit DirectIntegrator(const PropertyList &p) : GpuPathIntegrator(p, "direct", false, 1) {}
+    explicit DirectIntegrator(const PropertyList &p) : GpuPathIntegrator(p, "direct", false, 32) {}
 };
 
 NORI_REGISTER_CLASS(PathMisIntegrator, "path_mis");
