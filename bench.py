@@ -2,8 +2,9 @@
 configuration (8x8 level-6 icospheres = 5,242,882 triangles on a ground quad,
 2048x1024 sky+sun envmap, 512x512, 256 spp, max depth 12), plain path_mis.
 
-A step renders spp_per_step samples per pixel of the view (sample offsets
-advance from step to step, so K steps of 32 spp form the full 256-spp render).
+A step renders the configuration's whole job once: spp_per_step (default 256)
+samples per pixel of the 512x512 view in one persistent render call; sample
+offsets advance from step to step, so every step draws new samples.
 Inputs are resident in HBM before the timed region; the timed region is K
 render calls, each ending in a stream synchronisation.  After it, a second leg
 measures the GPU time to relMSE <= 1e-3 (validation_tests/compare_exr.py:102)
@@ -111,13 +112,14 @@ def time_to_rel_mse(gpu, ng, cam, depth, ref, start_spp, growth, max_spp, thresh
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp-per-step", type=int, default=32)
+    ap.add_argument("--spp-per-step", type=int, default=256)
     ap.add_argument("--depth", type=int, default=12)
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--batch", type=int, default=1 << 21)
-    ap.add_argument("--no-autotune", action="store_true", help="keep --batch and the default chunk length")
+    ap.add_argument("--no-autotune", action="store_true", help="keep --batch and --chunk-iters")
+    ap.add_argument("--chunk-iters", type=int, default=8, help="pool iterations per done-flag check (--no-autotune)")
     ap.add_argument("--guided", action="store_true",
                     help="time guided rendering (online-trained network) instead of plain path_mis")
     ap.add_argument("--tune-spp", type=int, default=0,
@@ -163,6 +165,7 @@ def main():
     log(f"[bench] upload: BVH8 {info.nodes} nodes depth {info.max_depth} ({info.node_bytes / 1e6:.1f} MB nodes, "
         f"{info.tri_bytes / 1e6:.1f} MB triangles), build {info.build_ms:.0f} ms, total {time.time() - t0:.1f}s")
     gpu.set_option("batch", args.batch)
+    gpu.set_option("chunk_iters", args.chunk_iters)
     spp = args.spp_per_step
     tune = None
     if not args.no_autotune:
@@ -284,7 +287,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"Disney icosphere grid 8x8, level {args.subdiv}, {len(scene.indices)} triangles, "
-                        f"2048x1024 sky+sun envmap, {cam.width}x{cam.height}, {args.steps * spp} spp, "
+                        f"2048x1024 sky+sun envmap, {cam.width}x{cam.height}, {spp} spp per step, "
                         f"max depth {args.depth}, {'guided' if args.guided else 'plain'} path_mis",
             "spp_per_step": spp,
             "path_pool": tune["pool"] if tune else args.batch,

@@ -249,7 +249,9 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * iterations between done-flag checks and between guide training rounds,
  * default 8), "sampler" (0: pcg32 per pixel, the default; 1: the reference GPU's
  * Owen-scrambled Sobol', shaders/RNG.hlsli), "sort" (0/1: class-sorted shading
- * launches, default 1), "timing_mode" (0/1), "count_traversal" (0/1), "guiding" (0/1:
+ * launches, default 1), "persistent" (0/1: traversal kernels as a grid of
+ * resident waves that take rays from a counter, default 0), "timing_mode"
+ * (0/1), "count_traversal" (0/1), "guiding" (0/1:
  * guided sampling with one-sample MIS and online training, one round per
  * pool's worth of samples started), "guide_train_steps" (Adam steps per
  * training round, default 4),

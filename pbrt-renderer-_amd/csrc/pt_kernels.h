@@ -90,6 +90,8 @@ struct ViewTab {
 //   cnt[3], cnt[4]: the two free-slot queues
 //   cnt[5]:         done flag of the render (no samples left, no live paths)
 //   cnt[6..8]:      shading-class queues of the iteration (light, surface, hair)
+//   cnt[9], cnt[10]: work counters of the persistent extension and shadow
+//                   traversals (zero between launches)
 //   stats[0..7]:    extension rays, shadow rays, unused, bounce iterations,
 //                   extension nodes, extension triangles, shadow nodes, shadow
 //                   triangles
@@ -126,9 +128,11 @@ void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *s
 // receive the summed BVH8 nodes visited and triangles tested.
 // q_cls (optional): sort the positions into the shading-class queues q_cls +
 // {0, 1, 2} * cls_stride with counts cls_cnt[0..2] (scenes without media).
+// work (optional): a zeroed counter; with it the traversal runs as a grid of
+// resident waves that take positions from the counter (persistent form).
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, uint32_t cap,
                    unsigned long long *count_stats, uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cls_stride,
-                   hipStream_t st);
+                   uint32_t *work, hipStream_t st);
 // Shading of the paths at positions [0, *cnt_in) of p: survivors to p_out
 // (count cnt_out); with q_free, the ids of paths that ended to q_free.  Bounce
 // and view come from p.meta, the view's depth limits from views.
@@ -142,7 +146,7 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
                          float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
-                   unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st);
+                   unsigned long long *count_stats, const GuideShadeDev &g, uint32_t *work, hipStream_t st);
 // Guiding: BF16 features of the hits at positions [0, *count) (zeros where
 // guiding does not apply).
 void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *count, bf16_t *X, uint32_t cap,

@@ -10,6 +10,9 @@
 #include "bsdf_dev.h"
 #include "pt_kernels.h"
 #include "trace_dev.h"
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 namespace ng {
@@ -136,7 +139,7 @@ __global__ void k_pool_start(Counters c, uint32_t *q_free0, uint32_t cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cap) q_free0[i] = i;
     if (i == 0) {
-        for (int k = 0; k < 9; ++k) c.cnt[k] = 0;
+        for (int k = 0; k < 11; ++k) c.cnt[k] = 0;
         c.cnt[3] = cap;
         c.next[0] = 0;
     }
@@ -233,6 +236,7 @@ __global__ void k_iter_end(Counters c, uint32_t qi) {
     c.cnt[2] = 0;
     c.cnt[3 + qi] = 0;
     c.cnt[6] = c.cnt[7] = c.cnt[8] = 0;
+    c.cnt[9] = c.cnt[10] = 0;
     c.cnt[5] = (c.next[0] >= c.next[1] && c.cnt[qi ^ 1u] == 0) ? 1u : 0u;
 }
 
@@ -355,6 +359,119 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
     if (COUNT) count_add(stats, 4, nn, nt);
 }
 
+// Persistent traversal: a fixed grid of resident waves works through the
+// queue positions [0, n).  A wave takes positions 64 at a time from the work
+// counter (one atomic per batch) and hands them to its lanes as they become
+// free, so a lane that finishes a short ray starts the next one at once
+// instead of idling until the wave's longest ray ends.  The work counter is
+// zero at entry (the iteration-end and counter kernels clear it).
+struct RayFeed {
+    uint32_t next, end; // wave-uniform: unhanded positions [next, end) of the wave's batch
+    bool exhausted;     // wave-uniform: the counter has passed n
+
+    __device__ __forceinline__ void init() {
+        next = end = 0;
+        exhausted = false;
+    }
+    // Lanes with pos == NONE and !done get a position (or become done).
+    // Returns the mask of lanes that received a new position.
+    __device__ __forceinline__ uint64_t refill(uint32_t *work, uint32_t n, uint32_t &pos, bool &done) {
+        const uint32_t lane = __lane_id();
+        const uint64_t lt = (1ull << lane) - 1ull;
+        uint64_t fresh = 0;
+        uint64_t need = __ballot(!done && pos == 0xFFFFFFFFu);
+        while (need) {
+            if (next >= end) {
+                if (exhausted) {
+                    if (pos == 0xFFFFFFFFu) done = true;
+                    break;
+                }
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(work, 64u);
+                base = __shfl(base, 0);
+                if (base >= n) {
+                    exhausted = true;
+                    if (pos == 0xFFFFFFFFu) done = true;
+                    break;
+                }
+                next = base;
+                end = min(base + 64u, n);
+            }
+            const uint32_t avail = end - next;
+            const uint32_t rank = (uint32_t)__popcll(need & lt);
+            if (!done && pos == 0xFFFFFFFFu && rank < avail) {
+                pos = next + rank;
+                fresh |= 1ull << lane;
+            }
+            next += min((uint32_t)__popcll(need), avail);
+            need = __ballot(!done && pos == 0xFFFFFFFFu);
+        }
+        return __ballot(fresh != 0);
+    }
+};
+
+// Closest hit of the paths at positions [0, *count), persistent form.
+template <bool COUNT, bool ALPHA>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_extend_p(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
+                                                          uint32_t *work, unsigned long long *stats) {
+    extern __shared__ uint2 stk[];
+    const uint32_t n = *count;
+    Bvh8Walker<false, COUNT> w;
+    w.cnt_nodes = w.cnt_tris = 0;
+    uint32_t pos = 0xFFFFFFFFu, seed = 0;
+    bool done = false;
+    RayFeed feed;
+    feed.init();
+    while (true) {
+        const uint64_t fresh = feed.refill(work, n, pos, done);
+        if ((fresh >> __lane_id()) & 1ull) {
+            const float4 a = p.ray_o[pos], b = p.ray_d[pos];
+            w.start(xyz(a), xyz(b), a.w, b.w);
+            if (ALPHA) seed = alpha_seed_path(p.aux[pos]);
+        }
+        if (__ballot(pos != 0xFFFFFFFFu) == 0) break;
+        if (pos != 0xFFFFFFFFu) {
+            bool fin;
+            if (ALPHA) {
+                auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3, f3) {
+                    return alpha_accept(s, prim, mesh, u, v, seed, false);
+                };
+                fin = w.step(s.nodes, s.tris, stk + threadIdx.x, blockDim.x, acc);
+            } else {
+                AcceptAll acc;
+                fin = w.step(s.nodes, s.tris, stk + threadIdx.x, blockDim.x, acc);
+            }
+            if (fin) {
+                p.hit[pos] = make_float4(w.res.t, __uint_as_float(w.res.prim), w.res.u, w.res.v);
+                pos = 0xFFFFFFFFu;
+            }
+        }
+    }
+    if (COUNT) count_add(stats, 4, w.cnt_nodes, w.cnt_tris);
+}
+
+// Shading class of the hits at positions [0, *count): each position appended
+// to the queue of its class (light, surface, hair at q_cls, q_cls + cap,
+// q_cls + 2 cap; counts cls_cnt[0..2]).
+__global__ __launch_bounds__(256) void k_classify(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
+                                                  uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, n = *count;
+    if (i - threadIdx.x % 64u >= n) return;
+    int cls = 0;
+    if (i < n) {
+        const uint32_t prim = __float_as_uint(p.hit[i].y);
+        cls = CLS_LIGHT;
+        if (prim != MISS) {
+            const nori_material &mr = s.mats[s.tri_mesh[prim]];
+            cls = mr.isEmitter ? CLS_LIGHT : (mr.type == NORI_MAT_HAIR ? CLS_HAIR : CLS_SURF);
+        }
+    }
+    for (int c = CLS_LIGHT; c <= CLS_HAIR; ++c) {
+        const uint32_t k = wave_append(cls_cnt + (c - 1), cls == c);
+        if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
+    }
+}
+
 // NEE contribution of a vertex: into the path radiance and, when the vertex
 // wrote a training record, into that record's snapshot (it is light that did
 // not arrive along the sampled direction).
@@ -426,6 +543,76 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
         }
     }
     if (COUNT) count_add(stats, 6, nn, nt);
+}
+
+// The shadow rays at queue positions [0, *count), persistent form (RayFeed).
+template <bool COUNT, bool VOL, bool PASS>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_p(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
+                                                          uint32_t *work, float4 *film, unsigned long long *stats,
+                                                          GuideShadeDev g) {
+    extern __shared__ uint2 stk[];
+    const uint32_t n = *count;
+    Bvh8Walker<true, COUNT> w;
+    w.cnt_nodes = w.cnt_tris = 0;
+    uint32_t pos = 0xFFFFFFFFu, seed = 0;
+    float trans = 1.0f;
+    bool done = false;
+    RayFeed feed;
+    feed.init();
+    while (true) {
+        const uint64_t fresh = feed.refill(work, n, pos, done);
+        if ((fresh >> __lane_id()) & 1ull) {
+            const float4 a = sh.o[pos], b = sh.d[pos];
+            const float tmin = sh.e[pos].x;
+            w.start(xyz(a), xyz(b), tmin, a.w);
+            trans = 1.0f;
+            if (PASS) seed = s.has_alpha ? alpha_seed_shadow(__float_as_uint(sh.e[pos].z)) : 0u;
+        }
+        if (__ballot(pos != 0xFFFFFFFFu) == 0) break;
+        if (pos != 0xFFFFFFFFu) {
+            bool fin;
+            if (PASS) {
+                // alpha-tested meshes first (they occlude when accepted), then
+                // mirror/dielectric meshes attenuate by 1 - Fresnel and let the
+                // ray through (shaders/RayQueryTrace.hlsli:96-124)
+                auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3 e1, f3 e2) {
+                    if (s.has_alpha && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE)
+                        return alpha_accept(s, prim, mesh, u, v, seed, true);
+                    if (s.pass_ior) {
+                        const float2 ior = s.pass_ior[mesh];
+                        if (ior.x > 0.0f) {
+                            const f3 ng = normalize(cross(e1, e2));
+                            trans *= 1.0f - fresnel_dielectric(fabsf(dot(w.d, ng)), ior.x, ior.y);
+                            return false;
+                        }
+                    }
+                    return true;
+                };
+                fin = w.step(s.nodes, s.tris, stk + threadIdx.x, blockDim.x, acc);
+            } else {
+                AcceptAll acc;
+                fin = w.step(s.nodes, s.tris, stk + threadIdx.x, blockDim.x, acc);
+            }
+            if (fin) {
+                if (w.res.prim == MISS) {
+                    const float4 c = sh.c[pos];
+                    f3 col = PASS ? xyz(c) * trans : xyz(c);
+                    if (VOL) {
+                        const float4 e = sh.e[pos];
+                        if (__float_as_uint(e.w) & 1u) {
+                            // transmittance through the media, with the ray's own stream
+                            uint64_t rr = pcg_seed((uint64_t)__float_as_uint(e.z), PCG_DEFAULT_STREAM);
+                            col = col * vol_transmittance(s.vols, s.n_vol, w.o, w.d, e.y, rr);
+                        }
+                    }
+                    film_add(film, __float_as_uint(sh.d[pos].w), col);
+                    guide_nee_add(g, __float_as_uint(c.w), col);
+                }
+                pos = 0xFFFFFFFFu;
+            }
+        }
+    }
+    if (COUNT) count_add(stats, 6, w.cnt_nodes, w.cnt_tris);
 }
 
 // One next-event shadow ray: occlusion over [tmin, tmax]; with media, the
@@ -1047,6 +1234,7 @@ __global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, u
     if (add_shadow) c.stats[1] += c.cnt[2];
     for (int k = 0; k < 9; ++k)
         if (clear_mask & (1u << k)) c.cnt[k] = 0;
+    c.cnt[9] = c.cnt[10] = 0; // persistent traversal work counters
 }
 
 __global__ void k_set_counter(uint32_t *cnt, uint32_t v) {
@@ -1191,10 +1379,46 @@ void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint
                            rows, seed, X, w_weight, wi_b);
 }
 
+// Grid of a persistent traversal kernel: the waves the device holds at once
+// (its occupancy at this LDS size times the CU count), at most one per 64
+// positions of capacity.
+static uint32_t resident_blocks(const void *kernel, size_t lds, uint32_t cap) {
+    static std::mutex mu;
+    static std::map<std::pair<const void *, size_t>, uint32_t> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kernel, lds});
+    uint32_t per_device;
+    if (it != cache.end()) {
+        per_device = it->second;
+    } else {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, TRACE_BLOCK, lds) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 1;
+        per_device = (uint32_t)(per_cu * cus);
+        cache[{kernel, lds}] = per_device;
+    }
+    return std::max(1u, std::min(per_device, blocks(cap, TRACE_BLOCK)));
+}
+
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, uint32_t cap,
                    unsigned long long *count_stats, uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cls_stride,
-                   hipStream_t st) {
+                   uint32_t *work, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
+    if (work) {
+        decltype(&k_extend_p<false, false>) kp[4] = {k_extend_p<false, false>, k_extend_p<false, true>,
+                                                     k_extend_p<true, false>, k_extend_p<true, true>};
+        const auto k = kp[(count_stats ? 2 : 0) | (s.has_alpha ? 1 : 0)];
+        hipLaunchKernelGGL(k, dim3(resident_blocks((const void *)k, lds, cap)), dim3(TRACE_BLOCK), lds, st, s, p,
+                           count, work, count_stats);
+        if (q_cls)
+            hipLaunchKernelGGL(k_classify, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, count, q_cls, cls_cnt,
+                               cls_stride);
+        return;
+    }
     decltype(&k_extend<false, false, false>) ks[8] = {
         k_extend<false, false, false>, k_extend<false, false, true>, k_extend<false, true, false>,
         k_extend<false, true, true>,   k_extend<true, false, false>, k_extend<true, false, true>,
@@ -1239,9 +1463,18 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
-                   unsigned long long *count_stats, const GuideShadeDev &g, hipStream_t st) {
+                   unsigned long long *count_stats, const GuideShadeDev &g, uint32_t *work, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
     const int v = (count_stats ? 4 : 0) | (s.n_vol > 0 ? 2 : 0) | (s.pass_ior || s.has_alpha ? 1 : 0);
+    if (work) {
+        decltype(&k_shadow_p<false, false, false>) kp[8] = {
+            k_shadow_p<false, false, false>, k_shadow_p<false, false, true>, k_shadow_p<false, true, false>,
+            k_shadow_p<false, true, true>,   k_shadow_p<true, false, false>, k_shadow_p<true, false, true>,
+            k_shadow_p<true, true, false>,   k_shadow_p<true, true, true>};
+        hipLaunchKernelGGL(kp[v], dim3(resident_blocks((const void *)kp[v], lds, cap)), dim3(TRACE_BLOCK), lds, st, s,
+                           sh, cnt_sh, work, film, count_stats, g);
+        return;
+    }
     decltype(&k_shadow<false, false, false>) ks[8] = {
         k_shadow<false, false, false>, k_shadow<false, false, true>, k_shadow<false, true, false>,
         k_shadow<false, true, true>,   k_shadow<true, false, false>, k_shadow<true, false, true>,

@@ -62,28 +62,44 @@ struct AcceptAll {
     __device__ __forceinline__ bool operator()(uint32_t, uint32_t, float, float, f3, f3) const { return true; }
 };
 
-// Traverse; ANY = occlusion query (returns on the first accepted hit).
-// nodes: 5 uint4 per node; tris: 3 float4 per triangle; stk: this lane's LDS
-// column (entry i at stk[i * stride]).  counters (optional): nodes, triangles.
+// One ray's traversal as a resumable state machine: start() sets it up,
+// each step() visits one node (its child boxes, then the triangles among its
+// hit children) or pops the stack, and returns true once the ray is finished
+// (res final).  ANY = occlusion query (finishes on the first accepted hit).
+// stk: this lane's LDS column (entry i at stk[i * stride]).
 // accept(prim, mesh, u, v, e1, e2) decides whether a candidate that passes the
 // triangle test counts as a hit (the any-hit programs of shaders/Shaders.hlsl:
 // 674-735 and shaders/RayQueryTrace.hlsli:43-124: alpha test, glass and
 // mirrors on shadow rays); a rejected candidate leaves the search as it was.
 // Every triangle is tested at most once per ray.
-template <bool ANY, bool COUNT, class Accept = AcceptAll>
-__device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
-                                                  f3 d, float tmin, float tmax, uint2 *stk, uint32_t stride,
-                                                  uint32_t *nnodes = nullptr, uint32_t *ntris = nullptr,
-                                                  Accept accept = Accept()) {
-    TraceOut res{tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
-    f3 rcp = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    const bool nx = rcp.x < 0.0f, ny = rcp.y < 0.0f, nz = rcp.z < 0.0f;
-    const uint32_t octinv = 7u - ((nx ? 4u : 0u) | (ny ? 2u : 0u) | (nz ? 1u : 0u));
-    const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
-    uint2 group = make_uint2(0u, (1u << (24u + octinv)) | 1u);
-    int sp = 0;
-    uint32_t cnt_nodes = 0, cnt_tris = 0;
-    while (true) {
+template <bool ANY, bool COUNT> struct Bvh8Walker {
+    f3 o, d, rcp;
+    float tmin;
+    uint32_t octinv;
+    bool nx, ny, nz;
+    TraceOut res;
+    uint2 group;
+    int sp;
+    uint32_t cnt_nodes, cnt_tris; // summed over every ray this walker ran
+
+    __device__ __forceinline__ void start(f3 o_, f3 d_, float tmin_, float tmax) {
+        o = o_;
+        d = d_;
+        tmin = tmin_;
+        res = TraceOut{tmax, 0.0f, 0.0f, 0xFFFFFFFFu};
+        rcp = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        nx = rcp.x < 0.0f;
+        ny = rcp.y < 0.0f;
+        nz = rcp.z < 0.0f;
+        octinv = 7u - ((nx ? 4u : 0u) | (ny ? 2u : 0u) | (nz ? 1u : 0u));
+        group = make_uint2(0u, (1u << (24u + octinv)) | 1u);
+        sp = 0;
+    }
+
+    template <class Accept>
+    __device__ __forceinline__ bool step(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, uint2 *stk,
+                                         uint32_t stride, Accept &accept) {
+        const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
         if (group.y & 0xFF000000u) {
             uint32_t bit = 31u - __builtin_clz(group.y & 0xFF000000u);
             group.y &= ~(1u << bit);
@@ -148,22 +164,36 @@ __device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ node
                         res.u = u;
                         res.v = v;
                         res.prim = prim;
-                        if (ANY) {
-                            if (COUNT) { *nnodes = cnt_nodes; *ntris = cnt_tris; }
-                            return res;
-                        }
+                        if (ANY) return true;
                     }
                 }
             }
         }
         if (!(group.y & 0xFF000000u)) {
-            if (sp == 0) break;
+            if (sp == 0) return true;
             --sp;
             group = stk[sp * stride];
         }
+        return false;
     }
-    if (COUNT) { *nnodes = cnt_nodes; *ntris = cnt_tris; }
-    return res;
+};
+
+// Whole traversal of one ray (counters, optional: nodes and triangles visited).
+template <bool ANY, bool COUNT, class Accept = AcceptAll>
+__device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
+                                                  f3 d, float tmin, float tmax, uint2 *stk, uint32_t stride,
+                                                  uint32_t *nnodes = nullptr, uint32_t *ntris = nullptr,
+                                                  Accept accept = Accept()) {
+    Bvh8Walker<ANY, COUNT> w;
+    w.cnt_nodes = w.cnt_tris = 0;
+    w.start(o, d, tmin, tmax);
+    while (!w.step(nodes, tris, stk, stride, accept)) {
+    }
+    if (COUNT) {
+        *nnodes = w.cnt_nodes;
+        *ntris = w.cnt_tris;
+    }
+    return w.res;
 }
 
 } // namespace ng

@@ -178,3 +178,23 @@ def test_guided_render_with_more_samples_than_the_sample_buffer(disney_small):
         assert g2.stats().guide_train_steps > 0
     finally:
         g2.close()
+
+
+def test_persistent_traversal_equals_one_thread_per_ray(pool_defaults, disney_small):
+    # the persistent extension and shadow kernels (resident waves refilling
+    # their lanes from a counter) trace the same rays to the same hits
+    gpu = pool_defaults
+    gpu.upload(disney_small)
+    view = ng.make_view(ng.lookat_camera_like(disney_small.camera, 64, 64), 8, 12)
+    _say("case: persistent traversal vs one thread per position")
+    gpu.set_option("persistent", 0)
+    gpu.reset_film()
+    ref = gpu.render(view, seed=31).copy()
+    try:
+        gpu.set_option("persistent", 1)
+        for batch in (1 << 21, 1000):
+            gpu.set_option("batch", batch)
+            gpu.reset_film()
+            _same_film(gpu.render(view, seed=31), ref, f"persistent, pool {batch}")
+    finally:
+        gpu.set_option("persistent", 0)
