@@ -32,6 +32,7 @@ extern "C" {
 #define NORI_MAT_MICROFACET 3
 #define NORI_MAT_DISNEY 4
 #define NORI_MAT_HAIR 5
+#define NORI_MAT_SUBSURFACE 6 /* random-walk subsurface (shaders/Subsurface.hlsli) */
 
 #define NORI_NO_TEXTURE 0xFFFFFFFFu
 

@@ -244,6 +244,25 @@ template <typename T> T disney_pdf(V3<T> wi, V3<T> wo, const MatT<T> &m) {
 }
 
 // ---------------------------------------------------------------- dispatch
+// Boundary reflection lobe of the random-walk subsurface material:
+// Beckmann with alpha = max(roughness^2, 1e-4), dielectric Fresnel
+// (shaders/Material.hlsli:48-57, 92-100).
+template <typename T> V3<T> sss_eval(V3<T> wi, V3<T> wo, const MatT<T> &m) {
+    if (wi.z <= T(0) || wo.z <= T(0)) return V3<T>(T(0));
+    const T a = std::max(m.roughness * m.roughness, T(1e-4));
+    const V3<T> wh = normalize(wi + wo);
+    const T Fr = fresnel_dielectric(dot(wh, wi), m.extIOR, m.intIOR);
+    const T G = smith_g1(wi, wh, a) * smith_g1(wo, wh, a);
+    return V3<T>(beckmann_d(wh, a) * Fr * G / std::max(T(4) * wi.z * wo.z, T(1e-10)));
+}
+template <typename T> T sss_pdf(V3<T> wi, V3<T> wo, const MatT<T> &m) {
+    if (wi.z <= T(0) || wo.z <= T(0)) return T(0);
+    const T a = std::max(m.roughness * m.roughness, T(1e-4));
+    const V3<T> wh = normalize(wi + wo);
+    const T Fr = fresnel_dielectric(dot(wh, wi), m.extIOR, m.intIOR);
+    return Fr * beckmann_d_cos(wh, a) / std::max(T(4) * dot(wh, wo), T(1e-10));
+}
+
 template <typename T> V3<T> material_eval(V3<T> wi, V3<T> wo, const MatT<T> &m) {
     switch (m.type) {
     case MAT_DIFFUSE:
@@ -251,6 +270,7 @@ template <typename T> V3<T> material_eval(V3<T> wi, V3<T> wo, const MatT<T> &m) 
         return m.albedo * Consts<T>::INV_PI;
     case MAT_MICROFACET: return microfacet_eval(wi, wo, m);
     case MAT_DISNEY: return disney_eval(wi, wo, m);
+    case MAT_SSS: return sss_eval(wi, wo, m);
     default: return V3<T>(T(0));
     }
 }
@@ -261,6 +281,7 @@ template <typename T> T material_pdf(V3<T> wi, V3<T> wo, const MatT<T> &m) {
         return wo.z * Consts<T>::INV_PI;
     case MAT_MICROFACET: return microfacet_pdf(wi, wo, m);
     case MAT_DISNEY: return disney_pdf(wi, wo, m);
+    case MAT_SSS: return sss_pdf(wi, wo, m);
     default: return T(0);
     }
 }

@@ -108,7 +108,7 @@ template <class Rng> static uint32_t shadow_word(const SceneData &s, Rng &rng) {
 // Texture maps at a hit (shaders/Shaders.hlsl:232-315) with the level of
 // detail from one view pixel's UV footprint at the segment's length
 // (shaders/GeometryUtils.hlsli:120-154; dims 0: level 0).
-static void apply_textures(const SceneData &s, uint32_t dims_max, uint32_t f, float bu, float bv, float hit_t,
+static float apply_textures(const SceneData &s, uint32_t dims_max, uint32_t f, float bu, float bv, float hit_t,
                            const MatRec &mr, V3f &N, MatT<float> &m) {
     const uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
     const float w = 1.0f - bu - bv;
@@ -150,6 +150,7 @@ static void apply_textures(const SceneData &s, uint32_t dims_max, uint32_t f, fl
     }
     if (mr.specularTex != 0xFFFFFFFFu) m.specular = look(mr.specularTex).r * 0.5f;
     if (mr.subsurfaceTex != 0xFFFFFFFFu) m.subsurface = look(mr.subsurfaceTex).r;
+    return foot;
 }
 
 // shaders/Emitter.hlsli:17-31
@@ -262,6 +263,137 @@ static V3f volume_nee_env(const SceneData &s, V3f P, V3f wo, const Volume &v, Rn
     return Le * fp / std::max(pe, 1e-20f) * w * tr * pass;
 }
 
+// Interpolated shading normal of triangle f at (u, v), the geometric normal
+// where the mesh has none or they cancel (shaders/GeometryUtils.hlsli:59-80).
+static V3f interp_normal(const SceneData &s, uint32_t f, float u, float v) {
+    const uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+    if (!s.nrm.empty()) {
+        V3f n0(s.nrm[3 * i0], s.nrm[3 * i0 + 1], s.nrm[3 * i0 + 2]);
+        V3f n1(s.nrm[3 * i1], s.nrm[3 * i1 + 1], s.nrm[3 * i1 + 2]);
+        V3f n2(s.nrm[3 * i2], s.nrm[3 * i2 + 1], s.nrm[3 * i2 + 2]);
+        V3f n = n0 * (1.0f - u - v) + n1 * u + n2 * v;
+        if (dot(n, n) >= 1e-8f) return normalize(n);
+    }
+    return normalize(cross(s.bvh.e1[f], s.bvh.e2[f]));
+}
+
+// Random walk through the interior of a subsurface mesh
+// (shaders/Subsurface.hlsli:48-141): free flights -ln(xi)/sigma_t, closest hits
+// from tmin 1e-4 up to the flight length, Fresnel reflection (and total
+// internal reflection) or refraction out at the mesh's own boundary, a ray
+// that reaches another mesh ends the walk, Henyey-Greenstein scattering that
+// multiplies the weight by the per-channel albedo, Russian roulette after the
+// fifth step, at most 256 steps.
+template <class Rng>
+static bool sss_walk(const SceneData &s, V3f pos, V3f dir, uint32_t mesh, float sigma_t, V3f alb, float g,
+                     float int_ior, float ext_ior, Rng &rng, V3f &exit_p, V3f &exit_d, V3f &exit_n, V3f &w) {
+    w = V3f(1.0f);
+    for (int step = 0; step < 256; ++step) {
+        const float dist = -std::log(std::max(rng.next_float(), 1e-10f)) / sigma_t;
+        const Ray r = mk_ray(pos, dir, 1e-4f, dist);
+        Hit h;
+        if (s.has_alpha) {
+            const uint32_t seed = alpha_seed(rng);
+            h = trace_accept(s.bvh, r, false,
+                             [&](uint32_t f, float u, float v) { return alpha_accept(s, f, u, v, seed, false); });
+        } else {
+            h = s.bvh.trace(r, false);
+        }
+        if (h.prim != 0xFFFFFFFFu) {
+            if (s.tri_mesh[h.prim] != mesh) {
+                w = V3f(0.0f);
+                return false;
+            }
+            const V3f bp = pos + dir * h.t;
+            V3f nb = interp_normal(s, h.prim, h.u, h.v);
+            if (dot(nb, dir) < 0.0f) nb = -nb;
+            const float fr = fresnel_dielectric(dot(dir, nb), int_ior, ext_ior);
+            if (rng.next_float() < fr) {
+                dir = normalize(reflect(dir, nb));
+                pos = bp - nb * 1e-4f;
+                continue;
+            }
+            const V3f out = refract(dir, -nb, int_ior / ext_ior);
+            if (dot(out, out) < 1e-8f) {
+                dir = normalize(reflect(dir, nb));
+                pos = bp - nb * 1e-4f;
+                continue;
+            }
+            exit_p = bp;
+            exit_d = normalize(out);
+            exit_n = nb;
+            return true;
+        }
+        pos = pos + dir * dist;
+        w = w * alb;
+        const float x1 = rng.next_float(), x2 = rng.next_float();
+        float ppdf;
+        dir = hg_sample(dir, g, x1, x2, ppdf);
+        if (step > 4) {
+            const float q = std::min(std::max(maxc(w), 0.05f), 0.95f);
+            if (rng.next_float() >= q) {
+                w = V3f(0.0f);
+                return false;
+            }
+            w = w / q;
+        }
+    }
+    w = V3f(0.0f);
+    return false;
+}
+
+// Next-event estimation at a subsurface entry with the boundary's reflection
+// lobe: one area-light and one envmap sample, balance heuristic
+// (shaders/Emitter.hlsli:43-153, 275-326).
+template <class Rng>
+static V3f sss_nee(const SceneData &s, V3f P, V3f N, V3f Ng, V3f Tg, V3f B, V3f wi, const MatT<float> &m, Rng &rng,
+                   const RenderParams &p, RenderStats &st) {
+    V3f L(0.0f);
+    const V3f so = offset_ray_origin(P, Ng, Ng);
+    if (!s.emitters.empty()) {
+        LightSample ls = sample_area_light(s, rng);
+        uint32_t tr_seed = shadow_word(s, rng);
+        V3f tl = ls.p - P;
+        float dist = length(tl);
+        V3f ww = tl / dist;
+        float ct = dot(N, ww), cl = dot(ls.n, -ww);
+        float pass = 0.0f;
+        if (ct > 0.0f && cl > 0.0f && (pass = shadow_pass_factor(s, so, ww, dist - 0.001f, st, tr_seed)) > 0.0f) {
+            V3f wol = to_local(ww, Tg, B, N);
+            V3f fv = material_eval(wi, wol, m);
+            float pb = material_pdf(wi, wol, m);
+            float pe = ls.pdfA * dist * dist / cl;
+            float w = balance_heuristic(pe, pb);
+            V3f tr = shadow_transmittance(s, so, ww, dist, tr_seed);
+            L += ls.Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w * tr * pass;
+        }
+    }
+    if (s.env.valid()) {
+        V3f dir, Le;
+        float pe;
+        if (p.env_cdf) {
+            float u1 = rng.next_float(), u2 = rng.next_float();
+            s.env.sample_cdf(u1, u2, dir, Le, pe);
+        } else {
+            uint32_t r = rng.next_uint();
+            float coin = rng.next_float(), jx = rng.next_float(), jy = rng.next_float();
+            s.env.sample_alias(r, coin, jx, jy, dir, Le, pe);
+        }
+        uint32_t tr_seed = shadow_word(s, rng);
+        float ct = dot(N, dir);
+        float pass = 0.0f;
+        if (pe > 0.0f && ct > 0.0f && (pass = shadow_pass_factor(s, so, dir, 1e20f, st, tr_seed)) > 0.0f) {
+            V3f wol = to_local(dir, Tg, B, N);
+            V3f fv = material_eval(wi, wol, m);
+            float pb = material_pdf(wi, wol, m);
+            float w = balance_heuristic(pe, pb);
+            V3f tr = shadow_transmittance(s, so, dir, 1e20f, tr_seed);
+            L += Le * fv * std::fabs(ct) / std::max(pe, 1e-20f) * w * tr * pass;
+        }
+    }
+    return L;
+}
+
 // RayGen body, shaders/Shaders.hlsl:55-627, for the surface materials of the
 // scene (diffuse, mirror, dielectric, microfacet, Disney).
 template <class Rng>
@@ -334,12 +466,11 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
         bool back = dot(Ng, ray.d) > 0.0f;
         if (back) Ng = -Ng;
         if (dot(N, Ng) < 0.0f) N = -N;
-        const V3f N_untextured = N;
-        (void)N_untextured;
+        float foot = 0.0f;
         if (!s.tex.empty() && !s.uv.empty() &&
             (mr.albedoTex != 0xFFFFFFFFu || mr.normalTex != 0xFFFFFFFFu || mr.roughnessTex != 0xFFFFFFFFu ||
              mr.specularTex != 0xFFFFFFFFu || mr.subsurfaceTex != 0xFFFFFFFFu))
-            apply_textures(s, std::max(cam.width, cam.height), f, h.u, h.v, h.t, mr, N, m);
+            foot = apply_textures(s, std::max(cam.width, cam.height), f, h.u, h.v, h.t, mr, N, m);
 
         if (mr.isEmitter) {
             V3f Le(mr.radiance[0], mr.radiance[1], mr.radiance[2]);
@@ -434,6 +565,67 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
             ray = mk_ray(P + on * 0.001f, nd, 0.0f, 1e20f);
             last_pdf = 0.0f;
             if (refr) eta *= etaI / etaT;
+        } else if (m.type == MAT_SSS) {
+            // Random-walk subsurface entry, shaders/Shaders.hlsl:513-612.
+            const V3f I = ray.d;
+            m.roughness *= 1.0f + (0.65f - 1.0f) * std::min(std::max(m.specular * 2.0f, 0.0f), 1.0f);
+            const float rough = m.roughness, aR = std::max(rough * rough, 1e-4f);
+            V3f wmL(0.0f, 0.0f, 1.0f);
+            if (rough > 1e-3f) {
+                float u0 = rng.next_float(), u1 = rng.next_float();
+                wmL = beckmann_sample(u0, u1, aR);
+            }
+            V3f wm = normalize(to_world(wmL, Tg, B, N));
+            float cos_m = dot(-I, wm);
+            if (cos_m <= 0.0f) {
+                wm = N;
+                wmL = V3f(0.0f, 0.0f, 1.0f);
+                cos_m = dot(-I, N);
+            }
+            const float g1i = rough > 1e-3f ? smith_g1(wi, wmL, aR) : 1.0f;
+            const float Fr = fresnel_dielectric(cos_m, m.extIOR, m.intIOR);
+            L += T * sss_nee(s, P, N, Ng, Tg, B, wi, m, rng, p, st);
+            if (rng.next_float() < Fr) {
+                const V3f refl = reflect(I, wm);
+                const V3f woL = to_local(refl, Tg, B, N);
+                if (woL.z <= 0.0f) break;
+                const float wgt = rough > 1e-3f ? g1i * smith_g1(woL, wmL, aR) * cos_m / std::max(wi.z * wmL.z, 1e-6f)
+                                                : 1.0f;
+                ray = mk_ray(P + Ng * 0.001f, refl, 0.0f, 1e20f);
+                T = T * wgt;
+                last_pdf = rough > 1e-3f ? Fr * beckmann_d_cos(wmL, aR) / std::max(4.0f * cos_m, 1e-6f) : 0.0f;
+            } else {
+                const V3f refr = refract(I, wm, m.extIOR / m.intIOR);
+                if (dot(refr, refr) < 1e-8f) {
+                    ray = mk_ray(P + Ng * 0.001f, reflect(I, wm), 0.0f, 1e20f);
+                    last_pdf = 0.0f;
+                } else {
+                    V3f a_blur = m.albedo, detail(1.0f);
+                    if (mr.albedoTex != 0xFFFFFFFFu && !s.tex.empty() && !s.uv.empty()) {
+                        const uint32_t i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+                        const float w0 = 1.0f - h.u - h.v;
+                        const float tu = s.uv[2 * i0] * w0 + s.uv[2 * i1] * h.u + s.uv[2 * i2] * h.v;
+                        const float tv = s.uv[2 * i0 + 1] * w0 + s.uv[2 * i1 + 1] * h.u + s.uv[2 * i2 + 1] * h.v;
+                        const Texture &tx = s.tex[mr.albedoTex];
+                        const Rgba c = sample(tx, tu, tv, lod_for(tx, foot) + 4.0f);
+                        a_blur = V3f(c.r, c.g, c.b);
+                        auto cl = [](float a, float b) { return std::min(std::max(a / std::max(b, 1e-3f), 0.5f), 2.0f); };
+                        detail = V3f(cl(m.albedo.x, a_blur.x), cl(m.albedo.y, a_blur.y), cl(m.albedo.z, a_blur.z));
+                    }
+                    const V3f A = a_blur * V3f(mr.sheen, mr.sheenTint, mr.clearcoat);
+                    const float sigma_t = 1.0f / std::max(m.subsurface, 1e-4f);
+                    const V3f A2 = A * A, A3 = A2 * A;
+                    const V3f ex = A * -5.09406f + A2 * 2.61188f - A3 * 4.31805f;
+                    const V3f alb(1.0f - std::exp(ex.x), 1.0f - std::exp(ex.y), 1.0f - std::exp(ex.z));
+                    V3f ep, ed, en, tw;
+                    if (!sss_walk(s, P - Ng * 0.001f, normalize(refr), mesh, sigma_t, alb, mr.anisotropic, m.intIOR,
+                                  m.extIOR, rng, ep, ed, en, tw))
+                        break;
+                    T = T * tw * detail;
+                    ray = mk_ray(offset_ray_origin(ep, en, en), ed, 0.0f, 1e20f);
+                    last_pdf = 0.0f;
+                }
+            }
         } else if (m.type == MAT_DIFFUSE || m.type == MAT_MICROFACET || m.type == MAT_DISNEY || m.type == MAT_HAIR) {
             const bool hair = m.type == MAT_HAIR;
             auto side = [&](V3f w) { return hair && dot(w, Ng) < 0.0f ? -Ng : Ng; }; // shaders/Emitter.hlsli:122-126

@@ -399,6 +399,15 @@ NG_HD void material_eval_pdf(f3 wi, f3 wo, const Mat &m, f3 &f, float &pdf) {
     } else if (m.type == NORI_MAT_MICROFACET) {
         f = microfacet_eval(wi, wo, m);
         pdf = microfacet_pdf(wi, wo, m);
+    } else if (m.type == NORI_MAT_SUBSURFACE) {
+        // the boundary's Beckmann reflection lobe (shaders/Material.hlsli:48-57, 92-100)
+        if (wi.z <= 0.0f || wo.z <= 0.0f) { f = mk(0.0f); pdf = 0.0f; return; }
+        const float a = fmaxf(m.roughness * m.roughness, 1e-4f);
+        const f3 wh = normalize(wi + wo);
+        const float Fr = fresnel_dielectric(dot(wh, wi), m.extIOR, m.intIOR);
+        const float G = smith_g1(wi, wh, a) * smith_g1(wo, wh, a);
+        f = mk(beckmann_d(wh, a) * Fr * G / fmaxf(4.0f * wi.z * wo.z, 1e-10f));
+        pdf = Fr * beckmann_d_cos(wh, a) / fmaxf(4.0f * dot(wh, wo), 1e-10f);
     } else {
         f = mk(0.0f);
         pdf = 0.0f;

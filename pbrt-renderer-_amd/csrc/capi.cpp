@@ -78,15 +78,15 @@ struct nori_ctx {
     PathsDev pd[2]{};                      // the two path-state buffers (active queue by parity)
     ShadowDev shdev{};
     uint32_t *qf[2] = {nullptr, nullptr};  // free path-id lists
-    uint32_t *qc = nullptr;                // shading-class queues (3 x cap)
+    uint32_t *qc = nullptr;                // shading-class queues (4 x cap)
     int sort_shading = 1;                  // option "sort": class-sorted shading launches
     // option "persistent": traversal as a grid of resident waves that refill
     // their lanes from a counter; off by default (measured slower than one
     // thread per position on the Disney grid, 474 vs 569 Msamples/s at a
     // 2^24-path pool)
     int persistent = 0;
-    uint32_t *work_ext() const { return persistent ? cdev.cnt + 9 : nullptr; }
-    uint32_t *work_shadow() const { return persistent ? cdev.cnt + 10 : nullptr; }
+    uint32_t *work_ext() const { return persistent ? cdev.cnt + 10 : nullptr; }
+    uint32_t *work_shadow() const { return persistent ? cdev.cnt + 11 : nullptr; }
     uint32_t sampler = 0;                  // option "sampler": 0 pcg32 per pixel, 1 Owen-scrambled Sobol'
     bool scene_has_hair = false;
     Counters cdev{};
@@ -255,7 +255,7 @@ struct nori_ctx {
         paths.release(); queues.release(); shadow.release();
         const size_t per = 5 * sizeof(float4) + 2 * sizeof(uint32_t);
         alloc(paths, (size_t)n * 2 * per);
-        alloc(queues, (size_t)n * 5 * sizeof(uint32_t));
+        alloc(queues, (size_t)n * 6 * sizeof(uint32_t));
         alloc(shadow, (size_t)n * 2 * 4 * sizeof(float4));
         for (int b = 0; b < 2; ++b) {
             float4 *pp = (float4 *)((char *)paths.p + (size_t)b * n * per);
@@ -627,6 +627,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         if (d->num_textures > NORI_MAX_TEXTURES) throw ArgError("scene: more than NORI_MAX_TEXTURES textures", NORI_E_LIMIT);
         if (d->num_textures && !d->textures) throw ArgError("scene: null texture array");
         s.has_alpha = 0;
+        s.has_sss = 0;
         for (uint32_t m = 0; m < d->num_meshes; ++m) {
             const nori_material &mr = mats[m];
             for (uint32_t ti : {mr.albedoTexIndex, mr.normalTexIndex, mr.roughnessTexIndex, mr.metallicTexIndex,
@@ -635,6 +636,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
                     throw ArgError("scene: material " + std::to_string(m) + " refers to texture " + std::to_string(ti) +
                                    " of " + std::to_string(d->num_textures));
             if (mr.alphaTexIndex != NORI_NO_TEXTURE) s.has_alpha = 1;
+            if (mr.type == NORI_MAT_SUBSURFACE && !mr.isEmitter) s.has_sss = 1;
         }
         if ((s.has_alpha || d->num_textures) && !d->uvs && d->num_textures)
             for (uint32_t m = 0; m < d->num_meshes; ++m)
@@ -653,7 +655,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
             s.tex = TexTables{c->upload(tu.tex.data(), tu.tex.size()), c->upload(tu.texel.data(), tu.texel.size()),
                               c->upload(tu.lut.data(), tu.lut.size()), (uint32_t)tu.tex.size()};
         }
-        if ((size_t)s.stack_depth * 64 * sizeof(uint2) > 64 * 1024)
+        if ((size_t)s.stack_depth * (s.has_sss ? 128 : 64) * sizeof(uint2) > 64 * 1024)
             throw ArgError("BVH8 deeper than the LDS stack allows", NORI_E_LIMIT);
         c->sdev = s;
         c->has_scene = true;

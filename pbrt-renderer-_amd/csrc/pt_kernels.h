@@ -37,6 +37,7 @@ struct SceneDev {
     uint32_t sobol;
     TexTables tex;        // scene textures (count 0: none)
     uint32_t has_alpha;   // some mesh has an alpha texture
+    uint32_t has_sss;     // some mesh has the random-walk subsurface material
 };
 
 // Guiding hooks of one render batch.  out: network outputs per queue position
@@ -89,8 +90,9 @@ struct ViewTab {
 //   cnt[2]:         shadow queue
 //   cnt[3], cnt[4]: the two free-slot queues
 //   cnt[5]:         done flag of the render (no samples left, no live paths)
-//   cnt[6..8]:      shading-class queues of the iteration (light, surface, hair)
-//   cnt[9], cnt[10]: work counters of the persistent extension and shadow
+//   cnt[6..9]:      shading-class queues of the iteration (light, surface, hair,
+//                   subsurface)
+//   cnt[10], cnt[11]: work counters of the persistent extension and shadow
 //                   traversals (zero between launches)
 //   stats[0..7]:    extension rays, shadow rays, unused, bounce iterations,
 //                   extension nodes, extension triangles, shadow nodes, shadow
@@ -127,7 +129,7 @@ void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *s
 // count_stats (optional): stats array whose [4],[5] (extend) / [6],[7] (shadow)
 // receive the summed BVH8 nodes visited and triangles tested.
 // q_cls (optional): sort the positions into the shading-class queues q_cls +
-// {0, 1, 2} * cls_stride with counts cls_cnt[0..2] (scenes without media).
+// {0, 1, 2, 3} * cls_stride with counts cls_cnt[0..3] (scenes without media).
 // work (optional): a zeroed counter; with it the traversal runs as a grid of
 // resident waves that take positions from the counter (persistent form).
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, uint32_t cap,
@@ -140,7 +142,8 @@ void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, c
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
                   uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
 // The same over the class queues k_extend sorted: one launch per class (the
-// hair launch only when `hair`), each compiled for its class alone.
+// hair launch only when `hair`, the subsurface one only when the scene has
+// subsurface materials), each compiled for its class alone.
 void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                          const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,

@@ -21,7 +21,9 @@ namespace {
 
 constexpr uint32_t MISS = 0xFFFFFFFFu;
 // Shading classes of a hit (the queues k_extend sorts positions into).
-constexpr int CLS_ALL = 0, CLS_LIGHT = 1, CLS_SURF = 2, CLS_HAIR = 3;
+// Shading classes; CLS_ALL / CLS_ALL_SSS shade every path of the queue (the
+// latter also the subsurface material, whose walk costs registers).
+constexpr int CLS_ALL = 0, CLS_LIGHT = 1, CLS_SURF = 2, CLS_HAIR = 3, CLS_SSS = 4, CLS_ALL_SSS = 5;
 constexpr int TRACE_BLOCK = 64;
 constexpr int SHADE_BLOCK = 128;
 
@@ -139,7 +141,7 @@ __global__ void k_pool_start(Counters c, uint32_t *q_free0, uint32_t cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cap) q_free0[i] = i;
     if (i == 0) {
-        for (int k = 0; k < 11; ++k) c.cnt[k] = 0;
+        for (int k = 0; k < 12; ++k) c.cnt[k] = 0;
         c.cnt[3] = cap;
         c.next[0] = 0;
     }
@@ -235,8 +237,8 @@ __global__ void k_iter_end(Counters c, uint32_t qi) {
     c.cnt[qi] = 0;
     c.cnt[2] = 0;
     c.cnt[3 + qi] = 0;
-    c.cnt[6] = c.cnt[7] = c.cnt[8] = 0;
-    c.cnt[9] = c.cnt[10] = 0;
+    c.cnt[6] = c.cnt[7] = c.cnt[8] = c.cnt[9] = 0;
+    c.cnt[10] = c.cnt[11] = 0;
     c.cnt[5] = (c.next[0] >= c.next[1] && c.cnt[qi ^ 1u] == 0) ? 1u : 0u;
 }
 
@@ -274,6 +276,14 @@ __device__ __forceinline__ void count_add(unsigned long long *stats, int k, uint
 // Closest hit of the paths at positions [0, *count).  SORT: the position is
 // also appended to the queue of its shading class (light, surface, hair at
 // q_cls, q_cls + cap, q_cls + 2 cap; counts cls_cnt[0..2]).
+// Shading class of a hit surface: emitters with the misses (light), hair,
+// the random-walk subsurface material, every other surface.
+__device__ __forceinline__ int shading_class(const nori_material &mr) {
+    if (mr.isEmitter) return CLS_LIGHT;
+    if (mr.type == NORI_MAT_HAIR) return CLS_HAIR;
+    return mr.type == NORI_MAT_SUBSURFACE ? CLS_SSS : CLS_SURF;
+}
+
 // Interpolated texture coordinates of triangle `prim` at barycentrics (u, v)
 // (shaders/GeometryUtils.hlsli GetInterpolatedUV: w = 1 - u - v).
 __device__ __forceinline__ void hit_uv(const SceneDev &s, uint32_t prim, float u, float v, float &tu, float &tv) {
@@ -346,12 +356,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
             cls = CLS_LIGHT;
             if (r.prim != MISS) {
                 const nori_material &mr = s.mats[s.tri_mesh[r.prim]];
-                cls = mr.isEmitter ? CLS_LIGHT : (mr.type == NORI_MAT_HAIR ? CLS_HAIR : CLS_SURF);
+                cls = shading_class(mr);
             }
         }
     }
     if (SORT) {
-        for (int c = CLS_LIGHT; c <= CLS_HAIR; ++c) {
+        for (int c = CLS_LIGHT; c <= CLS_SSS; ++c) {
             const uint32_t k = wave_append(cls_cnt + (c - 1), cls == c);
             if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
         }
@@ -463,10 +473,10 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev s, PathsDev p, const 
         cls = CLS_LIGHT;
         if (prim != MISS) {
             const nori_material &mr = s.mats[s.tri_mesh[prim]];
-            cls = mr.isEmitter ? CLS_LIGHT : (mr.type == NORI_MAT_HAIR ? CLS_HAIR : CLS_SURF);
+            cls = shading_class(mr);
         }
     }
-    for (int c = CLS_LIGHT; c <= CLS_HAIR; ++c) {
+    for (int c = CLS_LIGHT; c <= CLS_SSS; ++c) {
         const uint32_t k = wave_append(cls_cnt + (c - 1), cls == c);
         if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
     }
@@ -749,8 +759,9 @@ __device__ __forceinline__ bool mat_has_shading_tex(const nori_material &mr) {
 // (shaders/GeometryUtils.hlsli:120-154), albedo, a tangent-space normal map
 // on the frame from the triangle's UV derivatives, roughness (not for hair),
 // specular (x 0.5) and subsurface.
-__device__ void apply_textures(const SceneDev &s, uint32_t dims_max, uint32_t prim, float bu, float bv, float hit_t,
-                               const nori_material &mr, f3 &N, Mat &m) {
+// Returns the UV footprint it used.
+__device__ float apply_textures(const SceneDev &s, uint32_t dims_max, uint32_t prim, float bu, float bv, float hit_t,
+                                const nori_material &mr, f3 &N, Mat &m) {
     float tu, tv;
     hit_uv(s, prim, bu, bv, tu, tv);
     const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
@@ -793,13 +804,100 @@ __device__ void apply_textures(const SceneDev &s, uint32_t dims_max, uint32_t pr
         m.specular = tex_sample(s.tex, mr.specularTexIndex, tu, tv, tex_lod(s.tex, mr.specularTexIndex, foot), a).x * 0.5f;
     if (mr.subsurfaceTexIndex != NORI_NO_TEXTURE)
         m.subsurface = tex_sample(s.tex, mr.subsurfaceTexIndex, tu, tv, tex_lod(s.tex, mr.subsurfaceTexIndex, foot), a).x;
+    return foot;
+}
+
+// Interpolated shading normal of triangle prim at (u, v), the geometric
+// normal where the mesh has none or they cancel (shaders/GeometryUtils.hlsli:59-80).
+__device__ __forceinline__ f3 interp_normal(const SceneDev &s, uint32_t prim, float u, float v) {
+    const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+    if (s.nrm) {
+        const f3 n0 = f3{s.nrm[3 * i0], s.nrm[3 * i0 + 1], s.nrm[3 * i0 + 2]};
+        const f3 n1 = f3{s.nrm[3 * i1], s.nrm[3 * i1 + 1], s.nrm[3 * i1 + 2]};
+        const f3 n2 = f3{s.nrm[3 * i2], s.nrm[3 * i2 + 1], s.nrm[3 * i2 + 2]};
+        const f3 n = n0 * (1.0f - u - v) + n1 * u + n2 * v;
+        if (dot(n, n) >= 1e-8f) return normalize(n);
+    }
+    const f3 p0 = f3{s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
+    const f3 p1 = f3{s.pos[3 * i1], s.pos[3 * i1 + 1], s.pos[3 * i1 + 2]};
+    const f3 p2 = f3{s.pos[3 * i2], s.pos[3 * i2 + 1], s.pos[3 * i2 + 2]};
+    return normalize(cross_exact(sub_exact(p1, p0), sub_exact(p2, p0)));
+}
+
+// Random walk through the interior of a subsurface mesh (shaders/
+// Subsurface.hlsli:48-141): free flights of mean 1/sigma_t between
+// Henyey-Greenstein scatterings that multiply the weight by the per-channel
+// albedo, Fresnel reflection or refraction at the boundary, Russian roulette
+// after the fifth step, at most 256 steps.  A ray that reaches another mesh
+// ends the walk.  Closest hits use this lane's LDS stack column.  Returns true
+// when the walk leaves the surface (exit point, direction, outward normal,
+// weight).
+template <class Rng>
+__device__ bool sss_walk(const SceneDev &s, f3 pos, f3 dir, uint32_t mesh, float sigma_t, f3 alb, float g,
+                         float int_ior, float ext_ior, Rng &rng, uint2 *stk, uint32_t stride, f3 &exit_p, f3 &exit_d,
+                         f3 &exit_n, f3 &w) {
+    w = mk(1.0f);
+    for (int step = 0; step < 256; ++step) {
+        const float dist = -logf(fmaxf(rng_float(rng), 1e-10f)) / sigma_t;
+        TraceOut r;
+        if (s.has_alpha) {
+            const uint32_t seed = rng_state_hash(rng);
+            auto acc = [&](uint32_t prim, uint32_t m, float u, float v, f3, f3) {
+                return alpha_accept(s, prim, m, u, v, seed, false);
+            };
+            r = traverse_bvh8<false, false>(s.nodes, s.tris, pos, dir, 1e-4f, dist, stk, stride, nullptr, nullptr, acc);
+        } else {
+            r = traverse_bvh8<false, false>(s.nodes, s.tris, pos, dir, 1e-4f, dist, stk, stride);
+        }
+        if (r.prim != MISS) {
+            if (s.tri_mesh[r.prim] != mesh) {
+                w = mk(0.0f);
+                return false;
+            }
+            const f3 bp = pos + dir * r.t;
+            f3 nb = interp_normal(s, r.prim, r.u, r.v);
+            if (dot(nb, dir) < 0.0f) nb = -nb;
+            const float fr = fresnel_dielectric(dot(dir, nb), int_ior, ext_ior);
+            if (rng_float(rng) < fr) {
+                dir = normalize(reflect(dir, nb));
+                pos = bp - nb * 1e-4f;
+                continue;
+            }
+            const f3 out = refract(dir, -nb, int_ior / ext_ior);
+            if (dot(out, out) < 1e-8f) {
+                dir = normalize(reflect(dir, nb));
+                pos = bp - nb * 1e-4f;
+                continue;
+            }
+            exit_p = bp;
+            exit_d = normalize(out);
+            exit_n = nb;
+            return true;
+        }
+        pos = pos + dir * dist;
+        w = w * alb;
+        const float x1 = rng_float(rng), x2 = rng_float(rng);
+        float ppdf;
+        dir = hg_sample(dir, g, x1, x2, ppdf);
+        if (step > 4) {
+            const float q = fminf(fmaxf(maxc(w), 0.05f), 0.95f);
+            if (rng_float(rng) >= q) {
+                w = mk(0.0f);
+                return false;
+            }
+            w = w * (1.0f / q);
+        }
+    }
+    w = mk(0.0f);
+    return false;
 }
 
 // VOL: the scene has media; GUIDE: guided sampling and training records.
 // CLS: which paths the launch shades.  CLS_ALL takes positions [0, *cnt_in);
 // the others take the positions k_extend sorted into their queue (q_cls), so
 // each launch carries only its class's code: misses and emitter hits
-// (CLS_LIGHT), scattering surfaces but hair (CLS_SURF), hair (CLS_HAIR).
+// (CLS_LIGHT), scattering surfaces but hair (CLS_SURF), hair (CLS_HAIR), the
+// random-walk subsurface material (CLS_SSS).
 template <bool VOL, bool GUIDE, int CLS, bool SOBOL>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, PathsDev p_out,
                                                        const ViewTab *__restrict__ views,
@@ -807,9 +905,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *cnt_out,
                                                        uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
                                                        uint32_t *cnt_sh, float4 *film, GuideShadeDev g) {
+    // traversal stacks of the subsurface walks (allocated when the scene has
+    // subsurface materials: SHADE_BLOCK columns of the tree depth)
+    extern __shared__ uint2 shade_stk[];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *cnt_in) return;
-    const uint32_t pos = CLS == CLS_ALL ? i : q_cls[i];
+    constexpr bool ALL = CLS == CLS_ALL || CLS == CLS_ALL_SSS;
+    const uint32_t pos = ALL ? i : q_cls[i];
     const float4 ro = p.ray_o[pos], rd = p.ray_d[pos], hit = p.hit[pos], th = p.thr[pos], ax = p.aux[pos];
     const uint32_t meta = p.meta[pos], id = p.id[pos];
     const uint32_t bounce = meta & 0xFFFFu;
@@ -892,7 +994,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
 
     if (vol_scat) {
         // scattered in a medium: on to Russian roulette
-    } else if ((CLS == CLS_ALL || CLS == CLS_LIGHT) && prim == MISS) {
+    } else if ((ALL || CLS == CLS_LIGHT) && prim == MISS) {
         if (s.has_env) {
             f3 env = env_eval(s.env, d);
             f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
@@ -907,11 +1009,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         const uint32_t mesh = s.tri_mesh[prim];
         const nori_material &mr = s.mats[mesh];
         Mat m = load_mat(mr);
+        float foot = 0.0f;
         if (s.tex.count && s.uv && mat_has_shading_tex(mr)) {
             const nori_camera &vc = views[meta >> 16].cam;
-            apply_textures(s, vc.width > vc.height ? vc.width : vc.height, prim, hit.z, hit.w, hit.x, mr, N, m);
+            foot = apply_textures(s, vc.width > vc.height ? vc.width : vc.height, prim, hit.z, hit.w, hit.x, mr, N, m);
         }
-        const bool emitter = CLS == CLS_LIGHT ? true : (CLS == CLS_ALL ? mr.isEmitter != 0 : false);
+        const bool emitter = CLS == CLS_LIGHT ? true : (ALL ? mr.isEmitter != 0 : false);
         if (emitter) {
             f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
             if (last_pdf == 0.0f) {
@@ -933,7 +1036,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
             }
         } else {
             const uint32_t type = mr.type;
-            const bool hair = CLS == CLS_HAIR ? true : (CLS == CLS_ALL ? type == NORI_MAT_HAIR : false);
+            const bool hair = CLS == CLS_HAIR ? true : (ALL ? type == NORI_MAT_HAIR : false);
             f3 Tg, B;
             float hh = 0.0f;
             if (hair) hair_frame(s, prim, hit.z, hit.w, N, Tg, B, hh);
@@ -969,7 +1072,114 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 last_pdf = 0.0f;
                 if (refr) eta *= etaI / etaT;
                 alive = true;
-            } else if (guide_eligible(type) || hair) {
+            } else if ((CLS == CLS_SSS || CLS == CLS_ALL_SSS) && type == NORI_MAT_SUBSURFACE) {
+                // Random-walk subsurface entry, shaders/Shaders.hlsl:513-612: a
+                // Beckmann microfacet of the boundary, next-event estimation with
+                // its reflection lobe, then Fresnel reflection or refraction into
+                // the interior walk.
+                m.roughness *= 1.0f + (0.65f - 1.0f) * fminf(fmaxf(m.specular * 2.0f, 0.0f), 1.0f);
+                const float rough = m.roughness, aR = fmaxf(rough * rough, 1e-4f);
+                f3 wmL = mk(0.0f, 0.0f, 1.0f);
+                if (rough > 1e-3f) {
+                    const float u0 = rng_float(rng), u1 = rng_float(rng);
+                    wmL = beckmann_sample(u0, u1, aR);
+                }
+                f3 wm = normalize(to_world(wmL, Tg, B, N));
+                float cos_m = dot(-d, wm);
+                if (cos_m <= 0.0f) {
+                    wm = N;
+                    wmL = mk(0.0f, 0.0f, 1.0f);
+                    cos_m = dot(-d, N);
+                }
+                const float g1i = rough > 1e-3f ? smith_g1(wi, wmL, aR) : 1.0f;
+                const float Fr = fresnel_dielectric(cos_m, m.extIOR, m.intIOR);
+                so = offset_ray_origin(P, Ng, Ng);
+                if (s.n_emit > 0) {
+                    LightPt lp = sample_light(s, rng);
+                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : (s.has_alpha ? rng_state_hash(rng) : 0u);
+                    f3 tl = lp.p - P;
+                    float dist = length(tl);
+                    f3 ww = tl / dist;
+                    float ct = dot(N, ww), cl = dot(lp.n, -ww);
+                    if (ct > 0.0f && cl > 0.0f) {
+                        f3 f;
+                        float pb;
+                        material_eval_pdf(wi, to_local(ww, Tg, B, N), m, f, pb);
+                        float pe = lp.pdfA * dist * dist / cl;
+                        float w = balance_heuristic(pe, pb);
+                        ra = ShadowRec{so, ww, T * (lp.Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w)), 0.0f,
+                                       dist - 0.001f, dist, seed, tr_flag};
+                        sh_area = true;
+                    }
+                }
+                if (s.has_env) {
+                    uint32_t r = rng_u32(rng);
+                    float coin = rng_float(rng), jx = rng_float(rng), jy = rng_float(rng);
+                    f3 dir, Le;
+                    float pe;
+                    env_sample(s.env, r, coin, jx, jy, dir, Le, pe);
+                    uint32_t seed = VOL && s.n_vol ? rng_u32(rng) : (s.has_alpha ? rng_state_hash(rng) : 0u);
+                    float ct = dot(N, dir);
+                    if (pe > 0.0f && ct > 0.0f) {
+                        f3 f;
+                        float pb;
+                        material_eval_pdf(wi, to_local(dir, Tg, B, N), m, f, pb);
+                        float w = balance_heuristic(pe, pb);
+                        re = ShadowRec{so, dir, T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w)), 0.0f, 1e20f,
+                                       1e20f, seed, tr_flag};
+                        sh_env = true;
+                    }
+                }
+                if (rng_float(rng) < Fr) {
+                    const f3 refl = reflect(d, wm);
+                    const f3 woL = to_local(refl, Tg, B, N);
+                    if (woL.z > 0.0f) {
+                        const float wgt =
+                            rough > 1e-3f ? g1i * smith_g1(woL, wmL, aR) * cos_m / fmaxf(wi.z * wmL.z, 1e-6f) : 1.0f;
+                        new_o = P + Ng * 0.001f;
+                        new_d = refl;
+                        T *= wgt;
+                        last_pdf = rough > 1e-3f ? Fr * beckmann_d_cos(wmL, aR) / fmaxf(4.0f * cos_m, 1e-6f) : 0.0f;
+                        alive = true;
+                    }
+                } else {
+                    const f3 refr = refract(d, wm, m.extIOR / m.intIOR);
+                    if (dot(refr, refr) < 1e-8f) {
+                        new_o = P + Ng * 0.001f;
+                        new_d = reflect(d, wm);
+                        last_pdf = 0.0f;
+                        alive = true;
+                    } else {
+                        // interior parameters: scatter colour = albedo (blurred by
+                        // four mip levels where textured) x tint, mean free path
+                        // `subsurface`, phase anisotropy `anisotropic`
+                        f3 a_blur = m.albedo, detail = mk(1.0f);
+                        if (mr.albedoTexIndex != NORI_NO_TEXTURE && s.tex.count && s.uv) {
+                            float tu, tv, aa;
+                            hit_uv(s, prim, hit.z, hit.w, tu, tv);
+                            a_blur = tex_sample(s.tex, mr.albedoTexIndex, tu, tv,
+                                                tex_lod(s.tex, mr.albedoTexIndex, foot) + 4.0f, aa);
+                            detail = f3{fminf(fmaxf(m.albedo.x / fmaxf(a_blur.x, 1e-3f), 0.5f), 2.0f),
+                                        fminf(fmaxf(m.albedo.y / fmaxf(a_blur.y, 1e-3f), 0.5f), 2.0f),
+                                        fminf(fmaxf(m.albedo.z / fmaxf(a_blur.z, 1e-3f), 0.5f), 2.0f)};
+                        }
+                        const f3 A = a_blur * f3{mr.sheen, mr.sheenTint, mr.clearcoat};
+                        const float sigma_t = 1.0f / fmaxf(m.subsurface, 1e-4f);
+                        const f3 A2 = A * A, A3 = A2 * A;
+                        const f3 ex = -5.09406f * A + 2.61188f * A2 - 4.31805f * A3;
+                        const f3 alb = f3{1.0f - expf(ex.x), 1.0f - expf(ex.y), 1.0f - expf(ex.z)};
+                        f3 ep, ed, en, tw;
+                        if (sss_walk(s, P - Ng * 0.001f, normalize(refr), mesh, sigma_t, alb, mr.anisotropic,
+                                     m.intIOR, m.extIOR, rng, shade_stk + threadIdx.x, blockDim.x, ep, ed, en, tw)) {
+                            T = T * tw * detail;
+                            new_o = offset_ray_origin(ep, en, en);
+                            new_d = ed;
+                            last_pdf = 0.0f;
+                            alive = true;
+                        }
+                    }
+                }
+            } else if (CLS != CLS_SSS && (guide_eligible(type) || hair)) {
                 so = offset_ray_origin(P, Ng, Ng);
                 // Hair transmits: rays leave from the side of the fiber surface
                 // they head to (shaders/Emitter.hlsli:122-126).
@@ -1234,7 +1444,7 @@ __global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, u
     if (add_shadow) c.stats[1] += c.cnt[2];
     for (int k = 0; k < 9; ++k)
         if (clear_mask & (1u << k)) c.cnt[k] = 0;
-    c.cnt[9] = c.cnt[10] = 0; // persistent traversal work counters
+    c.cnt[10] = c.cnt[11] = 0; // persistent traversal work counters
 }
 
 __global__ void k_set_counter(uint32_t *cnt, uint32_t v) {
@@ -1428,21 +1638,34 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
                        q_cls, cls_cnt, cls_stride);
 }
 
-template <int CLS>
-static decltype(&k_shade<false, false, CLS, false>) shade_kernel(bool vol, bool guide, bool sobol) {
-    decltype(&k_shade<false, false, CLS, false>) ks[8] = {
-        k_shade<false, false, CLS, false>, k_shade<false, false, CLS, true>, k_shade<false, true, CLS, false>,
-        k_shade<false, true, CLS, true>,   k_shade<true, false, CLS, false>, k_shade<true, false, CLS, true>,
-        k_shade<true, true, CLS, false>,   k_shade<true, true, CLS, true>};
-    return ks[(vol ? 4 : 0) | (guide ? 2 : 0) | (sobol ? 1 : 0)];
+// Shading kernel of a sorted class (scenes without media).
+template <int CLS> static decltype(&k_shade<false, false, CLS, false>) shade_kernel(bool guide, bool sobol) {
+    decltype(&k_shade<false, false, CLS, false>) ks[4] = {k_shade<false, false, CLS, false>,
+                                                          k_shade<false, false, CLS, true>,
+                                                          k_shade<false, true, CLS, false>, k_shade<false, true, CLS, true>};
+    return ks[(guide ? 2 : 0) | (sobol ? 1 : 0)];
+}
+// Shading kernel over every path of the queue.
+static decltype(&k_shade<false, false, CLS_ALL, false>) shade_kernel_all(bool vol, bool guide, bool sobol, bool sss) {
+    decltype(&k_shade<false, false, CLS_ALL, false>) ks[16] = {
+        k_shade<false, false, CLS_ALL, false>,     k_shade<false, false, CLS_ALL, true>,
+        k_shade<false, true, CLS_ALL, false>,      k_shade<false, true, CLS_ALL, true>,
+        k_shade<true, false, CLS_ALL, false>,      k_shade<true, false, CLS_ALL, true>,
+        k_shade<true, true, CLS_ALL, false>,       k_shade<true, true, CLS_ALL, true>,
+        k_shade<false, false, CLS_ALL_SSS, false>, k_shade<false, false, CLS_ALL_SSS, true>,
+        k_shade<false, true, CLS_ALL_SSS, false>,  k_shade<false, true, CLS_ALL_SSS, true>,
+        k_shade<true, false, CLS_ALL_SSS, false>,  k_shade<true, false, CLS_ALL_SSS, true>,
+        k_shade<true, true, CLS_ALL_SSS, false>,   k_shade<true, true, CLS_ALL_SSS, true>};
+    return ks[(sss ? 8 : 0) | (vol ? 4 : 0) | (guide ? 2 : 0) | (sobol ? 1 : 0)];
 }
 
 void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
                   uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
     const bool guide = g.out != nullptr || g.rec != nullptr;
-    hipLaunchKernelGGL(shade_kernel<CLS_ALL>(s.n_vol > 0, guide, s.sobol != 0), dim3(blocks(cap, SHADE_BLOCK)),
-                       dim3(SHADE_BLOCK), 0, st, s, p, p_out, views, nullptr, cnt_in, cnt_out, q_free, cnt_free, sh,
+    const size_t lds = s.has_sss ? (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2) : 0;
+    hipLaunchKernelGGL(shade_kernel_all(s.n_vol > 0, guide, s.sobol != 0, s.has_sss != 0), dim3(blocks(cap, SHADE_BLOCK)),
+                       dim3(SHADE_BLOCK), lds, st, s, p, p_out, views, nullptr, cnt_in, cnt_out, q_free, cnt_free, sh,
                        cnt_sh, film, g);
 }
 
@@ -1453,13 +1676,17 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
     const bool guide = g.out != nullptr || g.rec != nullptr, sobol = s.sobol != 0;
     const dim3 grid(blocks(cap, SHADE_BLOCK)), block(SHADE_BLOCK);
     const size_t cs = cls_stride;
-    hipLaunchKernelGGL(shade_kernel<CLS_LIGHT>(false, guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls,
+    hipLaunchKernelGGL(shade_kernel<CLS_LIGHT>(guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls,
                        cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
-    hipLaunchKernelGGL(shade_kernel<CLS_SURF>(false, guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls + cs,
-                       cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    hipLaunchKernelGGL(shade_kernel<CLS_SURF>(guide, sobol), grid, block, 0, st, s, p, p_out, views,
+                       q_cls + cs, cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
     if (hair)
-        hipLaunchKernelGGL(shade_kernel<CLS_HAIR>(false, guide, sobol), grid, block, 0, st, s, p, p_out, views,
+        hipLaunchKernelGGL(shade_kernel<CLS_HAIR>(guide, sobol), grid, block, 0, st, s, p, p_out, views,
                            q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+    if (s.has_sss)
+        hipLaunchKernelGGL(shade_kernel<CLS_SSS>(guide, sobol), grid, block,
+                           (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2), st, s, p, p_out, views, q_cls + 3 * cs,
+                           cls_cnt + 3, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,

@@ -232,33 +232,40 @@ private:
     BSDFGPUData m_d;
 };
 
-/// "subsurface": parsed with the reference's properties.  The GPU path has no
-/// random-walk subsurface material; it shades these surfaces with their
-/// albedo as a diffuse BSDF.
+/// "subsurface": the random-walk subsurface material (src/cpu/subsurface.cpp:
+/// 1-92; the GPU record reuses fields as the reference does: albedo -> scatter
+/// colour A, subsurface -> mean free path, anisotropic -> HG g, sheen /
+/// sheenTint / clearcoat -> tint RGB).  eval/pdf give the boundary's Beckmann
+/// reflection lobe (the GPU BSDF query); the interior walk is part of the
+/// path tracer, so sample() returns no direction.
 class Subsurface : public GpuBSDF {
 public:
     explicit Subsurface(const PropertyList &p) {
         Color3f a = p.getColor("albedo", Color3f(0.8f, 0.5f, 0.4f));
         for (int k = 0; k < 3; ++k) m_d.albedo[k] = a[k];
-        m_radius = p.getFloat("radius", 0.1f);
+        m_d.subsurface = p.getFloat("radius", 0.1f);
         m_d.intIOR = p.getFloat("intIOR", 1.33f);
         m_d.extIOR = p.getFloat("extIOR", 1.000277f);
-        m_g = p.getFloat("g", 0.0f);
+        m_d.anisotropic = p.getFloat("g", 0.0f);
         m_d.roughness = p.getFloat("roughness", 0.3f);
+        Color3f tint = p.getColor("tint", Color3f(1.0f, 1.0f, 1.0f));
+        m_d.sheen = tint[0];
+        m_d.sheenTint = tint[1];
+        m_d.clearcoat = tint[2];
         m_d.specular = p.getFloat("specular", 0.0f);
         readTextures(p, m_d);
-        m_d.type = BSDFGPUData::DIFFUSE;
+        m_d.type = BSDFGPUData::SUBSURFACE;
     }
     bool isDiffuse() const override { return true; }
     BSDFGPUData getGPUData() const override { return m_d; }
     std::string toString() const override {
         return format("Subsurface[\n  albedo = %s,\n  radius = %f,\n  g = %f\n]",
-                      Color3f(m_d.albedo[0], m_d.albedo[1], m_d.albedo[2]).toString(), m_radius, m_g);
+                      Color3f(m_d.albedo[0], m_d.albedo[1], m_d.albedo[2]).toString(), m_d.subsurface,
+                      m_d.anisotropic);
     }
 
 private:
     BSDFGPUData m_d;
-    float m_radius, m_g;
 };
 
 NORI_REGISTER_CLASS(Diffuse, "diffuse");

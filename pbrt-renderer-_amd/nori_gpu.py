@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnori_gpu.so")
 
 NORI_OK = 0
-MAT_DIFFUSE, MAT_MIRROR, MAT_DIELECTRIC, MAT_MICROFACET, MAT_DISNEY, MAT_HAIR = range(6)
+MAT_DIFFUSE, MAT_MIRROR, MAT_DIELECTRIC, MAT_MICROFACET, MAT_DISNEY, MAT_HAIR, MAT_SUBSURFACE = range(7)
 NO_TEXTURE = 0xFFFFFFFF
 
 _f, _u32, _u64 = ctypes.c_float, ctypes.c_uint32, ctypes.c_uint64

@@ -305,3 +305,20 @@ def test_media_packing(tmp_path):
     assert m.vmin == [-1, 0, -2] and m.vmax == [1, 3, 2] and m.brick == 4
     assert np.allclose(m.sigma_s, 5.0) and math.isclose(m.g, 0.85, rel_tol=1e-6)
     assert np.array_equal(m.density, grid)
+
+
+def test_subsurface_bsdf_record(tmp_path):
+    # src/cpu/subsurface.cpp:1-92: albedo -> A, radius -> subsurface, g ->
+    # anisotropic, tint -> sheen / sheenTint / clearcoat, type 6
+    xml = ("<scene><camera type='perspective'/><mesh type='obj'><string name='filename' value='m.obj'/>"
+           "<bsdf type='subsurface'><color name='albedo' value='0.9 0.6 0.5'/><float name='radius' value='0.25'/>"
+           "<float name='g' value='0.3'/><float name='intIOR' value='1.4'/><color name='tint' value='1 0.8 0.6'/>"
+           "<float name='roughness' value='0.2'/></bsdf></mesh></scene>")
+    (tmp_path / "m.obj").write_text(QUAD)
+    hs = ng.XmlScene(text=xml, directory=str(tmp_path)).host_scene()
+    m = hs.materials[0]
+    assert m["type"] == ng.MAT_SUBSURFACE
+    assert np.allclose(m["albedo"], [0.9, 0.6, 0.5])
+    assert m["subsurface"] == pytest.approx(0.25) and m["anisotropic"] == pytest.approx(0.3)
+    assert m["intIOR"] == pytest.approx(1.4) and m["roughness"] == pytest.approx(0.2)
+    assert (m["sheen"], m["sheenTint"], m["clearcoat"]) == pytest.approx((1.0, 0.8, 0.6))
