@@ -80,3 +80,34 @@ def test_xml_li_is_unbiased(xml_scene):
     _say(f"  means {gm} {cm}")
     assert np.isfinite(g).all() and (g >= 0).all()
     assert np.allclose(gm, cm, rtol=0.05)
+
+
+def test_hair_file_scene_matches_oracle(tmp_path):
+    # a .hair strand file through the "hair" mesh plugin (tubes with fiber
+    # tangents), Chiang hair BSDF, flat grey environment: same samples as the oracle
+    from test_xml_scene import _hair_file
+    rng = np.random.default_rng(3)
+    strands = []
+    for k in range(40):
+        root = np.array([rng.uniform(-0.5, 0.5), 0.0, rng.uniform(-0.5, 0.5)])
+        pts = [root + np.array([0.05 * np.sin(j + k), 0.1 * j, 0.03 * j]) for j in range(12)]
+        strands.append(np.array(pts, np.float32))
+    _hair_file(tmp_path / "h.hair", strands, default_thickness=0.02)
+    xml = ("<scene><integrator type='path_mis'><integer name='maxDepth' value='8'/></integrator>"
+           "<camera type='perspective'><float name='fov' value='40'/><integer name='width' value='32'/>"
+           "<integer name='height' value='32'/><transform name='toWorld'><lookat origin='0, 0.6, 3' "
+           "target='0, 0.6, 0' up='0, 1, 0'/></transform></camera>"
+           "<sampler type='independent'><integer name='sampleCount' value='16'/></sampler>"
+           "<mesh type='hair'><string name='filename' value='h.hair'/><integer name='sides' value='5'/>"
+           "<bsdf type='hair_bsdf'/></mesh></scene>")
+    s = ng.XmlScene(text=xml, directory=str(tmp_path))
+    try:
+        _say("case: hair file 40 strands x 11 segments, 32x32 16spp")
+        img = s.render()
+        hs = s.host_scene()
+    finally:
+        s.close()
+    o, _ = orc.OracleScene(hs).render(hs.camera, 16, 8, seed=0)
+    e = orc.rel_mse(img, o[:, :3] / 16)
+    _say(f"  relMSE {e:.3e}")
+    assert e < 1e-3 and img.mean() > 0.01

@@ -322,3 +322,67 @@ def test_subsurface_bsdf_record(tmp_path):
     assert m["subsurface"] == pytest.approx(0.25) and m["anisotropic"] == pytest.approx(0.3)
     assert m["intIOR"] == pytest.approx(1.4) and m["roughness"] == pytest.approx(0.2)
     assert (m["sheen"], m["sheenTint"], m["clearcoat"]) == pytest.approx((1.0, 0.8, 0.6))
+
+
+def _hair_file(path, strands, thickness=None, default_thickness=0.1, flags_extra=0, signature=b"HAIR",
+               npoints_override=None):
+    """Writes a .hair file (Yuksel's format: 128-byte header, segments, points, thickness)."""
+    pts = np.concatenate([np.asarray(s, np.float32) for s in strands])
+    flags = 1 | 2 | (4 if thickness is not None else 0) | flags_extra
+    npts = len(pts) if npoints_override is None else npoints_override
+    hdr = signature + struct.pack("<IIII f f 3f", len(strands), npts, flags, 0, default_thickness, 1.0, 1, 1, 1)
+    hdr += b"\0" * (128 - len(hdr))
+    body = np.array([len(s) - 1 for s in strands], np.uint16).tobytes() + pts.tobytes()
+    if thickness is not None:
+        body += np.asarray(thickness, np.float32).tobytes()
+    path.write_bytes(hdr + body)
+
+
+HAIR_XML = ("<scene><camera type='perspective'/><mesh type='hair'><string name='filename' value='h.hair'/>"
+            "<integer name='sides' value='{sides}'/>{extra}<bsdf type='hair_bsdf'/></mesh></scene>")
+
+
+def test_hair_file_tubes(tmp_path):
+    # src/cpu/hair.cpp:150-322: (points) x sides vertices, 2 x sides triangles per
+    # segment, rings normal to the fiber tangent at radius thickness / 2,
+    # u along the strand, v = (1 - cos angle) / 2
+    s0 = [[0, 0, 0], [0, 1, 0], [0, 2, 0.5], [0, 3, 1.5]]
+    s1 = [[1, 0, 0], [1, 0, 1], [1, 0, 2]]
+    thick = [0.2, 0.2, 0.1, 0.1, 0.4, 0.4, 0.4]
+    _hair_file(tmp_path / "h.hair", [s0, s1], thick)
+    hs = ng.XmlScene(text=HAIR_XML.format(sides=4, extra=""), directory=str(tmp_path)).host_scene()
+    assert len(hs.positions) == (4 + 3) * 4 and len(hs.indices) == (3 + 2) * 2 * 4
+    assert hs.tangents is not None and hs.normals is not None and hs.uvs is not None
+    # ring j of strand 0 around control point j: |p - c| = thickness / 2, normal
+    # outward and orthogonal to the tangent, tangent = central difference
+    p, n, t, uv = hs.positions, hs.normals, hs.tangents, hs.uvs
+    for j in range(4):
+        ring = slice(4 * j, 4 * j + 4)
+        c = np.asarray(s0[j], np.float32)
+        assert np.allclose(np.linalg.norm(p[ring] - c, axis=1), thick[j] / 2, atol=1e-5)
+        assert np.allclose(np.linalg.norm(n[ring], axis=1), 1.0, atol=1e-5)
+        assert np.allclose((n[ring] * t[ring]).sum(1), 0.0, atol=1e-5)
+        assert np.allclose(np.cross(p[ring] - c, n[ring]), 0.0, atol=1e-5)
+        d = np.asarray(s0[min(j + 1, 3)], np.float32) - np.asarray(s0[max(j - 1, 0)], np.float32)
+        assert np.allclose(t[ring], d / np.linalg.norm(d), atol=1e-5)
+        assert np.allclose(uv[ring, 0], j / 3)
+        assert np.allclose(uv[ring, 1], [0.0, 0.5, 1.0, 0.5], atol=1e-6)  # angles 0, 90, 180, 270 degrees
+    # the second strand starts at vertex 16 and its triangles index its own rings
+    assert hs.indices[24:].min() >= 16
+    assert hs.materials[0]["type"] == ng.MAT_HAIR
+
+
+def test_hair_file_options_and_errors(tmp_path):
+    s0 = [[0, 0, 0], [0, 1, 0]]
+    s1 = [[1, 0, 0], [1, 1, 0], [1, 2, 0]]
+    _hair_file(tmp_path / "h.hair", [s0, s1], default_thickness=0.5)
+    hs = ng.XmlScene(text=HAIR_XML.format(sides=6, extra="<integer name='maxStrands' value='1'/>"
+                                          "<float name='radiusScale' value='2'/>"),
+                     directory=str(tmp_path)).host_scene()
+    assert len(hs.positions) == 2 * 6 and len(hs.indices) == 2 * 6
+    assert np.allclose(np.linalg.norm(hs.positions[:6] - np.array([0, 0, 0]), axis=1), 0.5)  # 0.5 / 2 * 2
+    for kw, needle in [(dict(signature=b"HAIX"), "signature"), (dict(npoints_override=6), "points")]:
+        _hair_file(tmp_path / "h.hair", [s0, s1], **kw)
+        with pytest.raises(ng.NoriError) as e:
+            ng.XmlScene(text=HAIR_XML.format(sides=6, extra=""), directory=str(tmp_path))
+        assert needle in str(e.value)
