@@ -250,7 +250,8 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * iterations between done-flag checks and between guide training rounds,
  * default 8), "sampler" (0: pcg32 per pixel, the default; 1: the reference GPU's
  * Owen-scrambled Sobol', shaders/RNG.hlsli), "sort" (0/1: class-sorted shading
- * launches, default 1), "persistent" (0/1: traversal kernels as a grid of
+ * launches, default 1), "aov" (0/1: denoiser feature films, nori_gpu_aov,
+ * default 0), "persistent" (0/1: traversal kernels as a grid of
  * resident waves that take rays from a counter, default 0), "timing_mode"
  * (0/1), "count_traversal" (0/1), "guiding" (0/1:
  * guided sampling with one-sample MIS and online training, one round per
@@ -260,6 +261,19 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * (learning rate x 1e6, default 2000), "guide_reset" (next use re-initialises
  * the network). */
 int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
+
+/* ---------------------------------------------------- denoiser features */
+#define NORI_AOV_ALBEDO 0
+#define NORI_AOV_NORMAL 1
+/* Denoiser feature films of the views of the last render (replaces the
+ * g_albedo / g_normal accumulation of shaders/Shaders.hlsl:99-106, 630-638):
+ * with option "aov" set, every sample adds the albedo and the shading normal
+ * of its first non-delta interaction (mirror and dielectric hits are looked
+ * through; a miss adds the envmap radiance and a zero normal, an emitter hit
+ * white and its normal).  rgba receives, per film entry as in
+ * nori_gpu_render, the RGB sum and the sample count; like the radiance film it
+ * accumulates until nori_gpu_reset_film. */
+int nori_gpu_aov(nori_ctx *ctx, int which, float *rgba);
 
 /* ----------------------------------------------------------- autotuner */
 /* Picks the path-pool size (option "batch") and the iterations between

@@ -40,7 +40,7 @@ def lib():
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_bvh_nodes.argtypes, L.orc_scene_bvh_nodes.restype = [vp], u32
         L.orc_trace.argtypes = [vp, vp, u32, i32, i32, vp, vp, vp, vp, i32, vp]
-        L.orc_render.argtypes = [vp, vp, u32, u32, u32, u32, u64, i32, i32, i32, vp, vp]
+        L.orc_render.argtypes = [vp, vp, u32, u32, u32, u32, u64, i32, i32, i32, vp, vp, vp, vp]
         L.orc_sobol.argtypes = [u32, u32, u32, u32, vp]
         L.orc_sobol_parts.argtypes = [u32, u32, vp, vp, vp]
         L.orc_camera_rays.argtypes = [vp, u32, vp, vp]
@@ -201,14 +201,17 @@ class OracleScene:
         return t, prim, u, v
 
     def render(self, camera, spp, max_depth, rr_depth=3, seed=0, sample_offset=0, env_cdf=False, nthreads=0,
-               sampler=0):
-        """sampler 0: pcg32 per pixel; 1: Owen-scrambled Sobol' (shaders/RNG.hlsli)."""
+               sampler=0, aov=False):
+        """sampler 0: pcg32 per pixel; 1: Owen-scrambled Sobol' (shaders/RNG.hlsli).
+        aov: also return the denoiser features (albedo, normal: RGBA sums per pixel)."""
         w, h = camera.width, camera.height
         acc = np.zeros((w * h, 4), np.float32)
         stats = np.zeros(3, np.uint64)
+        alb = np.zeros((w * h, 4), np.float32) if aov else None
+        nrm = np.zeros((w * h, 4), np.float32) if aov else None
         lib().orc_render(self.h, ctypes.byref(camera), spp, sample_offset, max_depth, rr_depth, seed, int(env_cdf),
-                         nthreads, int(sampler), _p(acc), _p(stats))
-        return acc, stats
+                         nthreads, int(sampler), _p(acc), _p(stats), _p(alb), _p(nrm))
+        return (acc, stats, alb, nrm) if aov else (acc, stats)
 
     def li(self, rays, seeds, max_depth, rr_depth=3):
         """Integrator::Li per ray (rays: n x 8 float32 o, mint, d, maxt)."""

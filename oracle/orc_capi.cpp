@@ -153,10 +153,10 @@ void orc_trace(void *hp, const float *rays, uint32_t n, int any_hit, int brute, 
 
 void orc_render(void *hp, const Camera *cam, uint32_t spp, uint32_t sample_offset, uint32_t max_depth,
                 uint32_t rr_depth, uint64_t seed, int env_cdf, int nthreads, int sampler, float *accum,
-                uint64_t *stats3) {
+                uint64_t *stats3, float *aov_albedo, float *aov_normal) {
     RenderParams p{spp, sample_offset, max_depth, rr_depth, seed, env_cdf, nthreads, sampler};
     RenderStats st;
-    render(((Handle *)hp)->s, *cam, p, accum, &st);
+    render(((Handle *)hp)->s, *cam, p, accum, &st, aov_albedo, aov_normal);
     if (stats3) { stats3[0] = st.samples; stats3[1] = st.extension_rays; stats3[2] = st.shadow_rays; }
 }
 

@@ -397,7 +397,10 @@ static V3f sss_nee(const SceneData &s, V3f P, V3f N, V3f Ng, V3f Tg, V3f B, V3f 
 // RayGen body, shaders/Shaders.hlsl:55-627, for the surface materials of the
 // scene (diffuse, mirror, dielectric, microfacet, Disney).
 template <class Rng>
-static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, const RenderParams &p, RenderStats &st) {
+static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, const RenderParams &p, RenderStats &st,
+                   Aov *aov = nullptr) {
+    Aov aov_unused;
+    Aov &av = aov ? *aov : aov_unused;
     V3f L(0.0f), T(1.0f);
     float last_pdf = 0.0f, eta = 1.0f;
     const uint32_t E = (uint32_t)s.emitters.size();
@@ -439,6 +442,7 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
         if (vol_scattered) {
             // scatter event: on to Russian roulette
         } else if (h.prim == 0xFFFFFFFFu) {
+            av.capture(s.env.valid() ? s.env.eval(ray.d) : V3f(0.0f), V3f(0.0f));
             if (s.env.valid()) {
                 V3f env = s.env.eval(ray.d);
                 if (last_pdf == 0.0f) L += T * env;
@@ -473,6 +477,7 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
             foot = apply_textures(s, std::max(cam.width, cam.height), f, h.u, h.v, h.t, mr, N, m);
 
         if (mr.isEmitter) {
+            av.capture(V3f(1.0f), N);
             V3f Le(mr.radiance[0], mr.radiance[1], mr.radiance[2]);
             if (last_pdf == 0.0f) {
                 L += T * Le;
@@ -567,6 +572,7 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
             if (refr) eta *= etaI / etaT;
         } else if (m.type == MAT_SSS) {
             // Random-walk subsurface entry, shaders/Shaders.hlsl:513-612.
+            av.capture(m.albedo, N);
             const V3f I = ray.d;
             m.roughness *= 1.0f + (0.65f - 1.0f) * std::min(std::max(m.specular * 2.0f, 0.0f), 1.0f);
             const float rough = m.roughness, aR = std::max(rough * rough, 1e-4f);
@@ -627,6 +633,7 @@ static V3f path_li(const SceneData &s, const Camera &cam, Ray ray, Rng &rng, con
                 }
             }
         } else if (m.type == MAT_DIFFUSE || m.type == MAT_MICROFACET || m.type == MAT_DISNEY || m.type == MAT_HAIR) {
+            av.capture(m.albedo, N);
             const bool hair = m.type == MAT_HAIR;
             auto side = [&](V3f w) { return hair && dot(w, Ng) < 0.0f ? -Ng : Ng; }; // shaders/Emitter.hlsli:122-126
             V3f so = offset_ray_origin(P, Ng, Ng);
@@ -709,16 +716,17 @@ V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, cons
 // then the path).
 template <class Rng>
 static V3f camera_sample(const SceneData &s, const Camera &cam, uint32_t x, uint32_t y, Rng &rng,
-                         const RenderParams &p, RenderStats &st) {
+                         const RenderParams &p, RenderStats &st, Aov *aov) {
     float jx = rng.next_float(), jy = rng.next_float();
     float ax = 0.5f, ay = 0.5f;
     if (cam.lens_radius > 0.0f) { ax = rng.next_float(); ay = rng.next_float(); }
     Ray ray;
     camera_ray(cam, (float)x + jx, (float)y + jy, ax, ay, ray);
-    return path_li(s, cam, ray, rng, p, st);
+    return path_li(s, cam, ray, rng, p, st, aov);
 }
 
-void render(const SceneData &s, const Camera &cam, const RenderParams &p, float *accum, RenderStats *stats) {
+void render(const SceneData &s, const Camera &cam, const RenderParams &p, float *accum, RenderStats *stats,
+            float *aov_albedo, float *aov_normal) {
     const uint32_t W = cam.width, H = cam.height, BS = 32;
     const uint32_t bx = (W + BS - 1) / BS, by = (H + BS - 1) / BS, nblocks = bx * by;
     std::atomic<uint32_t> next(0);
@@ -736,14 +744,23 @@ void render(const SceneData &s, const Camera &cam, const RenderParams &p, float 
                     float *a = accum + 4 * (size_t)pix;
                     for (uint32_t si = 0; si < p.spp; ++si) {
                         V3f L;
+                        Aov av;
                         if (p.sampler == 1) {
                             SobolRng rng(pix, p.sample_offset + si);
-                            L = camera_sample(s, cam, x, y, rng, p, st);
+                            L = camera_sample(s, cam, x, y, rng, p, st, &av);
                         } else {
                             Pcg32 rng = sample_stream(p.seed, pix, p.sample_offset + si);
-                            L = camera_sample(s, cam, x, y, rng, p, st);
+                            L = camera_sample(s, cam, x, y, rng, p, st, &av);
                         }
                         a[0] += L.x; a[1] += L.y; a[2] += L.z; a[3] += 1.0f;
+                        if (aov_albedo) {
+                            float *q = aov_albedo + 4 * (size_t)pix;
+                            q[0] += av.albedo.x; q[1] += av.albedo.y; q[2] += av.albedo.z; q[3] += 1.0f;
+                        }
+                        if (aov_normal) {
+                            float *q = aov_normal + 4 * (size_t)pix;
+                            q[0] += av.normal.x; q[1] += av.normal.y; q[2] += av.normal.z; q[3] += 1.0f;
+                        }
                         ++st.samples;
                     }
                 }

@@ -67,8 +67,25 @@ struct RenderStats {
     uint64_t extension_rays = 0, shadow_rays = 0, samples = 0;
 };
 
+// Denoiser features of one sample (shaders/Shaders.hlsl:99-106, 630-638):
+// albedo and normal at the first non-delta interaction (the envmap radiance
+// and a zero normal for a miss, white and the shading normal for an emitter).
+struct Aov {
+    V3f albedo{0.0f}, normal{0.0f};
+    bool done = false;
+    void capture(V3f a, V3f n) {
+        if (done) return;
+        albedo = a;
+        normal = n;
+        done = true;
+    }
+};
+
 void camera_ray(const Camera &cam, float px, float py, float ax, float ay, Ray &ray);
 V3f path_mis_li(const SceneData &s, const Camera &cam, Ray ray, Pcg32 &rng, const RenderParams &p, RenderStats &st);
-void render(const SceneData &s, const Camera &cam, const RenderParams &p, float *accum, RenderStats *stats);
+// accum: RGBA per pixel (radiance sum, sample count); aov_albedo / aov_normal
+// (optional): the same for the denoiser features.
+void render(const SceneData &s, const Camera &cam, const RenderParams &p, float *accum, RenderStats *stats,
+            float *aov_albedo = nullptr, float *aov_normal = nullptr);
 
 } // namespace orc

@@ -94,6 +94,16 @@ struct nori_ctx {
     // One film for all views of the last render, view after view.
     DevBuf film;
     std::vector<size_t> film_layout;
+    // option "aov": the film holds three parts of npix_all entries each (radiance,
+    // denoiser albedo, denoiser normal)
+    int aov_on = 0;
+    bool film_aov = false; // the allocated film has the feature parts
+    size_t npix_all = 0;
+    AovDev aov_dev() const {
+        if (!film_aov) return AovDev{nullptr, nullptr};
+        float4 *f = (float4 *)film.p;
+        return AovDev{f + npix_all, f + 2 * npix_all};
+    }
     DevBuf views_buf;
 
     // The bounce loop as a graph of two iterations (even, odd), kept while
@@ -101,10 +111,11 @@ struct nori_ctx {
     struct GraphKey {
         const void *paths, *film, *views, *guide_out, *guide_rec, *stats, *scene;
         uint32_t cap, sampler;
+        bool aov;
         bool operator==(const GraphKey &o) const {
             return paths == o.paths && film == o.film && views == o.views && guide_out == o.guide_out &&
                    guide_rec == o.guide_rec && stats == o.stats && scene == o.scene && cap == o.cap &&
-                   sampler == o.sampler;
+                   sampler == o.sampler && aov == o.aov;
         }
     };
     hipGraphExec_t graph = nullptr;
@@ -322,7 +333,7 @@ void run_bounces(nori_ctx *c, float4 *film, const ViewTab *views, uint32_t n, ui
         launch_extend(c->sdev, c->pd[qi], c->cdev.cnt + qi, n, nullptr, nullptr, nullptr, 0, c->work_ext(),
                       c->stream);
         launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, c->cdev.cnt + qi, c->cdev.cnt + qo, nullptr, nullptr,
-                     c->shdev, c->cdev.cnt + 2, film, n, none, c->stream);
+                     c->shdev, c->cdev.cnt + 2, film, n, none, AovDev{nullptr, nullptr}, c->stream);
         launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, film, 2 * n, nullptr, none, c->work_shadow(), c->stream);
         st.launches_closest++;
         st.launches_shadow++;
@@ -360,10 +371,10 @@ void pool_iteration(nori_ctx *c, uint32_t qi, float4 *film, const ViewTab *views
         Span t(c, 2);
         if (sorted)
             launch_shade_sorted(c->sdev, c->pd[qi], c->pd[qo], views, c->qc, c->cap, cnt + 6, c->scene_has_hair,
-                                cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev, cnt + 2, film, n, gsd, c->stream);
+                                cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev, cnt + 2, film, n, gsd, c->aov_dev(), c->stream);
         else
             launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, cnt + qi, cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev,
-                         cnt + 2, film, n, gsd, c->stream);
+                         cnt + 2, film, n, gsd, c->aov_dev(), c->stream);
     }
     {
         Span t(c, 1);
@@ -384,6 +395,7 @@ extern "C" {
 int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n, uint32_t max_depth, uint32_t rr_depth,
                 float *rgb3) {
     if (!c || n < 0 || (n > 0 && (!rays || !seeds || !rgb3)) || max_depth == 0) return NORI_E_ARG;
+    if (max_depth > 0x7FFFu) return fail(c, NORI_E_LIMIT, "li: max_depth above 32767");
     return guard(c, [&] {
         if (!c->has_scene) throw ArgError("li: no scene uploaded", NORI_E_STATE);
         if (n == 0) return;
@@ -677,7 +689,7 @@ int nori_gpu_reset_film(nori_ctx *c) {
     return guard(c, [&] {
         size_t npix = 0;
         for (size_t x : c->film_layout) npix += x;
-        if (c->film.p) launch_zero_f4((float4 *)c->film.p, npix, c->stream);
+        if (c->film.p) launch_zero_f4((float4 *)c->film.p, npix * (c->film_aov ? 3 : 1), c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
@@ -696,7 +708,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             const nori_view &v = views[vi];
             if (v.camera.width == 0 || v.camera.height == 0 || v.max_depth == 0)
                 throw ArgError("render: empty view");
-            if (v.max_depth > 0xFFFFu) throw ArgError("render: max_depth above 65535", NORI_E_LIMIT);
+            if (v.max_depth > 0x7FFFu) throw ArgError("render: max_depth above 32767", NORI_E_LIMIT);
             const uint64_t npix = (uint64_t)v.camera.width * v.camera.height;
             layout[vi] = (size_t)npix;
             vt[vi] = ViewTab{v.camera, v.spp, v.sample_offset, v.max_depth, v.rr_depth, (uint32_t)std::min<uint64_t>(npix, ~0u),
@@ -706,11 +718,14 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             total += npix * v.spp;
             max_depth = std::max(max_depth, v.max_depth);
         }
-        if (layout != c->film_layout) {
+        if (layout != c->film_layout || c->film_aov != (c->aov_on != 0)) {
+            const size_t parts = c->aov_on ? 3 : 1;
             c->film.release();
-            c->alloc(c->film, npix_all * sizeof(float4));
-            launch_zero_f4((float4 *)c->film.p, npix_all, c->stream);
+            c->alloc(c->film, parts * npix_all * sizeof(float4));
+            launch_zero_f4((float4 *)c->film.p, parts * npix_all, c->stream);
             c->film_layout = layout;
+            c->film_aov = c->aov_on != 0;
+            c->npix_all = (size_t)npix_all;
         }
         c->alloc(c->views_buf, std::max<size_t>((size_t)nviews, 16) * sizeof(ViewTab));
         HIPCHK(hipMemcpyAsync(c->views_buf.p, vt.data(), vt.size() * sizeof(ViewTab), hipMemcpyHostToDevice,
@@ -748,7 +763,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             const bool graphed = c->use_graph && !c->timing_mode;
             if (graphed) {
                 nori_ctx::GraphKey key{c->paths.p, c->film.p, c->views_buf.p, gsd.out, gsd.rec, cs,
-                                       (const void *)(uintptr_t)c->scene_gen, c->pool, c->sampler};
+                                       (const void *)(uintptr_t)c->scene_gen, c->pool, c->sampler, c->film_aov};
                 if (!c->graph || !(key == c->graph_key)) {
                     c->drop_graph();
                     hipGraph_t g = nullptr;
@@ -765,7 +780,8 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                 }
             }
             launch_pool_start(c->cdev, c->qf[0], c->pool, c->stream);
-            launch_film_count(dviews, (uint32_t)nviews, (uint32_t)npix_all, film, c->stream);
+            for (size_t part = 0; part < (c->film_aov ? 3u : 1u); ++part)
+                launch_film_count(dviews, (uint32_t)nviews, (uint32_t)npix_all, film + part * npix_all, c->stream);
             // Every slot runs at most max_depth iterations per sample and every
             // slot frees up within max_depth iterations, so the pool drains
             // within (samples / cap + 1) * max_depth iterations.
@@ -916,6 +932,15 @@ int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t 
     return rc;
 }
 
+int nori_gpu_aov(nori_ctx *c, int which, float *rgba) {
+    if (!c || (which != NORI_AOV_ALBEDO && which != NORI_AOV_NORMAL) || !rgba) return NORI_E_ARG;
+    return guard(c, [&] {
+        if (!c->film_aov) throw ArgError("aov: no render with option \"aov\" since the last film change", NORI_E_STATE);
+        const float4 *src = (const float4 *)c->film.p + (size_t)(which == NORI_AOV_ALBEDO ? 1 : 2) * c->npix_all;
+        HIPCHK(hipMemcpy(rgba, src, c->npix_all * sizeof(float4), hipMemcpyDeviceToHost));
+    });
+}
+
 int nori_gpu_render_stats(const nori_ctx *c, nori_render_stats *out) {
     if (!c || !out) return NORI_E_ARG;
     *out = c->last;
@@ -941,6 +966,9 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
     } else if (n == "sampler") {
         if (value != 0 && value != 1) return fail(c, NORI_E_ARG, "sampler must be 0 (pcg32) or 1 (Sobol')");
         c->sampler = (uint32_t)value;
+    } else if (n == "aov") {
+        c->aov_on = value != 0;
+        c->drop_graph();
     } else if (n == "sort") {
         c->sort_shading = value != 0;
         c->drop_graph();

@@ -50,6 +50,12 @@ struct GuideShadeDev {
     uint32_t cap, max_rec;
 };
 
+// Denoiser feature films (RGBA32F, entry per film entry; null: off): albedo
+// and normal at each sample's first non-delta interaction, summed.
+struct AovDev {
+    float4 *albedo, *normal;
+};
+
 // Path state, structure of arrays indexed by position in the active queue:
 // the live paths of an iteration occupy positions [0, count) of one of two
 // buffers, and shading moves the survivors, compacted, to the other.  `id`
@@ -60,7 +66,7 @@ struct PathsDev {
     float4 *hit;    // t, prim bits, u, v
     float4 *thr;    // throughput.xyz, pdf of the last BSDF sample (0 after delta / camera)
     float4 *aux;    // eta, film entry bits, rng state low/high bits
-    uint32_t *meta; // bounce (bits 0-15) | view (bits 16-31)
+    uint32_t *meta; // bounce (bits 0-14) | features taken (bit 15) | view (bits 16-31)
     uint32_t *id;
 };
 
@@ -140,14 +146,15 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
 // and view come from p.meta, the view's depth limits from views.
 void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
-                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
+                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
+                  hipStream_t st);
 // The same over the class queues k_extend sorted: one launch per class (the
 // hair launch only when `hair`, the subsurface one only when the scene has
 // subsurface materials), each compiled for its class alone.
 void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                          const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
-                         float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
+                         float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st);
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, uint32_t *work, hipStream_t st);
 // Guiding: BF16 features of the hits at positions [0, *count) (zeros where

@@ -904,7 +904,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                                                        const uint32_t *__restrict__ q_cls,
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *cnt_out,
                                                        uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
-                                                       uint32_t *cnt_sh, float4 *film, GuideShadeDev g) {
+                                                       uint32_t *cnt_sh, float4 *film, GuideShadeDev g, AovDev aov) {
     // traversal stacks of the subsurface walks (allocated when the scene has
     // subsurface materials: SHADE_BLOCK columns of the tree depth)
     extern __shared__ uint2 shade_stk[];
@@ -914,7 +914,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     const uint32_t pos = ALL ? i : q_cls[i];
     const float4 ro = p.ray_o[pos], rd = p.ray_d[pos], hit = p.hit[pos], th = p.thr[pos], ax = p.aux[pos];
     const uint32_t meta = p.meta[pos], id = p.id[pos];
-    const uint32_t bounce = meta & 0xFFFFu;
+    const uint32_t bounce = meta & 0x7FFFu; // bit 15: the denoiser features are taken
     const uint32_t max_depth = views[meta >> 16].max_depth, rr_depth = views[meta >> 16].rr_depth;
     const f3 o = xyz(ro), d = xyz(rd);
     f3 T = xyz(th);
@@ -929,6 +929,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     auto add_L = [&](f3 c) {
         film_add(film, pixel, c);
         if (GUIDE && g.rec && finite3(c)) lpath += c;
+    };
+    // Denoiser features at the first non-delta interaction
+    // (shaders/Shaders.hlsl:99-106, 195-200, 320-324, 475-479, 516-520).
+    bool aov_done = (meta & 0x8000u) != 0;
+    auto aov_take = [&](f3 albedo, f3 normal) {
+        if (!aov.albedo || aov_done) return;
+        film_add(aov.albedo, pixel, albedo);
+        film_add(aov.normal, pixel, normal);
+        aov_done = true;
     };
     f3 new_o = mk(0.0f), new_d = mk(0.0f);
     f3 so = mk(0.0f);
@@ -995,6 +1004,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     if (vol_scat) {
         // scattered in a medium: on to Russian roulette
     } else if ((ALL || CLS == CLS_LIGHT) && prim == MISS) {
+        if (aov.albedo) aov_take(s.has_env ? env_eval(s.env, d) : mk(0.0f), mk(0.0f));
         if (s.has_env) {
             f3 env = env_eval(s.env, d);
             f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
@@ -1016,6 +1026,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         }
         const bool emitter = CLS == CLS_LIGHT ? true : (ALL ? mr.isEmitter != 0 : false);
         if (emitter) {
+            aov_take(mk(1.0f), N);
             f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
             if (last_pdf == 0.0f) {
                 add_L(T * Le);
@@ -1077,6 +1088,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 // Beckmann microfacet of the boundary, next-event estimation with
                 // its reflection lobe, then Fresnel reflection or refraction into
                 // the interior walk.
+                aov_take(m.albedo, N);
                 m.roughness *= 1.0f + (0.65f - 1.0f) * fminf(fmaxf(m.specular * 2.0f, 0.0f), 1.0f);
                 const float rough = m.roughness, aR = fmaxf(rough * rough, 1e-4f);
                 f3 wmL = mk(0.0f, 0.0f, 1.0f);
@@ -1180,6 +1192,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     }
                 }
             } else if (CLS != CLS_SSS && (guide_eligible(type) || hair)) {
+                aov_take(m.albedo, N);
                 so = offset_ray_origin(P, Ng, Ng);
                 // Hair transmits: rays leave from the side of the fiber surface
                 // they head to (shaders/Emitter.hlsli:122-126).
@@ -1328,7 +1341,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         p_out.thr[k] = make_float4(T.x, T.y, T.z, last_pdf);
         const float2 rw = rng_words(rng);
         p_out.aux[k] = make_float4(eta, ax.y, rw.x, rw.y);
-        p_out.meta[k] = meta + 1u;
+        p_out.meta[k] = (meta + 1u) | (aov_done ? 0x8000u : 0u);
         p_out.id[k] = id;
     }
     if (q_free) {
@@ -1661,32 +1674,33 @@ static decltype(&k_shade<false, false, CLS_ALL, false>) shade_kernel_all(bool vo
 
 void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
-                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
+                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
+                  hipStream_t st) {
     const bool guide = g.out != nullptr || g.rec != nullptr;
     const size_t lds = s.has_sss ? (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2) : 0;
     hipLaunchKernelGGL(shade_kernel_all(s.n_vol > 0, guide, s.sobol != 0, s.has_sss != 0), dim3(blocks(cap, SHADE_BLOCK)),
                        dim3(SHADE_BLOCK), lds, st, s, p, p_out, views, nullptr, cnt_in, cnt_out, q_free, cnt_free, sh,
-                       cnt_sh, film, g);
+                       cnt_sh, film, g, aov);
 }
 
 void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                          const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
-                         float4 *film, uint32_t cap, const GuideShadeDev &g, hipStream_t st) {
+                         float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st) {
     const bool guide = g.out != nullptr || g.rec != nullptr, sobol = s.sobol != 0;
     const dim3 grid(blocks(cap, SHADE_BLOCK)), block(SHADE_BLOCK);
     const size_t cs = cls_stride;
     hipLaunchKernelGGL(shade_kernel<CLS_LIGHT>(guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls,
-                       cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+                       cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
     hipLaunchKernelGGL(shade_kernel<CLS_SURF>(guide, sobol), grid, block, 0, st, s, p, p_out, views,
-                       q_cls + cs, cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+                       q_cls + cs, cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
     if (hair)
         hipLaunchKernelGGL(shade_kernel<CLS_HAIR>(guide, sobol), grid, block, 0, st, s, p, p_out, views,
-                           q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+                           q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
     if (s.has_sss)
         hipLaunchKernelGGL(shade_kernel<CLS_SSS>(guide, sobol), grid, block,
                            (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2), st, s, p, p_out, views, q_cls + 3 * cs,
-                           cls_cnt + 3, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g);
+                           cls_cnt + 3, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,

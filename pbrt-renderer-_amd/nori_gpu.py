@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libnori_gpu.so")
 NORI_OK = 0
 MAT_DIFFUSE, MAT_MIRROR, MAT_DIELECTRIC, MAT_MICROFACET, MAT_DISNEY, MAT_HAIR, MAT_SUBSURFACE = range(7)
 NO_TEXTURE = 0xFFFFFFFF
+AOV_ALBEDO, AOV_NORMAL = 0, 1
 
 _f, _u32, _u64 = ctypes.c_float, ctypes.c_uint32, ctypes.c_uint64
 
@@ -135,7 +136,7 @@ EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
     "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
     "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_texture_sample",
-    "nori_gpu_render_stats",
+    "nori_gpu_render_stats", "nori_gpu_aov",
     "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_gpu_autotune", "nori_scene_generate", "nori_scene_get_desc",
     "nori_scene_get_camera", "nori_scene_free", "nori_bvh8_build_host",
 ]
@@ -166,6 +167,7 @@ def load_library(path=LIB_PATH):
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
         "nori_gpu_pcg32": (i32, [vp, _u64, _u64, _u64, i32, vp]),
         "nori_gpu_sobol": (i32, [vp, _u32, _u32, _u32, i32, vp]),
+        "nori_gpu_aov": (i32, [vp, i32, vp]),
         "nori_gpu_texture_sample": (i32, [vp, i32, vp, vp, vp, vp]),
         "nori_gpu_autotune": (i32, [vp, ctypes.POINTER(View), i32, _u32, ctypes.POINTER(TuneResult)]),
         "nori_gpu_render_stats": (i32, [vp, ctypes.POINTER(RenderStats)]),
@@ -395,6 +397,14 @@ class NoriGPU:
         npx = sum(v.camera.width * v.camera.height for v in views)
         out = np.zeros((npx, 4), np.float32) if readback else None
         self._check(self.lib.nori_gpu_render(self.h, arr, len(views), seed, _ptr(out)), "render")
+        self._npx = npx
+        return out
+
+    def aov(self, which):
+        """Denoiser feature film of the last render (option "aov"): which = AOV_ALBEDO
+        or AOV_NORMAL; per film entry the RGB sum and the sample count."""
+        out = np.zeros((getattr(self, "_npx", 0), 4), np.float32)
+        self._check(self.lib.nori_gpu_aov(self.h, int(which), _ptr(out)), "aov")
         return out
 
     def li(self, rays, seeds, max_depth, rr_depth=3):
