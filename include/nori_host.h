@@ -51,6 +51,25 @@ int nori_image_write(const char *filename, const float *rgb, uint32_t width, uin
 /* Read an RGB float image (.hdr RGBE or .pfm); *rgb is malloc'ed, release with nori_image_free. */
 int nori_image_read(const char *filename, float **rgb, uint32_t *width, uint32_t *height, char *err, int errlen);
 void nori_image_free(float *rgb);
+/* Linear-HDR comparison of two renders of the same size (the metrics of the
+ * reference's validation_tests/compare_exr.py:74-118): per-channel means of
+ * both images, their ratio (ours / ref; +inf where ref's mean is 1e-10 or
+ * less), mean and max |difference|; linear RMSE over all channels; relMSE =
+ * mean((ours - ref)^2 / (ref^2 + 1e-2)); PSNR after the Reinhard display curve
+ * (x 2^exposure_ev, max(x, 0), x / (1 + x), ^(1/2.2)), +inf when that MSE is
+ * 1e-10 or less.  Images: width * height * 3 floats. */
+typedef struct nori_image_metrics {
+    double mean_ours[3], mean_ref[3], ratio[3], mean_abs_diff[3], max_abs_diff[3];
+    double rmse, rel_mse, psnr_tonemapped;
+} nori_image_metrics;
+int nori_image_compare(const float *ours, const float *ref, uint32_t width, uint32_t height, float exposure_ev,
+                       nori_image_metrics *out);
+/* White-furnace statistics of validation_tests/disney/analyze_furnace.py:
+ * the pixels whose centre ray (vertical fov_deg, camera at (0, 0, cam_z)
+ * looking down -z) meets the sphere of `radius` at the origin, their
+ * per-channel mean and mean Rec.709 luminance; *count = their number. */
+int nori_furnace_stats(const float *rgb, uint32_t width, uint32_t height, float fov_deg, float cam_z, float radius,
+                       double mean[3], double *lum_mean, uint32_t *count);
 /* Comma-separated names of the registered plugin classes. */
 int nori_plugin_names(char *buf, int len);
 

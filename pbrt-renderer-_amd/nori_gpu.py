@@ -578,8 +578,39 @@ EXPORTS += ["nori_guide_params", "nori_guide_forward", "nori_guide_gradient", "n
 HOST_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnori_host.so")
 HOST_EXPORTS = ["nori_xml_load", "nori_xml_load_string", "nori_xml_free", "nori_xml_info_get", "nori_xml_desc",
                 "nori_xml_render", "nori_xml_li", "nori_image_write", "nori_image_read", "nori_image_free",
-                "nori_plugin_names"]
+                "nori_plugin_names", "nori_image_compare", "nori_furnace_stats"]
 _host = None
+
+
+class ImageMetrics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double * 3) for n in ("mean_ours", "mean_ref", "ratio", "mean_abs_diff", "max_abs_diff")] + \
+               [("rmse", ctypes.c_double), ("rel_mse", ctypes.c_double), ("psnr_tonemapped", ctypes.c_double)]
+
+
+def image_compare(ours, ref, width, height, exposure_ev=0.0):
+    """The compare_exr.py metrics of two (width*height, 3) float images (nori_image_compare)."""
+    a = np.ascontiguousarray(ours, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(ref, np.float32).reshape(-1, 3)
+    if len(a) != width * height or len(b) != width * height:
+        raise ValueError("image sizes do not match width * height")
+    m = ImageMetrics()
+    rc = load_host_library().nori_image_compare(_ptr(a), _ptr(b), width, height, exposure_ev, ctypes.byref(m))
+    if rc != NORI_OK:
+        raise NoriError(f"image_compare failed ({rc})")
+    return {k: (list(getattr(m, k)) if k in ("mean_ours", "mean_ref", "ratio", "mean_abs_diff", "max_abs_diff")
+                else getattr(m, k)) for k, _ in ImageMetrics._fields_}
+
+
+def furnace_stats(rgb, width, height, fov_deg=45.0, cam_z=4.0, radius=1.0):
+    """Sphere-masked mean, luminance mean and pixel count (nori_furnace_stats)."""
+    a = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    mean = (ctypes.c_double * 3)()
+    lum, cnt = ctypes.c_double(), ctypes.c_uint32()
+    rc = load_host_library().nori_furnace_stats(_ptr(a), width, height, fov_deg, cam_z, radius, mean,
+                                                 ctypes.byref(lum), ctypes.byref(cnt))
+    if rc != NORI_OK:
+        raise NoriError(f"furnace_stats failed ({rc})")
+    return list(mean), lum.value, cnt.value
 
 
 class XmlInfo(ctypes.Structure):
@@ -607,6 +638,10 @@ def load_host_library(path=HOST_LIB_PATH):
         "nori_image_write": (i32, [cp, vp, _u32, _u32, cp, i32]),
         "nori_image_read": (i32, [cp, ctypes.POINTER(vp), ctypes.POINTER(_u32), ctypes.POINTER(_u32), cp, i32]),
         "nori_image_free": (None, [vp]),
+        "nori_image_compare": (i32, [vp, vp, _u32, _u32, ctypes.c_float, ctypes.POINTER(ImageMetrics)]),
+        "nori_furnace_stats": (i32, [vp, _u32, _u32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(_u32)]),
         "nori_plugin_names": (i32, [cp, i32]),
     }
     for name, (res, args) in sig.items():
