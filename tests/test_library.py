@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_library_exports_every_declared_symbol():
     lib = ng.load_host_library()
-    names = _declared_functions("nori_host.h", "xml|image|plugin")
+    names = _declared_functions("nori_host.h", "xml|image|plugin|furnace")
     assert len(names) >= 11
     for n in names:
         assert hasattr(lib, n), n
