@@ -7,10 +7,12 @@
 #include "bsdf.h"
 #include "emitter.h"
 #include "gpu.h"
+#include "image.h"
 #include "medium.h"
 
 #include <cstring>
 #include <fstream>
+#include <unordered_map>
 
 NORI_NAMESPACE_BEGIN
 
@@ -89,6 +91,30 @@ void Accel::setMedium(const Medium *medium) {
 void Accel::build() {
     m_pos.clear(); m_nrm.clear(); m_uv.clear(); m_tan.clear(); m_idx.clear();
     m_meshTriOffset.clear(); m_meshTriCount.clear(); m_mats.clear(); m_triOffset.clear();
+    m_texPixels.clear(); m_texDesc.clear();
+    // Texture files of the materials (src/gpu/DXRApp_Textures.cpp:452-486):
+    // resolved against the scene's directory, one texture per resolved path
+    // (the first use decides sRGB: albedo maps are sRGB, the others linear); a
+    // file that does not load leaves the map unset.
+    std::unordered_map<std::string, uint32_t> texCache;
+    auto loadTexture = [&](const std::string &rel, bool srgb) -> uint32_t {
+        if (rel.empty()) return NORI_NO_TEXTURE;
+        const std::string key = getFileResolver()->resolve(rel);
+        auto it = texCache.find(key);
+        if (it != texCache.end()) return it->second;
+        uint32_t w = 0, h = 0;
+        std::vector<uint8_t> px;
+        std::string err;
+        if (!loadTextureRGBA8(key, w, h, px, err)) {
+            cerr << "[texture] Failed to load: " << key << " (" << err << ")" << endl;
+            return NORI_NO_TEXTURE;
+        }
+        const uint32_t idx = (uint32_t)m_texDesc.size();
+        m_texPixels.push_back(std::move(px));
+        m_texDesc.push_back(nori_texture{w, h, srgb ? 1u : 0u, nullptr});
+        texCache[key] = idx;
+        return idx;
+    };
     bool anyNormals = false, anyUV = false, anyTan = false;
     for (auto *m : m_meshes) {
         anyNormals |= m->getVertexNormals().cols() > 0;
@@ -122,7 +148,15 @@ void Accel::build() {
                 if (vi >= nv) throw NoriException("Mesh \"%s\": vertex index out of range", m->getName());
                 m_idx.push_back(v0 + vi);
             }
-        nori_material mat = toGPUMaterial(m->getBSDF()->getGPUData());
+        const BSDFGPUData gd = m->getBSDF()->getGPUData();
+        nori_material mat = toGPUMaterial(gd);
+        mat.albedoTexIndex = loadTexture(gd.albedoTexture, true);
+        mat.normalTexIndex = loadTexture(gd.normalTexture, false);
+        mat.roughnessTexIndex = loadTexture(gd.roughnessTexture, false);
+        mat.metallicTexIndex = loadTexture(gd.metallicTexture, false);
+        mat.specularTexIndex = loadTexture(gd.specularTexture, false);
+        mat.subsurfaceTexIndex = loadTexture(gd.subsurfaceTexture, false);
+        mat.alphaTexIndex = loadTexture(gd.alphaTexture, false);
         if (m->isEmitter()) {
             Color3f L = m->getEmitter()->getRadiance();
             mat.isEmitter = 1;
@@ -153,6 +187,9 @@ void Accel::build() {
     m_desc.env_height = m_env.empty() ? 0 : m_envH;
     m_desc.env_rgb = m_env.empty() ? nullptr : m_env.data();
     m_desc.env_scale = m_envScale;
+    for (size_t k = 0; k < m_texDesc.size(); ++k) m_texDesc[k].rgba = m_texPixels[k].data();
+    m_desc.num_textures = (uint32_t)m_texDesc.size();
+    m_desc.textures = m_texDesc.empty() ? nullptr : m_texDesc.data();
     m_desc.num_volumes = (uint32_t)m_vols.size();
     m_desc.volumes = m_vols.empty() ? nullptr : m_vols.data();
     m_uploaded = false;

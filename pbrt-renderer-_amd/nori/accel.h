@@ -53,6 +53,8 @@ private:
     std::vector<float> m_pos, m_nrm, m_uv, m_tan, m_env;
     std::vector<uint32_t> m_idx, m_meshTriOffset, m_meshTriCount;
     std::vector<nori_material> m_mats;
+    std::vector<std::vector<uint8_t>> m_texPixels; // decoded RGBA8 texture files
+    std::vector<nori_texture> m_texDesc;
     std::vector<nori_volume> m_vols;
     std::vector<float> m_volGrid;
     uint32_t m_envW = 0, m_envH = 0;

@@ -111,3 +111,37 @@ def test_hair_file_scene_matches_oracle(tmp_path):
     e = orc.rel_mse(img, o[:, :3] / 16)
     _say(f"  relMSE {e:.3e}")
     assert e < 1e-3 and img.mean() > 0.01
+
+
+def test_textured_xml_scene_matches_oracle(tmp_path):
+    # albedo (sRGB) and normal maps from PNG files on a floor quad under the
+    # flat grey environment and a small area light: same samples as the oracle
+    from png_writer import encode
+    rng = np.random.default_rng(5)
+    (tmp_path / "alb.png").write_bytes(encode(rng.integers(0, 256, (32, 32, 3)), 2, 8))
+    nrm = np.stack([rng.integers(96, 160, (16, 16)), rng.integers(96, 160, (16, 16)),
+                    np.full((16, 16), 230)], -1)
+    (tmp_path / "nrm.png").write_bytes(encode(nrm, 2, 8))
+    (tmp_path / "floor.obj").write_text(QUAD)
+    (tmp_path / "light.obj").write_text(LIGHT)
+    xml = ("<scene><integrator type='path_mis'><integer name='maxDepth' value='6'/></integrator>"
+           "<camera type='perspective'><float name='fov' value='50'/><integer name='width' value='32'/>"
+           "<integer name='height' value='32'/><transform name='toWorld'><lookat origin='0, 2, 2.5' "
+           "target='0, 0, 0' up='0, 1, 0'/></transform></camera>"
+           "<sampler type='independent'><integer name='sampleCount' value='16'/></sampler>"
+           "<mesh type='obj'><string name='filename' value='floor.obj'/><bsdf type='diffuse'>"
+           "<string name='albedoTexture' value='alb.png'/><string name='normalTexture' value='nrm.png'/></bsdf></mesh>"
+           "<mesh type='obj'><string name='filename' value='light.obj'/><emitter type='area'>"
+           "<color name='radiance' value='8 8 8'/></emitter></mesh></scene>")
+    s = ng.XmlScene(text=xml, directory=str(tmp_path))
+    try:
+        _say("case: textured xml scene 32x32 16spp")
+        img = s.render()
+        hs = s.host_scene()
+    finally:
+        s.close()
+    assert len(hs.textures) == 2 and hs.materials[0]["albedoTexIndex"] == 0
+    o, _ = orc.OracleScene(hs).render(hs.camera, 16, 6, seed=0)
+    e = orc.rel_mse(img, o[:, :3] / 16)
+    _say(f"  relMSE {e:.3e}")
+    assert e < 1e-4 and img.mean() > 0.01
