@@ -908,14 +908,14 @@ int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t 
     };
     int rc = NORI_OK;
     try {
-        // pools of 2^18 .. 2^24 paths, no larger than the calibration's samples
-        for (uint32_t lg = 18; lg <= 24; ++lg) {
+        // pools of 2^18 .. 2^25 paths, no larger than the calibration's samples
+        for (uint32_t lg = 18; lg <= 25; ++lg) {
             const uint32_t pool = 1u << lg;
             if (lg > 18 && (uint64_t)pool > samples) break;
             time_it(pool, chunk0);
         }
         const uint32_t bp = r.best_pool;
-        for (uint32_t chunk : {4u, 16u, 32u})
+        for (uint32_t chunk : {2u, 4u, 16u, 32u})
             if (chunk != chunk0) time_it(bp, chunk);
     } catch (const ArgError &e) {
         rc = e.code;
