@@ -305,10 +305,9 @@ __device__ __forceinline__ bool alpha_accept(const SceneDev &s, uint32_t prim, u
 #pragma clang fp contract(off)
     const uint32_t ti = s.mats[mesh].alphaTexIndex;
     if (ti == NORI_NO_TEXTURE || !s.uv) return true;
-    float tu, tv, a;
+    float tu, tv;
     hit_uv(s, prim, u, v, tu, tv);
-    const f3 c = tex_sample(s.tex, ti, tu, tv, 0.0f, a);
-    const float av = shadow ? (a < 1.0f ? a : c.x) : c.x;
+    const float av = tex_alpha_value(s.tex, ti, tu, tv, shadow);
     if (av < 0.01f) return false;
     const float xi = (float)pcg_hash(seed ^ pcg_hash(prim)) * 2.3283064365386963e-10f;
     return av * av >= xi;
