@@ -17,9 +17,12 @@ ImageRGB readImage(const std::string &filename);
 /// Writes .pfm or .exr (by extension); throws NoriException on failure.
 void writeImage(const std::string &filename, const ImageRGB &img);
 
-/// Decodes a texture file (PNG, binary PNM) to RGBA8, row 0 at the top; on
+/// Decodes a texture file (PNG, baseline JPEG, binary PNM) to RGBA8, row 0 at the top; on
 /// failure returns false with the reason in `err`.
 bool loadTextureRGBA8(const std::string &path, uint32_t &width, uint32_t &height, std::vector<uint8_t> &rgba,
                       std::string &err);
+/// Baseline JPEG decoder behind loadTextureRGBA8.
+bool decodeJpegRGBA8(const std::vector<uint8_t> &file, uint32_t &width, uint32_t &height, std::vector<uint8_t> &rgba,
+                     std::string &err);
 
 } // namespace nori

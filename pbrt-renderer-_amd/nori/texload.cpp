@@ -5,9 +5,10 @@
 //     and RGBA, non-interlaced, zlib streams inflated by the system zlib, the
 //     five scanline filters, tRNS transparency for palette images; 16-bit
 //     samples keep their high byte;
-//   * binary PNM: P5 (grey) and P6 (RGB), maxval up to 255.
-// Other formats (JPEG, ...) are not decoded: the caller reports the texture as
-// not loaded, as the reference does for a file stb_image refuses.
+//   * binary PNM: P5 (grey) and P6 (RGB), maxval up to 255;
+//   * baseline JPEG (jpeg.cpp).
+// Other formats are not decoded: the caller reports the texture as not loaded,
+// as the reference does for a file stb_image refuses.
 #include "image.h"
 
 #include <zlib.h>
@@ -188,7 +189,8 @@ bool loadTextureRGBA8(const std::string &path, uint32_t &width, uint32_t &height
     if (f.size() >= 8 && f[0] == 137 && f[1] == 'P' && f[2] == 'N' && f[3] == 'G')
         return decode_png(f, width, height, rgba, err);
     if (f.size() >= 2 && f[0] == 'P' && (f[1] == '5' || f[1] == '6')) return decode_pnm(f, width, height, rgba, err);
-    return fail(err, "unsupported image format (PNG and binary PNM are decoded)");
+    if (f.size() >= 2 && f[0] == 0xFF && f[1] == 0xD8) return decodeJpegRGBA8(f, width, height, rgba, err);
+    return fail(err, "unsupported image format (PNG, baseline JPEG and binary PNM are decoded)");
 }
 
 } // namespace nori
