@@ -113,15 +113,22 @@ def test_hair_file_scene_matches_oracle(tmp_path):
     assert e < 1e-3 and img.mean() > 0.01
 
 
-def test_textured_xml_scene_matches_oracle(tmp_path):
-    # albedo (sRGB) and normal maps from PNG files on a floor quad under the
-    # flat grey environment and a small area light: same samples as the oracle
-    from png_writer import encode
+@pytest.mark.parametrize("fmt", ["png", "jpg"])
+def test_textured_xml_scene_matches_oracle(tmp_path, fmt):
+    # albedo (sRGB) and normal maps from PNG or JPEG files on a floor quad under
+    # the flat grey environment and a small area light: same samples as the oracle
     rng = np.random.default_rng(5)
-    (tmp_path / "alb.png").write_bytes(encode(rng.integers(0, 256, (32, 32, 3)), 2, 8))
+    alb = rng.integers(0, 256, (32, 32, 3))
     nrm = np.stack([rng.integers(96, 160, (16, 16)), rng.integers(96, 160, (16, 16)),
                     np.full((16, 16), 230)], -1)
-    (tmp_path / "nrm.png").write_bytes(encode(nrm, 2, 8))
+    if fmt == "png":
+        from png_writer import encode
+        (tmp_path / "alb.png").write_bytes(encode(alb, 2, 8))
+        (tmp_path / "nrm.png").write_bytes(encode(nrm, 2, 8))
+    else:
+        Image = pytest.importorskip("PIL.Image")
+        Image.fromarray(alb.astype(np.uint8)).save(tmp_path / "alb.jpg", quality=90)
+        Image.fromarray(nrm.astype(np.uint8)).save(tmp_path / "nrm.jpg", quality=90)
     (tmp_path / "floor.obj").write_text(QUAD)
     (tmp_path / "light.obj").write_text(LIGHT)
     xml = ("<scene><integrator type='path_mis'><integer name='maxDepth' value='6'/></integrator>"
@@ -130,12 +137,12 @@ def test_textured_xml_scene_matches_oracle(tmp_path):
            "target='0, 0, 0' up='0, 1, 0'/></transform></camera>"
            "<sampler type='independent'><integer name='sampleCount' value='16'/></sampler>"
            "<mesh type='obj'><string name='filename' value='floor.obj'/><bsdf type='diffuse'>"
-           "<string name='albedoTexture' value='alb.png'/><string name='normalTexture' value='nrm.png'/></bsdf></mesh>"
+           f"<string name='albedoTexture' value='alb.{fmt}'/><string name='normalTexture' value='nrm.{fmt}'/></bsdf></mesh>"
            "<mesh type='obj'><string name='filename' value='light.obj'/><emitter type='area'>"
            "<color name='radiance' value='8 8 8'/></emitter></mesh></scene>")
     s = ng.XmlScene(text=xml, directory=str(tmp_path))
     try:
-        _say("case: textured xml scene 32x32 16spp")
+        _say(f"case: textured xml scene ({fmt}) 32x32 16spp")
         img = s.render()
         hs = s.host_scene()
     finally:
