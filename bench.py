@@ -144,7 +144,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus} is not supported; this build measures one GPU (use --gpus 1)",
+        print(f"bench.py: --gpus {args.gpus} is not supported; bench.py measures one GPU (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
 

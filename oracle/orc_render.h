@@ -5,7 +5,7 @@
 // This is a CPU restatement of RayGen as the `path_mis` integrator, run over
 // 32x32 image blocks (NORI_BLOCK_SIZE, include/block.h:17) on a std::thread pool.
 //
-// Sampling conventions that are this build's own (the HLSL draws from an
+// Sampling conventions chosen here rather than taken from the HLSL (it draws from an
 // Owen-scrambled Sobol' stream; shaders/RNG.hlsli:80-98):
 //   * one pcg32 stream per pixel, seed(render_seed ^ pixel, PCG32_DEFAULT_STREAM),
 //     sample s starting at advance(s << 20);

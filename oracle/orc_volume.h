@@ -11,7 +11,7 @@
 //   * volume NEE at scatter points (shaders/Emitter.hlsli:334-434), including
 //     its skip when minDist * min sigma_t > 8.
 //
-// Conventions that are this build's own:
+// Conventions chosen here (the reference leaves them open):
 //   * the brick majorant is the maximum density over every voxel the trilinear
 //     lookup can touch inside the brick (the brick's voxels plus a one-voxel
 //     apron), so density / majorant <= 1 everywhere; the reference takes the

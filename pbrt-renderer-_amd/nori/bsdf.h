@@ -7,7 +7,7 @@ NORI_NAMESPACE_BEGIN
 
 enum EMeasure { EUnknownMeasure = 0, ESolidAngle, EDiscrete };
 
-class AlphaTexture; // alpha-masked materials: not part of this build's GPU path
+class AlphaTexture; // the CPU integrators' alpha masks; the GPU path alpha-tests from the material's texture maps
 
 struct BSDFQueryRecord {
     Vector3f wi, wo;
