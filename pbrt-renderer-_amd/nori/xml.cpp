@@ -177,6 +177,10 @@ struct Reader {
 
 bool parseXml(const std::string &text, XmlDocument &doc, std::string &error, size_t &error_offset) {
     Reader r(text);
+    // a UTF-8 byte order mark before the document is skipped, as pugixml does
+    if (text.size() >= 3 && (unsigned char)text[0] == 0xEF && (unsigned char)text[1] == 0xBB &&
+        (unsigned char)text[2] == 0xBF)
+        r.i = 3;
     bool have_root = false;
     while (true) {
         r.skip_ws();

@@ -386,3 +386,17 @@ def test_hair_file_options_and_errors(tmp_path):
         with pytest.raises(ng.NoriError) as e:
             ng.XmlScene(text=HAIR_XML.format(sides=6, extra=""), directory=str(tmp_path))
         assert needle in str(e.value)
+
+
+def test_utf8_byte_order_mark_is_skipped(tmp_path):
+    # scene files saved with a UTF-8 BOM and an XML declaration load as pugixml
+    # loads them (the reference's own scenes/final_scene_real/scene2.xml starts so)
+    (tmp_path / "q.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    (tmp_path / "s.xml").write_bytes(b"\xef\xbb\xbf<?xml version='1.0' encoding='utf-8'?>\n<scene>"
+                                     b"<camera type='perspective'/><mesh type='obj'>"
+                                     b"<string name='filename' value='q.obj'/></mesh></scene>")
+    s = ng.XmlScene(str(tmp_path / "s.xml"))
+    try:
+        assert len(s.host_scene().materials) == 1
+    finally:
+        s.close()
