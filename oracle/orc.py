@@ -50,6 +50,8 @@ def lib():
         L.orc_env_eval_pdf.argtypes = [vp, u32, vp, vp, vp]
         L.orc_bsdf_batch.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_hair_fn.argtypes = [i32, u32, vp, vp, vp, vp, vp, vp]
+        L.orc_bsdf_bound.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, ctypes.c_double, vp, vp, vp, vp]
+        L.orc_microfacet_fn.argtypes = [i32, u32, vp, vp, vp, vp]
         L.orc_li.argtypes = [vp, u32, vp, vp, u32, u32, vp]
         _lib = L
     return _lib
@@ -143,6 +145,41 @@ def bsdf_batch(op, mats, wi, wo=None, u=None, precision=64, h=None):
     rgb, pdf, woo = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3))
     lib().orc_bsdf_batch(op, precision, n, _p(mats), _p(wi), _p(wo), _p(u), _p(h), _p(rgb), _p(pdf), _p(woo))
     return rgb, pdf, woo
+
+
+def bsdf_bound(op, mats, wi, wo=None, u=None, h=None, ulps=16.0, kind=0):
+    """Per-query forward-error bounds of an FP32 evaluation of bsdf_batch's op.
+    kind 0: the FP64 function's variation when each FP32 input coordinate moves
+    by +-ulps units in the last place (summed over coordinates); kind 1: the
+    running error analysis of the formula in FP32 (oracle/orc_fp32err.h).
+    Returns (bound_rgb n x 3, bound_pdf n, bound_wo n x 3, ambiguous n bool);
+    `ambiguous` (kind 1) marks queries with a comparison that FP32 rounding
+    could decide the other way."""
+    n = len(mats)
+    mats = np.ascontiguousarray(mats)
+    wi = np.ascontiguousarray(wi, np.float32)
+    wo = None if wo is None else np.ascontiguousarray(wo, np.float32)
+    if u is not None:
+        u = np.asarray(u, np.float32)
+        if u.shape[1] == 3:
+            u = np.c_[u, np.zeros(n, np.float32)]
+        u = np.ascontiguousarray(u, np.float32)
+    h = None if h is None else np.ascontiguousarray(h, np.float32)
+    brgb, bpdf, bwo, amb = np.zeros((n, 3)), np.zeros(n), np.zeros((n, 3)), np.zeros(n, np.uint8)
+    lib().orc_bsdf_bound(op, kind, n, _p(mats), _p(wi), _p(wo), _p(u), _p(h), float(ulps), _p(brgb), _p(bpdf),
+                         _p(bwo), _p(amb))
+    return brgb, bpdf, bwo, amb.astype(bool)
+
+
+def microfacet_fn(fn, v, wh, a):
+    """FP64 Beckmann pieces: fn 0 BeckmannD(wh, a), 1 BeckmannDCosTheta(wh, a), 2 SmithG1(v, wh, a)."""
+    wh = np.ascontiguousarray(np.atleast_2d(wh), np.float64)
+    n = len(wh)
+    v = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(v), (n, 3)), np.float64)
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))
+    out = np.zeros(n)
+    lib().orc_microfacet_fn(fn, n, _p(v), _p(wh), _p(a), _p(out))
+    return out
 
 
 class OracleScene:

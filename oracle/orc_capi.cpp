@@ -6,6 +6,7 @@
 #include "orc_render.h"
 #include <cstring>
 #include <thread>
+#include <vector>
 
 using namespace orc;
 
@@ -285,7 +286,83 @@ void orc_env_eval_pdf(void *hp, uint32_t n, const float *dir3, float *L3, float 
     }
 }
 
+} // extern "C"
+
 // ---------------------------------------------------------------- BSDFs
+namespace {
+// Runs f(i) for i in [0, n) on up to 16 host threads (contiguous ranges).
+template <class F> void par_for(uint32_t n, F f) {
+    unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        for (uint32_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint32_t step = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint32_t a = t * step, b = std::min(n, a + step);
+        if (a >= b) break;
+        th.emplace_back([=, &f] { for (uint32_t i = a; i < b; ++i) f(i); });
+    }
+    for (auto &x : th) x.join();
+}
+
+// The outputs of one query: rgb (f or the sample weight), pdf, sampled wo.
+struct BsdfOut {
+    V3d rgb, wo;
+    double pdf = 0;
+};
+template <typename T>
+BsdfOut bsdf_query(int op, const MatT<T> &m, V3<T> wi, V3<T> wo, const T *u, T h) {
+    BsdfOut r;
+    if (op == 0) {
+        r.rgb = V3d(material_eval_h(wi, wo, m, h));
+    } else if (op == 1) {
+        r.pdf = material_pdf_h(wi, wo, m, h);
+    } else {
+        V3<T> w2;
+        T pdf;
+        r.rgb = V3d(material_sample_h(wi, u, m, h, w2, pdf));
+        r.pdf = pdf;
+        r.wo = V3d(w2);
+    }
+    return r;
+}
+
+// The FP32 spacing at |x| (the distance to the next float away from zero).
+double f32_ulp(float x) {
+    const float a = std::fabs(x);
+    return (double)std::nextafter(a, INFINITY) - (double)a;
+}
+
+// Float parameters of a material record, by index (albedo r g b, intIOR,
+// extIOR, alpha, roughness, metallic, specular, specularTint, sheen,
+// sheenTint, subsurface, clearcoat, clearcoatGloss, betaN).
+constexpr int N_MAT_PARAMS = 16;
+double &mat_param(MatT<double> &m, int k) {
+    switch (k) {
+    case 0: return m.albedo.x;
+    case 1: return m.albedo.y;
+    case 2: return m.albedo.z;
+    case 3: return m.intIOR;
+    case 4: return m.extIOR;
+    case 5: return m.alpha;
+    case 6: return m.roughness;
+    case 7: return m.metallic;
+    case 8: return m.specular;
+    case 9: return m.specularTint;
+    case 10: return m.sheen;
+    case 11: return m.sheenTint;
+    case 12: return m.subsurface;
+    case 13: return m.clearcoat;
+    case 14: return m.clearcoatGloss;
+    default: return m.betaN;
+    }
+}
+} // namespace
+
+extern "C" {
+
 // op 0 eval (f rgb), 1 pdf, 2 sample (weight rgb, wo, pdf; u4: 4 uniforms per
 // query).  One material record per query; hv: hair offsets h (may be null).
 // precision 64 evaluates in double from the float inputs.
@@ -294,31 +371,138 @@ void orc_bsdf_batch(int op, int precision, uint32_t n, const void *mats, const f
     const MatRec *mr = (const MatRec *)mats;
     auto run = [&](auto zero) {
         using T = decltype(zero);
-        for (uint32_t i = 0; i < n; ++i) {
+        par_for(n, [&](uint32_t i) {
             MatT<T> m = MatT<T>::from(mr[i]);
             V3<T> wi{T(wi3[3 * i]), T(wi3[3 * i + 1]), T(wi3[3 * i + 2])};
-            T h = hv ? T(hv[i]) : T(0);
-            if (op == 0 || op == 1) {
-                V3<T> wo{T(wo3[3 * i]), T(wo3[3 * i + 1]), T(wo3[3 * i + 2])};
-                if (op == 0) {
-                    V3<T> f = material_eval_h(wi, wo, m, h);
-                    out_rgb[3 * i] = f.x; out_rgb[3 * i + 1] = f.y; out_rgb[3 * i + 2] = f.z;
-                } else {
-                    out_pdf[i] = material_pdf_h(wi, wo, m, h);
-                }
-            } else {
-                T u[4] = {T(u4[4 * i]), T(u4[4 * i + 1]), T(u4[4 * i + 2]), T(u4[4 * i + 3])};
-                V3<T> wo;
-                T pdf;
-                V3<T> w = material_sample_h(wi, u, m, h, wo, pdf);
-                out_rgb[3 * i] = w.x; out_rgb[3 * i + 1] = w.y; out_rgb[3 * i + 2] = w.z;
-                out_pdf[i] = pdf;
-                out_wo[3 * i] = wo.x; out_wo[3 * i + 1] = wo.y; out_wo[3 * i + 2] = wo.z;
-            }
-        }
+            V3<T> wo = op == 2 ? V3<T>() : V3<T>{T(wo3[3 * i]), T(wo3[3 * i + 1]), T(wo3[3 * i + 2])};
+            T u[4] = {0, 0, 0, 0};
+            if (op == 2)
+                for (int k = 0; k < 4; ++k) u[k] = T(u4[4 * i + k]);
+            const T h = hv ? T(hv[i]) : T(0);
+            BsdfOut r = bsdf_query(op, m, wi, wo, u, h);
+            if (op != 1) { out_rgb[3 * i] = r.rgb.x; out_rgb[3 * i + 1] = r.rgb.y; out_rgb[3 * i + 2] = r.rgb.z; }
+            if (op != 0) out_pdf[i] = r.pdf;
+            if (op == 2) { out_wo[3 * i] = r.wo.x; out_wo[3 * i + 1] = r.wo.y; out_wo[3 * i + 2] = r.wo.z; }
+        });
     };
     if (precision == 64) run(0.0);
     else run(0.0f);
+}
+
+// Per-query forward-error bound of an FP32 evaluation of orc_bsdf_batch's op,
+// from the FP64 function itself.  An FP32 evaluation whose rounding errors are
+// equivalent to moving each FP32 input by at most `ulps` units in the last
+// place (backward error) can differ from the exact result by at most the
+// function's variation over that neighbourhood; to first order that is the sum
+// over input coordinates of the largest change |f(x +- ulps*ulp_j e_j) - f(x)|
+// (a condition number times the input spacing, evaluated per query, with
+// branch changes inside the neighbourhood included).  Coordinates: the
+// direction components (step ulps * 2^-24, unit vectors), wo (ops 0, 1) or
+// u[0..3] (op 2), h for hair, and every non-zero float parameter of the
+// material record (step ulps * its own FP32 spacing).  Outputs (FP64):
+// bound_rgb n x 3 (ops 0 and 2), bound_pdf n (ops 1 and 2), bound_wo n x 3 (op 2).
+//
+// kind 1 instead runs the running error analysis of orc_fp32err.h: the bound
+// on |FP32 evaluation - exact| of the formula in the oracle's operation
+// order, with `amb[i]` = 1 where a comparison on the way could go the other
+// way in FP32 (`ulps` is unused).
+void orc_bsdf_bound(int op, int kind, uint32_t n, const void *mats, const float *wi3, const float *wo3,
+                    const float *u4, const float *hv, double ulps, double *bound_rgb, double *bound_pdf,
+                    double *bound_wo, uint8_t *amb) {
+    const MatRec *mr = (const MatRec *)mats;
+    if (kind == 1) {
+        par_for(n, [&](uint32_t i) {
+            Err::ambiguous() = false;
+            const MatT<Err> m = MatT<Err>::from(mr[i]);
+            const V3<Err> wi{Err(wi3[3 * i]), Err(wi3[3 * i + 1]), Err(wi3[3 * i + 2])};
+            const Err h = hv ? Err(hv[i]) : Err(0.0f);
+            V3<Err> rgb, wo;
+            Err pdf(0.0f);
+            if (op == 0 || op == 1) {
+                const V3<Err> w2{Err(wo3[3 * i]), Err(wo3[3 * i + 1]), Err(wo3[3 * i + 2])};
+                if (op == 0) rgb = material_eval_h(wi, w2, m, h);
+                else pdf = material_pdf_h(wi, w2, m, h);
+            } else {
+                Err u[4] = {Err(u4[4 * i]), Err(u4[4 * i + 1]), Err(u4[4 * i + 2]), Err(u4[4 * i + 3])};
+                rgb = material_sample_h(wi, u, m, h, wo, pdf);
+            }
+            if (bound_rgb) { bound_rgb[3 * i] = rgb.x.e; bound_rgb[3 * i + 1] = rgb.y.e; bound_rgb[3 * i + 2] = rgb.z.e; }
+            if (bound_pdf) bound_pdf[i] = pdf.e;
+            if (bound_wo) { bound_wo[3 * i] = wo.x.e; bound_wo[3 * i + 1] = wo.y.e; bound_wo[3 * i + 2] = wo.z.e; }
+            if (amb) amb[i] = Err::ambiguous() ? 1 : 0;
+        });
+        return;
+    }
+    par_for(n, [&](uint32_t i) {
+        const MatT<double> m0 = MatT<double>::from(mr[i]);
+        double x[11];
+        for (int k = 0; k < 3; ++k) x[k] = wi3[3 * i + k];
+        const int nv = op == 2 ? 4 : 3;
+        for (int k = 0; k < nv; ++k) x[3 + k] = op == 2 ? u4[4 * i + k] : wo3[3 * i + k];
+        x[7] = hv ? hv[i] : 0.0;
+        auto eval = [&](const double *xx, const MatT<double> &mm) {
+            V3d wi(xx[0], xx[1], xx[2]), wo(xx[3], xx[4], xx[5]);
+            double u[4] = {xx[3], xx[4], xx[5], xx[6]};
+            return bsdf_query(op, mm, wi, wo, u, xx[7]);
+        };
+        const BsdfOut base = eval(x, m0);
+        V3d brgb, bwo;
+        double bpdf = 0;
+        auto accumulate = [&](const BsdfOut &a, const BsdfOut &b, V3d &mr_, V3d &mw_, double &mp_) {
+            for (int c = 0; c < 3; ++c) {
+                mr_[c] = std::max(mr_[c], std::fabs(a.rgb[c] - base.rgb[c]));
+                mr_[c] = std::max(mr_[c], std::fabs(b.rgb[c] - base.rgb[c]));
+                mw_[c] = std::max(mw_[c], std::fabs(a.wo[c] - base.wo[c]));
+                mw_[c] = std::max(mw_[c], std::fabs(b.wo[c] - base.wo[c]));
+            }
+            mp_ = std::max(mp_, std::max(std::fabs(a.pdf - base.pdf), std::fabs(b.pdf - base.pdf)));
+        };
+        const int ncoord = 3 + nv + (m0.type == MAT_HAIR ? 1 : 0);
+        for (int j = 0; j < ncoord; ++j) {
+            const int k = j < 3 + nv ? j : 7;
+            const bool dir = k < 3 || (op != 2 && k < 6);
+            const double step = ulps * (dir ? std::ldexp(1.0, -24) : f32_ulp((float)x[k]));
+            if (step == 0.0) continue;
+            double xp[8], xm[8];
+            std::copy(x, x + 8, xp);
+            std::copy(x, x + 8, xm);
+            xp[k] += step;
+            xm[k] -= step;
+            V3d mr_, mw_;
+            double mp_ = 0;
+            accumulate(eval(xp, m0), eval(xm, m0), mr_, mw_, mp_);
+            brgb += mr_; bwo += mw_; bpdf += mp_;
+        }
+        for (int k = 0; k < N_MAT_PARAMS; ++k) {
+            MatT<double> mp = m0, mm = m0;
+            const double v = mat_param(mp, k);
+            if (v == 0.0) continue;
+            const double step = ulps * f32_ulp((float)v);
+            mat_param(mp, k) = v + step;
+            mat_param(mm, k) = v - step;
+            V3d mr_, mw_;
+            double mp_ = 0;
+            accumulate(eval(x, mp), eval(x, mm), mr_, mw_, mp_);
+            brgb += mr_; bwo += mw_; bpdf += mp_;
+        }
+        if (bound_rgb) { bound_rgb[3 * i] = brgb.x; bound_rgb[3 * i + 1] = brgb.y; bound_rgb[3 * i + 2] = brgb.z; }
+        if (bound_pdf) bound_pdf[i] = bpdf;
+        if (bound_wo) { bound_wo[3 * i] = bwo.x; bound_wo[3 * i + 1] = bwo.y; bound_wo[3 * i + 2] = bwo.z; }
+    });
+}
+
+// Scalar pieces of the Beckmann microfacet model in FP64 (closed-form checks;
+// shaders/Microfacet.hlsli:13-58): fn 0 BeckmannD(wh, a), 1 BeckmannDCosTheta(wh, a),
+// 2 SmithG1(v, wh, a).  v3, wh3: n x 3 directions; a: n roughnesses.
+void orc_microfacet_fn(int fn, uint32_t n, const double *v3, const double *wh3, const double *a, double *out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const V3d v(v3[3 * i], v3[3 * i + 1], v3[3 * i + 2]), wh(wh3[3 * i], wh3[3 * i + 1], wh3[3 * i + 2]);
+        switch (fn) {
+        case 0: out[i] = beckmann_d(wh, a[i]); break;
+        case 1: out[i] = beckmann_d_cos(wh, a[i]); break;
+        default: out[i] = smith_g1(v, wh, a[i]); break;
+        }
+    }
 }
 
 // Scalar pieces of the hair BCSDF in FP64 (closed-form checks):

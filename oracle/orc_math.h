@@ -10,6 +10,9 @@
 #include <cmath>
 #include <cstdint>
 #include <algorithm>
+// Err (FP32 running error analysis) and its std:: overloads must precede the
+// templates below, whose qualified std:: calls bind at their definition.
+#include "orc_fp32err.h"
 
 namespace orc {
 

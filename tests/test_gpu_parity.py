@@ -119,125 +119,7 @@ def test_envmap_alias_draws_bit_exact(gpu, disney_small):
     assert not bad.any()
 
 
-def _bsdf_tuples(n, seed, kind):
-    rng = np.random.default_rng(seed)
-    m = ng.default_materials(n)
-    m["albedo"] = rng.random((n, 3))
-    if kind == "disney":
-        m["type"] = ng.MAT_DISNEY
-        for k in ("metallic", "specular", "sheen", "clearcoat", "specularTint", "sheenTint", "clearcoatGloss",
-                  "subsurface"):
-            m[k] = rng.random(n)
-        m["roughness"] = 0.05 + 0.95 * rng.random(n)
-        m["anisotropic"] = 0.8 * rng.random(n)
-    elif kind == "microfacet":
-        m["type"] = ng.MAT_MICROFACET
-        m["alpha"] = 0.05 + 0.5 * rng.random(n)
-        m["albedo"] *= 0.5
-    elif kind == "hair":
-        m["type"] = ng.MAT_HAIR
-        m["roughness"] = rng.uniform(0.15, 0.9, n)
-        m["betaN"] = rng.uniform(0.15, 0.9, n)
-        m["alpha"] = rng.uniform(-4.0, 4.0, n)
-        m["intIOR"] = rng.uniform(1.4, 1.7, n)
-        m["albedo"] = rng.uniform(0.05, 1.0, (n, 3))
-    else:
-        m["type"] = ng.MAT_DIFFUSE
-
-    def hemi():
-        v = rng.normal(size=(n, 3))
-        if kind != "hair":  # hair scatters over the whole sphere
-            v[:, 2] = np.abs(v[:, 2])
-        return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
-
-    wi, wo = hemi(), hemi()
-    u = rng.random((n, 4)).astype(np.float32)
-    h = rng.uniform(-0.98, 0.98, n).astype(np.float32) if kind == "hair" else None
-    return m, wi, wo, u, h
-
-
-def _rel(a, b, floor):
-    return np.abs(a - b) / np.maximum(np.abs(b), floor)
-
-
-@pytest.mark.parametrize("kind", ["disney", "microfacet", "diffuse"])
-def test_bsdf_eval_pdf_vs_fp64(gpu, kind):
-    # FP32 GPU against the FP64 oracle on the same FP32 inputs.  Tolerance: 1e-5
-    # relative for 99.9% of the tuples, and for all of them the error is within
-    # 1e-5 relative plus 64 float ulps of the largest lobe value (the
-    # conditioning floor of the half-vector normalisation at grazing angles).
-    n = 1 << 20
-    m, wi, wo, _, _ = _bsdf_tuples(n, 21, kind)
-    _say(f"case: bsdf eval/pdf {kind} n={n}")
-    g_rgb, g_pdf, _ = gpu.bsdf_batch(0, m, wi, wo)  # op 0 writes f and pdf
-    o_rgb, _, _ = orc.bsdf_batch(0, m, wi, wo)
-    _, o_pdf, _ = orc.bsdf_batch(1, m, wi, wo)
-    for g, o, what in ((g_rgb, o_rgb, "f"), (g_pdf, o_pdf, "pdf")):
-        scale = np.abs(o).max(axis=-1, keepdims=True) if o.ndim > 1 else np.abs(o)
-        rel = _rel(g, o, 1e-30)
-        ok = rel <= 1e-5
-        frac = ok.mean()
-        _say(f"  {kind} {what}: frac<=1e-5 {frac:.6f} max rel {rel[np.abs(o) > 1e-6].max():.3g}")
-        assert frac >= 0.999, (kind, what, frac)
-        assert np.all(np.abs(g - o) <= 1e-5 * np.abs(o) + 64 * 1.2e-7 * scale + 1e-30), (kind, what)
-
-
-def test_hair_eval_pdf_vs_fp64(gpu):
-    # The hair lobes are exponentials of differences of O(1/v) terms
-    # (log I0(a) - b - 1/v, shaders/Hair.hlsli:55-66): FP32 loses about
-    # |a| ulps of relative accuracy there, so the bound is 1e-4 relative for
-    # 99% of the tuples and 2e-3 relative plus 1e-6 of the largest channel for all.
-    n = 1 << 20
-    m, wi, wo, _, h = _bsdf_tuples(n, 23, "hair")
-    _say(f"case: bsdf eval/pdf hair n={n}")
-    g_rgb, g_pdf, _ = gpu.bsdf_batch(0, m, wi, wo, h=h)
-    o_rgb, _, _ = orc.bsdf_batch(0, m, wi, wo, h=h)
-    _, o_pdf, _ = orc.bsdf_batch(1, m, wi, wo, h=h)
-    for g, o, what in ((g_rgb, o_rgb, "f"), (g_pdf, o_pdf, "pdf")):
-        scale = np.abs(o).max(axis=-1, keepdims=True) if o.ndim > 1 else np.abs(o)
-        rel = _rel(g, o, 1e-30)
-        big = np.abs(o) > 1e-6 * scale
-        q = np.quantile(rel[big], [0.5, 0.99, 0.999])
-        _say(f"  hair {what}: rel quantiles 50/99/99.9% {q} max {rel[big].max():.3g}")
-        assert q[1] <= 1e-4, (what, q)
-        assert np.all(np.abs(g - o) <= 2e-3 * np.abs(o) + 1e-6 * scale + 1e-30), what
-
-
-@pytest.mark.parametrize("kind", ["disney", "microfacet", "diffuse", "hair"])
-def test_bsdf_sample_vs_fp64(gpu, kind):
-    n = 1 << 19
-    m, wi, _, u, h = _bsdf_tuples(n, 22, kind)
-    _say(f"case: bsdf sample {kind} n={n}")
-    g_w, g_pdf, g_wo = gpu.bsdf_batch(2, m, wi, u=u, h=h)
-    o_w, o_pdf, o_wo = orc.bsdf_batch(2, m, wi, u=u, h=h)
-    both = (g_pdf > 0) & (o_pdf > 0)
-    assert np.mean((g_pdf > 0) == (o_pdf > 0)) > 0.9999
-    # the sampled direction: a lobe choice (u[0] against the lobe CDF) may flip
-    # by rounding for a handful of tuples
-    dw = np.abs(g_wo[both] - o_wo[both]).max(1)
-    _say(f"  {kind}: direction diff quantiles 99/99.99% {np.quantile(dw, [0.99, 0.9999])}")
-    # FP32 rounding of the half vector / longitudinal angle moves the direction by
-    # more where the mapping is ill-conditioned (grazing half vectors, narrow lobes)
-    tol99 = 1e-4 if kind == "hair" else 2e-5
-    assert np.quantile(dw, 0.99) < tol99, (kind, np.quantile(dw, 0.99))
-    assert np.quantile(dw, 0.9999) < 5e-4, (kind, np.quantile(dw, 0.9999))
-    # pdf and weight of the GPU's own sample, against the FP64 oracle evaluated
-    # at that same direction (the sampled direction itself is compared above;
-    # evaluating each side at its own direction would mix in the lobe's slope)
-    wo_g = g_wo[both]
-    e_f, _, _ = orc.bsdf_batch(0, m[both], wi[both], wo_g, h=None if h is None else h[both])
-    _, e_pdf, _ = orc.bsdf_batch(1, m[both], wi[both], wo_g, h=None if h is None else h[both])
-    rel = _rel(g_pdf[both], e_pdf, 1e-30)
-    _say(f"  {kind}: pdf rel quantiles 99/99.9% {np.quantile(rel, [0.99, 0.999])}")
-    # Disney's GGX lobe with alpha = roughness^2 down to 2.5e-3: sampled half
-    # vectors sit at the peak, where D ~ 1/(sin^2 + alpha^2 cos^2)^2 amplifies the
-    # FP32 rounding of the half vector (the FP32 oracle shows the same 3e-3 at
-    # the 99th percentile); the sample weight f cos / pdf is not affected.
-    ptol = 1e-2 if kind == "disney" else 1e-4
-    assert np.quantile(rel, 0.99) < ptol, (kind, np.quantile(rel, 0.99))
-    e_w = e_f * np.abs(wo_g[:, 2:3].astype(np.float64)) / np.maximum(e_pdf, 1e-30)[:, None]
-    relw = _rel(g_w[both], e_w, 1e-6)
-    assert np.quantile(relw, 0.99) < 1e-4, (kind, np.quantile(relw, 0.99))
+# BSDF eval/pdf/sample parity at 10^7 tuples per material: tests/test_gpu_bsdf.py.
 
 
 def test_hair_white_furnace_gpu(gpu):
