@@ -196,6 +196,26 @@ int nori_gpu_trace_device(nori_ctx *ctx, const nori_ray *d_rays, int n, nori_hit
 int nori_gpu_bsdf_batch(nori_ctx *ctx, int op, int n, const nori_material *mats, const float *wi3, const float *wo3,
                         const float *u4, const float *h, float *rgb3, float *pdf, float *wo3_out);
 
+/* Participating-media queries of volume `volume` of the uploaded scene on
+ * caller-given rays (replaces the functions of shaders/Volume.hlsl that the
+ * reference's RayGen calls per path vertex: NullCollisionFreeFlight :236-319,
+ * RatioTracking :326-372, the density lookup :95-102, SampleLocalMajorant
+ * :119-164; the Medium plugin's data is include/medium.h:37-63).
+ *   op 0 free flight over [tn, tf]: out3 = path weight (starting at 1),
+ *        t_out = collision distance (tf when none), flag = 1 when it scattered;
+ *   op 1 ratio-tracking transmittance over [tn, tf]: out3;
+ *   op 2 density at o3 (d3, tn, tf, seeds unused): t_out;
+ *   op 3 local majorant at o3 along d3: out3 = {majorant, distance to the
+ *        brick's exit, 0}.
+ * Ops 0 and 1 draw from pcg32 stream seed(seeds[i], PCG32_DEFAULT_STREAM) per
+ * query.  Host arrays.  NORI_E_ARG for an unknown volume. */
+#define NORI_MEDIUM_FREE_FLIGHT 0
+#define NORI_MEDIUM_RATIO_TRACKING 1
+#define NORI_MEDIUM_DENSITY 2
+#define NORI_MEDIUM_MAJORANT 3
+int nori_gpu_medium_batch(nori_ctx *ctx, int op, int volume, int n, const float *o3, const float *d3, const float *tn,
+                          const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag);
+
 /* Envmap alias sampling of the uploaded scene: for each i, a 32-bit integer r,
  * a coin and two jitters -> direction, radiance, solid-angle pdf and texel id.
  * The pdf equals EnvmapPdfDirection (shaders/Envmap.hlsli:109-135). */

@@ -14,6 +14,23 @@ namespace {
 struct Handle {
     SceneData s;
 };
+
+// Runs f(i) for i in [0, n) on up to 16 host threads (contiguous ranges).
+template <class F> void par_for(uint32_t n, F f) {
+    unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 4096 || nt == 1) {
+        for (uint32_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint32_t step = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const uint32_t a = t * step, b = std::min(n, a + step);
+        if (a >= b) break;
+        th.emplace_back([=, &f] { for (uint32_t i = a; i < b; ++i) f(i); });
+    }
+    for (auto &x : th) x.join();
+}
 } // namespace
 
 extern "C" {
@@ -168,7 +185,7 @@ void orc_volume_query(void *hp, int op, uint32_t vi, uint32_t n, const float *o3
                       const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag) {
     const SceneData &s = ((Handle *)hp)->s;
     const Volume &v = s.vols[vi];
-    for (uint32_t i = 0; i < n; ++i) {
+    par_for(n, [&](uint32_t i) {
         V3f o(o3[3 * i], o3[3 * i + 1], o3[3 * i + 2]), d(d3 ? d3[3 * i] : 0.f, d3 ? d3[3 * i + 1] : 0.f,
                                                           d3 ? d3[3 * i + 2] : 0.f);
         Pcg32 r;
@@ -190,7 +207,7 @@ void orc_volume_query(void *hp, int op, uint32_t vi, uint32_t n, const float *o3
             v.local_majorant(o, d, mu, bd);
             out3[3 * i] = mu; out3[3 * i + 1] = bd; out3[3 * i + 2] = 0.0f;
         }
-    }
+    });
 }
 
 // Henyey-Greenstein: op 0 eval(cos, g) -> out; op 1 sample(wo3, g, xi1, xi2) -> dir3, pdf.
@@ -290,23 +307,6 @@ void orc_env_eval_pdf(void *hp, uint32_t n, const float *dir3, float *L3, float 
 
 // ---------------------------------------------------------------- BSDFs
 namespace {
-// Runs f(i) for i in [0, n) on up to 16 host threads (contiguous ranges).
-template <class F> void par_for(uint32_t n, F f) {
-    unsigned nt = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (n < 4096 || nt == 1) {
-        for (uint32_t i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    const uint32_t step = (n + nt - 1) / nt;
-    for (unsigned t = 0; t < nt; ++t) {
-        const uint32_t a = t * step, b = std::min(n, a + step);
-        if (a >= b) break;
-        th.emplace_back([=, &f] { for (uint32_t i = a; i < b; ++i) f(i); });
-    }
-    for (auto &x : th) x.join();
-}
-
 // The outputs of one query: rgb (f or the sample weight), pdf, sampled wo.
 struct BsdfOut {
     V3d rgb, wo;

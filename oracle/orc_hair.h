@@ -10,23 +10,28 @@
 
 namespace orc {
 
-template <typename T> T hair_i0(T x) { // shaders/Hair.hlsli:26-40
-    T val = 0, x2i = 1, ifact = 1, i4 = 1;
-    for (int i = 0; i < 10; ++i) {
-        if (i > 0) {
-            x2i *= x * x;
-            ifact *= T(i);
-            i4 *= T(4);
-        }
-        val += x2i / (i4 * ifact * ifact);
+// I0(x) = sum_k (x^2/4)^k / (k!)^2 summed to convergence (26 terms: at x = 12
+// term 25 is 2e-16 of the sum), each term from the last by (x^2/4)/k^2.  The
+// reference (shaders/Hair.hlsli:26-40) stops at 10 terms, 2 % short at x = 12.
+template <typename T> T hair_i0(T x) {
+    const T q = T(0.25) * x * x;
+    T val = T(1), t = T(1);
+    for (int k = 1; k < 26; ++k) {
+        t *= q / T(k * k);
+        val += t;
     }
     return val;
 }
-template <typename T> T hair_log_i0(T x) { // :43-48
-    if (x > T(12)) return x + T(0.5) * (-std::log(T(2) * Consts<T>::PI) + std::log(T(1) / x) + T(1) / (T(8) * x));
+// log I0(x); above 12 its asymptotic expansion x - log(2 pi x)/2 + 1/(8x) +
+// 1/(16x^2).  The reference (shaders/Hair.hlsli:43-48) halves the 1/(8x) term.
+template <typename T> T hair_log_i0(T x) {
+    if (x > T(12)) {
+        const T r = T(1) / x;
+        return x - T(0.5) * std::log(T(2) * Consts<T>::PI * x) + r * (T(0.125) + T(0.0625) * r);
+    }
     return std::log(hair_i0(x));
 }
-template <typename T> T hair_mp(T cos_i, T cos_o, T sin_i, T sin_o, T v) { // :55-66
+template <typename T> T hair_mp(T cos_i, T cos_o, T sin_i, T sin_o, T v) { // shaders/Hair.hlsli:55-66
     T a = cos_i * cos_o / v, b = sin_i * sin_o / v;
     if (v <= T(0.1)) return std::exp(hair_log_i0(a) - b - T(1) / v + T(0.6931) + std::log(T(1) / (T(2) * v)));
     return (std::exp(-b) * hair_i0(a)) / (std::sinh(T(1) / v) * T(2) * v);

@@ -18,6 +18,14 @@
 //     maximum over the brick's own voxels only, where interpolation can exceed
 //     it and the clamped real-collision probability biases the estimate;
 //   * the brick edge is a configurable number of voxels (16 by default);
+//   * free flight and ratio tracking walk the brick grid as a 3D DDA: the brick
+//     index is carried across faces (the current brick +-1 along the axis the
+//     ray leaves by) and each exit distance is taken from the ray origin.  The
+//     reference re-derives the brick from the rounded position after stepping
+//     exactly onto a face (with a 1e-5 minimum step), so which brick the next
+//     step sees depends on the last bit of that position: two FP32
+//     implementations that round the position differently (a fused
+//     multiply-add) then diverge at brick faces;
 //   * the transmittance of a next-event shadow ray is estimated with its own
 //     pcg32 stream seed(s, PCG32_DEFAULT_STREAM), s being one 32-bit draw from the
 //     path's stream taken right after the light sample (only when the scene has
@@ -129,6 +137,53 @@ struct Volume {
     }
 };
 
+// A ray's walk through the majorant bricks (3D DDA).  Without a grid the whole
+// box is one brick with the global majorant.
+struct BrickWalk {
+    int bc[3] = {0, 0, 0};
+    bool grid = false;
+
+    void start(const Volume &v, V3f o, V3f d, float t) {
+        grid = v.has_grid();
+        if (!grid) return;
+        const V3f p = o + d * t;
+        const float pp[3] = {p.x, p.y, p.z}, lo[3] = {v.vmin.x, v.vmin.y, v.vmin.z};
+        const float sz[3] = {std::max(v.vmax.x - v.vmin.x, 1e-6f), std::max(v.vmax.y - v.vmin.y, 1e-6f),
+                             std::max(v.vmax.z - v.vmin.z, 1e-6f)};
+        const int m[3] = {(int)v.mW, (int)v.mH, (int)v.mD};
+        for (int i = 0; i < 3; ++i) bc[i] = std::min(std::max((int)((pp[i] - lo[i]) / sz[i] * (float)m[i]), 0), m[i] - 1);
+    }
+    float majorant(const Volume &v) const {
+        const float gmu = v.sigma_t_max();
+        return grid ? v.majorant[(size_t)bc[0] + (size_t)v.mW * ((size_t)bc[1] + (size_t)v.mH * bc[2])] * gmu : gmu;
+    }
+    // ray parameter where the ray leaves the current brick, and the axis it leaves by
+    float exit(const Volume &v, V3f o, V3f d, int &axis) const {
+        axis = -1;
+        if (!grid) return 1e20f;
+        const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, lo[3] = {v.vmin.x, v.vmin.y, v.vmin.z};
+        const float sz[3] = {std::max(v.vmax.x - v.vmin.x, 1e-6f), std::max(v.vmax.y - v.vmin.y, 1e-6f),
+                             std::max(v.vmax.z - v.vmin.z, 1e-6f)};
+        const float m[3] = {(float)v.mW, (float)v.mH, (float)v.mD};
+        float best = 1e20f;
+        for (int i = 0; i < 3; ++i) {
+            if (std::fabs(dd[i]) < 1e-20f) continue;
+            const float edge = lo[i] + (float)(bc[i] + (dd[i] > 0.0f ? 1 : 0)) * (sz[i] / m[i]);
+            const float ti = (edge - oo[i]) / dd[i];
+            if (ti < best) { best = ti; axis = i; }
+        }
+        return best;
+    }
+    // into the neighbouring brick across `axis`; false once the walk leaves the grid
+    bool advance(const Volume &v, V3f d, int axis) {
+        if (axis < 0) return false;
+        const float dd[3] = {d.x, d.y, d.z};
+        const int m[3] = {(int)v.mW, (int)v.mH, (int)v.mD};
+        bc[axis] += dd[axis] > 0.0f ? 1 : -1;
+        return bc[axis] >= 0 && bc[axis] < m[axis];
+    }
+};
+
 // shaders/Volume.hlsl:14-27
 inline bool ray_aabb(V3f o, V3f d, V3f bmin, V3f bmax, float &tn, float &tf) {
     V3f inv(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -188,27 +243,30 @@ inline uint32_t enumerate_segments(const std::vector<Volume> &vols, V3f o, V3f d
     return n;
 }
 
-// shaders/Volume.hlsl:236-319
+// shaders/Volume.hlsl:236-319, the bricks walked as a DDA (BrickWalk).  One
+// uniform per tentative collision, a second for the real/null decision.
 template <class Rng>
 inline bool null_collision_free_flight(const Volume &v, V3f o, V3f d, float tn, float tf, Rng &rng, V3f &weight,
                                        float &ts) {
     const float gmu = v.sigma_t_max();
     const V3f st = v.sigma_t();
     float t = tn;
+    BrickWalk w;
+    w.start(v, o, d, t);
     for (uint32_t step = 0; step < 512; ++step) {
-        V3f pos = o + d * t;
-        float mu, bd;
-        v.local_majorant(pos, d, mu, bd);
+        const float mu = w.majorant(v);
+        int axis;
+        const float texit = w.exit(v, o, d, axis);
         if (mu < 1e-8f) {
-            t += bd;
-            if (t >= tf) { ts = tf; return false; }
+            t = std::max(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) { ts = tf; return false; }
             continue;
         }
         float xi = std::max(rng.next_float(), 1e-10f);
         float dt = -std::log(xi) / mu;
-        if (dt > bd) {
-            t += bd;
-            if (t >= tf) { ts = tf; return false; }
+        if (t + dt > texit) {
+            t = std::max(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) { ts = tf; return false; }
             continue;
         }
         t += dt;
@@ -228,26 +286,28 @@ inline bool null_collision_free_flight(const Volume &v, V3f o, V3f d, float tn, 
     return false;
 }
 
-// shaders/Volume.hlsl:326-372
+// shaders/Volume.hlsl:326-372, the bricks walked as a DDA (BrickWalk).
 template <class Rng>
 inline V3f ratio_tracking(const Volume &v, V3f o, V3f d, float tn, float tf, Rng &rng) {
     const V3f st = v.sigma_t();
     V3f tr(1.0f);
     float t = tn;
+    BrickWalk w;
+    w.start(v, o, d, t);
     for (uint32_t step = 0; step < 512; ++step) {
-        V3f pos = o + d * t;
-        float mu, bd;
-        v.local_majorant(pos, d, mu, bd);
+        const float mu = w.majorant(v);
+        int axis;
+        const float texit = w.exit(v, o, d, axis);
         if (mu < 1e-8f) {
-            t += bd;
-            if (t >= tf) return tr;
+            t = std::max(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) return tr;
             continue;
         }
         float xi = std::max(rng.next_float(), 1e-10f);
         float dt = -std::log(xi) / mu;
-        if (dt > bd) {
-            t += bd;
-            if (t >= tf) return tr;
+        if (t + dt > texit) {
+            t = std::max(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) return tr;
             continue;
         }
         t += dt;

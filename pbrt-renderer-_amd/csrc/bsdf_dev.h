@@ -193,20 +193,27 @@ NG_HD void disney_eval_pdf(f3 wi, f3 wo, const Mat &m, const DisneyPre &q, f3 &f
 // phi = atan2(w.z, w.y); h in [-1, 1] is the offset across the fiber.  Material
 // fields: roughness = beta_m, betaN = beta_n, alpha = scale tilt in degrees,
 // intIOR = eta, albedo = fiber colour (mapped to sigma_a by Chiang eq. 9).
-NG_HD float hair_i0(float x) { // shaders/Hair.hlsli:26-40 (10-term series)
-    float val = 0.0f, x2i = 1.0f, ifact = 1.0f, i4 = 1.0f;
-    for (int i = 0; i < 10; ++i) {
-        if (i > 0) {
-            x2i *= x * x;
-            ifact *= (float)i;
-            i4 *= 4.0f;
-        }
-        val += x2i / (i4 * ifact * ifact);
+// I0(x) = sum_k (x^2/4)^k / (k!)^2 for x <= 12, summed to convergence: term k
+// from term k-1 by the factor (x^2/4)/k^2 (no overflowing powers or
+// factorials); at x = 12 term 25 is 2e-16 of the sum.  The reference
+// (shaders/Hair.hlsli:26-40) stops at 10 terms, 2 % short of I0 at x = 12.
+NG_HD float hair_i0(float x) {
+    const float q = 0.25f * x * x;
+    float val = 1.0f, t = 1.0f;
+    for (int k = 1; k < 26; ++k) {
+        t *= q / (float)(k * k);
+        val += t;
     }
     return val;
 }
+// log I0(x); above 12 its asymptotic expansion x - log(2 pi x)/2 + 1/(8x) +
+// 1/(16x^2) (remainder under 4e-5).  The reference (:43-48) writes the 1/(8x)
+// term halved, 1/(16x) short.
 NG_HD float hair_log_i0(float x) {
-    if (x > 12.0f) return x + 0.5f * (-logf(2.0f * PI) + logf(1.0f / x) + 1.0f / (8.0f * x));
+    if (x > 12.0f) {
+        const float r = 1.0f / x;
+        return x - 0.5f * logf(2.0f * PI * x) + r * (0.125f + 0.0625f * r);
+    }
     return logf(hair_i0(x));
 }
 NG_HD float hair_mp(float cos_i, float cos_o, float sin_i, float sin_o, float v) { // :55-66

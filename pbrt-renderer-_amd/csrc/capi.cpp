@@ -1059,6 +1059,44 @@ int nori_gpu_bsdf_batch(nori_ctx *c, int op, int n, const nori_material *mats, c
     });
 }
 
+int nori_gpu_medium_batch(nori_ctx *c, int op, int volume, int n, const float *o3, const float *d3, const float *tn,
+                          const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag) {
+    if (!c || n < 0 || op < 0 || op > 3 || volume < 0) return NORI_E_ARG;
+    const bool ray = op != 2, interval = op <= 1;
+    if (n > 0 && (!o3 || (ray && !d3) || (interval && (!tn || !tf || !seeds)) || (op != 2 && !out3) ||
+                  ((op == 0 || op == 2) && !t_out) || (op == 0 && !flag)))
+        return NORI_E_ARG;
+    return guard(c, [&] {
+        if (!c->has_scene || (uint32_t)volume >= c->sdev.n_vol)
+            throw ArgError("medium_batch: no such volume in the uploaded scene", NORI_E_ARG);
+        if (n == 0) return;
+        ScopedBufs b(8);
+        auto up = [&](DevBuf &d, const void *src, size_t bytes) {
+            HIPCHK(hipMalloc(&d.p, bytes));
+            d.bytes = bytes;
+            if (src) HIPCHK(hipMemcpy(d.p, src, bytes, hipMemcpyHostToDevice));
+            return d.p;
+        };
+        const size_t v1 = (size_t)n * sizeof(float), v3 = 3 * v1;
+        void *dO = up(b[0], o3, v3);
+        void *dD = ray ? up(b[1], d3, v3) : nullptr;
+        void *dN = interval ? up(b[2], tn, v1) : nullptr;
+        void *dF = interval ? up(b[3], tf, v1) : nullptr;
+        void *dS = interval ? up(b[4], seeds, (size_t)n * sizeof(uint64_t)) : nullptr;
+        void *dOut = up(b[5], nullptr, v3);
+        void *dT = up(b[6], nullptr, v1);
+        void *dFl = up(b[7], nullptr, (size_t)n * sizeof(int));
+        launch_medium_batch(c->sdev.vols, (uint32_t)volume, op, (uint32_t)n, (const float *)dO, (const float *)dD,
+                            (const float *)dN, (const float *)dF, (const uint64_t *)dS, (float *)dOut, (float *)dT,
+                            (int *)dFl, c->stream);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+        if (op != 2) HIPCHK(hipMemcpy(out3, dOut, v3, hipMemcpyDeviceToHost));
+        if (op == 0 || op == 2) HIPCHK(hipMemcpy(t_out, dT, v1, hipMemcpyDeviceToHost));
+        if (op == 0) HIPCHK(hipMemcpy(flag, dFl, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    });
+}
+
 int nori_gpu_envmap_sample(nori_ctx *c, int n, const uint32_t *r, const float *coin, const float *jx, const float *jy,
                            float *dir3, float *rgb3, float *pdf, uint32_t *texel) {
     if (!c || n < 0 || (n > 0 && (!r || !coin || !jx || !jy || !dir3 || !rgb3 || !pdf || !texel))) return NORI_E_ARG;

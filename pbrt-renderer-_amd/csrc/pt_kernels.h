@@ -175,6 +175,9 @@ void launch_counters(Counters c, uint32_t add_ext_from, uint32_t add_shadow, uin
 void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st);
 void launch_zero_f4(float4 *p, size_t n, hipStream_t st);
 
+void launch_medium_batch(const VolumeDev *vols, uint32_t vi, int op, uint32_t n, const float *o3, const float *d3,
+                         const float *tn, const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag,
+                         hipStream_t st);
 void launch_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo,
                        const float *u, const float *hv, float *rgb, float *pdf, float *wo_out, hipStream_t st);
 void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,

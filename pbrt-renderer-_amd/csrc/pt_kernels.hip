@@ -1494,6 +1494,39 @@ __global__ void k_bsdf_batch(int op, uint32_t n, const nori_material *mats, cons
     }
 }
 
+// Media queries of one volume on caller-given rays and seeds (the functions of
+// shaders/Volume.hlsl:95-372): op 0 free flight over [tn, tf] (weight, distance,
+// scattered flag), 1 ratio-tracking transmittance over [tn, tf], 2 density at
+// o, 3 local majorant at o along d (majorant, distance to the brick's exit).
+// Each query draws from its own pcg32 stream seed(seeds[i], default stream).
+__global__ void k_medium_batch(const VolumeDev *vols, uint32_t vi, int op, uint32_t n, const float *o3,
+                               const float *d3, const float *tn, const float *tf, const uint64_t *seeds, float *out3,
+                               float *t_out, int *flag) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const VolumeDev &v = vols[vi];
+    const f3 o = f3{o3[3 * i], o3[3 * i + 1], o3[3 * i + 2]};
+    const f3 d = d3 ? f3{d3[3 * i], d3[3 * i + 1], d3[3 * i + 2]} : mk(0.0f);
+    uint64_t rng = pcg_seed(seeds ? seeds[i] : 0ull, PCG_DEFAULT_STREAM);
+    if (op == 0) {
+        f3 w = mk(1.0f);
+        float ts;
+        const bool sc = vol_free_flight(v, o, d, tn[i], tf[i], rng, w, ts);
+        out3[3 * i] = w.x; out3[3 * i + 1] = w.y; out3[3 * i + 2] = w.z;
+        t_out[i] = ts;
+        flag[i] = sc ? 1 : 0;
+    } else if (op == 1) {
+        const f3 w = v.hetero ? vol_ratio_tracking(v, o, d, tn[i], tf[i], rng) : mk(0.0f);
+        out3[3 * i] = w.x; out3[3 * i + 1] = w.y; out3[3 * i + 2] = w.z;
+    } else if (op == 2) {
+        t_out[i] = vol_density(v, o);
+    } else {
+        float mu, bd;
+        vol_local_majorant(v, o, d, mu, bd);
+        out3[3 * i] = mu; out3[3 * i + 1] = bd; out3[3 * i + 2] = 0.0f;
+    }
+}
+
 __global__ void k_env_sample(EnvDev e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,
                              const float *jy, float *dir, float *rgb, float *pdf, uint32_t *texel) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1738,6 +1771,13 @@ void launch_zero_f4(float4 *p, size_t n, hipStream_t st) {
 void launch_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo, const float *u,
                        const float *hv, float *rgb, float *pdf, float *wo_out, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_bsdf_batch, dim3(blocks(n, 256)), dim3(256), 0, st, op, n, mats, wi, wo, u, hv, rgb, pdf, wo_out);
+}
+
+void launch_medium_batch(const VolumeDev *vols, uint32_t vi, int op, uint32_t n, const float *o3, const float *d3,
+                         const float *tn, const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag,
+                         hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_medium_batch, dim3(blocks(n, 128)), dim3(128), 0, st, vols, vi, op, n, o3, d3, tn, tf,
+                              seeds, out3, t_out, flag);
 }
 
 void launch_env_sample(const EnvDev &e, uint32_t n, const uint32_t *r, const float *coin, const float *jx,

@@ -10,7 +10,15 @@
 // voxels plus a one-voxel apron), so density / majorant <= 1.
 //
 // Floating-point contraction is off in these functions, as in the CPU
-// restatement, so both evaluate the same roundings.
+// restatement, so both evaluate the same roundings; positions along a ray are
+// formed by vol_point, not by the (contracted) f3 operators.
+//
+// Free flight and ratio tracking walk the majorant bricks as a 3D DDA: the
+// brick index is carried across faces (the current brick +-1 along the axis the
+// ray leaves by) and each exit distance is measured from the ray origin.  The
+// reference (shaders/Volume.hlsl:119-164, 236-372) instead steps exactly onto a
+// face and re-derives the brick from the rounded position (with a 1e-5 minimum
+// step), so the brick seen next hangs on the position's last bit.
 #pragma once
 #include "nori_math.h"
 
@@ -28,6 +36,12 @@ struct VolumeDev {
 
 NG_HD f3 vol_sigma_t(const VolumeDev &v) { return v.sigma_a + v.sigma_s; }
 NG_HD float vol_sigma_t_max(const VolumeDev &v) { return maxc(vol_sigma_t(v)); }
+
+// o + d t, rounded per component without contraction
+NG_HD f3 vol_point(f3 o, f3 d, float t) {
+#pragma clang fp contract(off)
+    return mk(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+}
 
 NG_HD float vol_density(const VolumeDev &v, f3 p) {
 #pragma clang fp contract(off)
@@ -83,6 +97,56 @@ NG_HD void vol_local_majorant(const VolumeDev &v, f3 pos, f3 dir, float &mu, flo
     }
     bd = best;
 }
+
+// A ray's walk through the majorant bricks (3D DDA); without a grid the whole
+// box is one brick with the global majorant.
+struct BrickWalk {
+    int bc[3];
+    bool grid;
+
+    NG_HD void start(const VolumeDev &v, f3 o, f3 d, float t) {
+#pragma clang fp contract(off)
+        grid = v.hetero && v.majorant;
+        bc[0] = bc[1] = bc[2] = 0;
+        if (!grid) return;
+        const f3 p = vol_point(o, d, t);
+        const float pp[3] = {p.x, p.y, p.z}, lo[3] = {v.vmin.x, v.vmin.y, v.vmin.z};
+        const float sz[3] = {fmaxf(v.vmax.x - v.vmin.x, 1e-6f), fmaxf(v.vmax.y - v.vmin.y, 1e-6f),
+                             fmaxf(v.vmax.z - v.vmin.z, 1e-6f)};
+        const int m[3] = {(int)v.mW, (int)v.mH, (int)v.mD};
+        for (int i = 0; i < 3; ++i) bc[i] = min(max((int)((pp[i] - lo[i]) / sz[i] * (float)m[i]), 0), m[i] - 1);
+    }
+    NG_HD float majorant(const VolumeDev &v) const {
+        const float gmu = vol_sigma_t_max(v);
+        return grid ? v.majorant[(size_t)bc[0] + (size_t)v.mW * ((size_t)bc[1] + (size_t)v.mH * bc[2])] * gmu : gmu;
+    }
+    // ray parameter where the ray leaves the current brick, and the axis it leaves by
+    NG_HD float exit(const VolumeDev &v, f3 o, f3 d, int &axis) const {
+#pragma clang fp contract(off)
+        axis = -1;
+        if (!grid) return 1e20f;
+        const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z}, lo[3] = {v.vmin.x, v.vmin.y, v.vmin.z};
+        const float sz[3] = {fmaxf(v.vmax.x - v.vmin.x, 1e-6f), fmaxf(v.vmax.y - v.vmin.y, 1e-6f),
+                             fmaxf(v.vmax.z - v.vmin.z, 1e-6f)};
+        const float m[3] = {(float)v.mW, (float)v.mH, (float)v.mD};
+        float best = 1e20f;
+        for (int i = 0; i < 3; ++i) {
+            if (fabsf(dd[i]) < 1e-20f) continue;
+            const float edge = lo[i] + (float)(bc[i] + (dd[i] > 0.0f ? 1 : 0)) * (sz[i] / m[i]);
+            const float ti = (edge - oo[i]) / dd[i];
+            if (ti < best) { best = ti; axis = i; }
+        }
+        return best;
+    }
+    // into the neighbouring brick across `axis`; false once the walk leaves the grid
+    NG_HD bool advance(const VolumeDev &v, f3 d, int axis) {
+        if (axis < 0) return false;
+        const float dd[3] = {d.x, d.y, d.z};
+        const int m[3] = {(int)v.mW, (int)v.mH, (int)v.mD};
+        bc[axis] += dd[axis] > 0.0f ? 1 : -1;
+        return bc[axis] >= 0 && bc[axis] < m[axis];
+    }
+};
 
 // shaders/Volume.hlsl:14-27
 NG_HD bool vol_ray_aabb(f3 o, f3 d, f3 bmin, f3 bmax, float &tn, float &tf) {
@@ -145,74 +209,80 @@ NG_HD uint32_t vol_segments(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, floa
     return n;
 }
 
-// shaders/Volume.hlsl:236-319
+// shaders/Volume.hlsl:236-319, the bricks walked as a DDA (BrickWalk).  One
+// uniform per tentative collision, a second for the real/null decision.
 template <class R>
 NG_HD bool vol_free_flight(const VolumeDev &v, f3 o, f3 d, float tn, float tf, R &rng, f3 &weight, float &ts) {
 #pragma clang fp contract(off)
     const float gmu = vol_sigma_t_max(v);
     const f3 st = vol_sigma_t(v);
     float t = tn;
+    BrickWalk w;
+    w.start(v, o, d, t);
     for (uint32_t step = 0; step < 512; ++step) {
-        f3 pos = o + d * t;
-        float mu, bd;
-        vol_local_majorant(v, pos, d, mu, bd);
+        const float mu = w.majorant(v);
+        int axis;
+        const float texit = w.exit(v, o, d, axis);
         if (mu < 1e-8f) {
-            t += bd;
-            if (t >= tf) { ts = tf; return false; }
+            t = fmaxf(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) { ts = tf; return false; }
             continue;
         }
         float xi = fmaxf(rng_float(rng), 1e-10f);
         float dt = -logf(xi) / mu;
-        if (dt > bd) {
-            t += bd;
-            if (t >= tf) { ts = tf; return false; }
+        if (t + dt > texit) {
+            t = fmaxf(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) { ts = tf; return false; }
             continue;
         }
         t += dt;
         if (t >= tf) { ts = tf; return false; }
-        float dens = vol_density(v, o + d * t);
+        float dens = vol_density(v, vol_point(o, d, t));
         float p_real = fminf(fmaxf((dens * gmu) / fmaxf(mu, 1e-20f), 0.0f), 1.0f);
         if (p_real >= 1.0f - 1e-6f || rng_float(rng) < p_real) {
             float inv = 1.0f / fmaxf(gmu, 1e-20f);
-            weight = weight * mk(st.x * inv, st.y * inv, st.z * inv);
+            weight = mk(weight.x * (st.x * inv), weight.y * (st.y * inv), weight.z * (st.z * inv));
             ts = t;
             return true;
         }
         float den = fmaxf(mu * (1.0f - p_real), 1e-20f);
-        weight = weight * mk((mu - dens * st.x) / den, (mu - dens * st.y) / den, (mu - dens * st.z) / den);
+        weight = mk(weight.x * ((mu - dens * st.x) / den), weight.y * ((mu - dens * st.y) / den),
+                    weight.z * ((mu - dens * st.z) / den));
     }
     ts = tf;
     return false;
 }
 
-// shaders/Volume.hlsl:326-372
+// shaders/Volume.hlsl:326-372, the bricks walked as a DDA (BrickWalk).
 template <class R>
 NG_HD f3 vol_ratio_tracking(const VolumeDev &v, f3 o, f3 d, float tn, float tf, R &rng) {
 #pragma clang fp contract(off)
     const f3 st = vol_sigma_t(v);
     f3 tr = mk(1.0f);
     float t = tn;
+    BrickWalk w;
+    w.start(v, o, d, t);
     for (uint32_t step = 0; step < 512; ++step) {
-        f3 pos = o + d * t;
-        float mu, bd;
-        vol_local_majorant(v, pos, d, mu, bd);
+        const float mu = w.majorant(v);
+        int axis;
+        const float texit = w.exit(v, o, d, axis);
         if (mu < 1e-8f) {
-            t += bd;
-            if (t >= tf) return tr;
+            t = fmaxf(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) return tr;
             continue;
         }
         float xi = fmaxf(rng_float(rng), 1e-10f);
         float dt = -logf(xi) / mu;
-        if (dt > bd) {
-            t += bd;
-            if (t >= tf) return tr;
+        if (t + dt > texit) {
+            t = fmaxf(t, texit);
+            if (t >= tf || !w.advance(v, d, axis)) return tr;
             continue;
         }
         t += dt;
         if (t >= tf) return tr;
-        float dens = vol_density(v, o + d * t);
+        float dens = vol_density(v, vol_point(o, d, t));
         float inv = 1.0f / fmaxf(mu, 1e-20f);
-        tr = tr * mk((mu - dens * st.x) * inv, (mu - dens * st.y) * inv, (mu - dens * st.z) * inv);
+        tr = mk(tr.x * ((mu - dens * st.x) * inv), tr.y * ((mu - dens * st.y) * inv), tr.z * ((mu - dens * st.z) * inv));
         if (maxc(tr) < 1e-6f) return mk(0.0f);
     }
     return tr;

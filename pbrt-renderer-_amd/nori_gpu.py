@@ -135,7 +135,7 @@ class SceneOptions(ctypes.Structure):
 EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
     "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
-    "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_texture_sample",
+    "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_medium_batch", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_texture_sample",
     "nori_gpu_render_stats", "nori_gpu_aov",
     "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_gpu_autotune", "nori_scene_generate", "nori_scene_get_desc",
     "nori_scene_get_camera", "nori_scene_free", "nori_bvh8_build_host",
@@ -164,6 +164,7 @@ def load_library(path=LIB_PATH):
         "nori_gpu_bsdf_batch": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_li": (i32, [vp, vp, vp, i32, ctypes.c_uint32, ctypes.c_uint32, vp]),
         "nori_gpu_envmap_sample": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "nori_gpu_medium_batch": (i32, [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
         "nori_gpu_pcg32": (i32, [vp, _u64, _u64, _u64, i32, vp]),
         "nori_gpu_sobol": (i32, [vp, _u32, _u32, _u32, i32, vp]),
@@ -466,6 +467,26 @@ class NoriGPU:
         self._check(self.lib.nori_gpu_bsdf_batch(self.h, op, n, _ptr(mats), _ptr(wi), _ptr(wo), _ptr(u), _ptr(h),
                                                  _ptr(rgb), _ptr(pdf), _ptr(woo)), "bsdf_batch")
         return rgb, pdf, woo
+
+    def medium_batch(self, op, volume, o, d=None, tn=None, tf=None, seeds=None):
+        """Media queries of volume `volume` on given rays: op 0 free flight
+        -> (weight n x 3, distance, scattered); 1 ratio tracking -> transmittance
+        n x 3; 2 density at o -> n; 3 local majorant at o along d -> n x 3
+        {majorant, distance to the brick's exit, 0}."""
+        o = np.ascontiguousarray(o, np.float32)
+        n = len(o)
+        d = None if d is None else np.ascontiguousarray(d, np.float32)
+        tn = None if tn is None else np.ascontiguousarray(tn, np.float32)
+        tf = None if tf is None else np.ascontiguousarray(tf, np.float32)
+        seeds = None if seeds is None else np.ascontiguousarray(seeds, np.uint64)
+        out3, t, flag = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32)
+        self._check(self.lib.nori_gpu_medium_batch(self.h, op, volume, n, _ptr(o), _ptr(d), _ptr(tn), _ptr(tf),
+                                                   _ptr(seeds), _ptr(out3), _ptr(t), _ptr(flag)), "medium_batch")
+        if op == 0:
+            return out3, t, flag.astype(bool)
+        if op == 2:
+            return t
+        return out3
 
     def envmap_sample(self, r, coin, jx, jy):
         n = len(r)

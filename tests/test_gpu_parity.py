@@ -124,7 +124,7 @@ def test_envmap_alias_draws_bit_exact(gpu, disney_small):
 
 def test_hair_white_furnace_gpu(gpu):
     # GPU quadrature of f |cos| for sigma_a = 0 against the same quadrature of the
-    # FP64 oracle (which the CPU suite pins to within the truncated-I0 loss of 1)
+    # FP64 oracle (which the CPU suite pins to 1 within 1e-4)
     from test_hair_oracle import hair_dir, hair_mats, sphere_quadrature
     _say("case: hair furnace quadrature")
     dirs, w = sphere_quadrature(256, 512)
@@ -139,7 +139,7 @@ def test_hair_white_furnace_gpu(gpu):
         ge, oe = ((g * cz) * w[:, None]).sum(0), ((o * cz) * w[:, None]).sum(0)
         _say(f"  beta {beta}: gpu {ge} oracle {oe} pdf mass {(gp * w).sum():.6f}")
         assert np.allclose(ge, oe, rtol=1e-5)
-        assert 0.99 < ge[0] < 1.0 + 1e-4
+        assert abs(ge[0] - 1.0) < 1e-3
 
 
 def _render_pair(gpu, scene, w, h, spp, depth, seed=7):

@@ -26,36 +26,100 @@ def _render(gpu, scene, cam, spp, depth, seed):
     return gpu.render(ng.make_view(cam, spp, depth), seed=seed)
 
 
+@pytest.fixture(scope="module")
+def c4_full():
+    # the medium workload's grid: 256^3 densities, 16^3-voxel bricks (16^3 majorants)
+    sc = ng.generate_scene(4, env_width=256, env_height=128, width=32, height=32, variant=1)
+    m = sc.media[0]
+    assert m.density.shape == (256, 256, 256) and m.brick == 16
+    return sc
+
+
+def _medium_queries(sc, n, seed):
+    """Rays from points in and around the medium's box in random directions,
+    clipped to the box, with per-query pcg32 seeds."""
+    m = sc.media[0]
+    lo, hi = np.asarray(m.vmin, np.float64), np.asarray(m.vmax, np.float64)
+    rng = np.random.default_rng(seed)
+    o = lo + (hi - lo) * rng.uniform(-0.2, 1.2, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o, d = o.astype(np.float32), d.astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / d.astype(np.float64)
+        t0, t1 = (lo - o) * inv, (hi - o) * inv
+    tn = np.maximum(np.minimum(t0, t1).max(1), 0.0)
+    tf = np.maximum(t0, t1).min(1)
+    keep = tf > tn
+    seeds = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+    return o[keep], d[keep], tn[keep].astype(np.float32), tf[keep].astype(np.float32), seeds[keep]
+
+
+def test_medium_functions_match_oracle(gpu, c4_full):
+    # The reference's per-vertex media functions (shaders/Volume.hlsl:95-372) on
+    # identical rays and random streams: lookups bit-exact, free flight and ratio
+    # tracking on the same collision sequence.
+    sc = c4_full
+    gpu.upload(sc)
+    o_sc = orc.OracleScene(sc)
+    o, d, tn, tf, seeds = _medium_queries(sc, 1 << 20, 41)
+    n = len(o)
+    _say(f"case: medium lookups n={n}")
+    p = o + d * ((tn + tf) / 2)[:, None]
+    g_den = gpu.medium_batch(2, 0, p)
+    _, o_den, _ = o_sc.volume_query(2, 0, p)
+    assert np.array_equal(g_den.view(np.uint32), o_den.view(np.uint32))
+    g_maj = gpu.medium_batch(3, 0, p, d)
+    o_maj, _, _ = o_sc.volume_query(3, 0, p, d)
+    assert np.array_equal(g_maj.view(np.uint32), o_maj.view(np.uint32))
+    _say(f"case: free flight n={n}")
+    g_w, g_t, g_sc = gpu.medium_batch(0, 0, o, d, tn, tf, seeds)
+    o_w, o_t, o_sc_ = o_sc.volume_query(0, 0, o, d, tn, tf, seeds)
+    o_sc_ = o_sc_.astype(bool)
+    same = (g_sc == o_sc_) & (np.abs(g_t - o_t) <= 1e-5 * np.abs(o_t) + 1e-6)
+    wclose = np.all(np.abs(g_w - o_w) <= 1e-4 * np.abs(o_w) + 1e-7, axis=1)
+    bits = np.mean(g_t.view(np.uint32) == o_t.view(np.uint32))
+    _say(f"  scattered {o_sc_.mean():.4f}; same outcome and distance {same.mean():.6f} "
+         f"(bit-equal distances {bits:.6f}); weights within 1e-4 {wclose[same].mean():.6f}")
+    assert same.mean() > 1 - 1e-4
+    assert wclose[same].mean() > 1 - 1e-4
+    _say(f"case: ratio tracking n={n}")
+    g_tr = gpu.medium_batch(1, 0, o, d, tn, tf, seeds)
+    o_tr, _, _ = o_sc.volume_query(1, 0, o, d, tn, tf, seeds)
+    trc = np.all(np.abs(g_tr - o_tr) <= 1e-4 * np.abs(o_tr) + 1e-6, axis=1)
+    _say(f"  mean transmittance {o_tr.mean():.5f}; within 1e-4 {trc.mean():.6f}")
+    assert trc.mean() > 1 - 1e-4
+
+
 @pytest.mark.parametrize("variant", [0, 2])
 def test_medium_scene_same_samples(gpu, variant):
-    # config 4 at a 32^3 grid: same pcg32 streams on both sides
-    sc = ng.generate_scene(4, grid=32, env_width=128, env_height=64, width=48, height=48, variant=variant)
+    # the medium scene at a 64^3 grid: same pcg32 streams on both sides, compared
+    # per pixel (bricks carried across faces keep FP32 rounding differences from
+    # switching a path's collision sequence)
+    sc = ng.generate_scene(4, grid=64, env_width=128, env_height=64, width=48, height=48, variant=variant)
     _say(f"case: medium scene g={sc.media[0].g:.1f} 48x48 16spp depth 64")
     g = _render(gpu, sc, sc.camera, 16, 64, 3)
     o, _ = orc.OracleScene(sc).render(sc.camera, 16, 64, seed=3)
     gi, oi = g[:, :3] / 16, o[:, :3] / 16
     e = orc.rel_mse(gi, oi)
-    close = np.all(np.abs(gi - oi) <= 1e-3 * np.abs(oi) + 1e-4, axis=1)
+    close = np.all(np.abs(gi - oi) <= 1e-3 * np.abs(oi) + 1e-5, axis=1)
     _say(f"  relMSE {e:.3e}, close {close.mean():.4f}, means {gi.mean(0)} {oi.mean(0)}")
-    # A path diverges from its oracle twin once an FP32 rounding difference in a
-    # free-flight distance (logf, the trilinear lookup) flips one of the tens of
-    # collision decisions per segment in this dense medium, so per-pixel
-    # agreement is partial; the estimates agree in the mean.
     assert np.isfinite(gi).all()
-    assert np.allclose(gi.mean(0), oi.mean(0), rtol=5e-3)
-    assert e < 2e-2
+    assert close.mean() > 0.9
+    assert e < 1e-3
 
 
-def test_medium_scene_converges_to_oracle(gpu):
-    sc = ng.generate_scene(4, grid=32, env_width=128, env_height=64, width=24, height=24, variant=1)
-    _say("case: medium scene independent seeds 24x24 1024spp")
-    g = _render(gpu, sc, sc.camera, 1024, 64, 21)
-    o, _ = orc.OracleScene(sc).render(sc.camera, 1024, 64, seed=22)
-    gi, oi = g[:, :3] / 1024, o[:, :3] / 1024
+def test_medium_full_grid_converges_to_oracle(gpu, c4_full):
+    # the 256^3 grid with 16^3 bricks at 32x32: the GPU at 65,536 spp against the
+    # oracle at 4,096 spp (independent seeds); relMSE <= 1e-3
+    sc = c4_full
+    _say("case: medium 256^3 32x32, GPU 65536 spp vs oracle 4096 spp")
+    g = _render(gpu, sc, sc.camera, 65536, 64, 21)
+    o, _ = orc.OracleScene(sc).render(sc.camera, 4096, 64, seed=22)
+    gi, oi = g[:, :3] / 65536, o[:, :3] / 4096
     e = orc.rel_mse(gi, oi)
     _say(f"  relMSE {e:.3e}, means {gi.mean(0)} {oi.mean(0)}")
-    assert np.allclose(gi.mean(0), oi.mean(0), rtol=0.03)
-    assert e < 2e-2
+    assert e <= 1e-3
 
 
 def test_beer_lambert_gpu(gpu):

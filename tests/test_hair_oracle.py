@@ -2,11 +2,18 @@
 shaders/Hair.hlsli evaluates it) before it checks the GPU path.
 
 Pins: library routines (scipy's I0 and logistic distribution), closed forms of
-the published method (the 10-term I0 series is I0 minus its known tail, the
-asymptotic log I0), normalisation integrals computed by quadrature, the white
-furnace (sigma_a = 0 loses only what the truncated I0 loses), the lobe
-attenuations in closed form, the sampler's density (chi-square), and the
-symmetries and lobe shifts the model fixes.
+the published method (the I0 series summed to convergence equals I0, the
+asymptotic expansion of log I0 to its O(x^-3) remainder), normalisation
+integrals computed by quadrature, the white furnace (sigma_a = 0: 1 to 1e-4),
+the lobe attenuations in closed form, the sampler's density (chi-square), and
+the symmetries and lobe shifts the model fixes.
+
+Deviation from shaders/Hair.hlsli:26-48, kept in the oracle and the device
+alike: the reference sums 10 terms of the I0 series (2 % short of I0 at
+x = 12) and halves the 1/(8x) term of the log I0 asymptote (PBRT's form), so
+its white furnace reaches only 0.9911 (final_report/report.html:1798-1802);
+here I0 is summed to convergence and the asymptote carries 1/(8x) + 1/(16x^2),
+which brings the furnace to 1 within 1e-4.
 """
 import math
 
@@ -45,27 +52,20 @@ def sphere_quadrature(nt, nphi):
 
 
 # ------------------------------------------------------------------ pieces
-def test_i0_series_is_i0_minus_its_tail():
-    # shaders/Hair.hlsli:26-40 sums the first 10 terms of I0(x) = sum (x/2)^2k / (k!)^2
+def test_i0_is_the_bessel_function():
+    # the series sum_k (x/2)^2k / (k!)^2 summed to convergence on [0, 12]
     x = np.linspace(0.0, 12.0, 97)
-    got = orc.hair_fn(0, x)
-    tail = np.array([sum(math.exp(2 * k * math.log(v / 2) - 2 * math.lgamma(k + 1)) for k in range(10, 80))
-                     if v > 0 else 0.0 for v in x])
-    assert np.allclose(got, special.i0(x) - tail, rtol=1e-12, atol=0)
-    small = x <= 2.0
-    assert np.allclose(got[small], special.i0(x[small]), rtol=1e-10)
+    assert np.allclose(orc.hair_fn(0, x), special.i0(x), rtol=1e-13, atol=0)
 
 
 def test_log_i0():
     x = np.linspace(0.1, 12.0, 50)
-    assert np.allclose(orc.hair_fn(1, x), np.log(orc.hair_fn(0, x)), rtol=1e-14)
-    # Asymptotic branch above 12 (shaders/Hair.hlsli:43-48, as PBRT writes it):
-    # x + (-log(2 pi) + log(1/x) + 1/(8x)) / 2.  The true expansion is
-    # log I0(x) = x - log(2 pi x)/2 + 1/(8x) + 1/(16x^2) + O(x^-3), so the branch
-    # falls short by 1/(16x) + 1/(16x^2) + O(x^-3).
+    assert np.allclose(orc.hair_fn(1, x), np.log(special.i0(x)), rtol=1e-13)
+    # Asymptotic branch above 12: log I0(x) = x - log(2 pi x)/2 + 1/(8x) + 1/(16x^2)
+    # + 25/(384 x^3) + O(x^-4); the branch stops after 1/(16x^2).
     x = np.linspace(12.01, 400.0, 200)
     short = (np.log(special.i0e(x)) + x) - orc.hair_fn(1, x)
-    assert np.abs(short - 1 / (16 * x) - 1 / (16 * x * x)).max() < 1 / 12.0 ** 3
+    assert np.all(short > 0) and np.all(short < 0.08 / x ** 3)
 
 
 def mp_textbook(ti, to, v):
@@ -86,8 +86,8 @@ def mp_mass(to, v):
 @pytest.mark.parametrize("v", [0.03, 0.08, 0.2, 0.8])
 def test_mp_is_textbook_lobe_with_truncated_i0(v):
     # The textbook lobe has unit mass.  The oracle's Mp (shaders/Hair.hlsli:55-66)
-    # is that lobe with I0 replaced by the 10-term series (v > 0.1), or, for
-    # v <= 0.1, by the log I0 of test_log_i0 and log 2 rounded to 0.6931.
+    # is that lobe with I0 as the series (v > 0.1), or, for v <= 0.1, with the
+    # log I0 of test_log_i0 and log 2 rounded to 0.6931 as the reference writes it.
     to = 0.4
     val = integrate.quad(lambda t: mp_textbook(t, to, v) * math.cos(t), -PI / 2, PI / 2, limit=400,
                          epsabs=1e-13)[0]
@@ -107,9 +107,10 @@ def test_mp_is_textbook_lobe_with_truncated_i0(v):
 @pytest.mark.parametrize("v", [0.03, 0.08, 0.2, 0.8])
 @pytest.mark.parametrize("theta_o", [0.0, 0.5, -1.0])
 def test_mp_mass(v, theta_o):
-    # what the truncations lose: under 1% of the lobe, never a gain
+    # unit mass, less what log 2 rounded to 0.6931 loses (4.7e-5) in the narrow
+    # lobes and the asymptote's O(a^-3) remainder
     m = mp_mass(theta_o, v)
-    assert 0.99 < m < 1.0 + 1e-6, m
+    assert abs(m - 1.0) < 1e-4, m
 
 
 @pytest.mark.parametrize("s", [0.05, 0.2, 0.6, 1.5])
@@ -144,20 +145,20 @@ def test_white_furnace(beta, theta_o, h):
     # colour 1 maps to sigma_a = 0 (shaders/Hair.hlsli:129-131): the four lobes'
     # attenuations then sum to one and every azimuthal lobe has unit mass, so
     # the integral of f |cos| over the sphere is a convex combination of the
-    # longitudinal masses: one, less the truncation losses of test_mp_mass.
+    # longitudinal masses: one, to the 1e-4 of test_mp_mass.
     # The same holds for the integral of the pdf.
     m = hair_mats(1, beta_m=beta, beta_n=beta)
     wi = hair_dir(theta_o, 0.3)
     e, p = _furnace(m, wi, h)
     assert np.allclose(e, e[0], rtol=1e-12)
-    assert 0.99 < e[0] < 1.0 + 1e-4, e
-    assert 0.99 < p < 1.0 + 1e-4, p
+    assert abs(e[0] - 1.0) < 1e-4, e
+    assert abs(p - 1.0) < 1e-4, p
 
 
 def test_pdf_mass_with_absorption():
     m = hair_mats(1, color=(0.6, 0.35, 0.2))
     e, p = _furnace(m, hair_dir(0.3, -1.0), 0.4)
-    assert 0.99 < p < 1.0 + 1e-4
+    assert abs(p - 1.0) < 1e-4
     assert (e < p).all() and (e > 0.0).all() and e[0] > e[1] > e[2]
 
 
@@ -273,8 +274,8 @@ def test_attenuation_closed_form(color):
     masses = [mp_mass(-2 * alpha, v0), mp_mass(alpha, v0 / 4), mp_mass(4 * alpha, 4 * v0), mp_mass(0.0, 4 * v0)]
     expect = sum(a * mm for a, mm in zip(A, masses))
     m = hair_mats(1, beta_m=bm, beta_n=bn, eta=eta, color=color)
-    # fine theta grid: the log-I0 branch jumps by 0.6% at a = 12 (test_log_i0),
-    # which Gauss-Legendre only resolves with many nodes
+    # fine theta grid: the log-I0 branch switches to its asymptote at a = 12
+    # (a 4e-5 step, test_log_i0), which Gauss-Legendre resolves with many nodes
     e, _ = _furnace(m, hair_dir(0.0, 0.0), 0.0, 1024, 384)
     assert np.allclose(e, expect, rtol=2e-5), (e, expect)
 
@@ -294,7 +295,7 @@ def _first_hits(o, cam):
 def test_hair_furnace_render():
     # Config-3 white furnace (colour 1 = sigma_a 0, hair only, uniform unit
     # envmap): pixels that see no hair are exactly 1, and hair pixels converge to
-    # 1 less the truncated-I0 loss of test_mp_mass per bounce.
+    # 1 (to the 1e-4 per bounce of test_mp_mass).
     sc = ng.generate_scene(3, strands=2000, variant=1, width=48, height=48)
     o = orc.OracleScene(sc)
     img, _ = o.render(sc.camera, 128, 32, seed=3)
@@ -312,4 +313,4 @@ def test_hair_furnace_render():
     far = ~near.reshape(-1)
     assert far.sum() > 200 and np.mean(L[far] == 1.0) > 0.99
     m = L[hit].mean()
-    assert abs(m - 1.0) < 3e-3, m
+    assert abs(m - 1.0) < 1e-3, m
