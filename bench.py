@@ -1,17 +1,24 @@
-"""Benchmark: Msamples/s of the MI355X wavefront path tracer on the Disney-grid
-configuration (8x8 level-6 icospheres = 5,242,882 triangles on a ground quad,
-2048x1024 sky+sun envmap, 512x512, 256 spp, max depth 12), plain path_mis.
+"""Benchmark: Msamples/s of the MI355X wavefront path tracer on the batched
+large scene, and time to relMSE <= 1e-3 (validation_tests/compare_exr.py:102),
+guided vs plain, on the Disney grid.
 
-A step renders the configuration's whole job once: spp_per_step (default 256)
-samples per pixel of the 512x512 view in one persistent render call; sample
-offsets advance from step to step, so every step draws new samples.
-Inputs are resident in HBM before the timed region; the timed region is K
-render calls, each ending in a stream synchronisation.  After it, a second leg
-measures the GPU time to relMSE <= 1e-3 (validation_tests/compare_exr.py:102)
-of plain and guided rendering against a high-spp plain reference.
+Headline (`value`): the batched textured scene, the largest workload that fits
+one GPU: 10,000,448 triangles (a 2048^2-cell heightfield terrain, 199 displaced
+icospheres, 20,000 alpha-tested hair cards), 128 procedural 1024^2 RGBA8
+textures (64 albedo, 64 tangent-space normal maps) plus the cards' alpha map,
+mixed Disney / Lambert / dielectric materials, a thin-lens camera (aperture
+radius 0.05, focus distance 5), sixteen views from 256^2 to 1920x1080 at 16 to
+256 spp rendered in one persistent render call, max depth 12, plain path_mis.
+A step renders all sixteen views once (1.85 G samples); sample offsets advance
+from step to step, so every step draws new samples.  Inputs are resident in
+HBM before the timed region; the timed region is K render calls, each ending
+in a stream synchronisation.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is not supported: this
-build runs on one GPU.
+Secondary legs: the Disney icosphere grid (5,242,882 triangles, 2048x1024
+sky+sun envmap, 512x512, 256 spp, max depth 12) plain; and the time to relMSE
+<= 1e-3 of plain and guided rendering of that grid at its 512x512 resolution.
+
+Prints one JSON line.  --gpus N > 1 is not supported: this build runs on one GPU.
 """
 import argparse
 import json
@@ -27,37 +34,74 @@ sys.path.insert(0, os.path.join(ROOT, "pbrt-renderer-_amd"))
 METRIC = "Msamples/s; time-to-relMSE≤1e-3, guided vs plain, on 1 MI355X"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES = 80, 48
-# per extension ray besides the traversal: ray read 32 B, hit write 16 B, queue index 4 B
+# per extension ray besides the traversal: ray read 32 B, hit write 16 B, class-queue entry 4 B
 EXT_IO_BYTES = 32 + 16 + 4
-# per shadow ray besides the traversal: origin/direction read 32 B
-SHADOW_IO_BYTES = 32
+# per alpha-tested candidate: 3 vertex indices 12 B, 3 UVs 24 B, a 2x2 texel footprint 16 B
+ALPHA_TEST_BYTES = 12 + 24 + 16
+BATCHED_SIZES = [(256, 256), (320, 240), (400, 300), (480, 270), (512, 512), (640, 360), (640, 480), (800, 450),
+                 (800, 600), (960, 540), (1024, 576), (1024, 768), (1280, 720), (1366, 768), (1600, 900),
+                 (1920, 1080)]
+BATCHED_SPP = list(range(16, 257, 16))
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, cam, spp, depth):
-    """The CPU oracle (path_mis restated on the CPU, Nori BVH2) on a bounded sample."""
+def host_cpu():
+    """Host cores this process may use, the machine's count and CPU model.  The
+    GPU box grants one GPU's share of its host (OMP_NUM_THREADS, 16 there); the
+    CPU baseline runs on every core of that share."""
+    nproc = os.cpu_count() or 1
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(usable, share) if share > 0 else usable
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": nproc, "affinity": usable, "share": share or None, "cpu_model": model}
+
+
+def cpu_baseline(scene, views, depth, budget_s):
+    """The CPU oracle (path_mis restated on the CPU over Nori's BVH2, 32x32
+    blocks on a std::thread pool) on a bounded sample of the same workload:
+    the views in order at 1 spp each (the first sample of every pixel) until
+    budget_s of wall time has been spent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
-    threads = min(os.cpu_count() or 1, 16)
+    threads, info = host_cpu()
     t0 = time.time()
     o = orc.OracleScene(scene)
     log(f"[bench] oracle scene (Nori BVH2) built in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    _, stats = o.render(cam, spp, depth, seed=1, nthreads=threads)
-    dt = time.time() - t0
-    return {"value": float(stats[0]) / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"{cam.width}x{cam.height} at {spp} spp (first {spp} samples of every pixel), "
-                      f"{int(stats[0])} samples in {dt:.2f}s"}
+    samples, dt, done = 0, 0.0, []
+    for k, v in enumerate(views):
+        t0 = time.time()
+        _, stats = o.render(v.camera, 1, depth, seed=1, nthreads=threads)
+        dt += time.time() - t0
+        samples += int(stats[0])
+        done.append(f"{v.camera.width}x{v.camera.height}")
+        if dt > budget_s:
+            break
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"views {', '.join(done)} at 1 spp (the first sample of every pixel), {samples} samples in "
+                      f"{dt:.2f}s", **info}
 
 
-def traffic_from_profiles():
-    """HBM bytes per extension ray of the closest-hit kernel from the committed rocprof counter summary
-    (its line `hbm_bytes_per_ray: N`: FETCH_SIZE + WRITE_SIZE per launch, scaled by the copy calibration,
-    divided by the rays of those launches)."""
-    path = os.path.join(ROOT, "profiles", "k_extend_disney_grid_traffic.md")
+def traffic_from_profiles(name):
+    """HBM bytes per extension ray of the closest-hit kernel from a committed
+    rocprof counter summary (its line `hbm_bytes_per_ray: N`: FETCH_SIZE +
+    WRITE_SIZE per launch, scaled by the copy calibration, divided by the rays
+    of those launches)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -84,11 +128,106 @@ def calibrate_copy(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
 
 
-def time_to_rel_mse(gpu, ng, cam, depth, ref, start_spp, growth, max_spp, thresh, guided, seed):
-    """Progressive render, checking relMSE against `ref` after 1, growth, growth^2, ...
-    times start_spp samples per pixel, until relMSE <= thresh.  Returns the GPU
-    milliseconds of the renders (every kernel, guiding inference and training
-    included), the spp reached and the convergence curve."""
+def measure(gpu, ng, make_views, steps, warmup, tune, tune_views, pool, chunk, label, timing_steps):
+    """Untimed autotune (or the given pool/chunk), an untimed traversal-count
+    pass, `warmup` untimed steps, `steps` timed steps (graph-replayed bounce
+    loop), then the first `timing_steps` of those steps again with HIP events
+    around every launch on the renderer's stream (kernel times)."""
+    gpu.set_option("batch", pool)
+    gpu.set_option("chunk_iters", chunk)
+    tuned = None
+    if tune:
+        r = gpu.autotune(tune_views, calib_spp=tune_views[0].spp)
+        tuned = {"pool": r.best_pool, "chunk_iters": r.best_chunk,
+                 "candidates": [[r.pool[k], r.chunk[k], round(r.msamples_per_s[k], 1)] for k in range(r.count)]}
+        log(f"[bench] {label} autotune: pool {r.best_pool}, chunk {r.best_chunk} "
+            f"({r.best_msamples_per_s:.1f} Msamples/s on the calibration renders)")
+    gpu.set_option("count_traversal", 1)
+    gpu.reset_film()
+    gpu.render(make_views(10 ** 6, True), seed=99, readback=False)
+    cs = gpu.stats()
+    gpu.set_option("count_traversal", 0)
+    per = lambda a, b: a / max(b, 1)  # noqa: E731
+    trav = {"ext_nodes_per_ray": per(cs.ext_nodes, cs.extension_rays), "ext_tris_per_ray": per(cs.ext_tris,
+                                                                                                 cs.extension_rays),
+            "ext_alpha_tests_per_ray": per(cs.ext_alpha_tests, cs.extension_rays),
+            "shadow_nodes_per_ray": per(cs.shadow_nodes, cs.shadow_rays),
+            "shadow_tris_per_ray": per(cs.shadow_tris, cs.shadow_rays),
+            "shadow_alpha_tests_per_ray": per(cs.shadow_alpha_tests, cs.shadow_rays)}
+    log(f"[bench] {label} per extension ray {trav['ext_nodes_per_ray']:.2f} nodes {trav['ext_tris_per_ray']:.2f} "
+        f"tris {trav['ext_alpha_tests_per_ray']:.3f} alpha tests; per shadow ray {trav['shadow_nodes_per_ray']:.2f} "
+        f"nodes {trav['shadow_tris_per_ray']:.2f} tris")
+    gpu.reset_film()
+    for w in range(warmup):
+        gpu.render(make_views((10 ** 6) + 64 + w * 256, False), seed=7, readback=False)
+        log(f"[bench] {label} warmup step {w + 1}/{warmup} done")
+    gpu.reset_film()
+    acc = {"samples": 0, "ext": 0, "sh": 0, "ms_gpu": 0.0}
+    t0 = time.perf_counter()
+    for k in range(steps):
+        gpu.render(make_views(k * 256, False), seed=7, readback=False)
+        s = gpu.stats()
+        acc["samples"] += s.samples
+        acc["ext"] += s.extension_rays
+        acc["sh"] += s.shadow_rays
+        acc["ms_gpu"] += s.ms_total
+        log(f"[bench] {label} step {k + 1}/{steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s")
+    wall = time.perf_counter() - t0
+    gpu.set_option("timing_mode", 1)
+    gpu.reset_film()
+    tm = {"ext": 0, "ms_ext": 0.0, "ms_sh": 0.0, "ms_shade": 0.0, "launch_ext": 0, "ms_total": 0.0}
+    for k in range(timing_steps):
+        gpu.render(make_views(k * 256, False), seed=7, readback=False)
+        s = gpu.stats()
+        tm["ext"] += s.extension_rays
+        tm["ms_ext"] += s.ms_trace_closest
+        tm["ms_sh"] += s.ms_trace_shadow
+        tm["ms_shade"] += s.ms_shade
+        tm["launch_ext"] += s.launches_closest
+        tm["ms_total"] += s.ms_total
+        log(f"[bench] {label} timing pass {k + 1}/{timing_steps}: extend {s.ms_trace_closest:.1f} ms, shadow "
+            f"{s.ms_trace_shadow:.1f} ms, shade {s.ms_shade:.1f} ms, total {s.ms_total:.1f} ms")
+    gpu.set_option("timing_mode", 0)
+    ext_b = NODE_BYTES * trav["ext_nodes_per_ray"] + TRI_BYTES * trav["ext_tris_per_ray"] + EXT_IO_BYTES + \
+        ALPHA_TEST_BYTES * trav["ext_alpha_tests_per_ray"]
+    bytes_launch = tm["ext"] * ext_b / max(tm["launch_ext"], 1)
+    ms_launch = tm["ms_ext"] / max(tm["launch_ext"], 1)
+    achieved = bytes_launch / (ms_launch / 1e3) / 1e9 if ms_launch > 0 else None
+    return {
+        "msamples_per_s": acc["samples"] / wall / 1e6, "ms_per_step": wall / steps * 1e3,
+        "samples_per_step": acc["samples"] // max(steps, 1),
+        "mrays_per_s": (acc["ext"] + acc["sh"]) / wall / 1e6,
+        "gpu_ms": {"extend": round(tm["ms_ext"], 2), "shadow": round(tm["ms_sh"], 2),
+                   "shade_raygen": round(tm["ms_shade"], 2), "render_total_timed_pass": round(tm["ms_total"], 2),
+                   "render_total": round(acc["ms_gpu"], 2)},
+        "traversal": {k: round(v, 4) for k, v in trav.items()},
+        "ext_bytes_per_ray": ext_b, "rays_per_launch": tm["ext"] / max(tm["launch_ext"], 1),
+        "bytes_per_launch": bytes_launch, "avg_launch_ms": ms_launch, "achieved_gbs": achieved,
+        "autotune": tuned,
+    }
+
+
+def roofline(m, profile, kernel):
+    per_ray = traffic_from_profiles(profile)
+    return {"kernel": kernel, "bound": "hbm",
+            "achieved": None if m["achieved_gbs"] is None else round(m["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": None if m["achieved_gbs"] is None else round(m["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": None if per_ray is None else round(per_ray * m["rays_per_launch"]),
+            "bytes_per_ray": round(m["ext_bytes_per_ray"], 1), "bytes_per_launch": round(m["bytes_per_launch"]),
+            "avg_launch_ms": round(m["avg_launch_ms"], 4)}
+
+
+def batched_views(ng, cam):
+    return [ng.make_view(ng.lookat_camera_like(cam, w, h), spp, 12) for (w, h), spp in zip(BATCHED_SIZES, BATCHED_SPP)]
+
+
+def time_to_rel_mse(gpu, ng, cam, depth, ref, ref_err, start_spp, max_spp, thresh, guided, seed):
+    """Progressive render against `ref` at 1, 2, 4, ... times start_spp
+    samples per pixel, until the relMSE (less the reference's own error
+    ref_err) reaches thresh or max_spp.  Returns the GPU milliseconds of the
+    renders (every kernel, guiding inference and training included), the spp
+    reached and the curve; past the last point the time to thresh is
+    extrapolated with relMSE ~ 1/N (labelled)."""
     gpu.set_option("guiding", 1 if guided else 0)
     if guided:
         gpu.set_option("guide_reset", 1)
@@ -96,46 +235,41 @@ def time_to_rel_mse(gpu, ng, cam, depth, ref, start_spp, growth, max_spp, thresh
     ms, spp, curve, e = 0.0, 0, [], float("inf")
     target = start_spp
     while spp < max_spp:
-        chunk = min(max(target - spp, 1), max_spp - spp)
+        chunk = min(target - spp, max_spp - spp)
         img = gpu.render(ng.make_view(cam, chunk, depth, sample_offset=spp), seed=seed)
         ms += gpu.stats().ms_total
         spp += chunk
-        e = rel_mse(img[:, :3] / img[:, 3:4], ref)
+        e = max(rel_mse(img[:, :3] / img[:, 3:4], ref) - ref_err, 0.0)
         curve.append((spp, round(ms, 2), float(f"{e:.4g}")))
         if e <= thresh:
             break
-        target = int(target * growth)
+        target *= 2
     gpu.set_option("guiding", 0)
-    return {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": e <= thresh, "curve": curve}
+    reached = e <= thresh
+    return {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": reached,
+            "ms_to_threshold": round(ms if reached else ms * e / thresh, 1), "extrapolated": not reached,
+            "curve": curve}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp-per-step", type=int, default=256)
-    ap.add_argument("--depth", type=int, default=12)
-    ap.add_argument("--subdiv", type=int, default=6)
-    ap.add_argument("--batch", type=int, default=1 << 21)
     ap.add_argument("--no-autotune", action="store_true", help="keep --batch and --chunk-iters")
-    ap.add_argument("--chunk-iters", type=int, default=8, help="pool iterations per done-flag check (--no-autotune)")
-    ap.add_argument("--batched", action="store_true",
-                    help="also render the batched textured scene (16 views, 256^2 .. 1920x1080, 16 .. 256 spp) in "
-                         "one persistent render call and report its throughput")
-    ap.add_argument("--guided", action="store_true",
-                    help="time guided rendering (online-trained network) instead of plain path_mis")
-    ap.add_argument("--tune-spp", type=int, default=0,
-                    help="spp of the autotuner's calibration renders (0: one step's spp, so the candidates run "
-                         "the timed steps' workload)")
+    ap.add_argument("--batch", type=int, default=1 << 24, help="path pool (--no-autotune)")
+    ap.add_argument("--chunk-iters", type=int, default=2, help="pool iterations per done-flag check (--no-autotune)")
+    ap.add_argument("--only", choices=["batched", "disney"], default=None,
+                    help="run one workload leg only (profiling); the JSON line then carries that leg as value")
+    ap.add_argument("--disney-steps", type=int, default=4)
+    ap.add_argument("--timing-steps", type=int, default=1,
+                    help="steps of the kernel-timing pass (HIP events around every launch)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-spp", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="wall-time budget of the CPU baseline sample")
     ap.add_argument("--no-time-to-mse", action="store_true", help="skip the time-to-relMSE leg")
-    ap.add_argument("--mse-res", type=int, default=64, help="image side of the time-to-relMSE leg")
-    ap.add_argument("--mse-ref-spp", type=int, default=1 << 20)
-    ap.add_argument("--mse-start-spp", type=int, default=256)
-    ap.add_argument("--mse-growth", type=float, default=1.25)
-    ap.add_argument("--mse-max-spp", type=int, default=1 << 18)
+    ap.add_argument("--mse-ref-spp", type=int, default=1 << 14)
+    ap.add_argument("--mse-start-spp", type=int, default=64)
+    ap.add_argument("--mse-max-spp", type=int, default=1 << 12)
     ap.add_argument("--mse-threshold", type=float, default=1e-3)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES bytes (counter calibration) and exit")
@@ -150,206 +284,124 @@ def main():
 
     import nori_gpu as ng
 
-    t0 = time.time()
-    scene = ng.generate_scene(2, subdiv=args.subdiv)
-    cam = scene.camera
-    log(f"[bench] scene: {len(scene.indices)} triangles, envmap {scene.env_rgb.shape[1]}x{scene.env_rgb.shape[0]}, "
-        f"{time.time() - t0:.1f}s")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, cam, args.cpu_spp, args.depth)
-        log(f"[bench] cpu baseline {cpu['value']:.3f} Msamples/s on {cpu['cores']} threads")
-
     gpu = ng.NoriGPU(0)
-    t0 = time.time()
-    gpu.upload(scene)
-    info = gpu.bvh_info()
-    log(f"[bench] upload: BVH8 {info.nodes} nodes depth {info.max_depth} ({info.node_bytes / 1e6:.1f} MB nodes, "
-        f"{info.tri_bytes / 1e6:.1f} MB triangles), build {info.build_ms:.0f} ms, total {time.time() - t0:.1f}s")
-    gpu.set_option("batch", args.batch)
-    gpu.set_option("chunk_iters", args.chunk_iters)
-    spp = args.spp_per_step
-    tune = None
-    if not args.no_autotune:
-        # Untimed: pool size and chunk length for this view, from short renders.
-        tspp = args.tune_spp or spp
-        r = gpu.autotune(ng.make_view(cam, tspp, args.depth), calib_spp=tspp)
-        tune = {"pool": r.best_pool, "chunk_iters": r.best_chunk,
-                "candidates": [[r.pool[k], r.chunk[k], round(r.msamples_per_s[k], 1)] for k in range(r.count)]}
-        log(f"[bench] autotune: pool {r.best_pool}, chunk {r.best_chunk} "
-            f"({r.best_msamples_per_s:.1f} Msamples/s on the calibration renders)")
-
-    # Traversal counts of this workload (instrumented kernels, untimed): average
-    # BVH8 nodes and triangles per extension and per shadow ray.
-    gpu.set_option("count_traversal", 1)
-    gpu.reset_film()
-    gpu.render(ng.make_view(cam, 4, args.depth, sample_offset=10 ** 6), seed=99, readback=False)
-    cs = gpu.stats()
-    gpu.set_option("count_traversal", 0)
-    ext_nodes = cs.ext_nodes / max(cs.extension_rays, 1)
-    ext_tris = cs.ext_tris / max(cs.extension_rays, 1)
-    sh_nodes = cs.shadow_nodes / max(cs.shadow_rays, 1)
-    sh_tris = cs.shadow_tris / max(cs.shadow_rays, 1)
-    log(f"[bench] per extension ray {ext_nodes:.2f} nodes {ext_tris:.2f} tris; per shadow ray {sh_nodes:.2f} nodes "
-        f"{sh_tris:.2f} tris")
-
-    if args.guided:
-        gpu.set_option("guiding", 1)
-    gpu.reset_film()
-    for w in range(args.warmup):
-        gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=(10 ** 6) + 64 + w * spp), seed=7,
-                   readback=False)
-        log(f"[bench] warmup step {w + 1}/{args.warmup} done")
-    gpu.reset_film()
-
-    # Timed region: K render calls (graph-replayed bounce loop, no per-kernel events).
-    acc = {"samples": 0, "ext": 0, "sh": 0, "ms_ext": 0.0, "ms_sh": 0.0, "ms_shade": 0.0, "launch_ext": 0,
-           "ms_gpu": 0.0}
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=k * spp), seed=7, readback=False)
-        s = gpu.stats()
-        acc["samples"] += s.samples
-        acc["ext"] += s.extension_rays
-        acc["sh"] += s.shadow_rays
-        acc["ms_gpu"] += s.ms_total
-        log(f"[bench] step {k + 1}/{args.steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s")
-    wall = time.perf_counter() - t0
-
-    # Kernel times: the same K steps again with HIP events around every launch
-    # on the renderer's stream (eager launches; events inside a graph are not used).
-    gpu.set_option("timing_mode", 1)
-    gpu.reset_film()
-    acc["ext_timed"] = 0
-    for k in range(args.steps):
-        gpu.render(ng.make_view(cam, spp, args.depth, sample_offset=k * spp), seed=7, readback=False)
-        s = gpu.stats()
-        acc["ext_timed"] += s.extension_rays
-        acc["ms_ext"] += s.ms_trace_closest
-        acc["ms_sh"] += s.ms_trace_shadow
-        acc["ms_shade"] += s.ms_shade
-        acc["launch_ext"] += s.launches_closest
-        log(f"[bench] timing pass {k + 1}/{args.steps}: extend {s.ms_trace_closest:.1f} ms, shadow "
-            f"{s.ms_trace_shadow:.1f} ms, shade {s.ms_shade:.1f} ms, total {s.ms_total:.1f} ms")
-    gpu.set_option("timing_mode", 0)
-    gpu.set_option("guiding", 0)
-
-    value = acc["samples"] / wall / 1e6
-
-    ext_bytes = acc["ext_timed"] * (NODE_BYTES * ext_nodes + TRI_BYTES * ext_tris + EXT_IO_BYTES)
-    achieved = ext_bytes / (acc["ms_ext"] / 1e3) / 1e9 if acc["ms_ext"] > 0 else None
-    traffic_per_ray = traffic_from_profiles()
-    rays_per_launch = acc["ext_timed"] / max(acc["launch_ext"], 1)
-    traffic = None if traffic_per_ray is None else round(traffic_per_ray * rays_per_launch)
-
-    ttm = None
-    if not args.no_time_to_mse:
-        # The same view at mse_res x mse_res: a plain reference at mse_ref_spp
-        # (independent seed), then plain and guided (network trained online from
-        # its initialisation) progressive renders until relMSE <= threshold.
-        # Plain path tracing of this scene is dominated by rare high-energy paths
-        # (sun caustics off glossy spheres), so 1e-3 takes ~1e5 spp; the leg runs
-        # at reduced resolution to keep the reference converged and the run short.
-        gpu.set_option("timing_mode", 0)
-        mcam = ng.lookat_camera_like(cam, args.mse_res, args.mse_res)
-        gpu.reset_film()
+    legs = {}
+    cpu = None
+    if args.only in (None, "batched"):
         t0 = time.time()
-        refimg = gpu.render(ng.make_view(mcam, args.mse_ref_spp, args.depth, sample_offset=1 << 24), seed=1234)
-        ref = refimg[:, :3] / refimg[:, 3:4]
-        log(f"[bench] relMSE reference: {args.mse_res}x{args.mse_res} at {args.mse_ref_spp} spp in "
+        sc = ng.generate_scene(5)
+        views = batched_views(ng, sc.camera)
+        log(f"[bench] batched scene: {len(sc.indices)} triangles, {len(sc.textures)} textures, "
+            f"{sum(v.camera.width * v.camera.height * v.spp for v in views)} samples per step, "
             f"{time.time() - t0:.1f}s")
-        legs = {}
-        for name, guided, seed in (("plain", False, 77), ("guided", True, 78)):
-            legs[name] = time_to_rel_mse(gpu, ng, mcam, args.depth, ref, args.mse_start_spp, args.mse_growth,
-                                         args.mse_max_spp, args.mse_threshold, guided, seed)
-            r = legs[name]
-            log(f"[bench] {name}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
-        # Where the threshold is not reached within mse_max_spp, the time to it
-        # is extrapolated from the last point with relMSE ~ 1/N (labelled).
-        for r in legs.values():
-            r["ms_to_threshold"] = r["ms"] if r["reached"] else round(r["ms"] * r["rel_mse"] / args.mse_threshold, 1)
-            r["extrapolated"] = not r["reached"]
-        ttm = {"threshold": args.mse_threshold, "image": f"{args.mse_res}x{args.mse_res}", "ref_spp": args.mse_ref_spp,
-               "plain": legs["plain"], "guided": legs["guided"],
-               "speedup_guided_over_plain": round(legs["plain"]["ms_to_threshold"] / legs["guided"]["ms_to_threshold"],
-                                                  3)}
+        if not args.no_cpu_baseline:
+            cpu = cpu_baseline(sc, views, 12, args.cpu_seconds)
+            log(f"[bench] cpu baseline {cpu['value']:.3f} Msamples/s on {cpu['cores']} threads "
+                f"(nproc {cpu['nproc']}, {cpu['cpu_model']})")
+        t0 = time.time()
+        gpu.upload(sc)
+        info = gpu.bvh_info()
+        log(f"[bench] upload: BVH8 {info.nodes} nodes depth {info.max_depth}, build {info.build_ms:.0f} ms, "
+            f"total {time.time() - t0:.1f}s")
 
+        def bviews(off, count_pass):
+            return [ng.make_view(v.camera, 1 if count_pass else v.spp, v.max_depth, sample_offset=off)
+                    for v in views]
+
+        tune_views = [ng.make_view(v.camera, 8, v.max_depth, sample_offset=0x40000000) for v in views]
+        m = measure(gpu, ng, bviews, args.steps, args.warmup, not args.no_autotune, tune_views, args.batch,
+                    args.chunk_iters, "batched", min(args.timing_steps, args.steps))
+        m["roofline"] = roofline(m, "k_extend_batched_traffic.md", "k_extend<ALPHA> (BVH8 closest hit, alpha test)")
+        m["workload"] = (f"batched textured scene, {len(sc.indices):,} triangles (2048^2 heightfield, 199 displaced "
+                         f"icospheres, 20,000 alpha-tested cards), 128 1024^2 RGBA8 textures, thin lens r 0.05 "
+                         f"focus 5, 16 views 256^2 .. 1920x1080 at 16 .. 256 spp in one render call, max depth 12, "
+                         f"plain path_mis")
+        legs["batched"] = m
+        del sc
+
+    if args.only in (None, "disney"):
+        t0 = time.time()
+        sc = ng.generate_scene(2)
+        cam = sc.camera
+        log(f"[bench] Disney grid: {len(sc.indices)} triangles, envmap {sc.env_rgb.shape[1]}x"
+            f"{sc.env_rgb.shape[0]}, {time.time() - t0:.1f}s")
+        gpu.upload(sc)
+        if cpu is None and not args.no_cpu_baseline:
+            cpu = cpu_baseline(sc, [ng.make_view(cam, 1, 12)] * 64, 12, args.cpu_seconds)
+
+        def dviews(off, count_pass):
+            return [ng.make_view(cam, 4 if count_pass else 256, 12, sample_offset=off)]
+
+        steps = args.steps if args.only == "disney" else args.disney_steps
+        m = measure(gpu, ng, dviews, steps, args.warmup, not args.no_autotune, [ng.make_view(cam, 256, 12)],
+                    args.batch, args.chunk_iters, "disney", min(args.timing_steps, steps))
+        m["roofline"] = roofline(m, "k_extend_disney_grid_traffic.md", "k_extend (BVH8 closest hit)")
+        m["workload"] = (f"Disney icosphere grid 8x8, level 6, {len(sc.indices):,} triangles, 2048x1024 sky+sun "
+                         f"envmap, 512x512, 256 spp per step, max depth 12, plain path_mis")
+        legs["disney"] = m
+        if not args.no_time_to_mse and args.only is None:
+            # Reference: two independent plain renders of mse_ref_spp/2 each;
+            # their mutual relMSE is four times the full reference's own error,
+            # which is subtracted from each leg's measured relMSE.
+            gpu.set_option("guiding", 0)
+            half = args.mse_ref_spp // 2
+            refs = []
+            t0 = time.time()
+            for k in range(2):
+                gpu.reset_film()
+                img = gpu.render(ng.make_view(cam, half, 12, sample_offset=(1 << 24) + k * half), seed=1234 + k)
+                refs.append(img[:, :3] / img[:, 3:4])
+            ref = 0.5 * (refs[0] + refs[1])
+            ref_err = rel_mse(refs[0], refs[1]) / 4
+            log(f"[bench] relMSE reference: 512x512 at {args.mse_ref_spp} spp in {time.time() - t0:.1f}s, own "
+                f"relMSE {ref_err:.3g}")
+            tt = {}
+            for name, guided, seed in (("plain", False, 77), ("guided", True, 78)):
+                tt[name] = time_to_rel_mse(gpu, ng, cam, 12, ref, ref_err, args.mse_start_spp, args.mse_max_spp,
+                                           args.mse_threshold, guided, seed)
+                r = tt[name]
+                log(f"[bench] {name}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
+            legs["time_to_rel_mse"] = {
+                "threshold": args.mse_threshold, "image": "512x512", "ref_spp": args.mse_ref_spp,
+                "ref_rel_mse": ref_err, "plain": tt["plain"], "guided": tt["guided"],
+                "speedup_guided_over_plain": round(tt["plain"]["ms_to_threshold"] / tt["guided"]["ms_to_threshold"],
+                                                   3)}
+
+    head = legs["batched"] if "batched" in legs else legs["disney"]
     result = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(head["msamples_per_s"], 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(head["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": f"Disney icosphere grid 8x8, level {args.subdiv}, {len(scene.indices)} triangles, "
-                        f"2048x1024 sky+sun envmap, {cam.width}x{cam.height}, {spp} spp per step, "
-                        f"max depth {args.depth}, {'guided' if args.guided else 'plain'} path_mis",
-            "spp_per_step": spp,
-            "path_pool": tune["pool"] if tune else args.batch,
-            "autotune": tune,
-        },
-        "mrays_per_s": round((acc["ext"] + acc["sh"]) / wall / 1e6, 2),
-        "gpu_ms": {"extend": round(acc["ms_ext"], 2), "shadow": round(acc["ms_sh"], 2),
-                   "shade_raygen": round(acc["ms_shade"], 2), "render_total": round(acc["ms_gpu"], 2)},
-        "traversal": {"ext_nodes_per_ray": round(ext_nodes, 3), "ext_tris_per_ray": round(ext_tris, 3),
-                      "shadow_nodes_per_ray": round(sh_nodes, 3), "shadow_tris_per_ray": round(sh_tris, 3)},
-        "roofline": {
-            "kernel": "k_extend (BVH8 closest-hit traversal)",
-            "bound": "hbm",
-            "achieved": None if achieved is None else round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": None if achieved is None else round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_launch": round(ext_bytes / max(acc["launch_ext"], 1)),
-            "avg_launch_ms": round(acc["ms_ext"] / max(acc["launch_ext"], 1), 4),
-        },
+        "config": {"workload": head["workload"], "samples_per_step": head["samples_per_step"],
+                   "path_pool": head["autotune"]["pool"] if head["autotune"] else args.batch,
+                   "autotune": head["autotune"]},
+        "mrays_per_s": round(head["mrays_per_s"], 2),
+        "gpu_ms": head["gpu_ms"],
+        "traversal": head["traversal"],
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
-        "time_to_rel_mse": ttm,
     }
     if cpu:
-        result["gpu_over_cpu"] = round(value / cpu["value"], 1)
-    if args.batched:
-        result["batched"] = batched_leg(gpu, args, log)
+        result["gpu_over_cpu"] = round(head["msamples_per_s"] / cpu["value"], 1)
+    if "batched" in legs and "disney" in legs:
+        d = legs["disney"]
+        result["disney_grid"] = {"workload": d["workload"], "msamples_per_s": round(d["msamples_per_s"], 3),
+                                 "ms_per_step": round(d["ms_per_step"], 3), "mrays_per_s": round(d["mrays_per_s"], 2),
+                                 "gpu_ms": d["gpu_ms"], "traversal": d["traversal"], "roofline": d["roofline"],
+                                 "autotune": d["autotune"]}
+    if "time_to_rel_mse" in legs:
+        result["time_to_rel_mse"] = legs["time_to_rel_mse"]
     print(json.dumps(result), flush=True)
     gpu.close()
-
-
-def batched_leg(gpu, args, log):
-    """The batched textured scene (9.4 M triangles, 128 1024^2 RGBA8 textures,
-    alpha-tested cards, thin lens): sixteen views of 256^2 .. 1920x1080 at
-    16 .. 256 spp in one persistent render call, after an untimed autotune on
-    the same views at 8 spp each."""
-    import nori_gpu as ng
-    t0 = time.time()
-    sc = ng.generate_scene(5)
-    log(f"[bench] batched scene: {len(sc.indices)} triangles, {len(sc.textures)} textures, "
-        f"{time.time() - t0:.1f}s")
-    gpu.upload(sc)
-    sizes = [(256, 256), (320, 240), (400, 300), (480, 270), (512, 512), (640, 360), (640, 480), (800, 450),
-             (800, 600), (960, 540), (1024, 576), (1024, 768), (1280, 720), (1366, 768), (1600, 900), (1920, 1080)]
-    spps = list(range(16, 257, 16))
-    views = [ng.make_view(ng.lookat_camera_like(sc.camera, w, h), spp, args.depth) for (w, h), spp in zip(sizes, spps)]
-    r = gpu.autotune(views, calib_spp=8)
-    samples = sum(v.camera.width * v.camera.height * v.spp for v in views)
-    gpu.reset_film()
-    gpu.render(views, seed=3, readback=False)  # warm-up (graph capture)
-    t = time.perf_counter()
-    gpu.render([ng.make_view(v.camera, v.spp, v.max_depth, sample_offset=v.spp) for v in views], seed=3,
-               readback=False)
-    ms = (time.perf_counter() - t) * 1e3
-    log(f"[bench] batched: {samples / ms / 1e3:.1f} Msamples/s over {samples} samples in {ms:.0f} ms")
-    return {"workload": "batched textured scene, 9.4 M triangles, 128 x 1024^2 textures, 16 views 256^2 .. "
-                        "1920x1080 at 16 .. 256 spp, one render call", "samples": samples, "ms": round(ms, 1),
-            "msamples_per_s": round(samples / ms / 1e3, 2), "path_pool": r.best_pool, "chunk_iters": r.best_chunk}
 
 
 if __name__ == "__main__":

@@ -253,6 +253,9 @@ typedef struct nori_render_stats {
     double ms_guide_infer;    /* guiding features + network inference          */
     double ms_guide_train;    /* guiding sample finalisation + training steps   */
     uint64_t guide_train_steps;
+    /* alpha-tested candidates (texture lookups of the any-hit filter) of the
+     * extension and the shadow rays, summed; with "count_traversal" only */
+    uint64_t ext_alpha_tests, shadow_alpha_tests;
 } nori_render_stats;
 int nori_gpu_render_stats(const nori_ctx *ctx, nori_render_stats *out);
 

@@ -107,15 +107,22 @@ struct nori_ctx {
     DevBuf views_buf;
 
     // The bounce loop as a graph of two iterations (even, odd), kept while
-    // the buffers it was captured on stay the same.
+    // nothing it was captured with changes: no context buffer (re)allocated
+    // since (buf_gen counts every allocation of alloc()), the same scene, and
+    // the same values baked into its kernel arguments.  Buffer addresses are
+    // not compared: a buffer freed and allocated again can return at the same
+    // address while another captured one moved.
+    uint64_t buf_gen = 0;
     struct GraphKey {
-        const void *paths, *film, *views, *guide_out, *guide_rec, *stats, *scene;
-        uint32_t cap, sampler;
-        bool aov;
+        uint64_t bufs, scene, npix_all;
+        uint32_t pool, cap, sampler, max_rec, guide_cap, rec_cap;
+        const void *stats;
+        bool aov, guiding, sorted, persistent;
         bool operator==(const GraphKey &o) const {
-            return paths == o.paths && film == o.film && views == o.views && guide_out == o.guide_out &&
-                   guide_rec == o.guide_rec && stats == o.stats && scene == o.scene && cap == o.cap &&
-                   sampler == o.sampler && aov == o.aov;
+            return bufs == o.bufs && scene == o.scene && npix_all == o.npix_all && pool == o.pool && cap == o.cap &&
+                   sampler == o.sampler && max_rec == o.max_rec && guide_cap == o.guide_cap && rec_cap == o.rec_cap &&
+                   stats == o.stats && aov == o.aov && guiding == o.guiding && sorted == o.sorted &&
+                   persistent == o.persistent;
         }
     };
     hipGraphExec_t graph = nullptr;
@@ -243,6 +250,7 @@ struct nori_ctx {
     void *alloc(DevBuf &b, size_t bytes) {
         if (b.bytes >= bytes && b.p) return b.p;
         b.release();
+        ++buf_gen; // a captured graph may hold the old address
         HIPCHK(hipMalloc(&b.p, bytes ? bytes : 16));
         b.bytes = bytes;
         return b.p;
@@ -451,9 +459,10 @@ int nori_gpu_create(nori_ctx **out, int device) {
     int rc = guard(c, [&] {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         c->alloc(c->counters, 16 * sizeof(uint32_t));
-        c->alloc(c->stats, 12 * sizeof(unsigned long long));
+        // 0..7 render counters, 8..11 sample claims (Counters::next), 12..13 alpha-test candidates
+        c->alloc(c->stats, 16 * sizeof(unsigned long long));
         HIPCHK(hipMemset(c->counters.p, 0, 16 * sizeof(uint32_t)));
-        HIPCHK(hipMemset(c->stats.p, 0, 12 * sizeof(unsigned long long)));
+        HIPCHK(hipMemset(c->stats.p, 0, 16 * sizeof(unsigned long long)));
         c->cdev = Counters{(uint32_t *)c->counters.p, (unsigned long long *)c->stats.p,
                            (unsigned long long *)c->stats.p + 8};
         HIPCHK(hipHostMalloc((void **)&c->h_flags, 2 * sizeof(uint32_t), hipHostMallocDefault));
@@ -762,8 +771,9 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             c->sdev.sobol = c->sampler;
             const bool graphed = c->use_graph && !c->timing_mode;
             if (graphed) {
-                nori_ctx::GraphKey key{c->paths.p, c->film.p, c->views_buf.p, gsd.out, gsd.rec, cs,
-                                       (const void *)(uintptr_t)c->scene_gen, c->pool, c->sampler, c->film_aov};
+                nori_ctx::GraphKey key{c->buf_gen, c->scene_gen, (uint64_t)c->npix_all, c->pool, c->cap, c->sampler,
+                                       gsd.max_rec, c->g_cap, c->g_rec_cap, cs, c->film_aov, gsd.out != nullptr,
+                                       c->sort_shading != 0, c->persistent != 0};
                 if (!c->graph || !(key == c->graph_key)) {
                     c->drop_graph();
                     hipGraph_t g = nullptr;
@@ -849,8 +859,10 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         st.ms_total = ms;
-        unsigned long long hs[8];
+        unsigned long long hs[14];
         HIPCHK(hipMemcpy(hs, c->stats.p, sizeof(hs), hipMemcpyDeviceToHost));
+        st.ext_alpha_tests = hs[12];
+        st.shadow_alpha_tests = hs[13];
         st.extension_rays = hs[0];
         st.shadow_rays = hs[1];
         st.bounces = hs[3];

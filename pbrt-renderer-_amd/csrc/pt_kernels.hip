@@ -273,6 +273,12 @@ __device__ __forceinline__ void count_add(unsigned long long *stats, int k, uint
     }
 }
 
+// Adds this lane's count into stats[k] (one atomic per wave).
+__device__ __forceinline__ void count_add1(unsigned long long *stats, int k, uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(stats + k, (unsigned long long)v);
+}
+
 // Closest hit of the paths at positions [0, *count).  SORT: the position is
 // also appended to the queue of its shading class (light, surface, hair at
 // q_cls, q_cls + cap, q_cls + 2 cap; counts cls_cnt[0..2]).
@@ -334,7 +340,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = *count;
     if ((COUNT || SORT) ? (i - threadIdx.x % 64u) >= n : i >= n) return;
-    uint32_t nn = 0, nt = 0;
+    uint32_t nn = 0, nt = 0, na = 0;
     int cls = 0;
     if (i < n) {
         float4 a = p.ray_o[i], b = p.ray_d[i];
@@ -342,6 +348,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
         if (ALPHA) {
             const uint32_t seed = alpha_seed_path(p.aux[i]);
             auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3, f3) {
+                if (COUNT && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE) ++na;
                 return alpha_accept(s, prim, mesh, u, v, seed, false);
             };
             r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x,
@@ -365,7 +372,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
             if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
         }
     }
-    if (COUNT) count_add(stats, 4, nn, nt);
+    if (COUNT) {
+        count_add(stats, 4, nn, nt);
+        if (ALPHA) count_add1(stats, 12, na);
+    }
 }
 
 // Persistent traversal: a fixed grid of resident waves works through the
@@ -509,7 +519,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = *count;
     if (COUNT ? (i - threadIdx.x % 64u) >= n : i >= n) return;
-    uint32_t nn = 0, nt = 0;
+    uint32_t nn = 0, nt = 0, na = 0;
     if (i < n) {
         float4 a = sh.o[i], b = sh.d[i], e = sh.e[i];
         float trans = 1.0f;
@@ -521,8 +531,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
             const f3 dir = xyz(b);
             const uint32_t seed = s.has_alpha ? alpha_seed_shadow(__float_as_uint(e.z)) : 0u;
             auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3 e1, f3 e2) {
-                if (s.has_alpha && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE)
+                if (s.has_alpha && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE) {
+                    if (COUNT) ++na;
                     return alpha_accept(s, prim, mesh, u, v, seed, true);
+                }
                 if (s.pass_ior) {
                     const float2 ior = s.pass_ior[mesh];
                     if (ior.x > 0.0f) {
@@ -551,7 +563,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
             guide_nee_add(g, __float_as_uint(c.w), col);
         }
     }
-    if (COUNT) count_add(stats, 6, nn, nt);
+    if (COUNT) {
+        count_add(stats, 6, nn, nt);
+        if (PASS) count_add1(stats, 13, na);
+    }
 }
 
 // The shadow rays at queue positions [0, *count), persistent form (RayFeed).
@@ -1447,8 +1462,10 @@ __global__ void k_guide_gather(const GuideSample *__restrict__ samples, const ui
 
 __global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (clear_mask & 0x1000u)
+    if (clear_mask & 0x1000u) {
         for (int k = 0; k < 8; ++k) c.stats[k] = 0;
+        c.stats[12] = c.stats[13] = 0; // alpha-test candidates (extension, shadow)
+    }
     if (ext_from < 2) {
         c.stats[0] += c.cnt[ext_from];
         c.stats[3] += 1;

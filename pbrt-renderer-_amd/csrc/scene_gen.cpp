@@ -515,18 +515,24 @@ void batched_scene(SceneData &s, Builder &b, Rng &rng, const nori_scene_options 
         m.normalTexIndex = 64;
         b.add_mesh(pos, nrm, tri, m, &uv);
     }
-    // displaced icospheres with spherical uv
-    std::vector<float> ipos;
-    std::vector<uint32_t> itri;
+    // displaced icospheres with spherical uv: the first 36 of every 199 one
+    // subdivision level finer (at the defaults 36 x 20,480 + 163 x 5,120
+    // triangles, so the scene holds 10,000,448 with the terrain and the cards)
+    std::vector<float> ipos, fpos;
+    std::vector<uint32_t> itri, ftri;
     icosphere_mesh(level, ipos, itri);
+    icosphere_mesh(level + 1, fpos, ftri);
+    const uint32_t nfine = (nsph * 36 + 198) / 199;
     for (uint32_t k = 0; k < nsph; ++k) {
+        const std::vector<float> &sp = k < nfine ? fpos : ipos;
+        const std::vector<uint32_t> &st = k < nfine ? ftri : itri;
         const float r = 0.5f + 1.5f * rng.next();
         const float x = -40.0f + 80.0f * rng.next(), z = -40.0f + 80.0f * rng.next();
         const f3 c = mk(x, terrain_height(x, z) + r * 0.8f, z);
         const uint32_t ns = 0x51u + k;
-        std::vector<float> pos(ipos.size()), nrm(ipos.size()), uv(ipos.size() / 3 * 2);
-        for (size_t v = 0; v < ipos.size(); v += 3) {
-            const f3 d = mk(ipos[v], ipos[v + 1], ipos[v + 2]);
+        std::vector<float> pos(sp.size()), nrm(sp.size()), uv(sp.size() / 3 * 2);
+        for (size_t v = 0; v < sp.size(); v += 3) {
+            const f3 d = mk(sp[v], sp[v + 1], sp[v + 2]);
             const float rr = r * (0.9f + 0.2f * value_noise(d.x * 3.0f + 7.0f, d.y * 3.0f, d.z * 3.0f, ns));
             const f3 p = c + d * rr;
             pos[v] = p.x; pos[v + 1] = p.y; pos[v + 2] = p.z;
@@ -553,7 +559,7 @@ void batched_scene(SceneData &s, Builder &b, Rng &rng, const nori_scene_options 
             m.intIOR = 1.5f;
             m.extIOR = 1.0f;
         }
-        b.add_mesh(pos, nrm, itri, m, &uv);
+        b.add_mesh(pos, nrm, st, m, &uv);
     }
     // hair cards: 0.08 x 0.8 quads standing on the terrain near the camera's
     // focus distance, alpha-tested with the strands texture

@@ -113,7 +113,8 @@ class RenderStats(ctypes.Structure):
                 ("ms_trace_shadow", ctypes.c_double), ("ms_shade", ctypes.c_double),
                 ("launches_closest", _u64), ("launches_shadow", _u64), ("ext_nodes", _u64), ("ext_tris", _u64),
                 ("shadow_nodes", _u64), ("shadow_tris", _u64), ("ms_guide_infer", ctypes.c_double),
-                ("ms_guide_train", ctypes.c_double), ("guide_train_steps", _u64)]
+                ("ms_guide_train", ctypes.c_double), ("guide_train_steps", _u64), ("ext_alpha_tests", _u64),
+                ("shadow_alpha_tests", _u64)]
 
 
 class TuneResult(ctypes.Structure):

@@ -88,3 +88,30 @@ def test_accumulates_and_requires_the_option(gpu, cornell):
         assert np.all(both[:, :3] >= first[:, :3] - 1e-6)
     finally:
         gpu.set_option("aov", 0)
+
+
+def test_graph_follows_a_film_layout_change(aov_gpu, cornell):
+    # The captured bounce loop bakes in the feature films' offsets (film +
+    # pixels of all views).  Render a large layout, then a smaller one without a
+    # new scene upload: the second render's films must equal an eager render's
+    # (graph off) of that layout.  Radiance arrives through float atomics (two
+    # next-event records of a vertex land in one launch), so sums agree to
+    # rounding; a stale offset would put features in the wrong place entirely.
+    gpu = aov_gpu
+    gpu.upload(cornell)
+    cam = cornell.camera
+    big = ng.lookat_camera_like(cam, 96, 80)
+    small = ng.lookat_camera_like(cam, 40, 24)
+    _say("case: aov graph, layout 96x80 then 40x24")
+    gpu.reset_film()
+    gpu.render(ng.make_view(big, 4, 8), seed=5)
+    out = {}
+    for graph in (1, 0):
+        gpu.set_option("graph", graph)
+        gpu.reset_film()
+        img = gpu.render([ng.make_view(small, 2, 8), ng.make_view(small, 2, 8, sample_offset=2)], seed=9)
+        out[graph] = (img, gpu.aov(ng.AOV_ALBEDO), gpu.aov(ng.AOV_NORMAL))
+    gpu.set_option("graph", 1)
+    for a, b in zip(out[1], out[0]):
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-7)
+    assert np.all(out[1][0][:, 3] == 2) and out[1][1][:, 3].max() > 0
