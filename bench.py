@@ -171,7 +171,8 @@ def measure(gpu, ng, make_views, steps, warmup, tune, tune_views, pool, chunk, l
         acc["ext"] += s.extension_rays
         acc["sh"] += s.shadow_rays
         acc["ms_gpu"] += s.ms_total
-        log(f"[bench] {label} step {k + 1}/{steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s")
+        log(f"[bench] {label} step {k + 1}/{steps}: {s.samples / s.ms_total / 1e3:.1f} Msamples/s, "
+            f"{s.extension_rays} extension rays in {s.launches_closest} launches")
     wall = time.perf_counter() - t0
     gpu.set_option("timing_mode", 1)
     gpu.reset_film()
@@ -310,7 +311,7 @@ def main():
 
         tune_views = [ng.make_view(v.camera, 8, v.max_depth, sample_offset=0x40000000) for v in views]
         m = measure(gpu, ng, bviews, args.steps, args.warmup, not args.no_autotune, tune_views, args.batch,
-                    args.chunk_iters, "batched", min(args.timing_steps, args.steps))
+                    args.chunk_iters, "batched", max(0, min(args.timing_steps, args.steps)))
         m["roofline"] = roofline(m, "k_extend_batched_traffic.md", "k_extend<ALPHA> (BVH8 closest hit, alpha test)")
         m["workload"] = (f"batched textured scene, {len(sc.indices):,} triangles (2048^2 heightfield, 199 displaced "
                          f"icospheres, 20,000 alpha-tested cards), 128 1024^2 RGBA8 textures, thin lens r 0.05 "
@@ -334,7 +335,7 @@ def main():
 
         steps = args.steps if args.only == "disney" else args.disney_steps
         m = measure(gpu, ng, dviews, steps, args.warmup, not args.no_autotune, [ng.make_view(cam, 256, 12)],
-                    args.batch, args.chunk_iters, "disney", min(args.timing_steps, steps))
+                    args.batch, args.chunk_iters, "disney", max(0, min(args.timing_steps, steps)))
         m["roofline"] = roofline(m, "k_extend_disney_grid_traffic.md", "k_extend (BVH8 closest hit)")
         m["workload"] = (f"Disney icosphere grid 8x8, level 6, {len(sc.indices):,} triangles, 2048x1024 sky+sun "
                          f"envmap, 512x512, 256 spp per step, max depth 12, plain path_mis")
