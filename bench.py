@@ -128,20 +128,25 @@ def calibrate_copy(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "launches": 3}), flush=True)
 
 
-def measure(gpu, ng, make_views, steps, warmup, tune, tune_views, pool, chunk, label, timing_steps):
+def measure(gpu, ng, make_views, steps, warmup, tune, tune_views, pool, chunk, traversal, waves, label, timing_steps):
     """Untimed autotune (or the given pool/chunk), an untimed traversal-count
     pass, `warmup` untimed steps, `steps` timed steps (graph-replayed bounce
     loop), then the first `timing_steps` of those steps again with HIP events
     around every launch on the renderer's stream (kernel times)."""
     gpu.set_option("batch", pool)
     gpu.set_option("chunk_iters", chunk)
+    gpu.set_option("traversal", traversal)
+    gpu.set_option("trace_waves", waves)
     tuned = None
     if tune:
         r = gpu.autotune(tune_views, calib_spp=tune_views[0].spp)
-        tuned = {"pool": r.best_pool, "chunk_iters": r.best_chunk,
-                 "candidates": [[r.pool[k], r.chunk[k], round(r.msamples_per_s[k], 1)] for k in range(r.count)]}
-        log(f"[bench] {label} autotune: pool {r.best_pool}, chunk {r.best_chunk} "
-            f"({r.best_msamples_per_s:.1f} Msamples/s on the calibration renders)")
+        tuned = {"pool": r.best_pool, "chunk_iters": r.best_chunk, "traversal": r.best_traversal,
+                 "trace_waves": r.best_trace_waves,
+                 "candidates [pool, chunk, traversal, waves per SIMD cap, Msamples/s]":
+                     [[r.pool[k], r.chunk[k], r.traversal[k], r.trace_waves[k], round(r.msamples_per_s[k], 1)]
+                      for k in range(r.count)]}
+        log(f"[bench] {label} autotune: pool {r.best_pool}, chunk {r.best_chunk}, traversal {r.best_traversal}, "
+            f"waves cap {r.best_trace_waves} ({r.best_msamples_per_s:.1f} Msamples/s on the calibration renders)")
     gpu.set_option("count_traversal", 1)
     gpu.reset_film()
     gpu.render(make_views(10 ** 6, True), seed=99, readback=False)
@@ -260,6 +265,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep --batch and --chunk-iters")
     ap.add_argument("--batch", type=int, default=1 << 24, help="path pool (--no-autotune)")
     ap.add_argument("--chunk-iters", type=int, default=2, help="pool iterations per done-flag check (--no-autotune)")
+    ap.add_argument("--traversal", type=int, default=1, help="triangle schedule of the traversal (--no-autotune)")
+    ap.add_argument("--trace-waves", type=int, default=0, help="waves-per-SIMD cap of traversal (--no-autotune)")
     ap.add_argument("--only", choices=["batched", "disney"], default=None,
                     help="run one workload leg only (profiling); the JSON line then carries that leg as value")
     ap.add_argument("--disney-steps", type=int, default=4)
@@ -311,7 +318,8 @@ def main():
 
         tune_views = [ng.make_view(v.camera, 8, v.max_depth, sample_offset=0x40000000) for v in views]
         m = measure(gpu, ng, bviews, args.steps, args.warmup, not args.no_autotune, tune_views, args.batch,
-                    args.chunk_iters, "batched", max(0, min(args.timing_steps, args.steps)))
+                    args.chunk_iters, args.traversal, args.trace_waves, "batched",
+                    max(0, min(args.timing_steps, args.steps)))
         m["roofline"] = roofline(m, "k_extend_batched_traffic.md", "k_extend<ALPHA> (BVH8 closest hit, alpha test)")
         m["workload"] = (f"batched textured scene, {len(sc.indices):,} triangles (2048^2 heightfield, 199 displaced "
                          f"icospheres, 20,000 alpha-tested cards), 128 1024^2 RGBA8 textures, thin lens r 0.05 "
@@ -335,7 +343,8 @@ def main():
 
         steps = args.steps if args.only == "disney" else args.disney_steps
         m = measure(gpu, ng, dviews, steps, args.warmup, not args.no_autotune, [ng.make_view(cam, 256, 12)],
-                    args.batch, args.chunk_iters, "disney", max(0, min(args.timing_steps, steps)))
+                    args.batch, args.chunk_iters, args.traversal, args.trace_waves, "disney",
+                    max(0, min(args.timing_steps, steps)))
         m["roofline"] = roofline(m, "k_extend_disney_grid_traffic.md", "k_extend (BVH8 closest hit)")
         m["workload"] = (f"Disney icosphere grid 8x8, level 6, {len(sc.indices):,} triangles, 2048x1024 sky+sun "
                          f"envmap, 512x512, 256 spp per step, max depth 12, plain path_mis")

@@ -275,7 +275,13 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * Owen-scrambled Sobol', shaders/RNG.hlsli), "sort" (0/1: class-sorted shading
  * launches, default 1), "aov" (0/1: denoiser feature films, nori_gpu_aov,
  * default 0), "persistent" (0/1: traversal kernels as a grid of
- * resident waves that take rays from a counter, default 0), "timing_mode"
+ * resident waves that take rays from a counter, default 0), "traversal" (0:
+ * a lane tests the triangles of a node's hit leaves right after the node; 1,
+ * the default: in closest-hit queries a wave-wide ballot postpones them while
+ * fewer than a fifth of the wave's active lanes hold triangles; the hits are
+ * the same either way), "trace_waves" (0 .. 8: at most this many waves per
+ * SIMD for the traversal kernels, by padding their LDS; 0, the default: as
+ * many as their LDS stacks allow), "timing_mode"
  * (0/1), "count_traversal" (0/1), "guiding" (0/1:
  * guided sampling with one-sample MIS and online training, one round per
  * pool's worth of samples started), "guide_train_steps" (Adam steps per
@@ -299,21 +305,27 @@ int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
 int nori_gpu_aov(nori_ctx *ctx, int which, float *rgba);
 
 /* ----------------------------------------------------------- autotuner */
-/* Picks the path-pool size (option "batch") and the iterations between
- * done-flag checks (option "chunk_iters") for a set of views by timing short
- * renders of them (calib_spp samples per pixel each, at sample offsets far
- * above any the caller uses) for every candidate, and sets the fastest.  It
- * has no counterpart in the reference (DXR sizes its own launches); it serves
- * callers that render the same views repeatedly.  The film is cleared
- * afterwards.  Candidates: pools 2^18 .. 2^22 (those no larger than four times
- * the calibration's samples), then chunks 4 .. 32 at the best pool. */
-#define NORI_TUNE_MAX 16
+/* Picks the path-pool size (option "batch"), the iterations between done-flag
+ * checks ("chunk_iters"), the traversal kernels' triangle schedule
+ * ("traversal") and their waves per SIMD ("trace_waves") for a set of views
+ * by timing short renders of them (calib_spp samples per pixel each, at
+ * sample offsets far above any the caller uses) for every candidate, and sets
+ * the fastest.  It has no counterpart in the reference (DXR sizes its own
+ * launches); it serves callers that render the same views repeatedly.  The
+ * film is cleared afterwards.  Candidates, each search at the best of the
+ * previous: pools 2^18 .. 2^24 (those no larger than the calibration's
+ * samples); chunks 2, 4, 16, 32; the other traversal schedule; waves-per-SIMD
+ * caps 2, 4, 6 (the traversal blocks' LDS stacks, stack depth x 64 lanes x 8
+ * bytes per wave, otherwise bound occupancy). */
+#define NORI_TUNE_MAX 32
 typedef struct nori_tune_result {
     uint32_t best_pool, best_chunk;
     double best_msamples_per_s;
     uint32_t count;                           /* candidates timed            */
     uint32_t pool[NORI_TUNE_MAX], chunk[NORI_TUNE_MAX];
     double msamples_per_s[NORI_TUNE_MAX];
+    uint32_t best_traversal, best_trace_waves;
+    uint32_t traversal[NORI_TUNE_MAX], trace_waves[NORI_TUNE_MAX];
 } nori_tune_result;
 int nori_gpu_autotune(nori_ctx *ctx, const nori_view *views, int nviews, uint32_t calib_spp, nori_tune_result *out);
 

@@ -85,6 +85,9 @@ struct nori_ctx {
     // thread per position on the Disney grid, 474 vs 569 Msamples/s at a
     // 2^24-path pool)
     int persistent = 0;
+    uint32_t traversal = 1; // option "traversal": triangle schedule of the BVH8 walk (SceneDev::traversal)
+    uint32_t trace_waves = 0; // option "trace_waves": waves-per-SIMD cap of the traversal kernels (0: none)
+    uint32_t lds_per_cu = 0;  // the device's LDS bytes per CU
     uint32_t *work_ext() const { return persistent ? cdev.cnt + 10 : nullptr; }
     uint32_t *work_shadow() const { return persistent ? cdev.cnt + 11 : nullptr; }
     uint32_t sampler = 0;                  // option "sampler": 0 pcg32 per pixel, 1 Owen-scrambled Sobol'
@@ -118,11 +121,12 @@ struct nori_ctx {
         uint32_t pool, cap, sampler, max_rec, guide_cap, rec_cap;
         const void *stats;
         bool aov, guiding, sorted, persistent;
+        uint32_t traversal;
         bool operator==(const GraphKey &o) const {
             return bufs == o.bufs && scene == o.scene && npix_all == o.npix_all && pool == o.pool && cap == o.cap &&
                    sampler == o.sampler && max_rec == o.max_rec && guide_cap == o.guide_cap && rec_cap == o.rec_cap &&
                    stats == o.stats && aov == o.aov && guiding == o.guiding && sorted == o.sorted &&
-                   persistent == o.persistent;
+                   persistent == o.persistent && traversal == o.traversal;
         }
     };
     hipGraphExec_t graph = nullptr;
@@ -458,6 +462,9 @@ int nori_gpu_create(nori_ctx **out, int device) {
     c->device = device;
     int rc = guard(c, [&] {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        int lds = 0;
+        HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
+        c->lds_per_cu = (uint32_t)lds;
         c->alloc(c->counters, 16 * sizeof(uint32_t));
         // 0..7 render counters, 8..11 sample claims (Counters::next), 12..13 alpha-test candidates
         c->alloc(c->stats, 16 * sizeof(unsigned long long));
@@ -676,8 +683,13 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
             s.tex = TexTables{c->upload(tu.tex.data(), tu.tex.size()), c->upload(tu.texel.data(), tu.texel.size()),
                               c->upload(tu.lut.data(), tu.lut.size()), (uint32_t)tu.tex.size()};
         }
-        if ((size_t)s.stack_depth * (s.has_sss ? 128 : 64) * sizeof(uint2) > 64 * 1024)
+        // stacks of 128 lanes (subsurface shading blocks) or of 64 lanes at twice
+        // the depth (postponing traversal) must fit 64 KiB of LDS
+        if ((size_t)s.stack_depth * 128 * sizeof(uint2) > 64 * 1024)
             throw ArgError("BVH8 deeper than the LDS stack allows", NORI_E_LIMIT);
+        s.traversal = c->traversal;
+        s.trace_waves = c->trace_waves;
+        s.lds_per_cu = c->lds_per_cu;
         c->sdev = s;
         c->has_scene = true;
         ++c->scene_gen;
@@ -773,7 +785,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             if (graphed) {
                 nori_ctx::GraphKey key{c->buf_gen, c->scene_gen, (uint64_t)c->npix_all, c->pool, c->cap, c->sampler,
                                        gsd.max_rec, c->g_cap, c->g_rec_cap, cs, c->film_aov, gsd.out != nullptr,
-                                       c->sort_shading != 0, c->persistent != 0};
+                                       c->sort_shading != 0, c->persistent != 0, c->traversal};
                 if (!c->graph || !(key == c->graph_key)) {
                     c->drop_graph();
                     hipGraph_t g = nullptr;
@@ -887,7 +899,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
 int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t calib_spp, nori_tune_result *out) {
     if (!c || !views || nviews <= 0 || !out || calib_spp == 0) return NORI_E_ARG;
     nori_tune_result r{};
-    const uint32_t batch0 = c->batch, chunk0 = c->chunk_iters;
+    const uint32_t batch0 = c->batch, chunk0 = c->chunk_iters, trav0 = c->traversal, waves0 = c->trace_waves;
     std::vector<nori_view> cal(views, views + nviews);
     uint64_t samples = 0;
     for (auto &v : cal) {
@@ -895,10 +907,16 @@ int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t 
         v.sample_offset = 0x40000000u;
         samples += (uint64_t)v.camera.width * v.camera.height * calib_spp;
     }
-    // Time one candidate: a warm-up render, then the best of two.
-    auto time_it = [&](uint32_t pool, uint32_t chunk) -> double {
+    auto apply = [&](uint32_t pool, uint32_t chunk, uint32_t trav, uint32_t waves) {
         c->batch = pool;
         c->chunk_iters = chunk;
+        c->traversal = c->sdev.traversal = trav;
+        c->trace_waves = c->sdev.trace_waves = waves;
+        c->drop_graph();
+    };
+    // Time one candidate: a warm-up render, then the best of two.
+    auto time_it = [&](uint32_t pool, uint32_t chunk, uint32_t trav, uint32_t waves) -> double {
+        apply(pool, chunk, trav, waves);
         double best = 0.0;
         for (int k = 0; k < 3; ++k) {
             int rc = nori_gpu_render(c, cal.data(), nviews, 0x7E57u + k, nullptr);
@@ -908,6 +926,8 @@ int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t 
         if (r.count < NORI_TUNE_MAX) {
             r.pool[r.count] = pool;
             r.chunk[r.count] = chunk;
+            r.traversal[r.count] = trav;
+            r.trace_waves[r.count] = waves;
             r.msamples_per_s[r.count] = best;
             ++r.count;
         }
@@ -915,6 +935,8 @@ int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t 
             r.best_msamples_per_s = best;
             r.best_pool = pool;
             r.best_chunk = chunk;
+            r.best_traversal = trav;
+            r.best_trace_waves = waves;
         }
         return best;
     };
@@ -925,21 +947,27 @@ int nori_gpu_autotune(nori_ctx *c, const nori_view *views, int nviews, uint32_t 
         for (uint32_t lg = 18; lg <= 24; ++lg) {
             const uint32_t pool = 1u << lg;
             if (lg > 18 && (uint64_t)pool > samples) break;
-            time_it(pool, chunk0);
+            time_it(pool, chunk0, trav0, waves0);
         }
         const uint32_t bp = r.best_pool;
         for (uint32_t chunk : {2u, 4u, 16u, 32u})
-            if (chunk != chunk0) time_it(bp, chunk);
+            if (chunk != chunk0) time_it(bp, chunk, trav0, waves0);
+        // the traversal kernels' triangle schedule and their waves per SIMD
+        // (the LDS stacks bound occupancy; a lower cap trades latency hiding
+        // for less cache contention)
+        const uint32_t bc = r.best_chunk;
+        time_it(bp, bc, trav0 ? 0u : 1u, waves0);
+        const uint32_t bt = r.best_traversal;
+        for (uint32_t w : {2u, 4u, 6u})
+            if (w != waves0) time_it(bp, bc, bt, w);
     } catch (const ArgError &e) {
         rc = e.code;
     }
     if (rc != NORI_OK) {
-        c->batch = batch0;
-        c->chunk_iters = chunk0;
+        apply(batch0, chunk0, trav0, waves0);
         return rc;
     }
-    c->batch = r.best_pool;
-    c->chunk_iters = r.best_chunk;
+    apply(r.best_pool, r.best_chunk, r.best_traversal, r.best_trace_waves);
     rc = nori_gpu_reset_film(c);
     *out = r;
     return rc;
@@ -987,6 +1015,17 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
         c->drop_graph();
     } else if (n == "persistent") {
         c->persistent = value != 0;
+        c->drop_graph();
+    } else if (n == "traversal") {
+        if (value != 0 && value != 1)
+            return fail(c, NORI_E_ARG, "traversal must be 0 (a node's triangles at once) or 1 (ballot-postponed)");
+        c->traversal = (uint32_t)value;
+        c->sdev.traversal = c->traversal;
+        c->drop_graph();
+    } else if (n == "trace_waves") {
+        if (value < 0 || value > 8) return fail(c, NORI_E_ARG, "trace_waves must be in [0, 8] (0: no cap)");
+        c->trace_waves = (uint32_t)value;
+        c->sdev.trace_waves = c->trace_waves;
         c->drop_graph();
     } else if (n == "chunk_iters") {
         if (value < 2 || value > 4096 || (value & 1)) return fail(c, NORI_E_ARG, "chunk_iters must be even, in [2, 4096]");

@@ -124,14 +124,14 @@ __device__ void camera_ray(const nori_camera &c, float px, float py, float ax, f
 
 // ------------------------------------------------------------------ kernels
 
-template <bool ANY>
+template <bool ANY, bool POST>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_batch(SceneDev s, const float4 *__restrict__ rays,
                                                              float4 *__restrict__ hits, uint32_t n) {
     extern __shared__ uint2 stk[];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 a = rays[2 * i], b = rays[2 * i + 1];
-    TraceOut r = traverse_bvh8<ANY, false>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x);
+    TraceOut r = traverse_bvh8<ANY, false, POST>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x);
     uint32_t prim = ANY ? (r.prim == MISS ? MISS : 0u) : r.prim;
     hits[i] = make_float4(r.t, __uint_as_float(prim), r.u, r.v);
 }
@@ -332,7 +332,7 @@ __device__ __forceinline__ uint32_t rng_state_hash(const SobolRng &r) { return p
 // scenes with media, else the path stream's state hash where the ray was made.
 __device__ __forceinline__ uint32_t alpha_seed_shadow(uint32_t word) { return pcg_hash(word ^ 0x2545F491u); }
 
-template <bool COUNT, bool SORT, bool ALPHA>
+template <bool COUNT, bool SORT, bool ALPHA, bool POST>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
                                                         unsigned long long *stats, uint32_t *q_cls,
                                                         uint32_t *cls_cnt, uint32_t cap) {
@@ -351,11 +351,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
                 if (COUNT && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE) ++na;
                 return alpha_accept(s, prim, mesh, u, v, seed, false);
             };
-            r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x,
-                                            &nn, &nt, acc);
+            r = traverse_bvh8<false, COUNT, POST>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x,
+                                                  blockDim.x, &nn, &nt, acc);
         } else {
-            r = traverse_bvh8<false, COUNT>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x, blockDim.x,
-                                            &nn, &nt);
+            r = traverse_bvh8<false, COUNT, POST>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x,
+                                                  blockDim.x, &nn, &nt);
         }
         p.hit[i] = make_float4(r.t, __uint_as_float(r.prim), r.u, r.v);
         if (SORT) {
@@ -1598,13 +1598,24 @@ inline uint32_t blocks(uint64_t n, int b) { return (uint32_t)((n + b - 1) / b); 
 
 } // namespace
 
+// LDS bytes of a traversal block (one wave): its lanes' stacks, one column of
+// stack_depth entries per lane and one more for a postponed triangle group in
+// closest-hit queries; with a waves-per-SIMD cap, padded so that no more than
+// that many of the kernel's blocks fit a CU's LDS.
+static size_t trace_lds(const SceneDev &s, bool closest = true) {
+    const size_t need = (size_t)(s.stack_depth + (closest && s.traversal ? 1u : 0u)) * TRACE_BLOCK * sizeof(uint2);
+    if (s.trace_waves == 0 || s.lds_per_cu == 0) return need;
+    const size_t per_cu = 4u * s.trace_waves; // 4 SIMDs per CU
+    const size_t cap = (((size_t)s.lds_per_cu + per_cu - 1u) / per_cu + 7u) & ~(size_t)7u;
+    return cap > need ? std::min<size_t>(cap, 65536) : need;
+}
+
 void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uint32_t n, int any_hit, hipStream_t st) {
     if (n == 0) return;
-    size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
-    if (any_hit)
-        hipLaunchKernelGGL(k_trace_batch<true>, dim3(blocks(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, rays, hits, n);
-    else
-        hipLaunchKernelGGL(k_trace_batch<false>, dim3(blocks(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, rays, hits, n);
+    decltype(&k_trace_batch<false, false>) ks[4] = {k_trace_batch<false, false>, k_trace_batch<false, true>,
+                                                    k_trace_batch<true, false>, k_trace_batch<true, true>};
+    hipLaunchKernelGGL(ks[(any_hit ? 2 : 0) | (s.traversal ? 1 : 0)], dim3(blocks(n, TRACE_BLOCK)), dim3(TRACE_BLOCK),
+                       trace_lds(s, !any_hit), st, s, rays, hits, n);
 }
 
 void launch_pool_start(Counters c, uint32_t *q_free0, uint32_t cap, hipStream_t st) {
@@ -1691,13 +1702,14 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
                                cls_stride);
         return;
     }
-    decltype(&k_extend<false, false, false>) ks[8] = {
-        k_extend<false, false, false>, k_extend<false, false, true>, k_extend<false, true, false>,
-        k_extend<false, true, true>,   k_extend<true, false, false>, k_extend<true, false, true>,
-        k_extend<true, true, false>,   k_extend<true, true, true>};
-    const int v = (count_stats ? 4 : 0) | (q_cls ? 2 : 0) | (s.has_alpha ? 1 : 0);
-    hipLaunchKernelGGL(ks[v], dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, p, count, count_stats,
-                       q_cls, cls_cnt, cls_stride);
+#define KX(c, q, a) k_extend<c, q, a, false>, k_extend<c, q, a, true>
+    decltype(&k_extend<false, false, false, false>) ks[16] = {
+        KX(false, false, false), KX(false, false, true), KX(false, true, false), KX(false, true, true),
+        KX(true, false, false),  KX(true, false, true),  KX(true, true, false),  KX(true, true, true)};
+#undef KX
+    const int v = (count_stats ? 8 : 0) | (q_cls ? 4 : 0) | (s.has_alpha ? 2 : 0) | (s.traversal ? 1 : 0);
+    hipLaunchKernelGGL(ks[v], dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), trace_lds(s), st, s, p, count,
+                       count_stats, q_cls, cls_cnt, cls_stride);
 }
 
 // Shading kernel of a sorted class (scenes without media).
@@ -1769,8 +1781,8 @@ void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_s
         k_shadow<false, false, false>, k_shadow<false, false, true>, k_shadow<false, true, false>,
         k_shadow<false, true, true>,   k_shadow<true, false, false>, k_shadow<true, false, true>,
         k_shadow<true, true, false>,   k_shadow<true, true, true>};
-    hipLaunchKernelGGL(ks[v], dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), lds, st, s, sh, cnt_sh, film,
-                       count_stats, g);
+    hipLaunchKernelGGL(ks[v], dim3(blocks(cap, TRACE_BLOCK)), dim3(TRACE_BLOCK), trace_lds(s, false), st, s, sh,
+                       cnt_sh, film, count_stats, g);
 }
 
 void launch_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st) {

@@ -13,7 +13,19 @@
 //     (Pharr, Jakob, Humphreys, PBRT 3e §3.9.2), so no box that holds an
 //     accepted hit, or an equal-t hit with a smaller id, is ever culled;
 //   * stack: the node groups (child base, hit bits | inner mask) of Ylitie et
-//     al. 2017, kept in LDS, one column per lane; its depth is the tree depth.
+//     al. 2017, kept in LDS, one column per lane; its depth is the tree depth;
+//   * schedule (POSTPONE, closest-hit queries): either a lane tests all
+//     triangles of a node's hit leaves right after visiting it, or, as Ylitie
+//     et al. 2017 (section 4) postpone them, a wave-wide ballot counts the
+//     lanes that hold triangles: while fewer than a fifth of the active lanes
+//     do and a lane still has child nodes to visit, its triangle group goes
+//     onto its stack and it keeps descending, so lanes testing triangles do
+//     not hold up the lanes walking nodes; otherwise it tests one triangle per
+//     step.  A lane keeps at most one postponed group on its stack (sharing it
+//     with node groups: a node group always has child hit bits in its top
+//     byte, a triangle group never does), so the stack needs one entry more
+//     than the tree depth.  Occlusion queries do not postpone: an early
+//     occluder ends them.
 #pragma once
 #include "nori_math.h"
 
@@ -72,14 +84,16 @@ struct AcceptAll {
 // 674-735 and shaders/RayQueryTrace.hlsli:43-124: alpha test, glass and
 // mirrors on shadow rays); a rejected candidate leaves the search as it was.
 // Every triangle is tested at most once per ray.
-template <bool ANY, bool COUNT> struct Bvh8Walker {
+template <bool ANY, bool COUNT, bool POSTPONE = false> struct Bvh8Walker {
     f3 o, d, rcp;
     float tmin;
     uint32_t octinv;
     bool nx, ny, nz;
     TraceOut res;
     uint2 group;
+    uint2 tg; // POSTPONE: pending triangles (first triangle, hit mask in bits 0..23); y = 0: none
     int sp;
+    bool postponed; // POSTPONE: a triangle group of this ray is on the stack
     uint32_t cnt_nodes, cnt_tris; // summed over every ray this walker ran
 
     __device__ __forceinline__ void start(f3 o_, f3 d_, float tmin_, float tmax) {
@@ -93,98 +107,152 @@ template <bool ANY, bool COUNT> struct Bvh8Walker {
         nz = rcp.z < 0.0f;
         octinv = 7u - ((nx ? 4u : 0u) | (ny ? 2u : 0u) | (nz ? 1u : 0u));
         group = make_uint2(0u, (1u << (24u + octinv)) | 1u);
+        tg = make_uint2(0u, 0u);
         sp = 0;
+        postponed = false;
+    }
+
+    // Visits the next hit child node of `group` (which has one): the rest of
+    // the group goes onto the stack, `group` becomes the node's hit inner
+    // children and the return value its hit leaf triangles (bits 0..23, from
+    // triangle *tri_base).
+    __device__ __forceinline__ uint32_t visit_node(const uint4 *__restrict__ nodes, uint2 *stk, uint32_t stride,
+                                                   uint32_t &tri_base) {
+        const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
+        uint32_t bit = 31u - __builtin_clz(group.y & 0xFF000000u);
+        group.y &= ~(1u << bit);
+        uint32_t slot = (bit - 24u) ^ octinv;
+        uint32_t node_idx = group.x + __builtin_popcount((group.y & 0xFFu) & ((1u << slot) - 1u));
+        if (group.y & 0xFF000000u) {
+            stk[sp * stride] = group;
+            ++sp;
+        }
+        const uint4 *np = nodes + 5 * (size_t)node_idx;
+        uint4 a = np[0], b = np[1], c = np[2], dd = np[3], e = np[4];
+        if (COUNT) ++cnt_nodes;
+        f3 p = f3{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
+        f3 sc = f3{__uint_as_float(((a.w) & 0xFFu) << 23), __uint_as_float(((a.w >> 8) & 0xFFu) << 23),
+                   __uint_as_float(((a.w >> 16) & 0xFFu) << 23)};
+        uint32_t imask = a.w >> 24;
+        // near/far plane words per axis, chosen by the ray octant
+        uint32_t nxl = nx ? dd.z : c.x, nxh = nx ? dd.w : c.y, fxl = nx ? c.x : dd.z, fxh = nx ? c.y : dd.w;
+        uint32_t nyl = ny ? e.x : c.z, nyh = ny ? e.y : c.w, fyl = ny ? c.z : e.x, fyh = ny ? c.w : e.y;
+        uint32_t nzl = nz ? e.z : dd.x, nzh = nz ? e.w : dd.y, fzl = nz ? dd.x : e.z, fzh = nz ? dd.y : e.w;
+        uint32_t hitmask = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            uint32_t meta = ((s < 4 ? b.z : b.w) >> (8 * (s & 3))) & 0xFFu;
+            if (meta == 0u) continue;
+            const int k = s & 3;
+            float qnx = byte_f(s < 4 ? nxl : nxh, k), qfx = byte_f(s < 4 ? fxl : fxh, k);
+            float qny = byte_f(s < 4 ? nyl : nyh, k), qfy = byte_f(s < 4 ? fyl : fyh, k);
+            float qnz = byte_f(s < 4 ? nzl : nzh, k), qfz = byte_f(s < 4 ? fzl : fzh, k);
+            float tnx = (__builtin_fmaf(qnx, sc.x, p.x) - o.x) * rcp.x;
+            float tfx = (__builtin_fmaf(qfx, sc.x, p.x) - o.x) * rcp.x;
+            float tny = (__builtin_fmaf(qny, sc.y, p.y) - o.y) * rcp.y;
+            float tfy = (__builtin_fmaf(qfy, sc.y, p.y) - o.y) * rcp.y;
+            float tnz = (__builtin_fmaf(qnz, sc.z, p.z) - o.z) * rcp.z;
+            float tfz = (__builtin_fmaf(qfz, sc.z, p.z) - o.z) * rcp.z;
+            float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+            // widened after the min with the current hit: a box entered at exactly
+            // res.t may hold an equal-t hit with a smaller primitive id
+            float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * far_scale;
+            if (tn <= tf) {
+                uint32_t is_inner = (meta & 0x18u) == 0x18u;
+                uint32_t bitidx = is_inner ? ((meta ^ octinv) & 0x1Fu) : (meta & 0x1Fu);
+                hitmask |= (meta >> 5) << bitidx;
+            }
+        }
+        group = make_uint2(b.x, (hitmask & 0xFF000000u) | imask);
+        tri_base = b.y;
+        return hitmask & 0x00FFFFFFu;
+    }
+
+    // Tests triangle tri_base + tb; true when an occlusion query is answered.
+    template <class Accept>
+    __device__ __forceinline__ bool test_triangle(const float4 *__restrict__ tris, uint32_t idx, Accept &accept) {
+        const float4 *tp = tris + 3 * (size_t)idx;
+        float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
+        if (COUNT) ++cnt_tris;
+        float t, u, v;
+        if (mt_test(o, d, f3{t0.x, t0.y, t0.z}, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}, tmin, res.t, t, u, v)) {
+            uint32_t prim = __float_as_uint(t0.w);
+            if (!accept(prim, __float_as_uint(t1.w), u, v, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}))
+                return false;
+            if (t < res.t || prim < res.prim) {
+                res.t = t;
+                res.u = u;
+                res.v = v;
+                res.prim = prim;
+                if (ANY) return true;
+            }
+        }
+        return false;
     }
 
     template <class Accept>
     __device__ __forceinline__ bool step(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, uint2 *stk,
                                          uint32_t stride, Accept &accept) {
-        const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
-        if (group.y & 0xFF000000u) {
-            uint32_t bit = 31u - __builtin_clz(group.y & 0xFF000000u);
-            group.y &= ~(1u << bit);
-            uint32_t slot = (bit - 24u) ^ octinv;
-            uint32_t node_idx = group.x + __builtin_popcount((group.y & 0xFFu) & ((1u << slot) - 1u));
+        if (!POSTPONE || ANY) {
             if (group.y & 0xFF000000u) {
-                stk[sp * stride] = group;
-                ++sp;
-            }
-            const uint4 *np = nodes + 5 * (size_t)node_idx;
-            uint4 a = np[0], b = np[1], c = np[2], dd = np[3], e = np[4];
-            if (COUNT) ++cnt_nodes;
-            f3 p = f3{__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
-            f3 sc = f3{__uint_as_float(((a.w) & 0xFFu) << 23), __uint_as_float(((a.w >> 8) & 0xFFu) << 23),
-                       __uint_as_float(((a.w >> 16) & 0xFFu) << 23)};
-            uint32_t imask = a.w >> 24;
-            // near/far plane words per axis, chosen by the ray octant
-            uint32_t nxl = nx ? dd.z : c.x, nxh = nx ? dd.w : c.y, fxl = nx ? c.x : dd.z, fxh = nx ? c.y : dd.w;
-            uint32_t nyl = ny ? e.x : c.z, nyh = ny ? e.y : c.w, fyl = ny ? c.z : e.x, fyh = ny ? c.w : e.y;
-            uint32_t nzl = nz ? e.z : dd.x, nzh = nz ? e.w : dd.y, fzl = nz ? dd.x : e.z, fzh = nz ? dd.y : e.w;
-            uint32_t hitmask = 0;
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                uint32_t meta = ((s < 4 ? b.z : b.w) >> (8 * (s & 3))) & 0xFFu;
-                if (meta == 0u) continue;
-                const int k = s & 3;
-                float qnx = byte_f(s < 4 ? nxl : nxh, k), qfx = byte_f(s < 4 ? fxl : fxh, k);
-                float qny = byte_f(s < 4 ? nyl : nyh, k), qfy = byte_f(s < 4 ? fyl : fyh, k);
-                float qnz = byte_f(s < 4 ? nzl : nzh, k), qfz = byte_f(s < 4 ? fzl : fzh, k);
-                float tnx = (__builtin_fmaf(qnx, sc.x, p.x) - o.x) * rcp.x;
-                float tfx = (__builtin_fmaf(qfx, sc.x, p.x) - o.x) * rcp.x;
-                float tny = (__builtin_fmaf(qny, sc.y, p.y) - o.y) * rcp.y;
-                float tfy = (__builtin_fmaf(qfy, sc.y, p.y) - o.y) * rcp.y;
-                float tnz = (__builtin_fmaf(qnz, sc.z, p.z) - o.z) * rcp.z;
-                float tfz = (__builtin_fmaf(qfz, sc.z, p.z) - o.z) * rcp.z;
-                float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
-                // widened after the min with the current hit: a box entered at exactly
-                // res.t may hold an equal-t hit with a smaller primitive id
-                float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * far_scale;
-                if (tn <= tf) {
-                    uint32_t is_inner = (meta & 0x18u) == 0x18u;
-                    uint32_t bitidx = is_inner ? ((meta ^ octinv) & 0x1Fu) : (meta & 0x1Fu);
-                    hitmask |= (meta >> 5) << bitidx;
+                uint32_t base;
+                uint32_t trimask = visit_node(nodes, stk, stride, base);
+                while (trimask) {
+                    uint32_t tb = __builtin_ctz(trimask);
+                    trimask &= trimask - 1u;
+                    if (test_triangle(tris, base + tb, accept)) return true;
                 }
             }
-            group = make_uint2(b.x, (hitmask & 0xFF000000u) | imask);
-            uint32_t trimask = hitmask & 0x00FFFFFFu;
-            while (trimask) {
-                uint32_t tb = __builtin_ctz(trimask);
-                trimask &= trimask - 1u;
-                const float4 *tp = tris + 3 * (size_t)(b.y + tb);
-                float4 t0 = tp[0], t1 = tp[1], t2 = tp[2];
-                if (COUNT) ++cnt_tris;
-                float t, u, v;
-                if (mt_test(o, d, f3{t0.x, t0.y, t0.z}, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}, tmin, res.t, t, u,
-                            v)) {
-                    uint32_t prim = __float_as_uint(t0.w);
-                    if (!accept(prim, __float_as_uint(t1.w), u, v, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}))
-                        continue;
-                    if (t < res.t || prim < res.prim) {
-                        res.t = t;
-                        res.u = u;
-                        res.v = v;
-                        res.prim = prim;
-                        if (ANY) return true;
-                    }
+            if (!(group.y & 0xFF000000u)) {
+                if (sp == 0) return true;
+                --sp;
+                group = stk[sp * stride];
+            }
+            return false;
+        }
+        // postponing schedule: a node visit or a pop, then at most one triangle
+        if (tg.y == 0u) {
+            if (group.y & 0xFF000000u) {
+                uint32_t base;
+                const uint32_t m = visit_node(nodes, stk, stride, base);
+                tg = make_uint2(base, m);
+            } else {
+                if (sp == 0) return true;
+                --sp;
+                const uint2 e = stk[sp * stride];
+                if (e.y & 0xFF000000u) {
+                    group = e;
+                } else {
+                    tg = e;
+                    postponed = false;
                 }
             }
         }
-        if (!(group.y & 0xFF000000u)) {
-            if (sp == 0) return true;
-            --sp;
-            group = stk[sp * stride];
+        const bool want = tg.y != 0u;
+        const uint64_t wants = __ballot(want), active = __ballot(1);
+        if (want) {
+            if (!postponed && (group.y & 0xFF000000u) && 5 * __popcll(wants) < __popcll(active)) {
+                stk[sp * stride] = tg; // postponed: few lanes of the wave would test triangles now
+                ++sp;
+                tg.y = 0u;
+                postponed = true;
+            } else {
+                const uint32_t tb = __builtin_ctz(tg.y);
+                tg.y &= tg.y - 1u;
+                if (test_triangle(tris, tg.x + tb, accept)) return true;
+            }
         }
         return false;
     }
 };
 
 // Whole traversal of one ray (counters, optional: nodes and triangles visited).
-template <bool ANY, bool COUNT, class Accept = AcceptAll>
+template <bool ANY, bool COUNT, bool POSTPONE = false, class Accept = AcceptAll>
 __device__ __forceinline__ TraceOut traverse_bvh8(const uint4 *__restrict__ nodes, const float4 *__restrict__ tris, f3 o,
                                                   f3 d, float tmin, float tmax, uint2 *stk, uint32_t stride,
                                                   uint32_t *nnodes = nullptr, uint32_t *ntris = nullptr,
                                                   Accept accept = Accept()) {
-    Bvh8Walker<ANY, COUNT> w;
+    Bvh8Walker<ANY, COUNT, POSTPONE> w;
     w.cnt_nodes = w.cnt_tris = 0;
     w.start(o, d, tmin, tmax);
     while (!w.step(nodes, tris, stk, stride, accept)) {

@@ -117,9 +117,14 @@ class RenderStats(ctypes.Structure):
                 ("shadow_alpha_tests", _u64)]
 
 
+TUNE_MAX = 32
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("best_pool", _u32), ("best_chunk", _u32), ("best_msamples_per_s", ctypes.c_double),
-                ("count", _u32), ("pool", _u32 * 16), ("chunk", _u32 * 16), ("msamples_per_s", ctypes.c_double * 16)]
+                ("count", _u32), ("pool", _u32 * TUNE_MAX), ("chunk", _u32 * TUNE_MAX),
+                ("msamples_per_s", ctypes.c_double * TUNE_MAX), ("best_traversal", _u32), ("best_trace_waves", _u32),
+                ("traversal", _u32 * TUNE_MAX), ("trace_waves", _u32 * TUNE_MAX)]
 
 
 class BvhInfo(ctypes.Structure):

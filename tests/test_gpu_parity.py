@@ -257,3 +257,33 @@ def test_render_hair_lit_same_samples(gpu):
     _say(f"  relMSE {e:.3e} close {close.mean():.4f} mean {gi.mean(0)} {oi.mean(0)}")
     assert e < 1e-3
     assert close.mean() > 0.9
+
+
+@pytest.mark.parametrize("which", ["cornell", "disney"])
+def test_trace_postponed_schedule_bit_exact(gpu, cornell, disney_small, which):
+    # option "traversal" 1: triangle groups postponed by a wave ballot; closest
+    # hits, ties and occlusion must not depend on the schedule
+    sc = cornell if which == "cornell" else disney_small
+    gpu.set_option("traversal", 1)
+    try:
+        gpu.upload(sc)
+        o = orc.OracleScene(sc)
+        _check_hits(gpu, o, _camera_rays(sc, 128, 128, 31), f"{which} camera, postponed")
+        lo, hi = ([-0.95, 0.05, -0.95], [0.95, 1.95, 3.0]) if which == "cornell" else ([-2, 0.01, -2], [2, 1.5, 2])
+        _check_hits(gpu, o, _random_rays(lo, hi, 1 << 17, 32), f"{which} random, postponed")
+    finally:
+        gpu.set_option("traversal", 0)
+
+
+def test_render_same_under_either_schedule(gpu, disney_small):
+    cam = disney_small.camera
+    cam.width, cam.height = 64, 64
+    gpu.upload(disney_small)
+    out = []
+    for t in (0, 1):
+        gpu.set_option("traversal", t)
+        gpu.reset_film()
+        out.append(gpu.render(ng.make_view(cam, 8, 12), seed=4))
+    gpu.set_option("traversal", 0)
+    _say("case: render disney 64x64 8spp under both traversal schedules")
+    assert np.allclose(out[0], out[1], rtol=1e-5, atol=1e-7)

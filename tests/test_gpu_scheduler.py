@@ -40,6 +40,8 @@ def pool_defaults(gpu):
     gpu.set_option("graph", 1)
     gpu.set_option("chunk_iters", 8)
     gpu.set_option("sort", 1)
+    gpu.set_option("traversal", 1)
+    gpu.set_option("trace_waves", 0)
 
 
 def test_multiview_equals_separate_renders(pool_defaults, cornell):
@@ -153,11 +155,13 @@ def test_autotune_keeps_the_fastest_timed_candidate(pool_defaults, cornell):
     gpu.reset_film()
     before = gpu.render(views, seed=11)
     r = gpu.autotune(views, calib_spp=64)
-    cands = [(r.pool[k], r.chunk[k], r.msamples_per_s[k]) for k in range(r.count)]
-    _say("  " + ", ".join(f"pool {p} chunk {c}: {m:.1f} Ms/s" for p, c, m in cands))
-    assert r.count >= 4 and all(m > 0 for _, _, m in cands)
-    best = max(cands, key=lambda x: x[2])
-    assert (r.best_pool, r.best_chunk) == best[:2]
+    cands = [(r.pool[k], r.chunk[k], r.traversal[k], r.trace_waves[k], r.msamples_per_s[k]) for k in range(r.count)]
+    _say("  " + ", ".join(f"pool {p} chunk {c} traversal {t} waves {w}: {m:.1f} Ms/s" for p, c, t, w, m in cands))
+    assert r.count >= 8 and all(m > 0 for *_, m in cands)
+    # every knob was searched: both schedules and a waves-per-SIMD cap were timed
+    assert {c[2] for c in cands} == {0, 1} and any(c[3] > 0 for c in cands)
+    best = max(cands, key=lambda x: x[4])
+    assert (r.best_pool, r.best_chunk, r.best_traversal, r.best_trace_waves) == best[:4]
     # the tuned options are in effect and the film was cleared: same image again
     after = gpu.render(views, seed=11)
     _same_film(after, before, "tuned")
