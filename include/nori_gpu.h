@@ -174,6 +174,21 @@ int nori_gpu_reset_film(nori_ctx *ctx);
 int nori_gpu_li(nori_ctx *ctx, const nori_ray *rays, const uint64_t *seeds, int n, uint32_t max_depth,
                 uint32_t rr_depth, float *rgb3);
 
+/* Allocation-free variant of nori_gpu_li.  No counterpart in the reference:
+ * for callers that cannot allocate device memory during the call (HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  rays,
+ * seeds and rgba4 are device pointers (n rays of 32 bytes, n pcg32 seeds, n x 4
+ * floats: the radiance Li of ray i in rgba4[4i .. 4i+2] and 1 in rgba4[4i+3]),
+ * as for nori_gpu_li's host arrays.  The path pool, queues, counters and view
+ * record live in a static 20 MiB device array per GPU (held until the process
+ * ends): at most NORI_LI_NOALLOC_MAX_RAYS rays per call (NORI_E_LIMIT above
+ * that).  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (several contexts or threads) race.  Returns when the results are in
+ * rgba4. */
+#define NORI_LI_NOALLOC_MAX_RAYS 65536
+int nori_gpu_li_noalloc(nori_ctx *ctx, const nori_ray *rays, const uint64_t *seeds, int n, uint32_t max_depth,
+                        uint32_t rr_depth, float *rgba4);
+
 /* Batch closest-hit (any_hit = 0) or occlusion (any_hit = 1) queries; rays and
  * hits are host arrays.  For any_hit, prim is 0 on a hit and 0xFFFFFFFF else.
  * The batch form of Accel::rayIntersect (src/cpu/accel.cpp:429-544). */

@@ -88,6 +88,7 @@ struct nori_ctx {
     uint32_t traversal = 1; // option "traversal": triangle schedule of the BVH8 walk (SceneDev::traversal)
     uint32_t trace_waves = 0; // option "trace_waves": waves-per-SIMD cap of the traversal kernels (0: none)
     uint32_t lds_per_cu = 0;  // the device's LDS bytes per CU
+    void *li_scratch = nullptr; // device address of nori_gpu_li_noalloc's static scratch on this device
     uint32_t *work_ext() const { return persistent ? cdev.cnt + 10 : nullptr; }
     uint32_t *work_shadow() const { return persistent ? cdev.cnt + 11 : nullptr; }
     uint32_t sampler = 0;                  // option "sampler": 0 pcg32 per pixel, 1 Owen-scrambled Sobol'
@@ -452,6 +453,41 @@ int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n,
     });
 }
 
+int nori_gpu_li_noalloc(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n, uint32_t max_depth,
+                        uint32_t rr_depth, float *rgba4) {
+    if (!c || n < 0 || (n > 0 && (!rays || !seeds || !rgba4)) || max_depth == 0) return NORI_E_ARG;
+    if (max_depth > 0x7FFFu) return fail(c, NORI_E_LIMIT, "li_noalloc: max_depth above 32767");
+    if ((uint32_t)n > LI_NOALLOC_SLOTS)
+        return fail(c, NORI_E_LIMIT, "li_noalloc: more than NORI_LI_NOALLOC_MAX_RAYS (65536) rays in one call; "
+                                     "its static scratch holds one pool of 65536 paths");
+    return guard(c, [&] {
+        if (!c->has_scene) throw ArgError("li_noalloc: no scene uploaded", NORI_E_STATE);
+        if (!c->li_scratch) throw ArgError("li_noalloc: no scratch on this device", NORI_E_STATE);
+        if (n == 0) return;
+        // Everything below launches kernels on the context stream over the
+        // caller's buffers and the static scratch: no allocation, no memset.
+        const LiScratch l = li_noalloc_layout(c->li_scratch);
+        SceneDev s = c->sdev;
+        s.sobol = 0; // caller rays come with pcg32 seeds (Nori's Independent sampler)
+        float4 *film = (float4 *)rgba4;
+        const uint32_t m = (uint32_t)n;
+        launch_zero_f4(film, (size_t)m, c->stream);
+        launch_li_setup(l, m, max_depth, rr_depth, c->stream);
+        launch_raygen_rays(l.pd[0], (const float4 *)rays, seeds, 0, m, film, c->stream);
+        const GuideShadeDev none{nullptr, nullptr, nullptr, nullptr, 0, 0};
+        for (uint32_t b = 0; b < max_depth; ++b) {
+            const uint32_t qi = b & 1u, qo = qi ^ 1u;
+            launch_extend(s, l.pd[qi], l.cnt.cnt + qi, m, nullptr, nullptr, nullptr, 0, nullptr, c->stream);
+            launch_shade(s, l.pd[qi], l.pd[qo], l.view, l.cnt.cnt + qi, l.cnt.cnt + qo, nullptr, nullptr, l.shd,
+                         l.cnt.cnt + 2, film, m, none, AovDev{nullptr, nullptr}, c->stream);
+            launch_shadow(s, l.shd, l.cnt.cnt + 2, film, 2 * m, nullptr, none, nullptr, c->stream);
+            launch_counters(l.cnt, qi, 1, (1u << qi) | 4u, c->stream);
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipGetLastError());
+    });
+}
+
 int nori_gpu_create(nori_ctx **out, int device) {
     if (!out) return NORI_E_ARG;
     *out = nullptr;
@@ -465,6 +501,9 @@ int nori_gpu_create(nori_ctx **out, int device) {
         int lds = 0;
         HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
         c->lds_per_cu = (uint32_t)lds;
+        // the allocation-free Li's static scratch on this device (guard made it current)
+        c->li_scratch = li_noalloc_scratch(c->device);
+        if (!c->li_scratch) throw HipError("hipGetSymbolAddress of the allocation-free Li scratch failed");
         c->alloc(c->counters, 16 * sizeof(uint32_t));
         // 0..7 render counters, 8..11 sample claims (Counters::next), 12..13 alpha-test candidates
         c->alloc(c->stats, 16 * sizeof(unsigned long long));

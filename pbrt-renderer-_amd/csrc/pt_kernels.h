@@ -178,6 +178,23 @@ void launch_counters(Counters c, uint32_t add_ext_from, uint32_t add_shadow, uin
 void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st);
 void launch_zero_f4(float4 *p, size_t n, hipStream_t st);
 
+// Allocation-free Li (nori_gpu_li_noalloc): a static device array holds one
+// pool of LI_NOALLOC_SLOTS paths, the counters and the view record.
+constexpr uint32_t LI_NOALLOC_SLOTS = 1u << 16;
+constexpr size_t LI_NOALLOC_USED = (size_t)LI_NOALLOC_SLOTS * (2 * (5 * 16 + 2 * 4) + 8 * 16) + 256 + sizeof(ViewTab);
+constexpr size_t LI_NOALLOC_BYTES = 20u << 20; // 20 MiB per GPU
+struct LiScratch {
+    PathsDev pd[2];
+    ShadowDev shd;
+    Counters cnt;
+    ViewTab *view;
+};
+// The scratch array's address on `device` (which must be current), looked up
+// once per device; null when the lookup fails.
+void *li_noalloc_scratch(int device);
+LiScratch li_noalloc_layout(void *base);
+void launch_li_setup(const LiScratch &l, uint32_t n, uint32_t max_depth, uint32_t rr_depth, hipStream_t st);
+
 void launch_medium_batch(const VolumeDev *vols, uint32_t vi, int op, uint32_t n, const float *o3, const float *d3,
                          const float *tn, const float *tf, const uint64_t *seeds, float *out3, float *t_out, int *flag,
                          hipStream_t st);

@@ -1831,4 +1831,67 @@ void launch_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32
     hipLaunchKernelGGL(k_pcg32, dim3(1), dim3(64), 0, st, initstate, initseq, advance, n, out);
 }
 
+// ------------------------------------------------- allocation-free Li
+// Scratch of nori_gpu_li_noalloc: one pool of LI_NOALLOC_SLOTS paths (two
+// path-state buffers, the shadow queue), the render counters and the view
+// record, in a file-scope device array so that the entry point allocates
+// nothing.  Each GPU holds its own copy until the process ends; its address
+// on a device comes from hipGetSymbolAddress, looked up once per device
+// (li_noalloc_scratch).  The array's name is never used as a device address.
+static __device__ __align__(256) unsigned char scratch[LI_NOALLOC_BYTES];
+
+void *li_noalloc_scratch(int device) {
+    static std::mutex mu;
+    static void *ptr[64] = {};
+    if (device < 0 || device >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return nullptr;
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        ptr[device] = p;
+    }
+    return ptr[device];
+}
+
+LiScratch li_noalloc_layout(void *base) {
+    static_assert(LI_NOALLOC_BYTES >= LI_NOALLOC_USED, "scratch smaller than its layout");
+    char *b = (char *)base;
+    const size_t n = LI_NOALLOC_SLOTS, per = 5 * sizeof(float4) + 2 * sizeof(uint32_t);
+    LiScratch l;
+    for (int k = 0; k < 2; ++k) {
+        float4 *pp = (float4 *)(b + (size_t)k * n * per);
+        uint32_t *pu = (uint32_t *)(pp + 5 * n);
+        l.pd[k] = PathsDev{pp, pp + n, pp + 2 * n, pp + 3 * n, pp + 4 * n, pu, pu + n};
+    }
+    float4 *sp = (float4 *)(b + 2 * n * per);
+    l.shd = ShadowDev{sp, sp + 2 * n, sp + 4 * n, sp + 6 * n};
+    char *tail = (char *)(sp + 8 * n);
+    l.cnt = Counters{(uint32_t *)tail, (unsigned long long *)(tail + 64), (unsigned long long *)(tail + 64) + 8};
+    l.view = (ViewTab *)(tail + 256);
+    return l;
+}
+
+// Zero counters and statistics, the first queue holds n paths, and the view
+// record carries the depth limits.
+__global__ void k_li_setup(Counters c, ViewTab *v, uint32_t n, uint32_t max_depth, uint32_t rr_depth) {
+    if (blockIdx.x != 0) return;
+    const uint32_t t = threadIdx.x;
+    if (t < 16) c.cnt[t] = 0;
+    if (t < 16) c.stats[t] = 0; // stats[8..15] are Counters::next
+    __syncthreads();
+    if (t == 0) {
+        c.cnt[0] = n;
+        ViewTab z{};
+        z.max_depth = max_depth;
+        z.rr_depth = rr_depth;
+        *v = z;
+    }
+}
+
+void launch_li_setup(const LiScratch &l, uint32_t n, uint32_t max_depth, uint32_t rr_depth, hipStream_t st) {
+    hipLaunchKernelGGL(k_li_setup, dim3(1), dim3(64), 0, st, l.cnt, l.view, n, max_depth, rr_depth);
+}
+
 } // namespace ng

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libnori_gpu.so")
 
 NORI_OK = 0
 MAT_DIFFUSE, MAT_MIRROR, MAT_DIELECTRIC, MAT_MICROFACET, MAT_DISNEY, MAT_HAIR, MAT_SUBSURFACE = range(7)
+LI_NOALLOC_MAX_RAYS = 65536  # NORI_LI_NOALLOC_MAX_RAYS
 NO_TEXTURE = 0xFFFFFFFF
 AOV_ALBEDO, AOV_NORMAL = 0, 1
 
@@ -141,6 +142,7 @@ class SceneOptions(ctypes.Structure):
 EXPORTS = [
     "nori_gpu_create", "nori_gpu_destroy", "nori_gpu_last_error", "nori_gpu_upload_scene", "nori_gpu_render",
     "nori_gpu_reset_film", "nori_gpu_trace", "nori_gpu_trace_device", "nori_gpu_bsdf_batch", "nori_gpu_li",
+    "nori_gpu_li_noalloc",
     "nori_gpu_envmap_sample", "nori_gpu_envmap_eval", "nori_gpu_medium_batch", "nori_gpu_pcg32", "nori_gpu_sobol", "nori_gpu_texture_sample",
     "nori_gpu_render_stats", "nori_gpu_aov",
     "nori_gpu_bvh_info", "nori_gpu_set_option", "nori_gpu_autotune", "nori_scene_generate", "nori_scene_get_desc",
@@ -169,6 +171,7 @@ def load_library(path=LIB_PATH):
         "nori_gpu_trace_device": (i32, [vp, vp, i32, vp, i32]),
         "nori_gpu_bsdf_batch": (i32, [vp, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_li": (i32, [vp, vp, vp, i32, ctypes.c_uint32, ctypes.c_uint32, vp]),
+        "nori_gpu_li_noalloc": (i32, [vp, vp, vp, i32, ctypes.c_uint32, ctypes.c_uint32, vp]),
         "nori_gpu_envmap_sample": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_medium_batch": (i32, [vp, i32, i32, i32, vp, vp, vp, vp, vp, vp, vp, vp]),
         "nori_gpu_envmap_eval": (i32, [vp, i32, vp, vp, vp]),
@@ -422,6 +425,12 @@ class NoriGPU:
         self._check(self.lib.nori_gpu_li(self.h, _ptr(rays), _ptr(seeds), len(rays), max_depth, rr_depth, _ptr(out)),
                     "li")
         return out
+
+    def li_noalloc(self, rays_dev, seeds_dev, n, max_depth, rr_depth, out_dev):
+        """nori_gpu_li_noalloc on device buffers given as integer addresses:
+        rays n x 8 float32, seeds n uint64, out n x 4 float32 (Li, count)."""
+        self._check(self.lib.nori_gpu_li_noalloc(self.h, ctypes.c_void_p(rays_dev), ctypes.c_void_p(seeds_dev), n,
+                                                 max_depth, rr_depth, ctypes.c_void_p(out_dev)), "li_noalloc")
 
     def reset_film(self):
         self._check(self.lib.nori_gpu_reset_film(self.h), "reset_film")
