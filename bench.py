@@ -227,34 +227,105 @@ def batched_views(ng, cam):
     return [ng.make_view(ng.lookat_camera_like(cam, w, h), spp, 12) for (w, h), spp in zip(BATCHED_SIZES, BATCHED_SPP)]
 
 
-def time_to_rel_mse(gpu, ng, cam, depth, ref, ref_err, start_spp, max_spp, thresh, guided, seed):
+def time_to_rel_mse(gpu, ng, cam, depth, ref, ref_err, start_spp, max_spp, thresh, guided, seed, train=()):
     """Progressive render against `ref` at 1, 2, 4, ... times start_spp
     samples per pixel, until the relMSE (less the reference's own error
-    ref_err) reaches thresh or max_spp.  Returns the GPU milliseconds of the
-    renders (every kernel, guiding inference and training included), the spp
-    reached and the curve; past the last point the time to thresh is
-    extrapolated with relMSE ~ 1/N (labelled)."""
+    ref_err) reaches thresh or max_spp.  Guided: the network starts from its
+    initialisation and trains online during the passes listed in `train`
+    ((spp, Adam learning rate x 1e6) each, 32 steps per pool round), which
+    count toward the image and the time; it is frozen afterwards.  Returns the
+    GPU milliseconds of the renders (every kernel, guiding inference and
+    training included), the spp reached and the curve; past the last point
+    the time to thresh is extrapolated with relMSE ~ 1/N (labelled)."""
     gpu.set_option("guiding", 1 if guided else 0)
-    if guided:
-        gpu.set_option("guide_reset", 1)
     gpu.reset_film()
     ms, spp, curve, e = 0.0, 0, [], float("inf")
-    target = start_spp
-    while spp < max_spp:
+
+    def rmse(img):
+        return max(rel_mse(img[:, :3] / img[:, 3:4], ref) - ref_err, 0.0)
+
+    steps = 0
+    if guided:
+        gpu.set_option("guide_reset", 1)
+        gpu.set_option("guide_train_steps", 32)
+        for chunk, lr in train:
+            gpu.set_option("guide_lr_micro", lr)
+            img = gpu.render(ng.make_view(cam, chunk, depth, sample_offset=spp), seed=seed)
+            st = gpu.stats()
+            ms += st.ms_total
+            steps += st.guide_train_steps
+            spp += chunk
+        gpu.set_option("guide_train_steps", 0)
+        if spp:
+            e = rmse(img)
+            curve.append((spp, round(ms, 2), float(f"{e:.4g}")))
+    target = max(start_spp, 2 * spp)
+    while spp < max_spp and e > thresh:
         chunk = min(target - spp, max_spp - spp)
         img = gpu.render(ng.make_view(cam, chunk, depth, sample_offset=spp), seed=seed)
         ms += gpu.stats().ms_total
         spp += chunk
-        e = max(rel_mse(img[:, :3] / img[:, 3:4], ref) - ref_err, 0.0)
+        e = rmse(img)
         curve.append((spp, round(ms, 2), float(f"{e:.4g}")))
-        if e <= thresh:
-            break
         target *= 2
     gpu.set_option("guiding", 0)
+    gpu.set_option("guide_train_steps", 4)
+    gpu.set_option("guide_lr_micro", 2000)
     reached = e <= thresh
-    return {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": reached,
-            "ms_to_threshold": round(ms if reached else ms * e / thresh, 1), "extrapolated": not reached,
-            "curve": curve}
+    out = {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": reached,
+           "ms_to_threshold": round(ms if reached else ms * e / thresh, 1), "extrapolated": not reached,
+           "curve": curve}
+    if guided:
+        out["train_steps"] = steps
+        out["train_passes"] = [list(t) for t in train]
+    return out
+
+
+def time_to_rel_mse_legs(gpu, ng, sc, args):
+    """Plain and guided time to relMSE <= threshold on the Disney grid at
+    512x512, depth 12, once with the reference's envmap sampling weights
+    (texel luminance) and once with the 3 x 3 footprint weights.  Per mode the
+    reference is two independent plain renders of mse_ref_spp/2 each; their
+    mutual relMSE is four times the full reference's own error, which is
+    subtracted from each leg's measured relMSE."""
+    cam = sc.camera
+    train = [(64, 2000)] * 8 + [(64, 500)] * 4
+    modes = {}
+    for mode, name in ((0, "texel"), (1, "footprint")):
+        gpu.set_option("env_sampling", mode)
+        gpu.upload(sc)
+        gpu.set_option("guiding", 0)
+        half = args.mse_ref_spp // 2
+        refs = []
+        t0 = time.time()
+        for k in range(2):
+            gpu.reset_film()
+            img = gpu.render(ng.make_view(cam, half, 12, sample_offset=(1 << 24) + k * half), seed=1234 + k)
+            refs.append(img[:, :3] / img[:, 3:4])
+        ref = 0.5 * (refs[0] + refs[1])
+        ref_err = rel_mse(refs[0], refs[1]) / 4
+        log(f"[bench] relMSE reference ({name} envmap sampling): 512x512 at {args.mse_ref_spp} spp in "
+            f"{time.time() - t0:.1f}s, own relMSE {ref_err:.3g}")
+        tt = {"ref_rel_mse": ref_err}
+        for leg, guided, seed in (("plain", False, 77), ("guided", True, 78)):
+            tt[leg] = time_to_rel_mse(gpu, ng, cam, 12, ref, ref_err, args.mse_start_spp, args.mse_max_spp,
+                                      args.mse_threshold, guided, seed, train)
+            r = tt[leg]
+            log(f"[bench] {name} {leg}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
+        tt["speedup_guided_over_plain"] = round(tt["plain"]["ms_to_threshold"] / tt["guided"]["ms_to_threshold"], 3)
+        modes[name] = tt
+    gpu.set_option("env_sampling", 0)  # for later uploads
+    t, f = modes["texel"], modes["footprint"]
+    return {"threshold": args.mse_threshold, "image": "512x512", "depth": 12, "ref_spp": args.mse_ref_spp,
+            "envmap_sampling": modes,
+            # the network's own gain, both legs with the same envmap sampling
+            "speedup_guided_over_plain": f["speedup_guided_over_plain"],
+            "speedup_guided_over_plain_texel_sampling": t["speedup_guided_over_plain"],
+            # footprint weights alone, plain against plain
+            "speedup_footprint_over_texel_plain": round(t["plain"]["ms_to_threshold"] /
+                                                        f["plain"]["ms_to_threshold"], 3),
+            "speedup_footprint_guided_over_texel_plain": round(t["plain"]["ms_to_threshold"] /
+                                                               f["guided"]["ms_to_threshold"], 3)}
 
 
 def main():
@@ -350,32 +421,7 @@ def main():
                          f"envmap, 512x512, 256 spp per step, max depth 12, plain path_mis")
         legs["disney"] = m
         if not args.no_time_to_mse and args.only is None:
-            # Reference: two independent plain renders of mse_ref_spp/2 each;
-            # their mutual relMSE is four times the full reference's own error,
-            # which is subtracted from each leg's measured relMSE.
-            gpu.set_option("guiding", 0)
-            half = args.mse_ref_spp // 2
-            refs = []
-            t0 = time.time()
-            for k in range(2):
-                gpu.reset_film()
-                img = gpu.render(ng.make_view(cam, half, 12, sample_offset=(1 << 24) + k * half), seed=1234 + k)
-                refs.append(img[:, :3] / img[:, 3:4])
-            ref = 0.5 * (refs[0] + refs[1])
-            ref_err = rel_mse(refs[0], refs[1]) / 4
-            log(f"[bench] relMSE reference: 512x512 at {args.mse_ref_spp} spp in {time.time() - t0:.1f}s, own "
-                f"relMSE {ref_err:.3g}")
-            tt = {}
-            for name, guided, seed in (("plain", False, 77), ("guided", True, 78)):
-                tt[name] = time_to_rel_mse(gpu, ng, cam, 12, ref, ref_err, args.mse_start_spp, args.mse_max_spp,
-                                           args.mse_threshold, guided, seed)
-                r = tt[name]
-                log(f"[bench] {name}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
-            legs["time_to_rel_mse"] = {
-                "threshold": args.mse_threshold, "image": "512x512", "ref_spp": args.mse_ref_spp,
-                "ref_rel_mse": ref_err, "plain": tt["plain"], "guided": tt["guided"],
-                "speedup_guided_over_plain": round(tt["plain"]["ms_to_threshold"] / tt["guided"]["ms_to_threshold"],
-                                                   3)}
+            legs["time_to_rel_mse"] = time_to_rel_mse_legs(gpu, ng, sc, args)
 
     head = legs["batched"] if "batched" in legs else legs["disney"]
     result = {

@@ -303,7 +303,15 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * training round, default 4),
  * "guide_train_batch" (samples per step, default 16384), "guide_lr_micro"
  * (learning rate x 1e6, default 2000), "guide_reset" (next use re-initialises
- * the network). */
+ * the network), "guide_min_rough_milli" (0 .. 1001: guided vertices are the
+ * diffuse ones and the microfacet / Disney ones whose alpha / roughness is at
+ * least this / 1000, default 500), "env_sampling" (envmap sampling weights,
+ * taking effect at the next nori_gpu_upload: 0, the default, the reference's
+ * texel luminance, src/gpu/DXRApp_Textures.cpp:281-390; 1, the largest
+ * luminance of each texel's 3 x 3 neighbourhood, which bounds the bilinear
+ * radiance inside the texel, so texels next to a small bright sun are drawn as
+ * often as the radiance they return calls for; images converge to the same
+ * values). */
 int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
 
 /* ---------------------------------------------------- denoiser features */

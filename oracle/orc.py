@@ -46,6 +46,7 @@ def lib():
         L.orc_camera_rays.argtypes = [vp, u32, vp, vp]
         L.orc_alias_build.argtypes = [vp, u32, vp, vp]
         L.orc_env_tables.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_env_footprint.argtypes = [vp, i32]
         L.orc_env_sample.argtypes = [vp, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
         L.orc_env_eval_pdf.argtypes = [vp, u32, vp, vp, vp]
         L.orc_bsdf_batch.argtypes = [i32, i32, u32, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -185,7 +186,9 @@ def microfacet_fn(fn, v, wh, a):
 class OracleScene:
     """The oracle's own copy of a flat scene: Nori BVH2, emitter CDFs, envmap tables."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, env_footprint=False):
+        """env_footprint: envmap sampling weights from each texel's 3 x 3
+        neighbourhood maximum (the GPU's option env_sampling = 1)."""
         self.scene = scene
         L = lib()
         env = scene.env_rgb
@@ -210,6 +213,8 @@ class OracleScene:
             _p(scene.mesh_tri_count), 0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
             _p(env), scene.env_scale, _p(getattr(scene, "uvs", None)), _p(getattr(scene, "tangents", None)),
             desc_vols[0], desc_vols[1], len(texs), ctypes.cast(tex_structs, ctypes.c_void_p))
+        if env_footprint:
+            L.orc_env_footprint(self.h, 1)
 
     def texture_sample(self, tex, uv, lod):
         """Trilinear lookups of the scene's textures: tex (n,), uv (n, 2), lod (n,) -> RGBA (n, 4)."""

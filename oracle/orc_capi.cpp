@@ -264,6 +264,14 @@ void orc_camera_rays(const Camera *cam, uint32_t n, const float *samples4, float
 // ---------------------------------------------------------------- envmap
 void orc_alias_build(const float *w, uint32_t n, float *prob, uint32_t *alias) { build_alias(w, n, prob, alias); }
 
+// Rebuild the scene's envmap tables with (1) or without (0) the footprint weights.
+void orc_env_footprint(void *hp, int footprint) {
+    Envmap &e = ((Handle *)hp)->s.env;
+    if (!e.valid()) return;
+    const std::vector<float> rgb = e.rgb;
+    e.build(e.W, e.H, rgb.data(), e.scale, footprint != 0);
+}
+
 void orc_env_tables(void *hp, float *marginal, float *conditional, float *ppix, float *prob, uint32_t *alias) {
     const Envmap &e = ((Handle *)hp)->s.env;
     if (marginal) std::memcpy(marginal, e.marginal.data(), e.marginal.size() * 4);

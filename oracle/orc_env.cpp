@@ -1,5 +1,6 @@
 // CPU oracle — test infrastructure only.  See orc_env.h.
 #include "orc_env.h"
+#include <algorithm>
 
 namespace orc {
 
@@ -27,10 +28,29 @@ void build_alias(const float *w, uint32_t n, float *prob, uint32_t *alias) {
     for (uint32_t i : small) { prob[i] = 1.0f; alias[i] = i; }
 }
 
-void Envmap::build(uint32_t w, uint32_t h, const float *rgb3, float sc) {
+void Envmap::build(uint32_t w, uint32_t h, const float *rgb3, float sc, bool footprint) {
     W = w; H = h; scale = sc;
     rgb.assign(rgb3, rgb3 + (size_t)w * h * 3);
     const float PI = 3.14159265358979323846f;
+    // Texel luminance; in footprint mode (no counterpart in the reference) the
+    // largest luminance of the 3 x 3 neighbourhood, U wrapped and V clamped,
+    // which bounds the bilinear radiance anywhere inside the texel.
+    std::vector<float> lum((size_t)W * H);
+    for (size_t i = 0; i < lum.size(); ++i) {
+        const float *p = &rgb[i * 3];
+        lum[i] = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+    }
+    if (footprint) {
+        std::vector<float> mx(lum.size());
+        for (int y = 0; y < (int)H; ++y)
+            for (int x = 0; x < (int)W; ++x) {
+                float v = 0.0f;
+                for (int yy = std::max(y - 1, 0); yy <= std::min(y + 1, (int)H - 1); ++yy)
+                    for (int k = -1; k <= 1; ++k) v = std::max(v, lum[(size_t)yy * W + (size_t)((x + k + (int)W) % (int)W)]);
+                mx[(size_t)y * W + x] = v;
+            }
+        lum.swap(mx);
+    }
     conditional.assign((size_t)H * (W + 1), 0.0f);
     std::vector<float> row_sums(H);
     for (uint32_t y = 0; y < H; ++y) {
@@ -40,8 +60,7 @@ void Envmap::build(uint32_t w, uint32_t h, const float *rgb3, float sc) {
         conditional[base] = 0.0f;
         float rs = 0.0f;
         for (uint32_t x = 0; x < W; ++x) {
-            const float *p = &rgb[((size_t)y * W + x) * 3];
-            float wgt = (0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2]) * st;
+            float wgt = lum[(size_t)y * W + x] * st;
             if (wgt < 0.0f) wgt = 0.0f;
             rs += wgt;
             conditional[base + x + 1] = rs;

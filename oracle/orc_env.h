@@ -26,7 +26,8 @@ struct Envmap {
     std::vector<uint32_t> alias_idx; // H*W
 
     bool valid() const { return W > 0 && H > 0; }
-    void build(uint32_t w, uint32_t h, const float *rgb3, float scale);
+    // footprint: 3 x 3 neighbourhood maximum weights (orc_env.cpp); false: the reference's
+    void build(uint32_t w, uint32_t h, const float *rgb3, float scale, bool footprint = false);
     V3f texel(int x, int y) const {
         const float *p = &rgb[((size_t)y * W + x) * 3];
         return V3f(p[0], p[1], p[2]);

@@ -79,7 +79,7 @@ Decoded decode(const double *o) {
     for (int j = 0; j < 8; ++j) { d.w[j] = std::exp(o[32 + j] - mx); sum += d.w[j]; }
     for (int j = 0; j < 8; ++j) d.w[j] /= sum;
     d.s = 1.0 / (1.0 + std::exp(-o[40]));
-    d.c = 0.1 + 0.8 * d.s;
+    d.c = 0.1 + 0.5 * d.s;
     return d;
 }
 inline double vmf(double k, double m) { return k / (2.0 * PI_D * (1.0 - std::exp(-2.0 * k))) * std::exp(k * (m - 1.0)); }
@@ -109,7 +109,7 @@ double guide_loss_dout(const double *out, const double w[3], double b, double we
     double P = d.c * q + (1.0 - d.c) * b;
     if (!(P > 0.0) || weight == 0.0) return 0.0;
     double dP = -weight / P;
-    g[40] = dP * (q - b) * 0.8 * d.s * (1.0 - d.s);
+    g[40] = dP * (q - b) * 0.5 * d.s * (1.0 - d.s);
     double dq = dP * d.c;
     for (int j = 0; j < 8; ++j) {
         g[32 + j] = dq * d.w[j] * (v[j] - q);

@@ -10,7 +10,7 @@
 //   MLP       64 -> 64 -> 64 -> 64 (ReLU) -> 64 outputs, of which 41 are used
 //   outputs   8 von Mises-Fisher lobes: mean = normalize(o[3j..3j+2]),
 //             kappa = exp(clamp(o[24+j], -4, 9)), weight = softmax(o[32..39]);
-//             selection probability c = 0.1 + 0.8 sigmoid(o[40])
+//             selection probability c = 0.1 + 0.5 sigmoid(o[40])
 //   density   P(w) = c q(w) + (1 - c) b(w), q the vMF mixture, b the BSDF pdf
 //             (one-sample MIS between the learned mixture and the BSDF)
 //   loss      -W log P(w) per training sample (W = throughput-normalised

@@ -121,12 +121,12 @@ struct nori_ctx {
         uint64_t bufs, scene, npix_all;
         uint32_t pool, cap, sampler, max_rec, guide_cap, rec_cap;
         const void *stats;
-        bool aov, guiding, sorted, persistent;
+        bool aov, guiding, records, sorted, persistent;
         uint32_t traversal;
         bool operator==(const GraphKey &o) const {
             return bufs == o.bufs && scene == o.scene && npix_all == o.npix_all && pool == o.pool && cap == o.cap &&
                    sampler == o.sampler && max_rec == o.max_rec && guide_cap == o.guide_cap && rec_cap == o.rec_cap &&
-                   stats == o.stats && aov == o.aov && guiding == o.guiding && sorted == o.sorted &&
+                   stats == o.stats && aov == o.aov && guiding == o.guiding && records == o.records && sorted == o.sorted &&
                    persistent == o.persistent && traversal == o.traversal;
         }
     };
@@ -149,8 +149,10 @@ struct nori_ctx {
     int guiding = 0;
     uint32_t g_train_steps = 4, g_train_batch = 16384, g_max_rec = 8;
     float g_lr = 2e-3f;
+    float g_min_rough = 0.5f; // guided vertices: diffuse, or alpha / roughness at least this
+    int env_footprint = 0;    // envmap sampling weights: 0 texel luminance, 1 3x3 neighbourhood maximum
     uint32_t g_train_seed = 1;
-    DevBuf g_rec, g_nrec, g_lpath, g_samples, g_scount;
+    DevBuf g_rec, g_nrec, g_lpath, g_samples, g_scount, g_pack;
     uint32_t g_rec_cap = 0;
     std::vector<hipEvent_t> ev_pool;
     struct Span {
@@ -213,6 +215,7 @@ struct nori_ctx {
         alloc(g_lpath, (size_t)n * sizeof(float4));
         alloc(g_samples, (size_t)n * 4 * sizeof(GuideSample));
         alloc(g_scount, 16);
+        alloc(g_pack, (size_t)n * sizeof(GuidePacked));
         g_rec_cap = n;
     }
     void ensure_guide_batch(uint32_t n) {
@@ -377,8 +380,9 @@ void pool_iteration(nori_ctx *c, uint32_t qi, float4 *film, const ViewTab *views
     }
     if (gsd.out) {
         Span t(c, 3);
-        launch_guide_feats(c->sdev, c->pd[qi], cnt + qi, (bf16_t *)c->g_act[0].p, n, c->stream);
-        guide_infer((const bf16_t *)c->g_act[0].p, c->gdev, (float *)c->g_out.p, n, cnt + qi, c->stream);
+        // sorted: only the surface queue can hold guided vertices
+        launch_guide_infer(c->sdev, c->pd[qi], sorted ? c->qc + c->cap : nullptr, sorted ? cnt + 7 : cnt + qi, n,
+                           c->gdev.w, c->gdev.params, (GuidePacked *)c->g_pack.p, c->stream);
     }
     {
         Span t(c, 2);
@@ -644,7 +648,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.has_env = 0;
         if (d->env_width && d->env_height && d->env_rgb) {
             EnvTables et;
-            build_env_tables(d->env_width, d->env_height, d->env_rgb, et);
+            build_env_tables(d->env_width, d->env_height, d->env_rgb, et, c->env_footprint != 0);
             s.env.texels = (const float4 *)c->upload(et.texels.data(), et.texels.size());
             s.env.ppix = c->upload(et.ppix.data(), et.ppix.size());
             s.env.alias = (const uint2 *)c->upload(et.alias.data(), et.alias.size());
@@ -729,6 +733,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.traversal = c->traversal;
         s.trace_waves = c->trace_waves;
         s.lds_per_cu = c->lds_per_cu;
+        s.guide_min_rough = c->g_min_rough;
         c->sdev = s;
         c->has_scene = true;
         ++c->scene_gen;
@@ -815,15 +820,18 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                 c->ensure_guide();
                 c->ensure_guide_batch(std::max(c->cap, c->g_train_batch));
                 c->ensure_guide_records(c->cap);
-                gsd = GuideShadeDev{(const float *)c->g_out.p, (GuideRecord *)c->g_rec.p, (uint32_t *)c->g_nrec.p,
-                                    (float4 *)c->g_lpath.p, c->cap, std::min(c->g_max_rec, max_depth)};
+                // training records only when this render trains
+                const bool rec = c->g_train_steps > 0;
+                gsd = GuideShadeDev{(const GuidePacked *)c->g_pack.p, rec ? (GuideRecord *)c->g_rec.p : nullptr,
+                                    rec ? (uint32_t *)c->g_nrec.p : nullptr, rec ? (float4 *)c->g_lpath.p : nullptr,
+                                    c->cap, std::min(c->g_max_rec, max_depth)};
                 launch_set_counter((uint32_t *)c->g_scount.p, 0, c->stream);
             }
             c->sdev.sobol = c->sampler;
             const bool graphed = c->use_graph && !c->timing_mode;
             if (graphed) {
                 nori_ctx::GraphKey key{c->buf_gen, c->scene_gen, (uint64_t)c->npix_all, c->pool, c->cap, c->sampler,
-                                       gsd.max_rec, c->g_cap, c->g_rec_cap, cs, c->film_aov, gsd.out != nullptr,
+                                       gsd.max_rec, c->g_cap, c->g_rec_cap, cs, c->film_aov, gsd.out != nullptr, gsd.rec != nullptr,
                                        c->sort_shading != 0, c->persistent != 0, c->traversal};
                 if (!c->graph || !(key == c->graph_key)) {
                     c->drop_graph();
@@ -1084,6 +1092,15 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
     } else if (n == "guide_lr_micro") {
         if (value <= 0 || value > 1000000) return fail(c, NORI_E_ARG, "guide_lr_micro out of range (0, 1e6]");
         c->g_lr = (float)value * 1e-6f;
+    } else if (n == "env_sampling") {
+        if (value != 0 && value != 1)
+            return fail(c, NORI_E_ARG, "env_sampling must be 0 (texel luminance) or 1 (3x3 footprint maximum)");
+        c->env_footprint = (int)value; // takes effect at the next scene upload
+    } else if (n == "guide_min_rough_milli") {
+        if (value < 0 || value > 1001) return fail(c, NORI_E_ARG, "guide_min_rough_milli out of range [0, 1001]");
+        c->g_min_rough = (float)value * 1e-3f;
+        c->sdev.guide_min_rough = c->g_min_rough;
+        c->drop_graph();
     } else if (n == "guide_reset") {
         c->guide_ready = false;
     } else {

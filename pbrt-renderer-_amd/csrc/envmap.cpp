@@ -7,14 +7,44 @@
 // marginal accumulated in double.  The alias table is Vose's method over those
 // probabilities; its worklists are stacks filled in texel order, so the table
 // (and every draw from it) is a deterministic function of the envmap.
+//
+// Footprint mode (not in the reference): the luminance of a texel is replaced
+// by the largest luminance of its 3 x 3 neighbourhood (U wrapped, V clamped).
+// Radiance is interpolated bilinearly between texel centres, so inside a texel
+// it never exceeds that maximum; a dim texel next to a bright one (the rim of
+// a small sun) is then drawn in proportion to the radiance it can return, and
+// next-event samples there no longer carry radiance / pdf ratios hundreds of
+// times the mean.
 #include "envmap.h"
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
 namespace ng {
 
-void build_env_tables(uint32_t W, uint32_t H, const float *rgb, EnvTables &t) {
+void build_env_tables(uint32_t W, uint32_t H, const float *rgb, EnvTables &t, bool footprint) {
     const float PI_F = 3.14159265358979323846f;
+    std::vector<float> lum((size_t)W * H);
+    for (size_t i = 0; i < (size_t)W * H; ++i) {
+        const float *p = rgb + i * 3;
+        lum[i] = 0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2];
+    }
+    if (footprint) {
+        std::vector<float> m(lum.size());
+        for (uint32_t y = 0; y < H; ++y)
+            for (uint32_t x = 0; x < W; ++x) {
+                float v = lum[(size_t)y * W + x];
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const uint32_t yy = (uint32_t)std::min(std::max((int)y + dy, 0), (int)H - 1);
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const uint32_t xx = (x + W + (uint32_t)(dx + 1) - 1u) % W;
+                        v = std::max(v, lum[(size_t)yy * W + xx]);
+                    }
+                }
+                m[(size_t)y * W + x] = v;
+            }
+        lum.swap(m);
+    }
     std::vector<float> cond((size_t)H * (W + 1)), rows(H), marg(H + 1);
     for (uint32_t y = 0; y < H; ++y) {
         float st = std::sin(PI_F * (y + 0.5f) / (float)H);
@@ -22,8 +52,7 @@ void build_env_tables(uint32_t W, uint32_t H, const float *rgb, EnvTables &t) {
         c[0] = 0.0f;
         float s = 0.0f;
         for (uint32_t x = 0; x < W; ++x) {
-            const float *p = rgb + ((size_t)y * W + x) * 3;
-            float w = (0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2]) * st;
+            float w = lum[(size_t)y * W + x] * st;
             s += w < 0.0f ? 0.0f : w;
             c[x + 1] = s;
         }

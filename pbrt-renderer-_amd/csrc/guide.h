@@ -9,11 +9,15 @@
 //             (sin, cos)(2^k pi p) (48); shading normal (3); direction to the
 //             viewer (3); roughness, metallic; 8 zeros
 //   outputs   mean_j = normalize(o[3j..3j+2]), kappa_j = exp(clamp(o[24+j], -4, 9)),
-//             weights = softmax(o[32..39]), c = 0.1 + 0.8 sigmoid(o[40])
+//             weights = softmax(o[32..39]), c = 0.1 + 0.5 sigmoid(o[40])
+//             (c at most 0.6: a BSDF-sampled direction's weight is at most
+//             1 / (1 - c) = 2.5 times the plain estimator's, however wrong q is)
 //   training  minimises -W log P(w_s) over samples the renderer records (the
 //             sample gradient of KL(target || P); Mueller et al., TOG 2019),
 //             W = luminance of the radiance that arrived along w_s divided by
-//             the throughput before the sampling vertex; Adam (Kingma & Ba).
+//             the throughput before the sampling vertex, less what an emitter
+//             or the envmap hit right at the next vertex gave (light NEE also
+//             samples); Adam (Kingma & Ba).
 // Matrix products use BF16 MFMA with FP32 accumulation
 // (v_mfma_f32_32x32x16_bf16 on gfx950); parameters are FP32 masters.
 #pragma once
@@ -45,7 +49,7 @@ struct GuideRecord {
     float4 wi_b;       // direction to the viewer, BSDF pdf of the sampled direction
     float4 w_valid;    // sampled direction (world)
     float4 thr;        // throughput before the vertex
-    float4 lsnap;      // path radiance collected up to and including this vertex's NEE
+    float4 lsnap;      // path radiance collected up to and including this vertex's NEE; bounce index
 };
 
 // Training sample after the path has finished (4 float4).
@@ -56,20 +60,31 @@ struct GuideSample {
     float4 w_weight;
 };
 
-NG_HD void guide_features(f3 p01, f3 n, f3 wi, float rough, float metal, float *f) {
-    const float pa[3] = {p01.x, p01.y, p01.z};
-    for (int a = 0; a < 3; ++a)
-        for (int k = 0; k < 8; ++k) {
-            float s, c;
-            sincosf((float)(1 << k) * PI * pa[a], &s, &c);
-            f[a * 16 + 2 * k] = s;
-            f[a * 16 + 2 * k + 1] = c;
+// Features 16s + 8h .. 16s + 8h + 7 of one query (s = 0..3, h = 0..1) in the
+// layout above: octaves 4h..4h+3 of axis s for s < 3, then the normal,
+// direction, roughness and metallic (h = 0) or the zero padding (h = 1).
+NG_HD void guide_feature_block(f3 p01, f3 n, f3 wi, float rough, float metal, int s, int h, float *f) {
+    if (s < 3) {
+        const float p = s == 0 ? p01.x : (s == 1 ? p01.y : p01.z);
+        for (int k = 0; k < 4; ++k) {
+            float sn, cs;
+            sincosf((float)(1 << (4 * h + k)) * PI * p, &sn, &cs);
+            f[2 * k] = sn;
+            f[2 * k + 1] = cs;
         }
-    f[48] = n.x; f[49] = n.y; f[50] = n.z;
-    f[51] = wi.x; f[52] = wi.y; f[53] = wi.z;
-    f[54] = rough;
-    f[55] = metal;
-    for (int k = 56; k < 64; ++k) f[k] = 0.0f;
+    } else if (h == 0) {
+        f[0] = n.x; f[1] = n.y; f[2] = n.z;
+        f[3] = wi.x; f[4] = wi.y; f[5] = wi.z;
+        f[6] = rough;
+        f[7] = metal;
+    } else {
+        for (int k = 0; k < 8; ++k) f[k] = 0.0f;
+    }
+}
+
+NG_HD void guide_features(f3 p01, f3 n, f3 wi, float rough, float metal, float *f) {
+    for (int s = 0; s < 4; ++s)
+        for (int h = 0; h < 2; ++h) guide_feature_block(p01, n, wi, rough, metal, s, h, f + 16 * s + 8 * h);
 }
 
 // Decoded mixture of one query.
@@ -97,7 +112,48 @@ NG_HD GuideMix guide_decode(const float *o) {
     }
     float inv = 1.0f / s;
     for (int j = 0; j < 8; ++j) g.w[j] *= inv;
-    g.c = 0.1f + 0.8f / (1.0f + expf(-o[40]));
+    g.c = 0.1f + 0.5f / (1.0f + expf(-o[40]));
+    return g;
+}
+
+// Mixture of one query as the render loop keeps it between inference and
+// shading (96 bytes instead of 64 FP32 outputs): FP16 unit lobe axes,
+// concentrations, lobe weights and c.  guide_unpack derives everything the
+// sampler and the density use from these stored values alone (axes normalised
+// again, weights divided by their sum), so sampling and density evaluation
+// describe the same distribution.
+struct alignas(16) GuidePacked {
+    _Float16 v[48];
+};
+
+NG_HD void guide_pack(const float *o, GuidePacked &p) {
+    const GuideMix g = guide_decode(o);
+    for (int j = 0; j < 8; ++j) {
+        p.v[3 * j] = (_Float16)g.mu[j].x;
+        p.v[3 * j + 1] = (_Float16)g.mu[j].y;
+        p.v[3 * j + 2] = (_Float16)g.mu[j].z;
+        p.v[24 + j] = (_Float16)g.kappa[j];
+        p.v[32 + j] = (_Float16)g.w[j];
+    }
+    p.v[40] = (_Float16)g.c;
+    for (int k = 41; k < 48; ++k) p.v[k] = (_Float16)0.0f;
+}
+
+NG_HD GuideMix guide_unpack(const GuidePacked &p) {
+    GuideMix g;
+    float s = 0.0f;
+    for (int j = 0; j < 8; ++j) {
+        const f3 r = mk((float)p.v[3 * j], (float)p.v[3 * j + 1], (float)p.v[3 * j + 2]);
+        g.mu[j] = r * (1.0f / fmaxf(sqrtf(dot(r, r)), 1e-12f));
+        const float k = (float)p.v[24 + j];
+        g.kappa[j] = k;
+        g.norm[j] = k / (2.0f * PI * (1.0f - expf(-2.0f * k)));
+        g.w[j] = (float)p.v[32 + j];
+        s += g.w[j];
+    }
+    const float inv = 1.0f / s;
+    for (int j = 0; j < 8; ++j) g.w[j] *= inv;
+    g.c = (float)p.v[40];
     return g;
 }
 
@@ -141,8 +197,8 @@ NG_HD float guide_loss_dout(const float *o, f3 w, float b, float weight, float *
     float P = m.c * q + (1.0f - m.c) * b;
     if (!(P > 0.0f) || weight == 0.0f) return 0.0f;
     float dP = -weight / P;
-    float s = (m.c - 0.1f) / 0.8f;
-    g[40] = dP * (q - b) * 0.8f * s * (1.0f - s);
+    float s = (m.c - 0.1f) / 0.5f;
+    g[40] = dP * (q - b) * 0.5f * s * (1.0f - s);
     float dq = dP * m.c;
     for (int j = 0; j < 8; ++j) {
         g[32 + j] = dq * m.w[j] * (v[j] - q);

@@ -7,27 +7,22 @@
 // the same k; the 32x32 FP32 result gives lane l column l%32, rows
 // 8(v/4) + 4(l/32) + v%4 for v = 0..15.  With M stored [n][k], both operand
 // fragments are single 16-byte loads.
-#include "guide.h"
+#include "guide_mlp.h"
 
 namespace ng {
 
 namespace {
 
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 constexpr int LAYER_BLOCK = 256; // 4 waves x 32 rows
-constexpr int WGRAD_CHUNK = 256; // rows per wgrad block
+constexpr int WGRAD_CHUNK = 128; // rows per wgrad block
 
-__device__ __forceinline__ bf16x8 load8(const bf16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
-
-__global__ __launch_bounds__(LAYER_BLOCK) void k_layer(const bf16_t *__restrict__ X, const bf16_t *__restrict__ M,
+__global__ __launch_bounds__(64) void k_layer(const bf16_t *__restrict__ X, const bf16_t *__restrict__ M,
                                                        const float *__restrict__ bias, void *Y,
                                                        const bf16_t *__restrict__ Hmask, int mode, uint32_t cap,
                                                        const uint32_t *count) {
     const uint32_t rows = count ? min(*count, cap) : cap;
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t row0 = (blockIdx.x * (LAYER_BLOCK / 64) + wave) * 32u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t row0 = blockIdx.x * 32u; // one wave per block: batches of training rows fill more CUs
     if (row0 >= rows) return;
     const uint32_t half = lane >> 5, col = lane & 31u;
     bf16x8 bfr[2][4], afr[4];
@@ -69,22 +64,8 @@ __global__ __launch_bounds__(LAYER_BLOCK) void k_layer(const bf16_t *__restrict_
     }
 }
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    const bf16_t x = (bf16_t)a, y = (bf16_t)b;
-    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
-}
-
-// Inference through all four layers in one kernel, one wave per 32 rows.  The
-// product is taken transposed, Z^T = W X^T (W as the A operand, rows of X as
-// the B columns), so a layer's outputs land one row per lane: lane l holds row
-// l%32 and outputs n = 32t + 8q + 4h + r in acc[t][4q + r] (h = l/32).  The
-// next layer's B fragment of k block s (k = 16s + 8h .. +7) is half in the
-// lane (q = 2(s%2) + h, offsets 4h..4h+3) and half in lane l^32, so one
-// exchange of four BF16 values per block replaces the activation round trip
-// through memory.  Rounding as in k_layer: FP32 accumulation, + bias, ReLU,
-// BF16 between layers; the last layer's 64 outputs are stored in FP32.
+// Inference through all four layers in one kernel, one wave per 32 rows
+// (mlp_forward_wave); the last layer's 64 outputs are stored in FP32.
 __global__ __launch_bounds__(LAYER_BLOCK) void k_mlp_infer(const bf16_t *__restrict__ X, const bf16_t *__restrict__ W,
                                                            const float *__restrict__ params, float *__restrict__ out,
                                                            uint32_t cap, const uint32_t *count) {
@@ -98,50 +79,15 @@ __global__ __launch_bounds__(LAYER_BLOCK) void k_mlp_infer(const bf16_t *__restr
     bf16x8 xb[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) xb[s] = live ? load8(X + (size_t)r * 64 + 16 * s + 8 * h) : bf16x8{};
-#pragma unroll
-    for (int l = 0; l < G_LAYERS; ++l) {
-        const bf16_t *Wl = W + l * 4096;
-        const float *bl = params + l * G_LAYER_PARAMS + 4096;
-        f32x16 acc[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            acc[t] = f32x16{};
-#pragma unroll
-            for (int s = 0; s < 4; ++s)
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load8(Wl + (32 * t + col) * 64 + 16 * s + 8 * h), xb[s],
-                                                                 acc[t], 0, 0, 0);
-        }
+    f32x16 acc[2];
+    mlp_forward_wave(xb, W, params, acc);
+    if (live) {
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const float val = acc[t][v] + bl[32 * t + 8 * (v >> 2) + 4 * h + (v & 3)];
-                acc[t][v] = l < G_LAYERS - 1 ? fmaxf(val, 0.0f) : val;
-            }
-        if (l == G_LAYERS - 1) {
-            if (live) {
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        *reinterpret_cast<float4 *>(out + (size_t)r * 64 + 32 * t + 8 * q + 4 * h) =
-                            make_float4(acc[t][4 * q], acc[t][4 * q + 1], acc[t][4 * q + 2], acc[t][4 * q + 3]);
-            }
-            break;
-        }
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int t = s >> 1, q0 = 2 * (s & 1);
-            // q0 block (offsets 0..3 of k block s), q0 + 1 block (offsets 4..7)
-            const uint32_t lo0 = pack_bf16x2(acc[t][4 * q0], acc[t][4 * q0 + 1]);
-            const uint32_t lo1 = pack_bf16x2(acc[t][4 * q0 + 2], acc[t][4 * q0 + 3]);
-            const uint32_t hi0 = pack_bf16x2(acc[t][4 * q0 + 4], acc[t][4 * q0 + 5]);
-            const uint32_t hi1 = pack_bf16x2(acc[t][4 * q0 + 6], acc[t][4 * q0 + 7]);
-            const uint32_t keep0 = h ? hi0 : lo0, keep1 = h ? hi1 : lo1;
-            const uint32_t recv0 = __shfl_xor(h ? lo0 : hi0, 32), recv1 = __shfl_xor(h ? lo1 : hi1, 32);
-            const u32x4 v4 = h ? u32x4{recv0, recv1, keep0, keep1} : u32x4{keep0, keep1, recv0, recv1};
-            xb[s] = __builtin_bit_cast(bf16x8, v4);
-        }
+            for (int q = 0; q < 4; ++q)
+                *reinterpret_cast<float4 *>(out + (size_t)r * 64 + 32 * t + 8 * q + 4 * h) =
+                    make_float4(acc[t][4 * q], acc[t][4 * q + 1], acc[t][4 * q + 2], acc[t][4 * q + 3]);
     }
 }
 
@@ -292,7 +238,7 @@ void guide_refresh(const GuideDev &g, hipStream_t st) {
 void guide_layer(const bf16_t *X, const bf16_t *M, const float *bias, void *Y, const bf16_t *Hmask, int mode,
                  uint32_t cap, const uint32_t *count, hipStream_t st) {
     if (cap == 0) return;
-    hipLaunchKernelGGL(k_layer, dim3(nblk(cap, 128)), dim3(LAYER_BLOCK), 0, st, X, M, bias, Y, Hmask, mode, cap, count);
+    hipLaunchKernelGGL(k_layer, dim3(nblk(cap, 32)), dim3(64), 0, st, X, M, bias, Y, Hmask, mode, cap, count);
 }
 
 void guide_infer(const bf16_t *X, const GuideDev &g, float *out, uint32_t cap, const uint32_t *count, hipStream_t st) {

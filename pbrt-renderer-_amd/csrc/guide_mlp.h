@@ -1,0 +1,87 @@
+// Device-side forward pass of the guiding network (see guide.h), one wave per
+// 32 rows, shared by the inference kernel over given features (guide.hip) and
+// the path-state kernel that computes the features itself (pt_kernels.hip).
+//
+// The product is taken transposed, Z^T = W X^T (W as the A operand of
+// v_mfma_f32_32x32x16_bf16, rows of X as the B columns), so a layer's outputs
+// land one row per lane: lane l holds row l%32 and outputs
+// n = 32t + 8q + 4h + r in acc[t][4q + r] (h = l/32).  The next layer's B
+// fragment of k block s (k = 16s + 8h .. +7) is half in the lane
+// (q = 2(s%2) + h, offsets 4h..4h+3) and half in lane l^32, so one exchange of
+// four BF16 values per block replaces the activation round trip through
+// memory.  Rounding as in the layer kernel: FP32 accumulation, + bias, ReLU,
+// BF16 between layers; the last layer's outputs stay FP32.
+#pragma once
+#include "guide.h"
+
+namespace ng {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 load8(const bf16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
+
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    const bf16_t x = (bf16_t)a, y = (bf16_t)b;
+    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+}
+
+// xb: the lane's input fragments (row l%32, k = 16s + 8(l/32) .. +7 for
+// s = 0..3).  On return acc holds the last layer's outputs in the layout above.
+// Every lane of the wave must call it (the exchanges are wave-wide).
+__device__ __forceinline__ void mlp_forward_wave(bf16x8 xb[4], const bf16_t *__restrict__ W,
+                                                 const float *__restrict__ params, f32x16 acc[2]) {
+    const uint32_t lane = threadIdx.x & 63u, h = lane >> 5, col = lane & 31u;
+#pragma unroll
+    for (int l = 0; l < G_LAYERS; ++l) {
+        const bf16_t *Wl = W + l * 4096;
+        const float *bl = params + l * G_LAYER_PARAMS + 4096;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            acc[t] = f32x16{};
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load8(Wl + (32 * t + col) * 64 + 16 * s + 8 * h), xb[s],
+                                                                 acc[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const float val = acc[t][v] + bl[32 * t + 8 * (v >> 2) + 4 * h + (v & 3)];
+                acc[t][v] = l < G_LAYERS - 1 ? fmaxf(val, 0.0f) : val;
+            }
+        if (l == G_LAYERS - 1) break;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int t = s >> 1, q0 = 2 * (s & 1);
+            // q0 block (offsets 0..3 of k block s), q0 + 1 block (offsets 4..7)
+            const uint32_t lo0 = pack_bf16x2(acc[t][4 * q0], acc[t][4 * q0 + 1]);
+            const uint32_t lo1 = pack_bf16x2(acc[t][4 * q0 + 2], acc[t][4 * q0 + 3]);
+            const uint32_t hi0 = pack_bf16x2(acc[t][4 * q0 + 4], acc[t][4 * q0 + 5]);
+            const uint32_t hi1 = pack_bf16x2(acc[t][4 * q0 + 6], acc[t][4 * q0 + 7]);
+            const uint32_t keep0 = h ? hi0 : lo0, keep1 = h ? hi1 : lo1;
+            const uint32_t recv0 = __shfl_xor(h ? lo0 : hi0, 32), recv1 = __shfl_xor(h ? lo1 : hi1, 32);
+            const u32x4 v4 = h ? u32x4{recv0, recv1, keep0, keep1} : u32x4{keep0, keep1, recv0, recv1};
+            xb[s] = __builtin_bit_cast(bf16x8, v4);
+        }
+    }
+}
+
+// The row's 64 outputs gathered into lane l%32's o[] (both lanes of the row
+// call it; lane l^32 gets the same array).
+__device__ __forceinline__ void mlp_row_outputs(const f32x16 acc[2], float o[64]) {
+    const uint32_t h = (threadIdx.x & 63u) >> 5;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const float mine = acc[t][v], other = __shfl_xor(mine, 32);
+            const int n = 32 * t + 8 * (v >> 2) + (v & 3);
+            o[n] = h ? other : mine;
+            o[n + 4] = h ? mine : other;
+        }
+}
+
+} // namespace ng

@@ -31,6 +31,7 @@ struct SceneDev {
     int has_env;
     EnvDev env;
     f3 bmin, binv; // scene bounds: position -> [0,1]^3 for the guiding features
+    float guide_min_rough; // guided vertices: diffuse, or microfacet alpha / Disney roughness at least this
     const VolumeDev *vols; // participating media (n_vol entries)
     uint32_t n_vol;
     // per mesh {extIOR, intIOR} of mirror/dielectric meshes, {0, 0} otherwise:
@@ -46,7 +47,7 @@ struct SceneDev {
 // Guiding hooks of one render batch.  out: network outputs per queue position
 // (null: plain BSDF sampling).  rec/nrec/lpath: training records (null: none).
 struct GuideShadeDev {
-    const float *out;
+    const GuidePacked *out; // per queue position: the mixture of a guided vertex
     GuideRecord *rec;
     uint32_t *nrec;
     float4 *lpath;
@@ -160,10 +161,12 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
                          float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st);
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, uint32_t *work, hipStream_t st);
-// Guiding: BF16 features of the hits at positions [0, *count) (zeros where
-// guiding does not apply).
-void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *count, bf16_t *X, uint32_t cap,
-                        hipStream_t st);
+// Guiding: the network's mixtures of the guided vertices (guide_vertex) among
+// the paths at positions q[0..*count) (q null: positions [0, *count)), with the
+// features computed from the path state; out[position] holds the packed
+// mixture.  W, params: BF16 weights [layer][out][in] and FP32 biases.
+void launch_guide_infer(const SceneDev &s, const PathsDev &p, const uint32_t *q, const uint32_t *count, uint32_t cap,
+                        const bf16_t *W, const float *params, GuidePacked *out, hipStream_t st);
 // Turn the records of the paths whose ids are listed in ids[0..*nids) into
 // training samples (appended at *count).
 void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, const uint32_t *ids, const uint32_t *nids,

@@ -8,6 +8,7 @@
 // (occlusion traversal, film accumulation).  Path state is structure-of-arrays
 // in float4s so every per-path access is a single 16-byte load or store.
 #include "bsdf_dev.h"
+#include "guide_mlp.h"
 #include "pt_kernels.h"
 #include "trace_dev.h"
 #include <algorithm>
@@ -660,6 +661,18 @@ __device__ __forceinline__ bool guide_eligible(uint32_t type) {
     return type == NORI_MAT_DIFFUSE || type == NORI_MAT_MICROFACET || type == NORI_MAT_DISNEY;
 }
 
+// Vertices that sample from the guide (and are recorded for its training):
+// non-emitting diffuse surfaces, and microfacet / Disney surfaces whose alpha /
+// roughness is at least s.guide_min_rough.  Glossier lobes are sampled well by
+// the BSDF alone, and the share of samples one-sample MIS gives the guide there
+// would be lost.  Inference and shading decide with this same predicate.
+__device__ __forceinline__ bool guide_vertex(const SceneDev &s, const nori_material &mr) {
+    if (mr.isEmitter) return false;
+    if (mr.type == NORI_MAT_DIFFUSE) return true;
+    if (mr.type == NORI_MAT_MICROFACET) return mr.alpha >= s.guide_min_rough;
+    return mr.type == NORI_MAT_DISNEY && mr.roughness >= s.guide_min_rough;
+}
+
 // Interpolated shading normal turned to the side of the geometric normal that
 // faces the ray (shaders/Shaders.hlsl:217-231).
 __device__ __forceinline__ void hit_frame(const SceneDev &s, uint32_t prim, float u, float v, f3 d, f3 &N, f3 &Ng,
@@ -944,6 +957,22 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         film_add(film, pixel, c);
         if (GUIDE && g.rec && finite3(c)) lpath += c;
     };
+    // An emitter or the envmap reached by the direction the previous vertex
+    // sampled: light that next-event estimation also covers there, so it is
+    // kept out of that vertex's training weight (it joins the record's
+    // snapshot, like the vertex's own NEE).  Trained on it, the guide would
+    // learn to aim at emitters NEE already samples, which raises its own MIS
+    // weight there and takes samples from indirect light.
+    auto add_direct = [&](f3 c) {
+        add_L(c);
+        if (GUIDE && g.rec && finite3(c) && bounce > 0) {
+            const uint32_t nr = g.nrec[id];
+            if (nr > 0 && nr <= g.max_rec) {
+                float4 &ls = g.rec[(size_t)(nr - 1) * g.cap + id].lsnap;
+                if (ls.w == (float)(bounce - 1)) ls = make_float4(ls.x + c.x, ls.y + c.y, ls.z + c.z, ls.w);
+            }
+        }
+    };
     // Denoiser features at the first non-delta interaction
     // (shaders/Shaders.hlsl:99-106, 195-200, 320-324, 475-479, 516-520).
     bool aov_done = (meta & 0x8000u) != 0;
@@ -1022,7 +1051,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         if (s.has_env) {
             f3 env = env_eval(s.env, d);
             f3 c = last_pdf == 0.0f ? T * env : T * env * balance_heuristic(last_pdf, env_pdf(s.env, d));
-            add_L(c);
+            add_direct(c);
         }
     } else {
         const f3 P = o + d * hit.x;
@@ -1043,7 +1072,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
             aov_take(mk(1.0f), N);
             f3 Le = f3{mr.radiance[0], mr.radiance[1], mr.radiance[2]};
             if (last_pdf == 0.0f) {
-                add_L(T * Le);
+                add_direct(T * Le);
             } else {
                 // shaders/Emitter.hlsli:91-103
                 f3 dv = P - o;
@@ -1057,7 +1086,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                         pe = (1.0f / (s.emit_area[k] * (float)s.n_emit)) * d2 / cl;
                     }
                 }
-                add_L(T * Le * balance_heuristic(last_pdf, pe));
+                add_direct(T * Le * balance_heuristic(last_pdf, pe));
             }
         } else {
             const uint32_t type = mr.type;
@@ -1217,9 +1246,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 };
                 // One-sample MIS with the learned mixture: the BSDF technique's
                 // density is P(w) = c q(w) + (1-c) b(w) wherever it enters MIS.
-                const bool guided = GUIDE && g.out != nullptr && !hair;
+                const bool guide_v = GUIDE && !hair && guide_vertex(s, mr);
+                const bool guided = guide_v && g.out != nullptr;
                 GuideMix mix;
-                if (guided) mix = guide_decode(g.out + (size_t)pos * 64);
+                if (guided) mix = guide_unpack(g.out[pos]);
                 auto tech_pdf = [&](f3 w_world, float b) { return guided ? mix.c * guide_q(mix, w_world) + (1.0f - mix.c) * b : b; };
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
@@ -1311,7 +1341,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                         }
                     }
                 }
-                if (GUIDE && alive && g.rec && !hair) {
+                if (guide_v && alive && g.rec) {
                     uint32_t k = g.nrec[id];
                     if (k < g.max_rec) {
                         GuideRecord &r = g.rec[(size_t)k * g.cap + id];
@@ -1321,7 +1351,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                         r.wi_b = make_float4(vv.x, vv.y, vv.z, b_pdf);
                         r.w_valid = make_float4(w_world.x, w_world.y, w_world.z, 1.0f);
                         r.thr = make_float4(T_before.x, T_before.y, T_before.z, 0.0f);
-                        r.lsnap = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
+                        r.lsnap = make_float4(lpath.x, lpath.y, lpath.z, (float)bounce);
                         g.nrec[id] = k + 1;
                         tag = id | 0x80000000u;
                     }
@@ -1364,28 +1394,57 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     }
 }
 
-// BF16 features of the surface hits in the queue, one row per queue position.
-__global__ void k_guide_feats(SceneDev s, PathsDev p, const uint32_t *__restrict__ count, bf16_t *X) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count) return;
-    const float4 hit = p.hit[i];
-    const uint32_t prim = __float_as_uint(hit.y);
-    float f[64];
+// Guide mixtures of the guided vertices among the queued paths, one wave per
+// 32 rows: both lanes of a row compute their half of its features from the
+// path state (guide_feature_block), the network runs in registers
+// (mlp_forward_wave) and the row's lane 0..31 packs the mixture.  Waves with
+// no guided vertex leave at once.
+__global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, const uint32_t *__restrict__ q,
+                                                     const uint32_t *__restrict__ count, uint32_t cap,
+                                                     const bf16_t *__restrict__ W, const float *__restrict__ params,
+                                                     GuidePacked *out) {
+    const uint32_t rows = min(*count, cap);
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t row0 = (blockIdx.x * 4u + wave) * 32u;
+    if (row0 >= rows) return;
+    const uint32_t h = lane >> 5, r = row0 + (lane & 31u);
+    uint32_t pos = 0;
     bool ok = false;
-    if (prim != MISS) {
-        const nori_material &mr = s.mats[s.tri_mesh[prim]];
-        if (!mr.isEmitter && guide_eligible(mr.type)) {
-            const f3 o = xyz(p.ray_o[i]), d = xyz(p.ray_d[i]);
-            const f3 P = o + d * hit.x;
-            f3 N, Ng;
-            bool back;
-            hit_frame(s, prim, hit.z, hit.w, d, N, Ng, back);
-            guide_features((P - s.bmin) * s.binv, N, -d, mr.roughness, mr.metallic, f);
-            ok = true;
+    f3 p01 = mk(0.0f), N = mk(0.0f), wi = mk(0.0f);
+    float rough = 0.0f, metal = 0.0f;
+    if (r < rows) {
+        pos = q ? q[r] : r;
+        const float4 hit = p.hit[pos];
+        const uint32_t prim = __float_as_uint(hit.y);
+        if (prim != MISS) {
+            const nori_material &mr = s.mats[s.tri_mesh[prim]];
+            if (guide_vertex(s, mr)) {
+                const f3 o = xyz(p.ray_o[pos]), d = xyz(p.ray_d[pos]);
+                f3 Ng;
+                bool back;
+                hit_frame(s, prim, hit.z, hit.w, d, N, Ng, back);
+                p01 = (o + d * hit.x - s.bmin) * s.binv;
+                wi = -d;
+                rough = mr.roughness;
+                metal = mr.metallic;
+                ok = true;
+            }
         }
     }
-    bf16_t *row = X + (size_t)i * 64;
-    for (int k = 0; k < 64; ++k) row[k] = (bf16_t)(ok ? f[k] : 0.0f);
+    if (__ballot(ok) == 0) return; // the same for every lane of the wave
+    bf16x8 xb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        float f[8];
+        guide_feature_block(p01, N, wi, rough, metal, b, (int)h, f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xb[b][k] = (bf16_t)f[k];
+    }
+    f32x16 acc[2];
+    mlp_forward_wave(xb, W, params, acc);
+    float o[64];
+    mlp_row_outputs(acc, o);
+    if (ok && h == 0) guide_pack(o, out[pos]);
 }
 
 // Training samples from the records of finished paths: the weight is the
@@ -1643,9 +1702,9 @@ void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *s
     if (n) hipLaunchKernelGGL(k_raygen_rays, dim3(blocks(n, 256)), dim3(256), 0, st, p, rays, seeds, first, n, film);
 }
 
-void launch_guide_feats(const SceneDev &s, const PathsDev &p, const uint32_t *count, bf16_t *X, uint32_t cap,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_feats, dim3(blocks(cap, 128)), dim3(128), 0, st, s, p, count, X);
+void launch_guide_infer(const SceneDev &s, const PathsDev &p, const uint32_t *q, const uint32_t *count, uint32_t cap,
+                        const bf16_t *W, const float *params, GuidePacked *out, hipStream_t st) {
+    if (cap) hipLaunchKernelGGL(k_guide_infer, dim3(blocks(cap, 128)), dim3(256), 0, st, s, p, q, count, cap, W, params, out);
 }
 
 void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, const uint32_t *ids, const uint32_t *nids,

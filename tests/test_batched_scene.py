@@ -16,7 +16,8 @@ def small_batched():
 
 def test_inventory(small_batched):
     sc = small_batched
-    assert len(sc.indices) == 2 * 32 * 32 + 12 * 20 * 4 ** 2 + 2 * 300
+    # the first ceil(36 * 12 / 199) = 3 icospheres one subdivision level finer
+    assert len(sc.indices) == 2 * 32 * 32 + 3 * 20 * 4 ** 3 + 9 * 20 * 4 ** 2 + 2 * 300
     assert len(sc.materials) == 1 + 12 + 1
     assert len(sc.textures) == 129
     assert sum(srgb for _, srgb in sc.textures) == 64

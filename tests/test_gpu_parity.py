@@ -119,6 +119,32 @@ def test_envmap_alias_draws_bit_exact(gpu, disney_small):
     assert not bad.any()
 
 
+def test_envmap_footprint_alias_draws_bit_exact(gpu, disney_small):
+    # option env_sampling = 1: tables from the 3 x 3 neighbourhood maxima
+    # (pinned against numpy in tests/test_env_footprint.py); same draws as the
+    # oracle's footprint tables, and different from the texel tables'
+    gpu.set_option("env_sampling", 1)
+    try:
+        gpu.upload(disney_small)
+    finally:
+        gpu.set_option("env_sampling", 0)
+    o = orc.OracleScene(disney_small, env_footprint=True)
+    rng = np.random.default_rng(10)
+    n = 1 << 18
+    r = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    coin, jx, jy = (rng.random(n).astype(np.float32) for _ in range(3))
+    _say("case: envmap footprint alias draws")
+    gd, gL, gpdf, gtex = gpu.envmap_sample(r, coin, jx, jy)
+    od, oL, opdf, otex = o.env_sample(r, coin, jx, jy, mode=0)
+    assert np.array_equal(gtex, otex)
+    assert np.allclose(gd, od, atol=2e-6)
+    assert np.allclose(gpdf, opdf, rtol=2e-5)
+    assert np.allclose(gL, oL, rtol=2e-5, atol=1e-6)
+    texel = orc.OracleScene(disney_small).env_sample(r, coin, jx, jy, mode=0)[3]
+    assert not np.array_equal(texel, otex)
+    gpu.upload(disney_small)  # texel tables again for the tests that follow
+
+
 # BSDF eval/pdf/sample parity at 10^7 tuples per material: tests/test_gpu_bsdf.py.
 
 
