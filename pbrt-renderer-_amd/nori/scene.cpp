@@ -10,6 +10,7 @@
 #include "sampler.h"
 
 #include <fstream>
+#include <sstream>
 
 NORI_NAMESPACE_BEGIN
 
@@ -82,18 +83,20 @@ void Scene::addChild(NoriObject *obj) {
 }
 
 std::string Scene::toString() const {
-    std::string meshes;
-    for (size_t i = 0; i < m_meshes.size(); ++i) {
-        meshes += std::string("  ") + indent(m_meshes[i]->toString(), 2);
-        if (i + 1 < m_meshes.size()) meshes += ",";
-        meshes += "\n";
-    }
-    return format("Scene[\n  integrator = %s,\n  sampler = %s\n  camera = %s,\n  medium = %s,\n  envmap = \"%s\",\n"
-                  "  meshes = {\n  %s  }\n]",
-                  m_integrator ? indent(m_integrator->toString()) : std::string("null"),
-                  m_sampler ? indent(m_sampler->toString()) : std::string("null"),
-                  m_camera ? indent(m_camera->toString()) : std::string("null"),
-                  m_medium ? indent(m_medium->toString()) : std::string("null"), m_envmap, indent(meshes, 2));
+    // one "name = value" line per member; nested descriptions indented under it
+    auto describe = [](const NoriObject *o) { return o ? indent(o->toString(), 2) : std::string("null"); };
+    std::ostringstream os;
+    os << "Scene[\n"
+       << "  integrator = " << describe(m_integrator) << ",\n"
+       << "  sampler = " << describe(m_sampler) << ",\n"
+       << "  camera = " << describe(m_camera) << ",\n"
+       << "  medium = " << describe(m_medium) << ",\n"
+       << "  envmap = \"" << m_envmap << "\",\n"
+       << "  meshes (" << m_meshes.size() << ") = [";
+    for (size_t i = 0; i < m_meshes.size(); ++i)
+        os << (i ? ",\n    " : "\n    ") << indent(m_meshes[i]->toString(), 4);
+    os << (m_meshes.empty() ? "]\n]" : "\n  ]\n]");
+    return os.str();
 }
 
 NORI_REGISTER_CLASS(Scene, "scene");

@@ -172,6 +172,17 @@ def test_obj_quads_and_dedup(tmp_path):
     assert [list(hs.positions[t].sum(0)) for t in tri[:2]] == [[2, 1, 0], [1, 2, 0]]
 
 
+def test_obj_vertex_and_triangle_order(tmp_path):
+    # vertices in order of the first use of each (p, t, n) triple, triangles in
+    # face order, polygons fanned from their first corner; negative indices
+    # count back from the last position read
+    obj = ("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nv 0.5 1.5 0\n"
+           "f 3 4 1\nf -5 -4 -3 -1 -2\n")
+    hs = _one_mesh("", tmp_path, obj)
+    assert np.allclose(hs.positions, [[1, 1, 0], [0, 1, 0], [0, 0, 0], [1, 0, 0], [0.5, 1.5, 0]])
+    assert hs.indices.reshape(-1, 3).tolist() == [[0, 1, 2], [2, 3, 0], [2, 0, 4], [2, 4, 1]]
+
+
 @pytest.mark.parametrize("xml,needle", [
     ("<scene><camera type='perspective'/>", "line 1"),
     ("<scene><camera type='nosuch'/></scene>", 'A constructor for class "nosuch" could not be found'),
