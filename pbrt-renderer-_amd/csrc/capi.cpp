@@ -632,6 +632,9 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.mats = c->upload(mats.data(), mats.size());
         s.n_emit = (uint32_t)emitters.size();
         s.emitters = c->upload(emitters.data(), emitters.size());
+        std::vector<uint32_t> mesh_emit(d->num_meshes, 0u);
+        for (uint32_t k = 0; k < (uint32_t)emitters.size(); ++k) mesh_emit[emitters[k]] = k;
+        s.mesh_emit = c->upload(mesh_emit.data(), mesh_emit.size());
         s.emit_cdf_off = c->upload(cdf_off.data(), cdf_off.size());
         s.emit_cdf = c->upload(cdf.data(), cdf.size());
         s.emit_area = c->upload(area.data(), area.size());

@@ -20,6 +20,7 @@ struct SceneDev {
     const uint32_t *tri_mesh;
     const nori_material *mats;
     const uint32_t *emitters;     // mesh ids of the emitters
+    const uint32_t *mesh_emit;    // per mesh: its index among the emitters (emitter meshes only)
     const uint32_t *emit_cdf_off; // per emitter: offset into emit_cdf
     const float *emit_cdf;        // per emitter: normalised area CDF (tris+1)
     const float *emit_area;       // per emitter: total area

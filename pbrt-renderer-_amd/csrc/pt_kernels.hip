@@ -1081,8 +1081,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     float dist = sqrtf(d2);
                     float cl = fabsf(dot(N, -dv / dist));
                     if (cl >= 1e-8f) {
-                        uint32_t k = 0;
-                        while (k + 1 < s.n_emit && s.emitters[k] != mesh) ++k;
+                        const uint32_t k = s.mesh_emit[mesh];
                         pe = (1.0f / (s.emit_area[k] * (float)s.n_emit)) * d2 / cl;
                     }
                 }

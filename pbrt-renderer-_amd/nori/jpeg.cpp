@@ -110,7 +110,7 @@ inline void idct_1d(const int64_t *in, int64_t out[8]) {
 }
 
 // Dequantised coefficients in natural order -> 8x8 samples (row stride `stride`).
-void idct_islow(const int32_t coef[64], uint8_t *out, size_t stride) {
+void idct_islow(const int64_t coef[64], uint8_t *out, size_t stride) {
     int64_t ws[64], col[8], res[8];
     for (int x = 0; x < 8; ++x) { // columns
         for (int k = 0; k < 8; ++k) col[k] = coef[k * 8 + x];
@@ -288,11 +288,16 @@ private:
     }
 
     bool block(Component &c, int bx, int by, std::string &err) {
-        int32_t coef[64] = {0};
+        // dequantised in 64 bits: 16-bit quantisation tables times 16-bit
+        // coefficients need 32 bits and the IDCT adds more
+        int64_t coef[64] = {0};
         const int t = decode(dc_[c.td]);
         if (t < 0 || t > 11) return fail(err, "corrupt JPEG data");
         c.pred += extend(receive(t), t);
-        coef[0] = c.pred * (int32_t)q_[c.tq][0];
+        // 8-bit data has |DC| <= 1024; a predictor drifting far past that is
+        // corrupt (and would overflow given enough blocks)
+        if (c.pred > 32767 || c.pred < -32768) return fail(err, "corrupt JPEG data (DC value out of range)");
+        coef[0] = (int64_t)c.pred * (int64_t)q_[c.tq][0];
         for (int k = 1; k < 64;) {
             const int rs = decode(ac_[c.ta]);
             if (rs < 0) return fail(err, "corrupt JPEG data");
@@ -304,7 +309,7 @@ private:
             }
             k += r;
             if (k > 63) return fail(err, "corrupt JPEG data");
-            coef[kZigzag[k]] = extend(receive(s), s) * (int32_t)q_[c.tq][kZigzag[k]];
+            coef[kZigzag[k]] = (int64_t)extend(receive(s), s) * (int64_t)q_[c.tq][kZigzag[k]];
             ++k;
         }
         idct_islow(coef, &c.plane[((size_t)by * 8 * c.bw + (size_t)bx) * 8], (size_t)c.bw * 8);

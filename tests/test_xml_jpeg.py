@@ -86,3 +86,36 @@ def test_progressive_and_corrupt_files_leave_the_map_unset(tmp_path):
     (tmp_path / "c.jpg").write_bytes(good[:len(good) // 3])  # cut inside the frame
     _decode(tmp_path, "c.jpg")  # must not crash; a truncated scan decodes as zeros or fails
     assert data[:2] == b"\xff\xd8"
+
+
+def _dc_drift_jpeg(blocks):
+    """Greyscale baseline JPEG, 8 x (8 blocks) pixels, written by hand: a 16-bit
+    quantisation table of 65535s, a DC Huffman table whose one code '0' means
+    category 11, an AC table whose one code '0' is end-of-block, and every
+    block's DC difference +2047, so the DC predictor climbs by 2047 a block."""
+    def seg(marker, body):
+        return bytes([0xFF, marker]) + (len(body) + 2).to_bytes(2, "big") + body
+    dqt = seg(0xDB, bytes([0x10]) + b"\xff\xff" * 64)
+    sof = seg(0xC0, bytes([8]) + (8).to_bytes(2, "big") + (8 * blocks).to_bytes(2, "big") + bytes([1, 1, 0x11, 0]))
+    counts = bytes([1] + [0] * 15)
+    dht = seg(0xC4, bytes([0x00]) + counts + bytes([11])) + seg(0xC4, bytes([0x10]) + counts + bytes([0]))
+    sos = seg(0xDA, bytes([1, 1, 0x00, 0, 63, 0]))
+    bits = ("0" + "1" * 11 + "0") * blocks
+    bits += "1" * (-len(bits) % 8)
+    data = bytearray()
+    for i in range(0, len(bits), 8):
+        b = int(bits[i:i + 8], 2)
+        data += bytes([b, 0]) if b == 0xFF else bytes([b])
+    return b"\xff\xd8" + dqt + sof + dht + sos + bytes(data) + b"\xff\xd9"
+
+
+def test_dc_predictor_drift_is_rejected(tmp_path):
+    # 8 blocks: predictor 16376, coefficient 16376 * 65535 (beyond 32 bits
+    # before the IDCT): decodes without overflow
+    (tmp_path / "ok.jpg").write_bytes(_dc_drift_jpeg(8))
+    got = _decode(tmp_path, "ok.jpg")
+    assert got is not None and got.shape == (8, 64, 4)
+    # 40 blocks: the predictor passes 32767 at block 17, which no 8-bit image
+    # produces: the file is refused and the map left unset
+    (tmp_path / "drift.jpg").write_bytes(_dc_drift_jpeg(40))
+    assert _decode(tmp_path, "drift.jpg") is None
