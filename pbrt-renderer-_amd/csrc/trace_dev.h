@@ -8,10 +8,12 @@
 //     evaluation bit for bit;
 //   * closest hit: smallest t in [tmin, tmax], equal t resolved to the smaller
 //     global primitive id (independent of traversal order);
-//   * box test: planes decoded with one fma exactly as the builder verified
-//     them, tfar (after clipping to the current hit) widened by (1 + 2*gamma_3)
-//     (Pharr, Jakob, Humphreys, PBRT 3e §3.9.2), so no box that holds an
-//     accepted hit, or an equal-t hit with a smaller id, is ever culled;
+//   * box test: the distance to each decoded plane fl(q sc + p) as one fma
+//     q (sc rcp) + (p - o) rcp, moved outward by a slack bounding its rounding
+//     errors (visit_node; rays with a zero direction component use the
+//     two-step decode), and the current hit distance widened by
+//     (1 + 2*gamma_3) (Pharr, Jakob, Humphreys, PBRT 3e §3.9.2), so no box that
+//     holds an accepted hit, or an equal-t hit with a smaller id, is ever culled;
 //   * stack: the node groups (child base, hit bits | inner mask) of Ylitie et
 //     al. 2017, kept in LDS, one column per lane; its depth is the tree depth;
 //   * schedule (POSTPONE, closest-hit queries): either a lane tests all
@@ -89,6 +91,7 @@ template <bool ANY, bool COUNT, bool POSTPONE = false> struct Bvh8Walker {
     float tmin;
     uint32_t octinv;
     bool nx, ny, nz;
+    bool fused; // every |rcp| below 1e28: the one-fma box test with its slack (else the two-step decode)
     TraceOut res;
     uint2 group;
     uint2 tg; // POSTPONE: pending triangles (first triangle, hit mask in bits 0..23); y = 0: none
@@ -105,11 +108,83 @@ template <bool ANY, bool COUNT, bool POSTPONE = false> struct Bvh8Walker {
         nx = rcp.x < 0.0f;
         ny = rcp.y < 0.0f;
         nz = rcp.z < 0.0f;
+        // a zero (or denormal-small) direction component makes its slab test a
+        // containment test of the origin, which the fused form cannot express
+        fused = fabsf(rcp.x) < 1e28f && fabsf(rcp.y) < 1e28f && fabsf(rcp.z) < 1e28f;
         octinv = 7u - ((nx ? 4u : 0u) | (ny ? 2u : 0u) | (nz ? 1u : 0u));
         group = make_uint2(0u, (1u << (24u + octinv)) | 1u);
         tg = make_uint2(0u, 0u);
         sp = 0;
         postponed = false;
+    }
+
+    // Hit bits of a node's children (bits 0..23 leaf triangles, 24..31 inner
+    // children by octant-sorted slot).  w: the near/far plane words per axis
+    // for this ray's octant (x lo, x hi, far x lo, far x hi, then y, z).
+    //
+    // FUSED: slab distances as one fma per plane, t = q A + B with A = sc rcp
+    // (exact: sc is a power of two) and B = (p - o) rcp, moved outward by a
+    // per-axis slack E that bounds every rounding between this form and the
+    // exact distance to the decoded plane fl(q sc + p): the plane's own
+    // rounding (u |p + q sc| |rcp|), those of p - o, of the product and of rcp
+    // (3u |B| and u q |A|), the final fma and the shifted constants
+    // (u (255 |A| + |B| + E) and u (|B| + E)).  E = 16u (|B| + 255 |A| +
+    // |p rcp|) covers their sum about three times over, so a near distance is
+    // never later and a far one never earlier than exact, and no box that the
+    // exact test enters is culled.  Otherwise (a ray with a zero direction
+    // component, |rcp| >= 1e28): the two-step decode (X - o) rcp with the far
+    // distance widened by (1 + 2 gamma_3), where an infinite rcp keeps the slab
+    // a containment test of the origin.
+    template <bool FUSED>
+    __device__ __forceinline__ uint32_t child_hits(const uint4 &b, const uint32_t *w, f3 p, f3 sc) const {
+        const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
+        const float u16 = 16.0f * 5.9604645e-08f;
+        f3 A, Bn, Bf;
+        if (FUSED) {
+            A = f3{sc.x * rcp.x, sc.y * rcp.y, sc.z * rcp.z};
+            const f3 B = f3{(p.x - o.x) * rcp.x, (p.y - o.y) * rcp.y, (p.z - o.z) * rcp.z};
+            const f3 E = f3{u16 * (fabsf(B.x) + __builtin_fmaf(255.0f, fabsf(A.x), fabsf(p.x * rcp.x))),
+                            u16 * (fabsf(B.y) + __builtin_fmaf(255.0f, fabsf(A.y), fabsf(p.y * rcp.y))),
+                            u16 * (fabsf(B.z) + __builtin_fmaf(255.0f, fabsf(A.z), fabsf(p.z * rcp.z)))};
+            Bn = f3{B.x - E.x, B.y - E.y, B.z - E.z};
+            Bf = f3{B.x + E.x, B.y + E.y, B.z + E.z};
+        }
+        // widened: a box entered at exactly res.t may hold an equal-t hit with a
+        // smaller primitive id
+        const float tcap = res.t * far_scale;
+        uint32_t hitmask = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            uint32_t meta = ((s < 4 ? b.z : b.w) >> (8 * (s & 3))) & 0xFFu;
+            if (meta == 0u) continue;
+            const int k = s & 3, h = s < 4 ? 0 : 1;
+            float qnx = byte_f(w[0 + h], k), qfx = byte_f(w[2 + h], k);
+            float qny = byte_f(w[4 + h], k), qfy = byte_f(w[6 + h], k);
+            float qnz = byte_f(w[8 + h], k), qfz = byte_f(w[10 + h], k);
+            float tn, tf;
+            if (FUSED) {
+                float tnx = __builtin_fmaf(qnx, A.x, Bn.x), tfx = __builtin_fmaf(qfx, A.x, Bf.x);
+                float tny = __builtin_fmaf(qny, A.y, Bn.y), tfy = __builtin_fmaf(qfy, A.y, Bf.y);
+                float tnz = __builtin_fmaf(qnz, A.z, Bn.z), tfz = __builtin_fmaf(qfz, A.z, Bf.z);
+                tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                tf = fminf(fminf(fminf(tfx, tfy), tfz), tcap);
+            } else {
+                float tnx = (__builtin_fmaf(qnx, sc.x, p.x) - o.x) * rcp.x;
+                float tfx = (__builtin_fmaf(qfx, sc.x, p.x) - o.x) * rcp.x;
+                float tny = (__builtin_fmaf(qny, sc.y, p.y) - o.y) * rcp.y;
+                float tfy = (__builtin_fmaf(qfy, sc.y, p.y) - o.y) * rcp.y;
+                float tnz = (__builtin_fmaf(qnz, sc.z, p.z) - o.z) * rcp.z;
+                float tfz = (__builtin_fmaf(qfz, sc.z, p.z) - o.z) * rcp.z;
+                tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
+                tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * far_scale;
+            }
+            if (tn <= tf) {
+                uint32_t is_inner = (meta & 0x18u) == 0x18u;
+                uint32_t bitidx = is_inner ? ((meta ^ octinv) & 0x1Fu) : (meta & 0x1Fu);
+                hitmask |= (meta >> 5) << bitidx;
+            }
+        }
+        return hitmask;
     }
 
     // Visits the next hit child node of `group` (which has one): the rest of
@@ -118,7 +193,6 @@ template <bool ANY, bool COUNT, bool POSTPONE = false> struct Bvh8Walker {
     // triangle *tri_base).
     __device__ __forceinline__ uint32_t visit_node(const uint4 *__restrict__ nodes, uint2 *stk, uint32_t stride,
                                                    uint32_t &tri_base) {
-        const float far_scale = 1.0f + 2.0f * 3.5762791e-07f; // 1 + 2*gamma_3, gamma_3 = 3u/(1-3u), u = 2^-24
         uint32_t bit = 31u - __builtin_clz(group.y & 0xFF000000u);
         group.y &= ~(1u << bit);
         uint32_t slot = (bit - 24u) ^ octinv;
@@ -138,31 +212,8 @@ template <bool ANY, bool COUNT, bool POSTPONE = false> struct Bvh8Walker {
         uint32_t nxl = nx ? dd.z : c.x, nxh = nx ? dd.w : c.y, fxl = nx ? c.x : dd.z, fxh = nx ? c.y : dd.w;
         uint32_t nyl = ny ? e.x : c.z, nyh = ny ? e.y : c.w, fyl = ny ? c.z : e.x, fyh = ny ? c.w : e.y;
         uint32_t nzl = nz ? e.z : dd.x, nzh = nz ? e.w : dd.y, fzl = nz ? dd.x : e.z, fzh = nz ? dd.y : e.w;
-        uint32_t hitmask = 0;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            uint32_t meta = ((s < 4 ? b.z : b.w) >> (8 * (s & 3))) & 0xFFu;
-            if (meta == 0u) continue;
-            const int k = s & 3;
-            float qnx = byte_f(s < 4 ? nxl : nxh, k), qfx = byte_f(s < 4 ? fxl : fxh, k);
-            float qny = byte_f(s < 4 ? nyl : nyh, k), qfy = byte_f(s < 4 ? fyl : fyh, k);
-            float qnz = byte_f(s < 4 ? nzl : nzh, k), qfz = byte_f(s < 4 ? fzl : fzh, k);
-            float tnx = (__builtin_fmaf(qnx, sc.x, p.x) - o.x) * rcp.x;
-            float tfx = (__builtin_fmaf(qfx, sc.x, p.x) - o.x) * rcp.x;
-            float tny = (__builtin_fmaf(qny, sc.y, p.y) - o.y) * rcp.y;
-            float tfy = (__builtin_fmaf(qfy, sc.y, p.y) - o.y) * rcp.y;
-            float tnz = (__builtin_fmaf(qnz, sc.z, p.z) - o.z) * rcp.z;
-            float tfz = (__builtin_fmaf(qfz, sc.z, p.z) - o.z) * rcp.z;
-            float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, tmin));
-            // widened after the min with the current hit: a box entered at exactly
-            // res.t may hold an equal-t hit with a smaller primitive id
-            float tf = fminf(fminf(fminf(tfx, tfy), tfz), res.t) * far_scale;
-            if (tn <= tf) {
-                uint32_t is_inner = (meta & 0x18u) == 0x18u;
-                uint32_t bitidx = is_inner ? ((meta ^ octinv) & 0x1Fu) : (meta & 0x1Fu);
-                hitmask |= (meta >> 5) << bitidx;
-            }
-        }
+        const uint32_t w[12] = {nxl, nxh, fxl, fxh, nyl, nyh, fyl, fyh, nzl, nzh, fzl, fzh};
+        const uint32_t hitmask = fused ? child_hits<true>(b, w, p, sc) : child_hits<false>(b, w, p, sc);
         group = make_uint2(b.x, (hitmask & 0xFF000000u) | imask);
         tri_base = b.y;
         return hitmask & 0x00FFFFFFu;
