@@ -15,8 +15,11 @@ HBM before the timed region; the timed region is K render calls, each ending
 in a stream synchronisation.
 
 Secondary legs: the Disney icosphere grid (5,242,882 triangles, 2048x1024
-sky+sun envmap, 512x512, 256 spp, max depth 12) plain; and the time to relMSE
-<= 1e-3 of plain and guided rendering of that grid at its 512x512 resolution.
+sky+sun envmap, 512x512, 256 spp, max depth 12) plain; the time to relMSE
+<= 1e-3 of plain and guided rendering of that grid at its 512x512 resolution;
+and the throughput of the Cornell box, the 50,000-strand hair scene (lit and
+white furnace) and the 256^3 heterogeneous medium (g = 0, 0.3, 0.8), each at
+its full size.
 
 Prints one JSON line.  --gpus N > 1 is not supported: this build runs on one GPU.
 """
@@ -328,6 +331,59 @@ def time_to_rel_mse_legs(gpu, ng, sc, args):
                                                                f["guided"]["ms_to_threshold"], 3)}
 
 
+CONFIG_LEGS = [
+    # name, generator config, generator options, spp, max depth, what it is
+    ("cornell", 1, {}, 64, 8, "Cornell box, 2 level-4 icospheres, 10,252 triangles, quad light, 256x256, 64 spp, "
+                              "max depth 8, RR from 3"),
+    ("hair_lit", 3, {}, 256, 32, "50,000 hair strands x 16 segments (1.6 M ribbon triangles), Chiang BCSDF, three "
+                                 "quad lights, 512x512, 256 spp, max depth 32"),
+    ("hair_furnace", 3, {"variant": 1}, 256, 32, "50,000 hair strands, sigma_a = 0, uniform unit envmap (white "
+                                                 "furnace), 512x512, 256 spp, max depth 32"),
+    ("volume_g0", 4, {"variant": 0}, 512, 64, "256^3 fBm density, max sigma_t 20, albedo 0.9, HG g = 0, 16^3 "
+                                              "brick majorants, diffuse box, sky+sun envmap, 512x512, 512 spp, "
+                                              "max depth 64"),
+    ("volume_g0.3", 4, {"variant": 1}, 512, 64, "as volume_g0 with HG g = 0.3"),
+    ("volume_g0.8", 4, {"variant": 2}, 512, 64, "as volume_g0 with HG g = 0.8"),
+]
+
+
+def config_legs(gpu, ng, steps, pool):
+    """Msamples/s of the other workload configurations, each at its full size:
+    one untimed frame, then `steps` timed frames (one render call each, new
+    sample offsets every frame), plain path_mis, at the pool and chunk length
+    the autotuner picks on the two large scenes."""
+    out = {}
+    for name, cfg, kw, spp, depth, what in CONFIG_LEGS:
+        t0 = time.time()
+        sc = ng.generate_scene(cfg, **kw)
+        gpu.upload(sc)
+        cam = sc.camera
+        samples = cam.width * cam.height * spp
+        gpu.set_option("batch", max(64, min(pool, (samples + 255) // 256 * 256)))
+        gpu.set_option("chunk_iters", 2)
+        gpu.set_option("traversal", 1)
+        gpu.set_option("trace_waves", 0)
+        gpu.reset_film()
+        gpu.render(ng.make_view(cam, spp, depth, sample_offset=1 << 22), seed=3, readback=False)
+        gpu.reset_film()
+        acc = {"samples": 0, "rays": 0}
+        t1 = time.perf_counter()
+        for k in range(steps):
+            gpu.render(ng.make_view(cam, spp, depth, sample_offset=k * spp), seed=5, readback=False)
+            s = gpu.stats()
+            acc["samples"] += s.samples
+            acc["rays"] += s.extension_rays + s.shadow_rays
+        wall = time.perf_counter() - t1
+        out[name] = {"workload": what, "triangles": int(len(sc.indices)), "msamples_per_s": round(acc["samples"] / wall
+                                                                                                  / 1e6, 2),
+                     "mrays_per_s": round(acc["rays"] / wall / 1e6, 2), "ms_per_frame": round(wall / steps * 1e3, 2),
+                     "frames": steps, "path_pool": min(pool, (samples + 255) // 256 * 256)}
+        log(f"[bench] {name}: {out[name]['msamples_per_s']} Msamples/s, {out[name]['mrays_per_s']} Mrays/s "
+            f"({steps} frames of {samples} samples, {time.time() - t0:.1f}s with scene set-up)")
+        del sc
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -338,7 +394,7 @@ def main():
     ap.add_argument("--chunk-iters", type=int, default=2, help="pool iterations per done-flag check (--no-autotune)")
     ap.add_argument("--traversal", type=int, default=1, help="triangle schedule of the traversal (--no-autotune)")
     ap.add_argument("--trace-waves", type=int, default=0, help="waves-per-SIMD cap of traversal (--no-autotune)")
-    ap.add_argument("--only", choices=["batched", "disney"], default=None,
+    ap.add_argument("--only", choices=["batched", "disney", "configs"], default=None,
                     help="run one workload leg only (profiling); the JSON line then carries that leg as value")
     ap.add_argument("--disney-steps", type=int, default=4)
     ap.add_argument("--timing-steps", type=int, default=1,
@@ -346,6 +402,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="wall-time budget of the CPU baseline sample")
     ap.add_argument("--no-time-to-mse", action="store_true", help="skip the time-to-relMSE leg")
+    ap.add_argument("--no-config-legs", action="store_true",
+                    help="skip the Cornell, hair and volume configurations' throughput")
+    ap.add_argument("--config-frames", type=int, default=3, help="timed frames per configuration leg")
     ap.add_argument("--mse-ref-spp", type=int, default=1 << 14)
     ap.add_argument("--mse-start-spp", type=int, default=64)
     ap.add_argument("--mse-max-spp", type=int, default=1 << 12)
@@ -422,6 +481,14 @@ def main():
         legs["disney"] = m
         if not args.no_time_to_mse and args.only is None:
             legs["time_to_rel_mse"] = time_to_rel_mse_legs(gpu, ng, sc, args)
+        del sc
+
+    if args.only in (None, "configs") and not args.no_config_legs:
+        legs["configs"] = config_legs(gpu, ng, max(1, args.config_frames), args.batch)
+        if args.only == "configs":
+            print(json.dumps({"metric": METRIC, "unit": "Msamples/s", "other_configs": legs["configs"]}), flush=True)
+            gpu.close()
+            return
 
     head = legs["batched"] if "batched" in legs else legs["disney"]
     result = {
@@ -454,6 +521,8 @@ def main():
                                  "ms_per_step": round(d["ms_per_step"], 3), "mrays_per_s": round(d["mrays_per_s"], 2),
                                  "gpu_ms": d["gpu_ms"], "traversal": d["traversal"], "roofline": d["roofline"],
                                  "autotune": d["autotune"]}
+    if "configs" in legs:
+        result["other_configs"] = legs["configs"]
     if "time_to_rel_mse" in legs:
         result["time_to_rel_mse"] = legs["time_to_rel_mse"]
     print(json.dumps(result), flush=True)
