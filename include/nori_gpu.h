@@ -160,7 +160,12 @@ int nori_gpu_upload_scene(nori_ctx *ctx, const nori_scene_desc *scene);
  * each view's device film (RGBA float: radiance sum, sample count).  If
  * accum_rgba is not NULL the films are copied to it afterwards, concatenated in
  * view order (4 floats per pixel).  Replaces the DispatchRays frame loop
- * (src/gpu/DXRApp.cpp:199-246) running RayGen (shaders/Shaders.hlsl:25-651). */
+ * (src/gpu/DXRApp.cpp:199-246) running RayGen (shaders/Shaders.hlsl:25-651).
+ * The film accumulates each contribution rounded to 2^-28 in 64-bit fixed
+ * point (clamped to +-2^30), so a render is bit-reproducible run to run and
+ * independent of the path pool, chunk length and traversal schedule; guided
+ * renders that train the network are not (the training batches depend on
+ * the order in which paths end). */
 int nori_gpu_render(nori_ctx *ctx, const nori_view *views, int nviews, uint64_t seed, float *accum_rgba);
 
 /* Zero the device films. */
@@ -179,8 +184,8 @@ int nori_gpu_li(nori_ctx *ctx, const nori_ray *rays, const uint64_t *seeds, int 
  * capture, tight serving loops, harnesses that forbid hipMalloc).  rays,
  * seeds and rgba4 are device pointers (n rays of 32 bytes, n pcg32 seeds, n x 4
  * floats: the radiance Li of ray i in rgba4[4i .. 4i+2] and 1 in rgba4[4i+3]),
- * as for nori_gpu_li's host arrays.  The path pool, queues, counters and view
- * record live in a static 20 MiB device array per GPU (held until the process
+ * as for nori_gpu_li's host arrays.  The path pool, queues, film entries,
+ * counters and view record live in a static 24 MiB device array per GPU (held until the process
  * ends): at most NORI_LI_NOALLOC_MAX_RAYS rays per call (NORI_E_LIMIT above
  * that).  Every call on one GPU shares that array: calls that overlap on one
  * GPU (several contexts or threads) race.  Returns when the results are in

@@ -95,8 +95,10 @@ struct nori_ctx {
     bool scene_has_hair = false;
     Counters cdev{};
 
-    // One film for all views of the last render, view after view.
-    DevBuf film;
+    // One film for all views of the last render, view after view: film_fx
+    // accumulates (fixed point, order-independent), film holds its RGBA32F
+    // resolve after every render.
+    DevBuf film, film_fx;
     std::vector<size_t> film_layout;
     // option "aov": the film holds three parts of npix_all entries each (radiance,
     // denoiser albedo, denoiser normal)
@@ -105,7 +107,7 @@ struct nori_ctx {
     size_t npix_all = 0;
     AovDev aov_dev() const {
         if (!film_aov) return AovDev{nullptr, nullptr};
-        float4 *f = (float4 *)film.p;
+        FilmFx *f = (FilmFx *)film_fx.p;
         return AovDev{f + npix_all, f + 2 * npix_all};
     }
     DevBuf views_buf;
@@ -341,7 +343,7 @@ struct Span {
 // The bounce loop of one batch of n caller rays whose state k_raygen_rays
 // wrote (Integrator::Li): extend, shade, shadow per bounce; views holds the
 // one entry with the depth limits.
-void run_bounces(nori_ctx *c, float4 *film, const ViewTab *views, uint32_t n, uint32_t max_depth,
+void run_bounces(nori_ctx *c, FilmFx *film, const ViewTab *views, uint32_t n, uint32_t max_depth,
                  nori_render_stats &st) {
     const GuideShadeDev none{nullptr, nullptr, nullptr, nullptr, 0, 0};
     for (uint32_t b = 0; b < max_depth; ++b) {
@@ -362,7 +364,7 @@ void run_bounces(nori_ctx *c, float4 *film, const ViewTab *views, uint32_t n, ui
 // [guide inference], shade (survivors compacted into buffer qi^1, ended ids to
 // free list qi^1), shadow, [guide training samples of the ended paths],
 // counters.
-void pool_iteration(nori_ctx *c, uint32_t qi, float4 *film, const ViewTab *views, const GuideShadeDev &gsd,
+void pool_iteration(nori_ctx *c, uint32_t qi, FilmFx *film, const ViewTab *views, const GuideShadeDev &gsd,
                     unsigned long long *cs) {
     const uint32_t qo = qi ^ 1u, n = c->pool;
     uint32_t *cnt = c->cdev.cnt;
@@ -420,20 +422,21 @@ int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n,
         c->spans.clear();
         c->ev_used = 0;
         launch_counters(c->cdev, 3, 0, 0x1000u, c->stream);
-        ScopedBufs b(4);
+        ScopedBufs b(5);
         size_t nr = (size_t)n;
         HIPCHK(hipMalloc(&b[0].p, nr * sizeof(nori_ray)));
         HIPCHK(hipMalloc(&b[1].p, nr * sizeof(uint64_t)));
         HIPCHK(hipMalloc(&b[2].p, nr * sizeof(float4)));
         HIPCHK(hipMalloc(&b[3].p, sizeof(ViewTab)));
+        HIPCHK(hipMalloc(&b[4].p, nr * sizeof(FilmFx)));
         ViewTab vt{};
         vt.max_depth = max_depth;
         vt.rr_depth = rr_depth;
         HIPCHK(hipMemcpyAsync(b[0].p, rays, nr * sizeof(nori_ray), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(b[1].p, seeds, nr * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(b[3].p, &vt, sizeof(ViewTab), hipMemcpyHostToDevice, c->stream));
-        float4 *film = (float4 *)b[2].p;
-        launch_zero_f4(film, nr, c->stream);
+        FilmFx *film = (FilmFx *)b[4].p;
+        launch_zero_fx(film, nr, c->stream);
         uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, nr);
         c->ensure_paths(cap);
         c->sdev.sobol = 0; // caller rays come with pcg32 seeds (Nori's Independent sampler)
@@ -445,8 +448,9 @@ int nori_gpu_li(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds, int n,
                                c->stream);
             run_bounces(c, film, (const ViewTab *)b[3].p, m, max_depth, st);
         }
+        launch_film_resolve(film, (float4 *)b[2].p, nr, c->stream);
         std::vector<float4> out(nr);
-        HIPCHK(hipMemcpyAsync(out.data(), film, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(out.data(), b[2].p, nr * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipGetLastError());
         for (size_t i = 0; i < nr; ++i) {
@@ -473,9 +477,9 @@ int nori_gpu_li_noalloc(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds
         const LiScratch l = li_noalloc_layout(c->li_scratch);
         SceneDev s = c->sdev;
         s.sobol = 0; // caller rays come with pcg32 seeds (Nori's Independent sampler)
-        float4 *film = (float4 *)rgba4;
+        FilmFx *film = l.film;
         const uint32_t m = (uint32_t)n;
-        launch_zero_f4(film, (size_t)m, c->stream);
+        launch_zero_fx(film, (size_t)m, c->stream);
         launch_li_setup(l, m, max_depth, rr_depth, c->stream);
         launch_raygen_rays(l.pd[0], (const float4 *)rays, seeds, 0, m, film, c->stream);
         const GuideShadeDev none{nullptr, nullptr, nullptr, nullptr, 0, 0};
@@ -487,6 +491,7 @@ int nori_gpu_li_noalloc(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds
             launch_shadow(s, l.shd, l.cnt.cnt + 2, film, 2 * m, nullptr, none, nullptr, c->stream);
             launch_counters(l.cnt, qi, 1, (1u << qi) | 4u, c->stream);
         }
+        launch_film_resolve(film, (float4 *)rgba4, (size_t)m, c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
         HIPCHK(hipGetLastError());
     });
@@ -536,12 +541,13 @@ void nori_gpu_destroy(nori_ctx *c) {
     c->free_scene();
     for (DevBuf *b : {&c->paths, &c->queues, &c->shadow, &c->counters, &c->stats, &c->scratch_rays, &c->scratch_hits,
                       &c->g_params, &c->g_m, &c->g_v, &c->g_grad, &c->g_w, &c->g_wt, &c->g_loss, &c->g_out, &c->g_ww,
-                      &c->g_wib, &c->g_rec, &c->g_nrec, &c->g_lpath, &c->g_samples, &c->g_scount})
+                      &c->g_wib, &c->g_rec, &c->g_nrec, &c->g_lpath, &c->g_samples, &c->g_scount, &c->g_pack})
         b->release();
     for (auto &b : c->g_act) b.release();
     for (auto &b : c->g_dz) b.release();
     c->drop_graph();
     c->film.release();
+    c->film_fx.release();
     c->views_buf.release();
     if (c->h_flags) (void)hipHostFree(c->h_flags);
     if (c->h_claimed) (void)hipHostFree(c->h_claimed);
@@ -758,6 +764,7 @@ int nori_gpu_reset_film(nori_ctx *c) {
         size_t npix = 0;
         for (size_t x : c->film_layout) npix += x;
         if (c->film.p) launch_zero_f4((float4 *)c->film.p, npix * (c->film_aov ? 3 : 1), c->stream);
+        if (c->film_fx.p) launch_zero_fx((FilmFx *)c->film_fx.p, npix * (c->film_aov ? 3 : 1), c->stream);
         HIPCHK(hipStreamSynchronize(c->stream));
     });
 }
@@ -789,8 +796,11 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
         if (layout != c->film_layout || c->film_aov != (c->aov_on != 0)) {
             const size_t parts = c->aov_on ? 3 : 1;
             c->film.release();
+            c->film_fx.release();
             c->alloc(c->film, parts * npix_all * sizeof(float4));
+            c->alloc(c->film_fx, parts * npix_all * sizeof(FilmFx));
             launch_zero_f4((float4 *)c->film.p, parts * npix_all, c->stream);
+            launch_zero_fx((FilmFx *)c->film_fx.p, parts * npix_all, c->stream);
             c->film_layout = layout;
             c->film_aov = c->aov_on != 0;
             c->npix_all = (size_t)npix_all;
@@ -812,7 +822,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
         launch_counters(c->cdev, 3, 0, 0x1000u, c->stream); // zero the stats
         HIPCHK(hipEventRecord(e0, c->stream));
         unsigned long long *cs = c->count_traversal ? c->cdev.stats : nullptr;
-        float4 *film = (float4 *)c->film.p;
+        FilmFx *film = (FilmFx *)c->film_fx.p;
         const ViewTab *dviews = (const ViewTab *)c->views_buf.p;
         if (total > 0) {
             const uint32_t cap = (uint32_t)std::min<uint64_t>(c->batch, (total + 255) / 256 * 256);
@@ -915,6 +925,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
             st.launches_closest = st.launches_shadow = iters;
         }
         st.samples = total;
+        launch_film_resolve(film, (float4 *)c->film.p, (c->film_aov ? 3 : 1) * (size_t)npix_all, c->stream);
         HIPCHK(hipEventRecord(e1, c->stream));
         HIPCHK(hipEventSynchronize(e1));
         HIPCHK(hipGetLastError());

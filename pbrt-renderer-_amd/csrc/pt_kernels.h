@@ -55,10 +55,26 @@ struct GuideShadeDev {
     uint32_t cap, max_rec;
 };
 
-// Denoiser feature films (RGBA32F, entry per film entry; null: off): albedo
-// and normal at each sample's first non-delta interaction, summed.
+// Film entry as the kernels accumulate it: each channel in Q35.28 fixed
+// point (64-bit two's complement, so signed contributions such as normals add
+// exactly), then the sample count.  Integer addition is associative, so the
+// sum does not depend on the order in which the atomics land and a render is
+// bit-reproducible whatever the pool size, iteration grouping or scheduling.
+// A contribution is rounded to the nearest 2^-28 and clamped to +-2^30; a sum
+// holds up to 2^35 in magnitude.  launch_film_resolve turns entries into
+// RGBA32F (sum, count).
+struct FilmFx {
+    unsigned long long c[3];
+    float n;
+    uint32_t pad;
+};
+constexpr float FILM_FX_SCALE = 268435456.0f; // 2^28
+constexpr float FILM_FX_CLAMP = 1073741824.0f; // 2^30
+
+// Denoiser feature films (entry per film entry; null: off): albedo and
+// normal at each sample's first non-delta interaction, summed.
 struct AovDev {
-    float4 *albedo, *normal;
+    FilmFx *albedo, *normal;
 };
 
 // Path state, structure of arrays indexed by position in the active queue:
@@ -128,7 +144,7 @@ void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Co
                   const uint32_t *cnt_free, uint32_t *cnt_act, uint32_t cap, const GuideShadeDev &g, hipStream_t st);
 // Film sample counts of a render: spp_v added to the fourth channel of every
 // entry of view v.
-void launch_film_count(const ViewTab *views, uint32_t nviews, uint32_t npix_all, float4 *film, hipStream_t st);
+void launch_film_count(const ViewTab *views, uint32_t nviews, uint32_t npix_all, FilmFx *film, hipStream_t st);
 // End of an iteration whose active buffer was qi: statistics, the counters of
 // parity qi cleared, done flag.
 void launch_iter_end(Counters c, uint32_t qi, hipStream_t st);
@@ -136,7 +152,7 @@ void launch_iter_end(Counters c, uint32_t qi, hipStream_t st);
 // per-ray pcg32 seeds at positions 0..n-1 of p; film entry first + i collects
 // ray i's radiance.
 void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *seeds, uint64_t first, uint32_t n,
-                        float4 *film, hipStream_t st);
+                        FilmFx *film, hipStream_t st);
 // count_stats (optional): stats array whose [4],[5] (extend) / [6],[7] (shadow)
 // receive the summed BVH8 nodes visited and triangles tested.
 // q_cls (optional): sort the positions into the shading-class queues q_cls +
@@ -151,7 +167,7 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
 // and view come from p.meta, the view's depth limits from views.
 void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
-                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
+                  uint32_t *cnt_sh, FilmFx *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
                   hipStream_t st);
 // The same over the class queues k_extend sorted: one launch per class (the
 // hair launch only when `hair`, the subsurface one only when the scene has
@@ -159,8 +175,8 @@ void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, c
 void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                          const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
-                         float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st);
-void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
+                         FilmFx *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st);
+void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, FilmFx *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, uint32_t *work, hipStream_t st);
 // Guiding: the network's mixtures of the guided vertices (guide_vertex) among
 // the paths at positions q[0..*count) (q null: positions [0, *count)), with the
@@ -181,15 +197,21 @@ void launch_guide_gather(const GuideSample *samples, const uint32_t *count, uint
 void launch_counters(Counters c, uint32_t add_ext_from, uint32_t add_shadow, uint32_t clear_mask, hipStream_t st);
 void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st);
 void launch_zero_f4(float4 *p, size_t n, hipStream_t st);
+void launch_zero_fx(FilmFx *p, size_t n, hipStream_t st);
+// out[e] = (c[0..2] * 2^-28, n) for e < n_entries.
+void launch_film_resolve(const FilmFx *film, float4 *out, size_t n_entries, hipStream_t st);
 
 // Allocation-free Li (nori_gpu_li_noalloc): a static device array holds one
-// pool of LI_NOALLOC_SLOTS paths, the counters and the view record.
+// pool of LI_NOALLOC_SLOTS paths, their fixed-point film entries, the counters
+// and the view record.
 constexpr uint32_t LI_NOALLOC_SLOTS = 1u << 16;
-constexpr size_t LI_NOALLOC_USED = (size_t)LI_NOALLOC_SLOTS * (2 * (5 * 16 + 2 * 4) + 8 * 16) + 256 + sizeof(ViewTab);
-constexpr size_t LI_NOALLOC_BYTES = 20u << 20; // 20 MiB per GPU
+constexpr size_t LI_NOALLOC_USED =
+    (size_t)LI_NOALLOC_SLOTS * (2 * (5 * 16 + 2 * 4) + 8 * 16 + sizeof(FilmFx)) + 256 + sizeof(ViewTab);
+constexpr size_t LI_NOALLOC_BYTES = 24u << 20; // 24 MiB per GPU
 struct LiScratch {
     PathsDev pd[2];
     ShadowDev shd;
+    FilmFx *film;
     Counters cnt;
     ViewTab *view;
 };

@@ -54,12 +54,18 @@ template <bool SOBOL> __device__ __forceinline__ PathRng<SOBOL> rng_resume(float
 }
 __device__ __forceinline__ bool finite3(f3 c) { return isfinite(c.x) && isfinite(c.y) && isfinite(c.z); }
 
-__device__ __forceinline__ void film_add(float4 *film, uint32_t pixel, f3 c) {
+// One channel in Q35.28: v * 2^28 is exact in FP32 (a power-of-two scale),
+// rounded to the nearest integer, clamped to +-2^58 (v to +-2^30).
+__device__ __forceinline__ unsigned long long film_fixed(float v) {
+    const float x = fminf(fmaxf(v, -FILM_FX_CLAMP), FILM_FX_CLAMP) * FILM_FX_SCALE;
+    return (unsigned long long)__float2ll_rn(x);
+}
+__device__ __forceinline__ void film_add(FilmFx *film, uint32_t pixel, f3 c) {
     if (!finite3(c)) return;
-    float *p = reinterpret_cast<float *>(film + pixel);
-    unsafeAtomicAdd(p + 0, c.x);
-    unsafeAtomicAdd(p + 1, c.y);
-    unsafeAtomicAdd(p + 2, c.z);
+    unsigned long long *p = film[pixel].c;
+    atomicAdd(p + 0, film_fixed(c.x));
+    atomicAdd(p + 1, film_fixed(c.y));
+    atomicAdd(p + 2, film_fixed(c.z));
 }
 
 // Wave-aggregated queue append: one atomic per wave; returns this lane's slot.
@@ -215,7 +221,7 @@ __global__ void k_regen_commit(Counters c, const uint32_t *__restrict__ cnt_free
 
 // Sample counts of a render: film entry e of view v gets spp_v (every pixel
 // of a view receives exactly spp samples).
-__global__ void k_film_count(const ViewTab *__restrict__ views, uint32_t nviews, uint32_t npix_all, float4 *film) {
+__global__ void k_film_count(const ViewTab *__restrict__ views, uint32_t nviews, uint32_t npix_all, FilmFx *film) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= npix_all) return;
     uint32_t lo = 0, hi = nviews;
@@ -224,7 +230,7 @@ __global__ void k_film_count(const ViewTab *__restrict__ views, uint32_t nviews,
         if (views[mid].film_off <= e) lo = mid;
         else hi = mid;
     }
-    film[e].w += (float)views[lo].spp;
+    film[e].n += (float)views[lo].spp;
 }
 
 // End of an iteration over active queue qi: statistics, the counters of parity
@@ -246,7 +252,7 @@ __global__ void k_iter_end(Counters c, uint32_t qi) {
 // Paths from caller rays (Integrator::Li): ray i -> position i, film entry first + i,
 // pcg32 stream seed(seeds[first + i], PCG32_DEFAULT_STREAM).
 __global__ void k_raygen_rays(PathsDev p, const float4 *__restrict__ rays, const uint64_t *__restrict__ seeds,
-                              uint64_t first, uint32_t n, float4 *film) {
+                              uint64_t first, uint32_t n, FilmFx *film) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t gi = first + i;
@@ -259,7 +265,7 @@ __global__ void k_raygen_rays(PathsDev p, const float4 *__restrict__ rays, const
                            __uint_as_float((uint32_t)(rng >> 32)));
     p.meta[i] = 0;
     p.id[i] = i;
-    film[gi].w += 1.0f;
+    film[gi].n += 1.0f;
 }
 
 // Adds this lane's traversal counts into stats[k], stats[k+1] (one atomic pair per wave).
@@ -515,7 +521,7 @@ __device__ __forceinline__ void guide_nee_add(const GuideShadeDev &g, uint32_t t
 // filter (alpha test, mirror/dielectric meshes attenuate instead of occluding).
 template <bool COUNT, bool VOL, bool PASS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
-                                                        float4 *film, unsigned long long *stats, GuideShadeDev g) {
+                                                        FilmFx *film, unsigned long long *stats, GuideShadeDev g) {
     extern __shared__ uint2 stk[];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = *count;
@@ -573,7 +579,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
 // The shadow rays at queue positions [0, *count), persistent form (RayFeed).
 template <bool COUNT, bool VOL, bool PASS>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_p(SceneDev s, ShadowDev sh, const uint32_t *__restrict__ count,
-                                                          uint32_t *work, float4 *film, unsigned long long *stats,
+                                                          uint32_t *work, FilmFx *film, unsigned long long *stats,
                                                           GuideShadeDev g) {
     extern __shared__ uint2 stk[];
     const uint32_t n = *count;
@@ -931,7 +937,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                                                        const uint32_t *__restrict__ q_cls,
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *cnt_out,
                                                        uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
-                                                       uint32_t *cnt_sh, float4 *film, GuideShadeDev g, AovDev aov) {
+                                                       uint32_t *cnt_sh, FilmFx *film, GuideShadeDev g, AovDev aov) {
     // traversal stacks of the subsurface walks (allocated when the scene has
     // subsurface materials: SHADE_BLOCK columns of the tree depth)
     extern __shared__ uint2 shade_stk[];
@@ -1687,7 +1693,7 @@ void launch_regen(const SceneDev &s, const PathsDev &p, const ViewTab *views, Co
     hipLaunchKernelGGL(k_regen_commit, dim3(1), dim3(64), 0, st, c, cnt_free, cnt_act);
 }
 
-void launch_film_count(const ViewTab *views, uint32_t nviews, uint32_t npix_all, float4 *film, hipStream_t st) {
+void launch_film_count(const ViewTab *views, uint32_t nviews, uint32_t npix_all, FilmFx *film, hipStream_t st) {
     if (npix_all && nviews)
         hipLaunchKernelGGL(k_film_count, dim3(blocks(npix_all, 256)), dim3(256), 0, st, views, nviews, npix_all, film);
 }
@@ -1697,7 +1703,7 @@ void launch_iter_end(Counters c, uint32_t qi, hipStream_t st) {
 }
 
 void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *seeds, uint64_t first, uint32_t n,
-                        float4 *film, hipStream_t st) {
+                        FilmFx *film, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_raygen_rays, dim3(blocks(n, 256)), dim3(256), 0, st, p, rays, seeds, first, n, film);
 }
 
@@ -1793,7 +1799,7 @@ static decltype(&k_shade<false, false, CLS_ALL, false>) shade_kernel_all(bool vo
 
 void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                   const uint32_t *cnt_in, uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
-                  uint32_t *cnt_sh, float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
+                  uint32_t *cnt_sh, FilmFx *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
                   hipStream_t st) {
     const bool guide = g.out != nullptr || g.rec != nullptr;
     const size_t lds = s.has_sss ? (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2) : 0;
@@ -1805,7 +1811,7 @@ void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, c
 void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, const ViewTab *views,
                          const uint32_t *q_cls, uint32_t cls_stride, const uint32_t *cls_cnt, bool hair,
                          uint32_t *cnt_out, uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh, uint32_t *cnt_sh,
-                         float4 *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st) {
+                         FilmFx *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov, hipStream_t st) {
     const bool guide = g.out != nullptr || g.rec != nullptr, sobol = s.sobol != 0;
     const dim3 grid(blocks(cap, SHADE_BLOCK)), block(SHADE_BLOCK);
     const size_t cs = cls_stride;
@@ -1822,7 +1828,7 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
                            cls_cnt + 3, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
 }
 
-void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, float4 *film, uint32_t cap,
+void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, FilmFx *film, uint32_t cap,
                    unsigned long long *count_stats, const GuideShadeDev &g, uint32_t *work, hipStream_t st) {
     size_t lds = (size_t)s.stack_depth * TRACE_BLOCK * sizeof(uint2);
     const int v = (count_stats ? 4 : 0) | (s.n_vol > 0 ? 2 : 0) | (s.pass_ior || s.has_alpha ? 1 : 0);
@@ -1853,6 +1859,25 @@ void launch_set_counter(uint32_t *cnt, uint32_t value, hipStream_t st) {
 
 void launch_zero_f4(float4 *p, size_t n, hipStream_t st) {
     if (n) hipLaunchKernelGGL(k_zero_f4, dim3(blocks(n, 256)), dim3(256), 0, st, p, n);
+}
+
+void launch_zero_fx(FilmFx *p, size_t n, hipStream_t st) {
+    static_assert(sizeof(FilmFx) == 2 * sizeof(float4), "film entry is two float4");
+    launch_zero_f4(reinterpret_cast<float4 *>(p), 2 * n, st);
+}
+
+__global__ void k_film_resolve(const FilmFx *__restrict__ film, float4 *__restrict__ out, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const FilmFx f = film[e];
+    const double k = 1.0 / 268435456.0; // 2^-28
+    out[e] = make_float4((float)((double)(long long)f.c[0] * k), (float)((double)(long long)f.c[1] * k),
+                         (float)((double)(long long)f.c[2] * k), f.n);
+}
+
+void launch_film_resolve(const FilmFx *film, float4 *out, size_t n_entries, hipStream_t st) {
+    if (n_entries)
+        hipLaunchKernelGGL(k_film_resolve, dim3(blocks(n_entries, 256)), dim3(256), 0, st, film, out, n_entries);
 }
 
 void launch_bsdf_batch(int op, uint32_t n, const nori_material *mats, const float *wi, const float *wo, const float *u,
@@ -1891,7 +1916,7 @@ void launch_pcg32(uint64_t initstate, uint64_t initseq, uint64_t advance, uint32
 
 // ------------------------------------------------- allocation-free Li
 // Scratch of nori_gpu_li_noalloc: one pool of LI_NOALLOC_SLOTS paths (two
-// path-state buffers, the shadow queue), the render counters and the view
+// path-state buffers, the shadow queue, the film entries), the render counters and the view
 // record, in a file-scope device array so that the entry point allocates
 // nothing.  Each GPU holds its own copy until the process ends; its address
 // on a device comes from hipGetSymbolAddress, looked up once per device
@@ -1925,7 +1950,8 @@ LiScratch li_noalloc_layout(void *base) {
     }
     float4 *sp = (float4 *)(b + 2 * n * per);
     l.shd = ShadowDev{sp, sp + 2 * n, sp + 4 * n, sp + 6 * n};
-    char *tail = (char *)(sp + 8 * n);
+    l.film = (FilmFx *)(sp + 8 * n);
+    char *tail = (char *)(l.film + n);
     l.cnt = Counters{(uint32_t *)tail, (unsigned long long *)(tail + 64), (unsigned long long *)(tail + 64) + 8};
     l.view = (ViewTab *)(tail + 256);
     return l;

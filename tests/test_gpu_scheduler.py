@@ -2,8 +2,9 @@
 many views of uneven sizes and spp equals the views rendered one at a time;
 the graph-replayed bounce loop equals eager launches; the image does not depend
 on the pool size (a sample runs on its own pcg32 stream in whatever slot and
-iteration it lands); deep paths and tiny pools drain.  Films are sums of float
-atomics, so equal sample sets agree to float rounding of the summation order."""
+iteration it lands); deep paths and tiny pools drain.  The film accumulates in
+fixed point, so equal sample sets give bit-identical films whatever the order
+in which their contributions land."""
 import sys
 
 import numpy as np
@@ -20,12 +21,17 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _same_film(a, b, label):
-    # per-pixel radiance sums over the same samples in another order
+def _same_film(a, b, label, exact=True):
+    # per-pixel radiance sums over the same samples in another order: equal
+    # bits; exact=False for films of differently compiled kernel instances,
+    # whose floating-point contractions may differ by an ulp per operation
     assert np.array_equal(a[:, 3], b[:, 3]), f"{label}: sample counts differ"
-    tol = 1e-5 * np.maximum(np.abs(a[:, :3]), np.abs(b[:, :3])) + 1e-5
-    bad = np.abs(a[:, :3] - b[:, :3]) > tol
-    assert bad.mean() < 1e-4, f"{label}: {bad.sum()} channel sums differ"
+    if exact:
+        diff = a[:, :3].view(np.uint32) != b[:, :3].view(np.uint32)
+    else:
+        diff = np.abs(a[:, :3] - b[:, :3]) > 1e-5 * np.maximum(np.abs(a[:, :3]), np.abs(b[:, :3])) + 1e-6
+    assert not diff.any(), f"{label}: {diff.sum()} channel sums differ, largest by " \
+                           f"{np.abs(a[:, :3] - b[:, :3]).max():.3g}"
 
 
 def _views(scene):
@@ -129,7 +135,10 @@ def test_class_sorted_shading_equals_one_launch(pool_defaults, config):
     one = gpu.render(view, seed=21)
     gpu.set_option("sort", 1)
     gpu.reset_film()
-    _same_film(gpu.render(view, seed=21), one, f"config {config} sorted")
+    # the sorted launches are other template instances of the shading kernel
+    # (one per class); the Disney grid's give the same bits, the hair class's
+    # round a few operations differently
+    _same_film(gpu.render(view, seed=21), one, f"config {config} sorted", exact=config == 2)
 
 
 def test_deep_paths_drain(pool_defaults):
