@@ -316,7 +316,10 @@ int nori_gpu_bvh_info(const nori_ctx *ctx, nori_bvh_info *out);
  * luminance of each texel's 3 x 3 neighbourhood, which bounds the bilinear
  * radiance inside the texel, so texels next to a small bright sun are drawn as
  * often as the radiance they return calls for; images converge to the same
- * values). */
+ * values), "bvh_collapse" (how the BVH2 becomes 8-wide nodes, taking effect at
+ * the next nori_gpu_upload: 0 greedy, largest-area child first; 1, the
+ * default, the SAH-optimal dynamic programme of Ylitie et al. 2017; the hits
+ * are the same either way). */
 int nori_gpu_set_option(nori_ctx *ctx, const char *name, int64_t value);
 
 /* ---------------------------------------------------- denoiser features */

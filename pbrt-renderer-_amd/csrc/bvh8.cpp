@@ -1,10 +1,21 @@
 // Host builder for the compressed BVH8 (see bvh8.h).
 //
 // Step 1: binned SAH BVH2 over triangle centroids (32 bins per axis, all three
-//         axes, leaves of at most 3 triangles), subtrees built in parallel.
-// Step 2: collapse: each BVH8 node takes a BVH2 node's two children and keeps
+//         axes), subtrees built in parallel.  For the greedy collapse its
+//         leaves hold at most 3 triangles (SAH decides); for the SAH-optimal
+//         collapse it is split down to single triangles and the collapse
+//         decides the leaves.
+// Step 2: collapse into 8-wide nodes, either
+//         greedy: each BVH8 node takes a BVH2 node's two children and keeps
 //         replacing its largest-area inner child by that child's two children
-//         until it has 8 children or only leaves.
+//         until it has 8 children or only leaves; or
+//         SAH-optimal (Ylitie, Karras, Laine 2017, section 3): a dynamic
+//         programme over the BVH2, bottom up, gives each node n the least SAH
+//         cost C(n, i) of covering its subtree with at most i roots (i = 1..8)
+//         — a root is a leaf of at most 3 triangles (cost A(n) c_tri P(n)) or
+//         an 8-wide node (A(n) c_node plus the best split of its two
+//         children's subtrees into at most 8 roots) — and the tree is read
+//         back top down from the recorded choices.
 // Step 3: per node, assign children to slots by ray octant (greedy on
 //         dot(child centre - node centre, octant direction)), quantise the child
 //         boxes conservatively and lay out inner children and leaf triangles
@@ -17,6 +28,7 @@
 #include <cstring>
 #include <deque>
 #include <stdexcept>
+#include <memory>
 #include <thread>
 
 namespace ng {
@@ -54,12 +66,16 @@ struct B2Node {
 
 constexpr int NBINS = 32;
 constexpr uint32_t MAX_LEAF = 3;
+// SAH weights of the optimal collapse: a node visit tests 8 child boxes, a
+// triangle test costs about a quarter of that in the traversal kernel.
+constexpr float C_NODE = 1.0f, C_TRI = 0.3f;
 
 struct B2Builder {
     const std::vector<Box> &tbox;
     const std::vector<float> &cent; // 3 per triangle
     std::vector<uint32_t> &ids;
     std::vector<B2Node> &nodes;
+    uint32_t max_leaf = MAX_LEAF;
     std::atomic<uint32_t> next{1};
     std::atomic<int> threads_left{0};
 
@@ -119,7 +135,7 @@ struct B2Builder {
             }
         }
         float split_cost = parea > 0 ? 1.0f + best_cost / parea : INFINITY;
-        if (n <= MAX_LEAF && (best_axis < 0 || (float)n <= split_cost)) {
+        if (n <= max_leaf && (best_axis < 0 || (float)n <= split_cost)) {
             node.leaf = true;
             return;
         }
@@ -150,11 +166,88 @@ struct B2Builder {
 
 inline float fmaf_(float a, float b, float c) { return std::fmaf(a, b, c); }
 
+// SAH-optimal collapse tables: cost[8n + i - 1] = C(n, i); split[8n + j - 1]
+// (j = 2..8) = roots given to the left child in the best split into at most j
+// roots (0: none recorded); fewer bit i - 1: C(n, i) = C(n, i - 1); bit 8: C(n, 1)
+// is a leaf.
+struct Collapse {
+    const std::vector<B2Node> &b2;
+    std::vector<float> cost;
+    std::vector<uint8_t> split;
+    std::vector<uint16_t> flags;
+
+    explicit Collapse(const std::vector<B2Node> &t) : b2(t), cost(8 * t.size()), split(8 * t.size()), flags(t.size()) {
+        // children always have larger indices than their parent
+        for (size_t n = t.size(); n-- > 0;) solve((uint32_t)n);
+    }
+    float dist(uint32_t n, int j, uint8_t &best_k) const {
+        const uint32_t l = b2[n].left, r = l + 1;
+        float best = INFINITY;
+        for (int k = 1; k < j; ++k) {
+            const float c = cost[8 * l + k - 1] + cost[8 * r + (j - k) - 1];
+            if (c < best) {
+                best = c;
+                best_k = (uint8_t)k;
+            }
+        }
+        return best;
+    }
+    void solve(uint32_t n) {
+        const B2Node &b = b2[n];
+        const float area = b.box.area();
+        const float leaf = b.count <= MAX_LEAF ? area * C_TRI * (float)b.count : INFINITY;
+        float *c = &cost[8 * (size_t)n];
+        uint16_t f = 0;
+        if (b.leaf) {
+            if (!(leaf < INFINITY)) throw std::runtime_error("BVH8: BVH2 leaf above the leaf size");
+            for (int i = 0; i < 8; ++i) c[i] = leaf;
+            flags[n] = (uint16_t)(0xFEu | 0x100u);
+            return;
+        }
+        for (int j = 2; j <= 8; ++j) {
+            uint8_t k = 0;
+            const float d = dist(n, j, k);
+            split[8 * (size_t)n + j - 1] = k;
+            c[j - 1] = d; // provisional: C_distribute(n, j)
+        }
+        const float internal = area * C_NODE + c[7];
+        if (leaf <= internal) {
+            c[0] = leaf;
+            f |= 0x100u;
+        } else {
+            c[0] = internal;
+        }
+        for (int i = 2; i <= 8; ++i)
+            if (c[i - 2] <= c[i - 1]) {
+                c[i - 1] = c[i - 2];
+                f |= (uint16_t)(1u << (i - 1));
+            }
+        flags[n] = f;
+    }
+    bool leaf(uint32_t n) const { return (flags[n] & 0x100u) != 0; }
+    // roots covering n's subtree with at most i of them
+    void roots(uint32_t n, int i, std::vector<uint32_t> &out) const {
+        while (i > 1 && (flags[n] & (1u << (i - 1)))) --i;
+        if (i == 1) {
+            out.push_back(n);
+            return;
+        }
+        children(n, i, out);
+    }
+    // the best split of n's two children's subtrees into at most j roots
+    void children(uint32_t n, int j, std::vector<uint32_t> &out) const {
+        const int k = split[8 * (size_t)n + j - 1];
+        roots(b2[n].left, k, out);
+        roots(b2[n].left + 1, j - k, out);
+    }
+};
+
 inline float exp2i(int e) { return std::ldexp(1.0f, e); }
 
 } // namespace
 
-void build_bvh8(const float *P, const uint32_t *I, const uint32_t *tri_mesh, uint32_t nt, Bvh8 &out, int nthreads) {
+void build_bvh8(const float *P, const uint32_t *I, const uint32_t *tri_mesh, uint32_t nt, Bvh8 &out, int nthreads,
+                int collapse) {
     auto t0 = std::chrono::steady_clock::now();
     out = Bvh8();
     if (nt == 0) throw std::invalid_argument("scene has no triangles");
@@ -171,11 +264,17 @@ void build_bvh8(const float *P, const uint32_t *I, const uint32_t *tri_mesh, uin
     std::vector<B2Node> b2(2 * (size_t)nt);
     {
         B2Builder bb{tbox, cent, ids, b2};
+        if (collapse == BVH8_COLLAPSE_SAH) bb.max_leaf = 1;
         int nth = nthreads > 0 ? nthreads : (int)std::max(1u, std::thread::hardware_concurrency());
         bb.threads_left = nth - 1;
         bb.build(0, 0, nt);
         b2.resize(bb.next.load());
     }
+
+    std::unique_ptr<Collapse> opt;
+    if (collapse == BVH8_COLLAPSE_SAH) opt.reset(new Collapse(b2));
+    // is BVH2 node c a leaf slot of the BVH8 (else an 8-wide node)
+    auto is_leaf = [&](uint32_t c) { return opt ? opt->leaf(c) : b2[c].leaf; };
 
     // Collapse + layout, breadth first.
     struct Item {
@@ -189,8 +288,10 @@ void build_bvh8(const float *P, const uint32_t *I, const uint32_t *tri_mesh, uin
         q.pop_front();
         out.max_depth = std::max(out.max_depth, it.depth);
         std::vector<uint32_t> ch;
-        if (b2[it.b2].leaf) {
+        if (is_leaf(it.b2)) {
             ch.push_back(it.b2);
+        } else if (opt) {
+            opt->children(it.b2, 8, ch);
         } else {
             ch.push_back(b2[it.b2].left);
             ch.push_back(b2[it.b2].left + 1);
@@ -252,7 +353,7 @@ void build_bvh8(const float *P, const uint32_t *I, const uint32_t *tri_mesh, uin
         // Children: inner nodes first get consecutive output indices in slot order.
         uint32_t n_inner = 0;
         for (int s = 0; s < 8; ++s)
-            if (child_in[s] >= 0 && !b2[ch[child_in[s]]].leaf) ++n_inner;
+            if (child_in[s] >= 0 && !is_leaf(ch[child_in[s]])) ++n_inner;
         node.child_base = (uint32_t)out.nodes.size();
         node.tri_base = (uint32_t)out.tris.size();
         uint32_t inner_k = 0, tri_off = 0;
@@ -269,7 +370,7 @@ void build_bvh8(const float *P, const uint32_t *I, const uint32_t *tri_mesh, uin
                 node.qlo[k][s] = (uint8_t)ql;
                 node.qhi[k][s] = (uint8_t)qh;
             }
-            if (!c.leaf) {
+            if (!is_leaf(ch[child_in[s]])) {
                 node.imask |= (uint8_t)(1u << s);
                 node.meta[s] = (uint8_t)((1u << 5) | (24u + (uint32_t)s));
                 pending.push_back({ch[child_in[s]], node.child_base + inner_k, it.depth + 1});

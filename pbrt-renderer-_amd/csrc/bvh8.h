@@ -46,8 +46,12 @@ struct Bvh8 {
     double build_ms = 0.0;
 };
 
+// How BVH2 subtrees become 8-wide nodes: greedy (largest-area child expanded
+// first) or the SAH-optimal dynamic programme of Ylitie et al. 2017.
+constexpr int BVH8_COLLAPSE_GREEDY = 0, BVH8_COLLAPSE_SAH = 1;
+
 // positions: 3*nv floats; indices: 3*nt global vertex ids; tri_mesh: nt mesh ids.
 void build_bvh8(const float *positions, const uint32_t *indices, const uint32_t *tri_mesh, uint32_t nt, Bvh8 &out,
-                int nthreads = 0);
+                int nthreads = 0, int collapse = BVH8_COLLAPSE_SAH);
 
 } // namespace ng

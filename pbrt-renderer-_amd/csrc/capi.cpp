@@ -92,6 +92,7 @@ struct nori_ctx {
     uint32_t *work_ext() const { return persistent ? cdev.cnt + 10 : nullptr; }
     uint32_t *work_shadow() const { return persistent ? cdev.cnt + 11 : nullptr; }
     uint32_t sampler = 0;                  // option "sampler": 0 pcg32 per pixel, 1 Owen-scrambled Sobol'
+    int bvh_collapse = BVH8_COLLAPSE_SAH;  // option "bvh_collapse" (next upload): 0 greedy, 1 SAH-optimal
     bool scene_has_hair = false;
     Counters cdev{};
 
@@ -623,7 +624,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
             }
 
         Bvh8 bvh;
-        build_bvh8(d->positions, d->indices, tri_mesh.data(), d->num_triangles, bvh);
+        build_bvh8(d->positions, d->indices, tri_mesh.data(), d->num_triangles, bvh, 0, c->bvh_collapse);
 
         c->free_scene();
         SceneDev s{};
@@ -644,7 +645,13 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         s.emit_cdf_off = c->upload(cdf_off.data(), cdf_off.size());
         s.emit_cdf = c->upload(cdf.data(), cdf.size());
         s.emit_area = c->upload(area.data(), area.size());
-        s.stack_depth = bvh.max_depth + 1;
+        // Stack entries per lane: the walker pushes the rest of a node's child
+        // group when it descends into one child, so the stack holds groups of
+        // strictly increasing tree levels, each owned by a node with inner
+        // children; a node on the deepest level has none.  A tree of D levels
+        // thus needs D - 1 entries (one more for a postponed triangle group,
+        // which trace_lds adds for the postponing kernels).
+        s.stack_depth = std::max(1u, bvh.max_depth - 1u);
         // Scene bounds map hit positions to [0,1]^3 for the guiding features.
         f3 lo = mk(INFINITY, INFINITY, INFINITY), hi = mk(-INFINITY, -INFINITY, -INFINITY);
         for (uint32_t i = 0; i < d->num_vertices; ++i) {
@@ -1115,6 +1122,10 @@ int nori_gpu_set_option(nori_ctx *c, const char *name, int64_t value) {
         c->g_min_rough = (float)value * 1e-3f;
         c->sdev.guide_min_rough = c->g_min_rough;
         c->drop_graph();
+    } else if (n == "bvh_collapse") {
+        if (value != BVH8_COLLAPSE_GREEDY && value != BVH8_COLLAPSE_SAH)
+            return fail(c, NORI_E_ARG, "bvh_collapse must be 0 (greedy) or 1 (SAH-optimal)");
+        c->bvh_collapse = (int)value; // takes effect at the next scene upload
     } else if (n == "guide_reset") {
         c->guide_ready = false;
     } else {

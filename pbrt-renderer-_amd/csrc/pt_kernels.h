@@ -25,7 +25,7 @@ struct SceneDev {
     const float *emit_cdf;        // per emitter: normalised area CDF (tris+1)
     const float *emit_area;       // per emitter: total area
     uint32_t n_emit;
-    uint32_t stack_depth;
+    uint32_t stack_depth; // traversal stack entries per lane (node groups; tree levels - 1)
     uint32_t traversal; // 0: a node's triangles right after it; 1: ballot-postponed in closest-hit queries
     uint32_t trace_waves; // traversal kernels: at most this many waves per SIMD (LDS padding); 0: no cap
     uint32_t lds_per_cu;  // LDS bytes per compute unit of the device

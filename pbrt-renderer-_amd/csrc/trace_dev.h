@@ -15,7 +15,10 @@
 //     (1 + 2*gamma_3) (Pharr, Jakob, Humphreys, PBRT 3e §3.9.2), so no box that
 //     holds an accepted hit, or an equal-t hit with a smaller id, is ever culled;
 //   * stack: the node groups (child base, hit bits | inner mask) of Ylitie et
-//     al. 2017, kept in LDS, one column per lane; its depth is the tree depth;
+//     al. 2017, kept in LDS, one column per lane; descending into one child
+//     pushes the rest of its group, so the entries belong to strictly
+//     increasing tree levels that have inner children: a tree of D levels
+//     needs D - 1 entries;
 //   * schedule (POSTPONE, closest-hit queries): either a lane tests all
 //     triangles of a node's hit leaves right after visiting it, or, as Ylitie
 //     et al. 2017 (section 4) postpone them, a wave-wide ballot counts the
@@ -25,8 +28,8 @@
 //     not hold up the lanes walking nodes; otherwise it tests one triangle per
 //     step.  A lane keeps at most one postponed group on its stack (sharing it
 //     with node groups: a node group always has child hit bits in its top
-//     byte, a triangle group never does), so the stack needs one entry more
-//     than the tree depth.  Occlusion queries do not postpone: an early
+//     byte, a triangle group never does), so the stack needs one entry more.
+//     Occlusion queries do not postpone: an early
 //     occluder ends them.
 #pragma once
 #include "nori_math.h"
