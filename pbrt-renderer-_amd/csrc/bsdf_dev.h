@@ -420,18 +420,22 @@ NG_HD void material_eval_pdf(f3 wi, f3 wo, const Mat &m, f3 &f, float &pdf) {
         pdf = 0.0f;
     }
 }
-// Dispatch including hair (h is ignored by the other materials).
+// Dispatch including hair (h is ignored by the other materials).  HAIR =
+// false compiles the hair BCSDF out, for callers that never see hair (its
+// per-lobe tables would otherwise cost registers and scratch everywhere).
+template <bool HAIR = true>
 NG_HD void material_eval_pdf(f3 wi, f3 wo, const Mat &m, float h, f3 &f, float &pdf) {
-    if (m.type == NORI_MAT_HAIR) hair_eval_pdf(wi, wo, m, h, f, pdf);
+    if (HAIR && m.type == NORI_MAT_HAIR) hair_eval_pdf(wi, wo, m, h, f, pdf);
     else material_eval_pdf(wi, wo, m, f, pdf);
 }
 NG_HD int material_sample_dims(uint32_t t) { return t == NORI_MAT_HAIR ? 4 : t == NORI_MAT_DISNEY ? 3 : 2; }
 
 // Returns f*|cos|/pdf; u holds material_sample_dims(type) uniforms.
+template <bool HAIR = true>
 NG_HD f3 material_sample(f3 wi, const float *u, const Mat &m, float h, f3 &wo, float &pdf) {
     wo = mk(0, 0, 1);
     pdf = 0.0f;
-    if (m.type == NORI_MAT_HAIR) return hair_sample(wi, u, m, h, wo, pdf);
+    if (HAIR && m.type == NORI_MAT_HAIR) return hair_sample(wi, u, m, h, wo, pdf);
     if (wi.z <= 0.0f) return mk(0.0f);
     if (m.type == NORI_MAT_DIFFUSE) {
         wo = cosine_sample_hemisphere(u[0], u[1]);

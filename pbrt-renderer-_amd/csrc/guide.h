@@ -6,7 +6,8 @@
 // probability c of one-sample MIS between q and the BSDF:
 //   P(w) = c q(w) + (1 - c) b(w).
 //   features  per axis of the scene-normalised position, 8 octaves of
-//             (sin, cos)(2^k pi p) (48); shading normal (3); direction to the
+//             (sin, cos)(2^k pi p) (48, octaves 1-3 and 5-7 by angle
+//             doubling); shading normal (3); direction to the
 //             viewer (3); roughness, metallic; 8 zeros
 //   outputs   mean_j = normalize(o[3j..3j+2]), kappa_j = exp(clamp(o[24+j], -4, 9)),
 //             weights = softmax(o[32..39]), c = 0.1 + 0.5 sigmoid(o[40])
@@ -65,12 +66,19 @@ struct GuideSample {
 // direction, roughness and metallic (h = 0) or the zero padding (h = 1).
 NG_HD void guide_feature_block(f3 p01, f3 n, f3 wi, float rough, float metal, int s, int h, float *f) {
     if (s < 3) {
+        // octave 4h from sincos, the next three by doubling the angle
+        // (sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x): each doubling
+        // about doubles the absolute error, to a few 1e-7 at the top octave,
+        // far below the features' BF16 rounding
         const float p = s == 0 ? p01.x : (s == 1 ? p01.y : p01.z);
+        float sn, cs;
+        sincosf((float)(1 << (4 * h)) * PI * p, &sn, &cs);
         for (int k = 0; k < 4; ++k) {
-            float sn, cs;
-            sincosf((float)(1 << (4 * h + k)) * PI * p, &sn, &cs);
             f[2 * k] = sn;
             f[2 * k + 1] = cs;
+            const float s2 = 2.0f * sn * cs;
+            cs = 1.0f - 2.0f * sn * sn;
+            sn = s2;
         }
     } else if (h == 0) {
         f[0] = n.x; f[1] = n.y; f[2] = n.z;
@@ -157,6 +165,30 @@ NG_HD GuideMix guide_unpack(const GuidePacked &p) {
     return g;
 }
 
+// The same decode one lobe at a time, straight from the packed record, for
+// callers that keep the 24-register packed form live instead of the unpacked
+// mixture: guide_lobe_* and guide_q_packed / guide_sample_packed compute the
+// same values as guide_unpack followed by guide_q / guide_sample.
+NG_HD float guide_packed_inv_wsum(const GuidePacked &p) {
+    float s = 0.0f;
+    for (int j = 0; j < 8; ++j) s += (float)p.v[32 + j];
+    return 1.0f / s;
+}
+NG_HD f3 guide_lobe_mu(const GuidePacked &p, int j) {
+    const f3 r = mk((float)p.v[3 * j], (float)p.v[3 * j + 1], (float)p.v[3 * j + 2]);
+    return r * (1.0f / fmaxf(sqrtf(dot(r, r)), 1e-12f));
+}
+NG_HD float guide_q_packed(const GuidePacked &p, float inv_wsum, f3 w) {
+    float q = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float k = (float)p.v[24 + j];
+        const float norm = k / (2.0f * PI * (1.0f - expf(-2.0f * k)));
+        q += ((float)p.v[32 + j] * inv_wsum) * norm * expf(k * (dot(guide_lobe_mu(p, j), w) - 1.0f));
+    }
+    return q;
+}
+
 NG_HD float guide_q(const GuideMix &g, f3 w) {
     float q = 0.0f;
     for (int j = 0; j < 8; ++j) q += g.w[j] * g.norm[j] * expf(g.kappa[j] * (dot(g.mu[j], w) - 1.0f));
@@ -176,12 +208,46 @@ NG_HD f3 vmf_sample(f3 mu, float k, float u1, float u2) {
     return normalize(mu * cw + (t * cp + s * sp) * sw);
 }
 
-// Pick a lobe by weight with u in [0,1), then sample it.
+// Pick a lobe by weight with u in [0,1): the first j with ul < w_0 + .. + w_j
+// (summed in order; the last lobe when rounding leaves ul above the total),
+// then sample it.  Unrolled with static indices only, so the mixture stays in
+// registers.
 NG_HD f3 guide_sample(const GuideMix &g, float ul, float u1, float u2) {
-    int j = 0;
-    float acc = g.w[0];
-    while (j < 7 && ul >= acc) acc += g.w[++j];
-    return vmf_sample(g.mu[j], g.kappa[j], u1, u2);
+    f3 mu = g.mu[0];
+    float k = g.kappa[0], acc = g.w[0];
+    bool found = ul < acc;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        if (!found) {
+            mu = g.mu[j];
+            k = g.kappa[j];
+            acc += g.w[j];
+            found = ul < acc;
+        }
+    }
+    return vmf_sample(mu, k, u1, u2);
+}
+NG_HD f3 guide_sample_packed(const GuidePacked &p, float inv_wsum, float ul, float u1, float u2) {
+    int sel = 0;
+    float acc = (float)p.v[32] * inv_wsum;
+    bool found = ul < acc;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        if (!found) {
+            sel = j;
+            acc += (float)p.v[32 + j] * inv_wsum;
+            found = ul < acc;
+        }
+    }
+    f3 mu = mk(0.0f);
+    float k = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j == sel) {
+            mu = guide_lobe_mu(p, j);
+            k = (float)p.v[24 + j];
+        }
+    return vmf_sample(mu, k, u1, u2);
 }
 
 // Loss -W log P and its gradient w.r.t. the outputs (41 used, rest zero).

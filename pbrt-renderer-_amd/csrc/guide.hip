@@ -80,7 +80,7 @@ __global__ __launch_bounds__(LAYER_BLOCK) void k_mlp_infer(const bf16_t *__restr
 #pragma unroll
     for (int s = 0; s < 4; ++s) xb[s] = live ? load8(X + (size_t)r * 64 + 16 * s + 8 * h) : bf16x8{};
     f32x16 acc[2];
-    mlp_forward_wave(xb, W, params, acc);
+    mlp_forward_wave(xb, W, params + 4096, G_LAYER_PARAMS, acc);
     if (live) {
 #pragma unroll
         for (int t = 0; t < 2; ++t)
@@ -220,6 +220,18 @@ __global__ void k_mixture(const float *out64, uint32_t n, const float *u3, const
     q[i] = qq;
     c[i] = m.c;
     P[i] = m.c * qq + (1.0f - m.c) * bpdf[i];
+    // The renderer keeps the mixture packed and evaluates it lobe by lobe
+    // (guide_q_packed, guide_sample_packed); they must agree with
+    // guide_unpack + guide_q / guide_sample on the same record, else P is NaN.
+    GuidePacked pk;
+    guide_pack(o, pk);
+    const GuideMix mu = guide_unpack(pk);
+    const float inv = guide_packed_inv_wsum(pk);
+    const float q1 = guide_q(mu, w), q2 = guide_q_packed(pk, inv, w);
+    const f3 d1 = guide_sample(mu, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2]);
+    const f3 d2 = guide_sample_packed(pk, inv, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2]);
+    const f3 dd = d1 - d2;
+    if (!(fabsf(q1 - q2) <= 1e-5f * fabsf(q1) + 1e-30f) || !(dot(dd, dd) <= 1e-10f)) P[i] = __int_as_float(0x7fc00000);
 }
 
 inline uint32_t nblk(uint64_t n, int b) { return (uint32_t)((n + b - 1) / b); }

@@ -28,15 +28,17 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
 }
 
 // xb: the lane's input fragments (row l%32, k = 16s + 8(l/32) .. +7 for
-// s = 0..3).  On return acc holds the last layer's outputs in the layout above.
-// Every lane of the wave must call it (the exchanges are wave-wide).
+// s = 0..3).  W: BF16 weights [layer][out][in]; bias: layer l's 64 biases at
+// bias + l * bias_stride (G_LAYER_PARAMS in the FP32 parameter block, 64 in a
+// packed copy).  On return acc holds the last layer's outputs in the layout
+// above.  Every lane of the wave must call it (the exchanges are wave-wide).
 __device__ __forceinline__ void mlp_forward_wave(bf16x8 xb[4], const bf16_t *__restrict__ W,
-                                                 const float *__restrict__ params, f32x16 acc[2]) {
+                                                 const float *__restrict__ bias, int bias_stride, f32x16 acc[2]) {
     const uint32_t lane = threadIdx.x & 63u, h = lane >> 5, col = lane & 31u;
 #pragma unroll
     for (int l = 0; l < G_LAYERS; ++l) {
         const bf16_t *Wl = W + l * 4096;
-        const float *bl = params + l * G_LAYER_PARAMS + 4096;
+        const float *bl = bias + l * bias_stride;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             acc[t] = f32x16{};

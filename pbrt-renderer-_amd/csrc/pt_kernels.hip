@@ -944,6 +944,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *cnt_in) return;
     constexpr bool ALL = CLS == CLS_ALL || CLS == CLS_ALL_SSS;
+    // only these launches can meet hair, so only they compile the hair BCSDF
+    constexpr bool HAIR_CODE = ALL || CLS == CLS_HAIR;
     const uint32_t pos = ALL ? i : q_cls[i];
     const float4 ro = p.ray_o[pos], rd = p.ray_d[pos], hit = p.hit[pos], th = p.thr[pos], ax = p.aux[pos];
     const uint32_t meta = p.meta[pos], id = p.id[pos];
@@ -1253,9 +1255,18 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                 // density is P(w) = c q(w) + (1-c) b(w) wherever it enters MIS.
                 const bool guide_v = GUIDE && !hair && guide_vertex(s, mr);
                 const bool guided = guide_v && g.out != nullptr;
-                GuideMix mix;
-                if (guided) mix = guide_unpack(g.out[pos]);
-                auto tech_pdf = [&](f3 w_world, float b) { return guided ? mix.c * guide_q(mix, w_world) + (1.0f - mix.c) * b : b; };
+                // the packed mixture (24 registers) stays live across both
+                // next-event estimates; lobes are decoded where evaluated
+                GuidePacked mix;
+                float mix_inv = 0.0f, mix_c = 0.0f;
+                if (guided) {
+                    mix = g.out[pos];
+                    mix_inv = guide_packed_inv_wsum(mix);
+                    mix_c = (float)mix.v[40];
+                }
+                auto tech_pdf = [&](f3 w_world, float b) {
+                    return guided ? mix_c * guide_q_packed(mix, mix_inv, w_world) + (1.0f - mix_c) * b : b;
+                };
                 // Area-light NEE, shaders/Emitter.hlsli:43-89, 105-153
                 if (s.n_emit > 0) {
                     LightPt lp = sample_light(s, rng);
@@ -1267,7 +1278,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     if ((hair || ct > 0.0f) && cl > 0.0f) {
                         f3 f;
                         float pb;
-                        material_eval_pdf(wi, to_local(ww, Tg, B, N), m, hh, f, pb);
+                        material_eval_pdf<HAIR_CODE>(wi, to_local(ww, Tg, B, N), m, hh, f, pb);
                         pb = tech_pdf(ww, pb);
                         float pe = lp.pdfA * dist * dist / cl;
                         float w = balance_heuristic(pe, pb);
@@ -1288,7 +1299,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     if (pe > 0.0f && (hair || ct > 0.0f)) {
                         f3 f;
                         float pb;
-                        material_eval_pdf(wi, to_local(dir, Tg, B, N), m, hh, f, pb);
+                        material_eval_pdf<HAIR_CODE>(wi, to_local(dir, Tg, B, N), m, hh, f, pb);
                         pb = tech_pdf(dir, pb);
                         float w = balance_heuristic(pe, pb);
                         re = ShadowRec{origin(dir), dir, T * (Le * f * (fabsf(ct) / fmaxf(pe, 1e-20f) * w)), 0.0f, 1e20f,
@@ -1307,7 +1318,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     u[3] = hair ? rng_float(rng) : 0.0f;
                     f3 wo;
                     float pdf;
-                    f3 wgt = material_sample(wi, u, m, hh, wo, pdf);
+                    f3 wgt = material_sample<HAIR_CODE>(wi, u, m, hh, wo, pdf);
                     if (pdf > 0.0f && !is_zero(wgt)) {
                         new_d = to_world(wo, Tg, B, N);
                         new_o = origin(new_d);
@@ -1319,9 +1330,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                     }
                 } else {
                     f3 wo;
-                    if (rng_float(rng) < mix.c) {
+                    if (rng_float(rng) < mix_c) {
                         float ul = rng_float(rng), u1 = rng_float(rng), u2 = rng_float(rng);
-                        w_world = guide_sample(mix, ul, u1, u2);
+                        w_world = guide_sample_packed(mix, mix_inv, ul, u1, u2);
                         wo = to_local(w_world, Tg, B, N);
                     } else {
                         float u[4];
@@ -1330,7 +1341,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
                         u[2] = type == NORI_MAT_DISNEY ? rng_float(rng) : 0.0f;
                         u[3] = 0.0f;
                         float pdf;
-                        material_sample(wi, u, m, 0.0f, wo, pdf);
+                        material_sample<false>(wi, u, m, 0.0f, wo, pdf);
                         w_world = to_world(wo, Tg, B, N);
                     }
                     if (wo.z > 0.0f) {
@@ -1399,57 +1410,72 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
     }
 }
 
-// Guide mixtures of the guided vertices among the queued paths, one wave per
-// 32 rows: both lanes of a row compute their half of its features from the
-// path state (guide_feature_block), the network runs in registers
-// (mlp_forward_wave) and the row's lane 0..31 packs the mixture.  Waves with
-// no guided vertex leave at once.
+// Guide mixtures of the guided vertices among the queued paths.  A block
+// stages the network (BF16 weights, FP32 biases: 33 KB) in LDS once, then each
+// of its four waves takes tiles of 32 rows in turn: both lanes of a row
+// compute their half of its features from the path state
+// (guide_feature_block), the network runs in registers (mlp_forward_wave) and
+// the row's lane 0..31 packs the mixture.  Tiles without a guided vertex are
+// skipped.
+constexpr uint32_t GUIDE_INFER_BLOCKS = 1024; // four per CU: the LDS copies allow four
 __global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, const uint32_t *__restrict__ q,
                                                      const uint32_t *__restrict__ count, uint32_t cap,
                                                      const bf16_t *__restrict__ W, const float *__restrict__ params,
                                                      GuidePacked *out) {
+    __shared__ __align__(16) bf16_t sW[G_LAYERS * 4096];
+    __shared__ float sB[G_LAYERS * 64];
     const uint32_t rows = min(*count, cap);
+    if (blockIdx.x * 128u >= rows) return; // the same for the whole block: no tile of its own
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(W);
+        uint4 *dst = reinterpret_cast<uint4 *>(sW);
+        for (uint32_t i = threadIdx.x; i < G_LAYERS * 4096 / 8; i += 256u) dst[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < G_LAYERS * 64; i += 256u)
+            sB[i] = params[(i >> 6) * G_LAYER_PARAMS + 4096 + (i & 63u)];
+    }
+    __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t row0 = (blockIdx.x * 4u + wave) * 32u;
-    if (row0 >= rows) return;
-    const uint32_t h = lane >> 5, r = row0 + (lane & 31u);
-    uint32_t pos = 0;
-    bool ok = false;
-    f3 p01 = mk(0.0f), N = mk(0.0f), wi = mk(0.0f);
-    float rough = 0.0f, metal = 0.0f;
-    if (r < rows) {
-        pos = q ? q[r] : r;
-        const float4 hit = p.hit[pos];
-        const uint32_t prim = __float_as_uint(hit.y);
-        if (prim != MISS) {
-            const nori_material &mr = s.mats[s.tri_mesh[prim]];
-            if (guide_vertex(s, mr)) {
-                const f3 o = xyz(p.ray_o[pos]), d = xyz(p.ray_d[pos]);
-                f3 Ng;
-                bool back;
-                hit_frame(s, prim, hit.z, hit.w, d, N, Ng, back);
-                p01 = (o + d * hit.x - s.bmin) * s.binv;
-                wi = -d;
-                rough = mr.roughness;
-                metal = mr.metallic;
-                ok = true;
+    const uint32_t h = lane >> 5;
+    for (uint32_t row0 = (blockIdx.x * 4u + wave) * 32u; row0 < rows; row0 += gridDim.x * 128u) {
+        const uint32_t r = row0 + (lane & 31u);
+        uint32_t pos = 0;
+        bool ok = false;
+        f3 p01 = mk(0.0f), N = mk(0.0f), wi = mk(0.0f);
+        float rough = 0.0f, metal = 0.0f;
+        if (r < rows) {
+            pos = q ? q[r] : r;
+            const float4 hit = p.hit[pos];
+            const uint32_t prim = __float_as_uint(hit.y);
+            if (prim != MISS) {
+                const nori_material &mr = s.mats[s.tri_mesh[prim]];
+                if (guide_vertex(s, mr)) {
+                    const f3 o = xyz(p.ray_o[pos]), d = xyz(p.ray_d[pos]);
+                    f3 Ng;
+                    bool back;
+                    hit_frame(s, prim, hit.z, hit.w, d, N, Ng, back);
+                    p01 = (o + d * hit.x - s.bmin) * s.binv;
+                    wi = -d;
+                    rough = mr.roughness;
+                    metal = mr.metallic;
+                    ok = true;
+                }
             }
         }
-    }
-    if (__ballot(ok) == 0) return; // the same for every lane of the wave
-    bf16x8 xb[4];
+        if (__ballot(ok) == 0) continue; // the same for every lane of the wave
+        bf16x8 xb[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        float f[8];
-        guide_feature_block(p01, N, wi, rough, metal, b, (int)h, f);
+        for (int b = 0; b < 4; ++b) {
+            float f[8];
+            guide_feature_block(p01, N, wi, rough, metal, b, (int)h, f);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) xb[b][k] = (bf16_t)f[k];
+            for (int k = 0; k < 8; ++k) xb[b][k] = (bf16_t)f[k];
+        }
+        f32x16 acc[2];
+        mlp_forward_wave(xb, sW, sB, 64, acc);
+        float o[64];
+        mlp_row_outputs(acc, o);
+        if (ok && h == 0) guide_pack(o, out[pos]);
     }
-    f32x16 acc[2];
-    mlp_forward_wave(xb, W, params, acc);
-    float o[64];
-    mlp_row_outputs(acc, o);
-    if (ok && h == 0) guide_pack(o, out[pos]);
 }
 
 // Training samples from the records of finished paths: the weight is the
@@ -1709,7 +1735,9 @@ void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *s
 
 void launch_guide_infer(const SceneDev &s, const PathsDev &p, const uint32_t *q, const uint32_t *count, uint32_t cap,
                         const bf16_t *W, const float *params, GuidePacked *out, hipStream_t st) {
-    if (cap) hipLaunchKernelGGL(k_guide_infer, dim3(blocks(cap, 128)), dim3(256), 0, st, s, p, q, count, cap, W, params, out);
+    if (cap)
+        hipLaunchKernelGGL(k_guide_infer, dim3(std::min(blocks(cap, 128), GUIDE_INFER_BLOCKS)), dim3(256), 0, st, s, p, q,
+                           count, cap, W, params, out);
 }
 
 void launch_guide_finalize(const SceneDev &s, const GuideShadeDev &g, const uint32_t *ids, const uint32_t *nids,
