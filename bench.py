@@ -39,8 +39,9 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES = 80, 48
 # per extension ray besides the traversal: ray read 32 B, hit write 16 B, class-queue entry 4 B
 EXT_IO_BYTES = 32 + 16 + 4
-# per alpha-tested candidate: 3 vertex indices 12 B, 3 UVs 24 B, a 2x2 texel footprint 16 B
-ALPHA_TEST_BYTES = 12 + 24 + 16
+# per alpha-tested candidate: its 32-B alpha record (3 UVs, texture level 0) and a 2x2 footprint of
+# decoded (red, alpha) texels, 8 B each
+ALPHA_TEST_BYTES = 32 + 4 * 8
 BATCHED_SIZES = [(256, 256), (320, 240), (400, 300), (480, 270), (512, 512), (640, 360), (640, 480), (800, 450),
                  (800, 600), (960, 540), (1024, 576), (1024, 768), (1280, 720), (1366, 768), (1600, 900),
                  (1920, 1080)]

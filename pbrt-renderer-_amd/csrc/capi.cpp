@@ -629,7 +629,6 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         c->free_scene();
         SceneDev s{};
         s.nodes = (const uint4 *)c->upload(bvh.nodes.data(), bvh.nodes.size());
-        s.tris = (const float4 *)c->upload(bvh.tris.data(), bvh.tris.size());
         s.pos = c->upload(d->positions, 3 * (size_t)d->num_vertices);
         s.nrm = d->normals ? c->upload(d->normals, 3 * (size_t)d->num_vertices) : nullptr;
         s.uv = d->uvs ? c->upload(d->uvs, 2 * (size_t)d->num_vertices) : nullptr;
@@ -741,7 +740,49 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
             }
             s.tex = TexTables{c->upload(tu.tex.data(), tu.tex.size()), c->upload(tu.texel.data(), tu.texel.size()),
                               c->upload(tu.lut.data(), tu.lut.size()), (uint32_t)tu.tex.size()};
+            // One alpha record per triangle of an alpha-tested mesh, named by
+            // the flags word of its BVH8 triangle record.
+            if (s.has_alpha) {
+                // decoded level 0 of every alpha texture
+                std::vector<float2> planes;
+                std::vector<uint32_t> plane_off(d->num_textures, ~0u);
+                for (uint32_t m = 0; m < d->num_meshes; ++m) {
+                    const uint32_t ti = mats[m].alphaTexIndex;
+                    if (ti == NORI_NO_TEXTURE || plane_off[ti] != ~0u) continue;
+                    const TexDev &td = tu.tex[ti];
+                    plane_off[ti] = (uint32_t)planes.size();
+                    const float *cl = tu.lut.data() + (td.srgb ? 0 : 256), *la = tu.lut.data() + 256;
+                    for (size_t k = 0; k < (size_t)td.w * td.h; ++k) {
+                        const uint32_t v = tu.texel[td.off[0] + k];
+                        planes.push_back(make_float2(cl[v & 0xFFu], la[v >> 24]));
+                    }
+                }
+                s.alpha_planes = c->upload(planes.data(), planes.size());
+                std::vector<AlphaTri> at;
+                for (TriRec &r : bvh.tris) {
+                    const uint32_t ti = mats[r.mesh].alphaTexIndex;
+                    if (ti == NORI_NO_TEXTURE) continue;
+                    if (at.size() >= TRI_ALPHA_MASK - 1u) throw ArgError("scene: too many alpha-tested triangles", NORI_E_LIMIT);
+                    AlphaTri a{};
+                    for (int k = 0; k < 3; ++k) {
+                        const uint32_t vi = d->indices[3 * (size_t)r.prim + k];
+                        a.uv[2 * k] = d->uvs[2 * (size_t)vi];
+                        a.uv[2 * k + 1] = d->uvs[2 * (size_t)vi + 1];
+                    }
+                    const TexDev &td = tu.tex[ti];
+                    a.off0 = plane_off[ti];
+                    a.size = (td.w - 1u) | (td.h - 1u) << 15;
+                    at.push_back(a);
+                    r.pad = (uint32_t)at.size();
+                }
+                s.alpha_tris = c->upload(at.data(), at.size());
+            }
         }
+        // Shadow rays pass mirror and dielectric meshes (flag bit TRI_PASS).
+        if (any_pass)
+            for (TriRec &r : bvh.tris)
+                if (pass_ior[2 * (size_t)r.mesh] > 0.0f) r.pad |= TRI_PASS;
+        s.tris = (const float4 *)c->upload(bvh.tris.data(), bvh.tris.size());
         // stacks of 128 lanes (subsurface shading blocks) or of 64 lanes at twice
         // the depth (postponing traversal) must fit 64 KiB of LDS
         if ((size_t)s.stack_depth * 128 * sizeof(uint2) > 64 * 1024)

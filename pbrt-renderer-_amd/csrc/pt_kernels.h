@@ -9,6 +9,18 @@
 
 namespace ng {
 
+// Flags word of a BVH8 triangle record (TriRec::pad, read with the triangle):
+// bits 0..30 the alpha record's index + 1 (0: not alpha-tested), bit 31 set
+// for mirror and dielectric meshes, which shadow rays pass with Fresnel
+// attenuation.
+constexpr uint32_t TRI_ALPHA_MASK = 0x7FFFFFFFu, TRI_PASS = 0x80000000u;
+struct AlphaTri {
+    float uv[6];      // (u, v) of the triangle's vertices 0, 1, 2
+    uint32_t off0;    // offset of the alpha texture's level 0 in SceneDev::alpha_planes
+    uint32_t size;    // (width - 1) | (height - 1) << 15
+};
+static_assert(sizeof(AlphaTri) == 32, "alpha record is 32 bytes");
+
 struct SceneDev {
     const uint4 *nodes;  // 5 uint4 per BVH8 node
     const float4 *tris;  // 3 float4 per triangle (leaf order)
@@ -38,6 +50,15 @@ struct SceneDev {
     // per mesh {extIOR, intIOR} of mirror/dielectric meshes, {0, 0} otherwise:
     // shadow rays pass those with Fresnel attenuation (null: none in the scene)
     const float2 *pass_ior;
+    // Per triangle of an alpha-tested mesh, in the order the flags word of its
+    // BVH8 triangle record names (TRI_ALPHA_MASK bits = index + 1): the three
+    // UVs and the alpha texture's level 0, so the alpha test reads one record
+    // instead of the material, vertex indices, UVs and texture record.
+    const struct AlphaTri *alpha_tris;
+    // Level 0 of each alpha texture decoded, one float2 per texel: the red
+    // channel (sRGB-decoded where the texture is sRGB) and alpha, the values
+    // the texture tables give those channels.
+    const float2 *alpha_planes;
     // path sampler: 0 pcg32 per pixel, 1 Owen-scrambled Sobol' (shaders/RNG.hlsli)
     uint32_t sobol;
     TexTables tex;        // scene textures (count 0: none)

@@ -313,14 +313,20 @@ __device__ __forceinline__ void hit_uv(const SceneDev &s, uint32_t prim, float u
 // The reference threads xi through the traversal order (a PCGHash chain); here
 // xi = pcg_hash(seed ^ pcg_hash(prim)) * 2^-32 depends only on the ray's seed
 // and the triangle, so the outcome does not depend on the traversal order.
-__device__ __forceinline__ bool alpha_accept(const SceneDev &s, uint32_t prim, uint32_t mesh, float u, float v,
+// The candidate's triangle flags word (TRI_ALPHA_MASK) names its alpha record
+// (zero: not alpha-tested, accepted); the UVs and texel lookups are those of
+// hit_uv and tex_alpha_value, bit for bit.
+__device__ __forceinline__ bool alpha_accept(const SceneDev &s, uint32_t prim, uint32_t flags, float u, float v,
                                              uint32_t seed, bool shadow) {
 #pragma clang fp contract(off)
-    const uint32_t ti = s.mats[mesh].alphaTexIndex;
-    if (ti == NORI_NO_TEXTURE || !s.uv) return true;
-    float tu, tv;
-    hit_uv(s, prim, u, v, tu, tv);
-    const float av = tex_alpha_value(s.tex, ti, tu, tv, shadow);
+    const uint32_t ai = flags & TRI_ALPHA_MASK;
+    if (ai == 0u) return true;
+    const AlphaTri a = s.alpha_tris[ai - 1u];
+    const float w = 1.0f - u - v;
+    const float tu = a.uv[0] * w + a.uv[2] * u + a.uv[4] * v;
+    const float tv = a.uv[1] * w + a.uv[3] * u + a.uv[5] * v;
+    const float av = alpha_plane_value(s.alpha_planes + a.off0, (a.size & 0x7FFFu) + 1u, ((a.size >> 15) & 0x7FFFu) + 1u,
+                                       tu, tv, shadow);
     if (av < 0.01f) return false;
     const float xi = (float)pcg_hash(seed ^ pcg_hash(prim)) * 2.3283064365386963e-10f;
     return av * av >= xi;
@@ -354,9 +360,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
         TraceOut r;
         if (ALPHA) {
             const uint32_t seed = alpha_seed_path(p.aux[i]);
-            auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3, f3) {
-                if (COUNT && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE) ++na;
-                return alpha_accept(s, prim, mesh, u, v, seed, false);
+            auto acc = [&](uint32_t prim, uint32_t, uint32_t flags, float u, float v, f3, f3) {
+                if (COUNT && (flags & TRI_ALPHA_MASK)) ++na;
+                return alpha_accept(s, prim, flags, u, v, seed, false);
             };
             r = traverse_bvh8<false, COUNT, POST>(s.nodes, s.tris, xyz(a), xyz(b), a.w, b.w, stk + threadIdx.x,
                                                   blockDim.x, &nn, &nt, acc);
@@ -459,8 +465,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend_p(SceneDev s, PathsDev p
         if (pos != 0xFFFFFFFFu) {
             bool fin;
             if (ALPHA) {
-                auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3, f3) {
-                    return alpha_accept(s, prim, mesh, u, v, seed, false);
+                auto acc = [&](uint32_t prim, uint32_t, uint32_t flags, float u, float v, f3, f3) {
+                    return alpha_accept(s, prim, flags, u, v, seed, false);
                 };
                 fin = w.step(s.nodes, s.tris, stk + threadIdx.x, blockDim.x, acc);
             } else {
@@ -537,18 +543,16 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow(SceneDev s, ShadowDev sh
             // ray through (shaders/RayQueryTrace.hlsli:96-124)
             const f3 dir = xyz(b);
             const uint32_t seed = s.has_alpha ? alpha_seed_shadow(__float_as_uint(e.z)) : 0u;
-            auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3 e1, f3 e2) {
-                if (s.has_alpha && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE) {
+            auto acc = [&](uint32_t prim, uint32_t mesh, uint32_t flags, float u, float v, f3 e1, f3 e2) {
+                if (flags & TRI_ALPHA_MASK) {
                     if (COUNT) ++na;
-                    return alpha_accept(s, prim, mesh, u, v, seed, true);
+                    return alpha_accept(s, prim, flags, u, v, seed, true);
                 }
-                if (s.pass_ior) {
+                if (flags & TRI_PASS) {
                     const float2 ior = s.pass_ior[mesh];
-                    if (ior.x > 0.0f) {
-                        const f3 ng = normalize(cross(e1, e2));
-                        trans *= 1.0f - fresnel_dielectric(fabsf(dot(dir, ng)), ior.x, ior.y);
-                        return false;
-                    }
+                    const f3 ng = normalize(cross(e1, e2));
+                    trans *= 1.0f - fresnel_dielectric(fabsf(dot(dir, ng)), ior.x, ior.y);
+                    return false;
                 }
                 return true;
             };
@@ -606,16 +610,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_shadow_p(SceneDev s, ShadowDev 
                 // alpha-tested meshes first (they occlude when accepted), then
                 // mirror/dielectric meshes attenuate by 1 - Fresnel and let the
                 // ray through (shaders/RayQueryTrace.hlsli:96-124)
-                auto acc = [&](uint32_t prim, uint32_t mesh, float u, float v, f3 e1, f3 e2) {
-                    if (s.has_alpha && s.mats[mesh].alphaTexIndex != NORI_NO_TEXTURE)
-                        return alpha_accept(s, prim, mesh, u, v, seed, true);
-                    if (s.pass_ior) {
+                auto acc = [&](uint32_t prim, uint32_t mesh, uint32_t flags, float u, float v, f3 e1, f3 e2) {
+                    if (flags & TRI_ALPHA_MASK) return alpha_accept(s, prim, flags, u, v, seed, true);
+                    if (flags & TRI_PASS) {
                         const float2 ior = s.pass_ior[mesh];
-                        if (ior.x > 0.0f) {
-                            const f3 ng = normalize(cross(e1, e2));
-                            trans *= 1.0f - fresnel_dielectric(fabsf(dot(w.d, ng)), ior.x, ior.y);
-                            return false;
-                        }
+                        const f3 ng = normalize(cross(e1, e2));
+                        trans *= 1.0f - fresnel_dielectric(fabsf(dot(w.d, ng)), ior.x, ior.y);
+                        return false;
                     }
                     return true;
                 };
@@ -875,8 +876,8 @@ __device__ bool sss_walk(const SceneDev &s, f3 pos, f3 dir, uint32_t mesh, float
         TraceOut r;
         if (s.has_alpha) {
             const uint32_t seed = rng_state_hash(rng);
-            auto acc = [&](uint32_t prim, uint32_t m, float u, float v, f3, f3) {
-                return alpha_accept(s, prim, m, u, v, seed, false);
+            auto acc = [&](uint32_t prim, uint32_t, uint32_t flags, float u, float v, f3, f3) {
+                return alpha_accept(s, prim, flags, u, v, seed, false);
             };
             r = traverse_bvh8<false, false>(s.nodes, s.tris, pos, dir, 1e-4f, dist, stk, stride, nullptr, nullptr, acc);
         } else {

@@ -84,27 +84,24 @@ NG_HD f3 tex_sample(const TexTables &t, uint32_t ti, float u, float v, float lod
 
 // The value the alpha test reads at level 0 (shaders/Shaders.hlsl:674-735):
 // the bilinear red channel for radiance rays, and for shadow rays the alpha
-// channel where it is below 1, else red.  The same arithmetic as tex_sample
-// at lod 0 on the channels it needs.
-NG_HD float tex_alpha_value(const TexTables &t, uint32_t ti, float u, float v, bool shadow) {
+// channel where it is below 1, else red; the same texels, weights and sums as
+// tex_sample at lod 0 on those channels, read from a decoded plane of level 0
+// ((red, alpha) per texel, the values the texture tables give them).
+NG_HD float alpha_plane_value(const float2 *plane, uint32_t tw, uint32_t th, float u, float v, bool shadow) {
 #pragma clang fp contract(off)
-    const TexDev &d = t.tex[ti];
-    const int w = (int)d.w, h = (int)d.h;
-    const float x = u * (float)d.w - 0.5f, y = v * (float)d.h - 0.5f;
+    const int w = (int)tw, h = (int)th;
+    const float x = u * (float)tw - 0.5f, y = v * (float)th - 0.5f;
     const float x0 = floorf(x), y0 = floorf(y);
     const float fx = x - x0, fy = y - y0, gx = 1.0f - fx, gy = 1.0f - fy;
     int xa = (int)x0 % w, ya = (int)y0 % h;
     if (xa < 0) xa += w;
     if (ya < 0) ya += h;
     const int xb = xa + 1 == w ? 0 : xa + 1, yb = ya + 1 == h ? 0 : ya + 1;
-    const uint32_t *row_a = t.texel + d.off[0] + (uint32_t)ya * (uint32_t)w;
-    const uint32_t *row_b = t.texel + d.off[0] + (uint32_t)yb * (uint32_t)w;
-    const uint32_t v00 = row_a[xa], v10 = row_a[xb], v01 = row_b[xa], v11 = row_b[xb];
-    const float *cl = t.lut + (d.srgb ? 0 : 256);
-    const float red = (cl[v00 & 0xFFu] * gx + cl[v10 & 0xFFu] * fx) * gy + (cl[v01 & 0xFFu] * gx + cl[v11 & 0xFFu] * fx) * fy;
+    const float2 *row_a = plane + (uint32_t)ya * (uint32_t)w, *row_b = plane + (uint32_t)yb * (uint32_t)w;
+    const float2 v00 = row_a[xa], v10 = row_a[xb], v01 = row_b[xa], v11 = row_b[xb];
+    const float red = (v00.x * gx + v10.x * fx) * gy + (v01.x * gx + v11.x * fx) * fy;
     if (!shadow) return red;
-    const float *la = t.lut + 256;
-    const float a = (la[v00 >> 24] * gx + la[v10 >> 24] * fx) * gy + (la[v01 >> 24] * gx + la[v11 >> 24] * fx) * fy;
+    const float a = (v00.y * gx + v10.y * fx) * gy + (v01.y * gx + v11.y * fx) * fy;
     return a < 1.0f ? a : red;
 }
 

@@ -76,7 +76,9 @@ __device__ __forceinline__ float byte_f(uint32_t w, int k) { return (float)((w >
 
 // Candidate filter that accepts every triangle (opaque geometry).
 struct AcceptAll {
-    __device__ __forceinline__ bool operator()(uint32_t, uint32_t, float, float, f3, f3) const { return true; }
+    __device__ __forceinline__ bool operator()(uint32_t, uint32_t, uint32_t, float, float, f3, f3) const {
+        return true;
+    }
 };
 
 // One ray's traversal as a resumable state machine: start() sets it up,
@@ -84,7 +86,7 @@ struct AcceptAll {
 // hit children) or pops the stack, and returns true once the ray is finished
 // (res final).  ANY = occlusion query (finishes on the first accepted hit).
 // stk: this lane's LDS column (entry i at stk[i * stride]).
-// accept(prim, mesh, u, v, e1, e2) decides whether a candidate that passes the
+// accept(prim, mesh, flags, u, v, e1, e2) decides whether a candidate that passes the
 // triangle test counts as a hit (the any-hit programs of shaders/Shaders.hlsl:
 // 674-735 and shaders/RayQueryTrace.hlsli:43-124: alpha test, glass and
 // mirrors on shadow rays); a rejected candidate leaves the search as it was.
@@ -231,7 +233,8 @@ template <bool ANY, bool COUNT, bool POSTPONE = false> struct Bvh8Walker {
         float t, u, v;
         if (mt_test(o, d, f3{t0.x, t0.y, t0.z}, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}, tmin, res.t, t, u, v)) {
             uint32_t prim = __float_as_uint(t0.w);
-            if (!accept(prim, __float_as_uint(t1.w), u, v, f3{t1.x, t1.y, t1.z}, f3{t2.x, t2.y, t2.z}))
+            if (!accept(prim, __float_as_uint(t1.w), __float_as_uint(t2.w), u, v, f3{t1.x, t1.y, t1.z},
+                        f3{t2.x, t2.y, t2.z}))
                 return false;
             if (t < res.t || prim < res.prim) {
                 res.t = t;
