@@ -14,9 +14,11 @@ The network starts from its fixed initialisation and is trained online by
 guided renders (twelve passes of 256 spp, Adam at 2e-3 then 5e-4); it is then
 frozen and compared with plain sampling at 1024 spp on independent seeds,
 against a reference of 2 x 16384 plain spp whose own error (a quarter of the
-relMSE between its halves) is subtracted.  Measured on MI355X at 96 x 96 with
-this schedule: 1.8-1.9x lower relMSE at equal spp, 1.2x more time per sample
-(training runs to run vary by about 10%).
+relMSE between its halves) is subtracted.  Measured on MI355X at 128 x 128
+with this schedule: 1.84-1.92x lower relMSE at equal spp and 1.10x the time
+per sample, 1.67-1.75x lower relMSE at equal GPU time (six training runs;
+longer training, larger batches or a wider range for the guide's sample share
+c did not move it).
 """
 import sys
 
@@ -93,8 +95,8 @@ def test_trained_guide_beats_plain(trained):
     gain_time = (e_p * t_p) / (e_g * t_g)
     _say(f"  after {steps} training steps: relMSE plain {e_p:.4g} ({t_p:.1f} ms) guided {e_g:.4g} ({t_g:.1f} ms); "
          f"equal-spp gain {gain_spp:.2f}x, equal-time gain {gain_time:.2f}x")
-    assert gain_spp > 1.4
-    assert gain_time > 1.1
+    assert gain_spp > 1.5
+    assert gain_time > 1.35
 
 
 def test_guided_render_unbiased_per_region(trained):
