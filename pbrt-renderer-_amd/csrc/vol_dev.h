@@ -138,13 +138,22 @@ struct BrickWalk {
         }
         return best;
     }
-    // into the neighbouring brick across `axis`; false once the walk leaves the grid
+    // into the neighbouring brick across `axis`; false once the walk leaves the
+    // grid (static indices only, so the walker stays in registers)
     NG_HD bool advance(const VolumeDev &v, f3 d, int axis) {
-        if (axis < 0) return false;
-        const float dd[3] = {d.x, d.y, d.z};
-        const int m[3] = {(int)v.mW, (int)v.mH, (int)v.mD};
-        bc[axis] += dd[axis] > 0.0f ? 1 : -1;
-        return bc[axis] >= 0 && bc[axis] < m[axis];
+        if (axis == 0) {
+            bc[0] += d.x > 0.0f ? 1 : -1;
+            return bc[0] >= 0 && bc[0] < (int)v.mW;
+        }
+        if (axis == 1) {
+            bc[1] += d.y > 0.0f ? 1 : -1;
+            return bc[1] >= 0 && bc[1] < (int)v.mH;
+        }
+        if (axis == 2) {
+            bc[2] += d.z > 0.0f ? 1 : -1;
+            return bc[2] >= 0 && bc[2] < (int)v.mD;
+        }
+        return false;
     }
 };
 
@@ -189,15 +198,21 @@ struct VolSeg {
     float tn, tf;
 };
 
+// The segment [tn, tf] of ray (o, d) inside one medium's box, clipped to
+// [0, tmax] (one step of vol_segments); false when empty.
+NG_HD bool vol_segment(const VolumeDev &v, f3 o, f3 d, float tmax, float &tn, float &tf) {
+    if (!vol_ray_aabb(o, d, v.vmin, v.vmax, tn, tf)) return false;
+    tn = fmaxf(tn, 0.0f);
+    tf = fminf(tf, tmax);
+    return tf > tn;
+}
+
 // shaders/Volume.hlsl:187-222
 NG_HD uint32_t vol_segments(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, VolSeg seg[8]) {
     uint32_t n = 0;
     for (uint32_t i = 0; i < nv && n < 8; ++i) {
         float tn, tf;
-        if (!vol_ray_aabb(o, d, vols[i].vmin, vols[i].vmax, tn, tf)) continue;
-        tn = fmaxf(tn, 0.0f);
-        tf = fminf(tf, tmax);
-        if (tf <= tn) continue;
+        if (!vol_segment(vols[i], o, d, tmax, tn, tf)) continue;
         uint32_t k = n;
         for (; k > 0; --k) {
             if (seg[k - 1].tn <= tn) break;
@@ -295,6 +310,15 @@ NG_HD bool vol_sample_free_flight(const VolumeDev *vols, uint32_t nv, f3 o, f3 d
     ts = tmax;
     vi = 0;
     if (nv == 0) return false;
+    if (nv == 1) {
+        // one medium: its segment directly (the segment list would be a
+        // dynamically indexed array, kept in scratch memory)
+        float tn, tf, th;
+        if (!vol_segment(vols[0], o, d, tmax, tn, tf) || vol_sigma_t_max(vols[0]) < 1e-8f) return false;
+        if (!vol_free_flight(vols[0], o, d, tn, tf, rng, weight, th)) return false;
+        ts = th;
+        return true;
+    }
     VolSeg seg[8];
     uint32_t n = vol_segments(vols, nv, o, d, tmax, seg);
     for (uint32_t k = 0; k < n; ++k) {
@@ -315,6 +339,16 @@ template <class R>
 NG_HD f3 vol_transmittance(const VolumeDev *vols, uint32_t nv, f3 o, f3 d, float tmax, R &rng) {
 #pragma clang fp contract(off)
     if (nv == 0) return mk(1.0f);
+    if (nv == 1) {
+        // one medium: its segment directly, as in vol_sample_free_flight
+        float tn, tf;
+        const VolumeDev &v = vols[0];
+        const f3 st = vol_sigma_t(v);
+        if (!vol_segment(v, o, d, tmax, tn, tf) || maxc(st) < 1e-8f) return mk(1.0f);
+        const f3 tr = v.hetero ? vol_ratio_tracking(v, o, d, tn, tf, rng)
+                               : mk(expf(-st.x * (tf - tn)), expf(-st.y * (tf - tn)), expf(-st.z * (tf - tn)));
+        return maxc(tr) < 1e-6f ? mk(0.0f) : tr;
+    }
     VolSeg seg[8];
     uint32_t n = vol_segments(vols, nv, o, d, tmax, seg);
     f3 tr = mk(1.0f);
