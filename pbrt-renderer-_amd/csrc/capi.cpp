@@ -650,7 +650,7 @@ int nori_gpu_upload_scene(nori_ctx *c, const nori_scene_desc *d) {
         // children; a node on the deepest level has none.  A tree of D levels
         // thus needs D - 1 entries (one more for a postponed triangle group,
         // which trace_lds adds for the postponing kernels).
-        s.stack_depth = std::max(1u, bvh.max_depth - 1u);
+        s.stack_depth = bvh.max_depth > 1u ? bvh.max_depth - 1u : 1u;
         // Scene bounds map hit positions to [0,1]^3 for the guiding features.
         f3 lo = mk(INFINITY, INFINITY, INFINITY), hi = mk(-INFINITY, -INFINITY, -INFINITY);
         for (uint32_t i = 0; i < d->num_vertices; ++i) {
