@@ -16,8 +16,9 @@ in a stream synchronisation.
 
 Secondary legs: the Disney icosphere grid (5,242,882 triangles, 2048x1024
 sky+sun envmap, 512x512, 256 spp, max depth 12) plain; the time to relMSE
-<= 1e-3 of plain and guided rendering of that grid at its 512x512 resolution;
-and the throughput of the Cornell box, the 50,000-strand hair scene (lit and
+<= 1e-3 of plain and guided rendering of that grid at its 512x512 resolution
+and of the sun room (guide_scenes.py, a scene lit through a wall patch that
+next-event estimation cannot reach) at 256x256; and the throughput of the Cornell box, the 50,000-strand hair scene (lit and
 white furnace) and the 256^3 heterogeneous medium (g = 0, 0.3, 0.8), each at
 its full size.
 
@@ -239,8 +240,10 @@ def time_to_rel_mse(gpu, ng, cam, depth, ref, ref_err, start_spp, max_spp, thres
     ((spp, Adam learning rate x 1e6) each, 32 steps per pool round), which
     count toward the image and the time; it is frozen afterwards.  Returns the
     GPU milliseconds of the renders (every kernel, guiding inference and
-    training included), the spp reached and the curve; past the last point
-    the time to thresh is extrapolated with relMSE ~ 1/N (labelled)."""
+    training included), the spp reached and the curve.  The time to thresh
+    is interpolated between the two points around the crossing (a straight
+    line in log relMSE over log time); past the last point it is
+    extrapolated with relMSE ~ 1/N (labelled)."""
     gpu.set_option("guiding", 1 if guided else 0)
     gpu.reset_film()
     ms, spp, curve, e = 0.0, 0, [], float("inf")
@@ -276,8 +279,17 @@ def time_to_rel_mse(gpu, ng, cam, depth, ref, ref_err, start_spp, max_spp, thres
     gpu.set_option("guide_train_steps", 4)
     gpu.set_option("guide_lr_micro", 2000)
     reached = e <= thresh
+    if not reached:
+        t_thr = ms * e / thresh
+    elif len(curve) >= 2 and curve[-2][2] > thresh and e > 0.0:
+        # crossing between the last two points, on the straight line through
+        # them in log relMSE over log time
+        (_, t1, e1), (_, t2, e2) = curve[-2], curve[-1]
+        t_thr = t1 * (t2 / t1) ** (np.log(e1 / thresh) / np.log(e1 / e2))
+    else:
+        t_thr = ms
     out = {"ms": round(ms, 2), "spp": spp, "rel_mse": e, "reached": reached,
-           "ms_to_threshold": round(ms if reached else ms * e / thresh, 1), "extrapolated": not reached,
+           "ms_to_threshold": round(float(t_thr), 1), "extrapolated": not reached,
            "curve": curve}
     if guided:
         out["train_steps"] = steps
@@ -330,6 +342,46 @@ def time_to_rel_mse_legs(gpu, ng, sc, args):
                                                         f["plain"]["ms_to_threshold"], 3),
             "speedup_footprint_guided_over_texel_plain": round(t["plain"]["ms_to_threshold"] /
                                                                f["guided"]["ms_to_threshold"], 3)}
+
+
+def sun_room_legs(gpu, ng, args):
+    """Plain and guided time to relMSE <= threshold on the sun room (a closed
+    room lit through a roof opening, the sun reaching most surfaces only via a
+    sunlit wall patch that next-event estimation cannot see), 256x256, depth 8,
+    footprint envmap weights.  The guided leg trains the network from its
+    initialisation online for four passes of 64 spp (32 Adam steps per pool
+    round, learning rate 2e-3, the last pass 5e-4), which count toward the
+    image and the time, then renders with the network frozen.  Reference: two
+    independent plain renders of 32768 spp, own error subtracted as in the
+    Disney grid legs."""
+    from guide_scenes import sun_room
+
+    sc = sun_room(width=256, height=256)
+    cam, depth, half = sc.camera, 8, 1 << 15
+    gpu.set_option("env_sampling", 1)
+    gpu.upload(sc)
+    gpu.set_option("guiding", 0)
+    refs = []
+    t0 = time.time()
+    for k in range(2):
+        gpu.reset_film()
+        img = gpu.render(ng.make_view(cam, half, depth, sample_offset=(1 << 24) + k * half), seed=2234 + k)
+        refs.append(img[:, :3] / img[:, 3:4])
+    ref = 0.5 * (refs[0] + refs[1])
+    ref_err = rel_mse(refs[0], refs[1]) / 4
+    log(f"[bench] relMSE reference (sun room): 256x256 at {2 * half} spp in {time.time() - t0:.1f}s, "
+        f"own relMSE {ref_err:.3g}")
+    train = [(64, 2000)] * 3 + [(64, 500)]
+    tt = {"threshold": args.mse_threshold, "image": "256x256", "depth": depth, "ref_spp": 2 * half,
+          "envmap_sampling": "footprint", "ref_rel_mse": ref_err}
+    for leg, guided, seed in (("plain", False, 87), ("guided", True, 88)):
+        tt[leg] = time_to_rel_mse(gpu, ng, cam, depth, ref, ref_err, 1024, 1 << 14, args.mse_threshold, guided,
+                                  seed, train)
+        r = tt[leg]
+        log(f"[bench] sun room {leg}: relMSE {r['rel_mse']:.3g} at {r['spp']} spp after {r['ms']:.0f} ms GPU time")
+    tt["speedup_guided_over_plain"] = round(tt["plain"]["ms_to_threshold"] / tt["guided"]["ms_to_threshold"], 3)
+    gpu.set_option("env_sampling", 0)  # for later uploads
+    return tt
 
 
 CONFIG_LEGS = [
@@ -483,6 +535,8 @@ def main():
         if not args.no_time_to_mse and args.only is None:
             legs["time_to_rel_mse"] = time_to_rel_mse_legs(gpu, ng, sc, args)
         del sc
+        if not args.no_time_to_mse and args.only is None:
+            legs["time_to_rel_mse"]["sun_room"] = sun_room_legs(gpu, ng, args)
 
     if args.only in (None, "configs") and not args.no_config_legs:
         legs["configs"] = config_legs(gpu, ng, max(1, args.config_frames), args.batch)
