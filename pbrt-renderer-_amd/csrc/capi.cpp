@@ -89,8 +89,8 @@ struct nori_ctx {
     uint32_t trace_waves = 0; // option "trace_waves": waves-per-SIMD cap of the traversal kernels (0: none)
     uint32_t lds_per_cu = 0;  // the device's LDS bytes per CU
     void *li_scratch = nullptr; // device address of nori_gpu_li_noalloc's static scratch on this device
-    uint32_t *work_ext() const { return persistent ? cdev.cnt + 10 : nullptr; }
-    uint32_t *work_shadow() const { return persistent ? cdev.cnt + 11 : nullptr; }
+    uint32_t *work_ext() const { return persistent ? cnt_slot(cdev.cnt, 10) : nullptr; }
+    uint32_t *work_shadow() const { return persistent ? cnt_slot(cdev.cnt, 11) : nullptr; }
     uint32_t sampler = 0;                  // option "sampler": 0 pcg32 per pixel, 1 Owen-scrambled Sobol'
     int bvh_collapse = BVH8_COLLAPSE_SAH;  // option "bvh_collapse" (next upload): 0 greedy, 1 SAH-optimal
     bool scene_has_hair = false;
@@ -349,11 +349,11 @@ void run_bounces(nori_ctx *c, FilmFx *film, const ViewTab *views, uint32_t n, ui
     const GuideShadeDev none{nullptr, nullptr, nullptr, nullptr, 0, 0};
     for (uint32_t b = 0; b < max_depth; ++b) {
         uint32_t qi = b & 1u, qo = qi ^ 1u;
-        launch_extend(c->sdev, c->pd[qi], c->cdev.cnt + qi, n, nullptr, nullptr, nullptr, 0, c->work_ext(),
+        launch_extend(c->sdev, c->pd[qi], cnt_slot(c->cdev.cnt, qi), n, nullptr, nullptr, nullptr, 0, c->work_ext(),
                       c->stream);
-        launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, c->cdev.cnt + qi, c->cdev.cnt + qo, nullptr, nullptr,
-                     c->shdev, c->cdev.cnt + 2, film, n, none, AovDev{nullptr, nullptr}, c->stream);
-        launch_shadow(c->sdev, c->shdev, c->cdev.cnt + 2, film, 2 * n, nullptr, none, c->work_shadow(), c->stream);
+        launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, cnt_slot(c->cdev.cnt, qi), cnt_slot(c->cdev.cnt, qo), nullptr,
+                     nullptr, c->shdev, cnt_slot(c->cdev.cnt, 2), film, n, none, AovDev{nullptr, nullptr}, c->stream);
+        launch_shadow(c->sdev, c->shdev, cnt_slot(c->cdev.cnt, 2), film, 2 * n, nullptr, none, c->work_shadow(), c->stream);
         st.launches_closest++;
         st.launches_shadow++;
         launch_counters(c->cdev, qi, 1, (1u << qi) | 4u, c->stream);
@@ -371,38 +371,39 @@ void pool_iteration(nori_ctx *c, uint32_t qi, FilmFx *film, const ViewTab *views
     uint32_t *cnt = c->cdev.cnt;
     {
         Span t(c, 2);
-        launch_regen(c->sdev, c->pd[qi], views, c->cdev, c->qf[qi], cnt + 3 + qi, cnt + qi, n, gsd, c->stream);
+        launch_regen(c->sdev, c->pd[qi], views, c->cdev, c->qf[qi], cnt_slot(cnt, 3 + qi), cnt_slot(cnt, qi), n, gsd,
+                     c->stream);
     }
     // Scenes without media shade in class-sorted launches (a path's class is
     // known once its hit is; with media, free flight decides first).
     const bool sorted = c->sort_shading && c->sdev.n_vol == 0;
     {
         Span t(c, 0);
-        launch_extend(c->sdev, c->pd[qi], cnt + qi, n, cs, sorted ? c->qc : nullptr, cnt + 6, c->cap, c->work_ext(),
+        launch_extend(c->sdev, c->pd[qi], cnt_slot(cnt, qi), n, cs, sorted ? c->qc : nullptr, cnt_slot(cnt, 6), c->cap, c->work_ext(),
                       c->stream);
     }
     if (gsd.out) {
         Span t(c, 3);
         // sorted: only the surface queue can hold guided vertices
-        launch_guide_infer(c->sdev, c->pd[qi], sorted ? c->qc + c->cap : nullptr, sorted ? cnt + 7 : cnt + qi, n,
+        launch_guide_infer(c->sdev, c->pd[qi], sorted ? c->qc + c->cap : nullptr, sorted ? cnt_slot(cnt, 7) : cnt_slot(cnt, qi), n,
                            c->gdev.w, c->gdev.params, (GuidePacked *)c->g_pack.p, c->stream);
     }
     {
         Span t(c, 2);
         if (sorted)
-            launch_shade_sorted(c->sdev, c->pd[qi], c->pd[qo], views, c->qc, c->cap, cnt + 6, c->scene_has_hair,
-                                cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev, cnt + 2, film, n, gsd, c->aov_dev(), c->stream);
+            launch_shade_sorted(c->sdev, c->pd[qi], c->pd[qo], views, c->qc, c->cap, cnt_slot(cnt, 6), c->scene_has_hair,
+                                cnt_slot(cnt, qo), c->qf[qo], cnt_slot(cnt, 3 + qo), c->shdev, cnt_slot(cnt, 2), film, n, gsd, c->aov_dev(), c->stream);
         else
-            launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, cnt + qi, cnt + qo, c->qf[qo], cnt + 3 + qo, c->shdev,
-                         cnt + 2, film, n, gsd, c->aov_dev(), c->stream);
+            launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, cnt_slot(cnt, qi), cnt_slot(cnt, qo), c->qf[qo],
+                         cnt_slot(cnt, 3 + qo), c->shdev, cnt_slot(cnt, 2), film, n, gsd, c->aov_dev(), c->stream);
     }
     {
         Span t(c, 1);
-        launch_shadow(c->sdev, c->shdev, cnt + 2, film, 2 * n, cs, gsd, c->work_shadow(), c->stream);
+        launch_shadow(c->sdev, c->shdev, cnt_slot(cnt, 2), film, 2 * n, cs, gsd, c->work_shadow(), c->stream);
     }
     if (gsd.rec) {
         Span t(c, 4);
-        launch_guide_finalize(c->sdev, gsd, c->qf[qo], cnt + 3 + qo, n, (GuideSample *)c->g_samples.p,
+        launch_guide_finalize(c->sdev, gsd, c->qf[qo], cnt_slot(cnt, 3 + qo), n, (GuideSample *)c->g_samples.p,
                               (uint32_t *)c->g_scount.p, 4 * c->g_rec_cap, c->stream);
     }
     launch_iter_end(c->cdev, qi, c->stream);
@@ -486,10 +487,10 @@ int nori_gpu_li_noalloc(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds
         const GuideShadeDev none{nullptr, nullptr, nullptr, nullptr, 0, 0};
         for (uint32_t b = 0; b < max_depth; ++b) {
             const uint32_t qi = b & 1u, qo = qi ^ 1u;
-            launch_extend(s, l.pd[qi], l.cnt.cnt + qi, m, nullptr, nullptr, nullptr, 0, nullptr, c->stream);
-            launch_shade(s, l.pd[qi], l.pd[qo], l.view, l.cnt.cnt + qi, l.cnt.cnt + qo, nullptr, nullptr, l.shd,
-                         l.cnt.cnt + 2, film, m, none, AovDev{nullptr, nullptr}, c->stream);
-            launch_shadow(s, l.shd, l.cnt.cnt + 2, film, 2 * m, nullptr, none, nullptr, c->stream);
+            launch_extend(s, l.pd[qi], cnt_slot(l.cnt.cnt, qi), m, nullptr, nullptr, nullptr, 0, nullptr, c->stream);
+            launch_shade(s, l.pd[qi], l.pd[qo], l.view, cnt_slot(l.cnt.cnt, qi), cnt_slot(l.cnt.cnt, qo), nullptr, nullptr,
+                         l.shd, cnt_slot(l.cnt.cnt, 2), film, m, none, AovDev{nullptr, nullptr}, c->stream);
+            launch_shadow(s, l.shd, cnt_slot(l.cnt.cnt, 2), film, 2 * m, nullptr, none, nullptr, c->stream);
             launch_counters(l.cnt, qi, 1, (1u << qi) | 4u, c->stream);
         }
         launch_film_resolve(film, (float4 *)rgba4, (size_t)m, c->stream);
@@ -514,10 +515,10 @@ int nori_gpu_create(nori_ctx **out, int device) {
         // the allocation-free Li's static scratch on this device (guard made it current)
         c->li_scratch = li_noalloc_scratch(c->device);
         if (!c->li_scratch) throw HipError("hipGetSymbolAddress of the allocation-free Li scratch failed");
-        c->alloc(c->counters, 16 * sizeof(uint32_t));
+        c->alloc(c->counters, CNT_SLOTS * CNT_STRIDE * sizeof(uint32_t));
         // 0..7 render counters, 8..11 sample claims (Counters::next), 12..13 alpha-test candidates
         c->alloc(c->stats, 16 * sizeof(unsigned long long));
-        HIPCHK(hipMemset(c->counters.p, 0, 16 * sizeof(uint32_t)));
+        HIPCHK(hipMemset(c->counters.p, 0, CNT_SLOTS * CNT_STRIDE * sizeof(uint32_t)));
         HIPCHK(hipMemset(c->stats.p, 0, 16 * sizeof(unsigned long long)));
         c->cdev = Counters{(uint32_t *)c->counters.p, (unsigned long long *)c->stats.p,
                            (unsigned long long *)c->stats.p + 8};
@@ -929,7 +930,7 @@ int nori_gpu_render(nori_ctx *c, const nori_view *views, int nviews, uint64_t se
                 iters += c->chunk_iters;
                 // Progress of the previous chunk (read while this one runs):
                 // done flag and samples claimed so far.
-                HIPCHK(hipMemcpyAsync(c->h_flags + (k & 1u), c->cdev.cnt + 5, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                HIPCHK(hipMemcpyAsync(c->h_flags + (k & 1u), cnt_slot(c->cdev.cnt, 5), sizeof(uint32_t), hipMemcpyDeviceToHost,
                                       c->stream));
                 HIPCHK(hipMemcpyAsync(c->h_claimed + (k & 1u), c->cdev.next, sizeof(unsigned long long),
                                       hipMemcpyDeviceToHost, c->stream));

@@ -133,7 +133,7 @@ struct ViewTab {
     unsigned long long first;
 };
 
-// Counters on the device.
+// Counters on the device (cnt[i] below is counter i, cnt_slot(cnt, i)).
 //   cnt[0], cnt[1]: the two active path queues (ping-pong by iteration parity)
 //   cnt[2]:         shadow queue
 //   cnt[3], cnt[4]: the two free-slot queues
@@ -147,11 +147,17 @@ struct ViewTab {
 //                   triangles
 //   next[0..3]:     next unclaimed global sample, total samples, render seed,
 //                   number of views
+// Counter i is cnt[i * CNT_STRIDE]: each sits in its own 256-B block.  The
+// shading and extension kernels' waves append to several of them with
+// returning atomics, and atomics to one cache line are serviced one after
+// another.
+constexpr uint32_t CNT_STRIDE = 64, CNT_SLOTS = 16;
 struct Counters {
     uint32_t *cnt;
     unsigned long long *stats;
     unsigned long long *next;
 };
+__host__ __device__ inline uint32_t *cnt_slot(uint32_t *cnt, uint32_t i) { return cnt + (size_t)i * CNT_STRIDE; }
 
 void launch_trace_batch(const SceneDev &s, const float4 *rays, float4 *hits, uint32_t n, int any_hit, hipStream_t st);
 // Path pool start: free list 0 = every path id, other queues empty, sample
@@ -177,7 +183,7 @@ void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *s
 // count_stats (optional): stats array whose [4],[5] (extend) / [6],[7] (shadow)
 // receive the summed BVH8 nodes visited and triangles tested.
 // q_cls (optional): sort the positions into the shading-class queues q_cls +
-// {0, 1, 2, 3} * cls_stride with counts cls_cnt[0..3] (scenes without media).
+// {0, 1, 2, 3} * cls_stride with counts cls_cnt[{0, 1, 2, 3} * CNT_STRIDE] (scenes without media).
 // work (optional): a zeroed counter; with it the traversal runs as a grid of
 // resident waves that take positions from the counter (persistent form).
 void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, uint32_t cap,
@@ -227,7 +233,8 @@ void launch_film_resolve(const FilmFx *film, float4 *out, size_t n_entries, hipS
 // and the view record.
 constexpr uint32_t LI_NOALLOC_SLOTS = 1u << 16;
 constexpr size_t LI_NOALLOC_USED =
-    (size_t)LI_NOALLOC_SLOTS * (2 * (5 * 16 + 2 * 4) + 8 * 16 + sizeof(FilmFx)) + 256 + sizeof(ViewTab);
+    (size_t)LI_NOALLOC_SLOTS * (2 * (5 * 16 + 2 * 4) + 8 * 16 + sizeof(FilmFx)) + CNT_SLOTS * CNT_STRIDE * 4 + 256 +
+    sizeof(ViewTab);
 constexpr size_t LI_NOALLOC_BYTES = 24u << 20; // 24 MiB per GPU
 struct LiScratch {
     PathsDev pd[2];

@@ -148,8 +148,8 @@ __global__ void k_pool_start(Counters c, uint32_t *q_free0, uint32_t cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cap) q_free0[i] = i;
     if (i == 0) {
-        for (int k = 0; k < 12; ++k) c.cnt[k] = 0;
-        c.cnt[3] = cap;
+        for (uint32_t k = 0; k < 12; ++k) *cnt_slot(c.cnt, k) = 0;
+        *cnt_slot(c.cnt, 3) = cap;
         c.next[0] = 0;
     }
 }
@@ -238,15 +238,14 @@ __global__ void k_film_count(const ViewTab *__restrict__ views, uint32_t nviews,
 // iteration, done = every sample claimed and no path left for the next one.
 __global__ void k_iter_end(Counters c, uint32_t qi) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    c.stats[0] += c.cnt[qi];
-    c.stats[1] += c.cnt[2];
+    c.stats[0] += *cnt_slot(c.cnt, qi);
+    c.stats[1] += *cnt_slot(c.cnt, 2);
     c.stats[3] += 1;
-    c.cnt[qi] = 0;
-    c.cnt[2] = 0;
-    c.cnt[3 + qi] = 0;
-    c.cnt[6] = c.cnt[7] = c.cnt[8] = c.cnt[9] = 0;
-    c.cnt[10] = c.cnt[11] = 0;
-    c.cnt[5] = (c.next[0] >= c.next[1] && c.cnt[qi ^ 1u] == 0) ? 1u : 0u;
+    *cnt_slot(c.cnt, qi) = 0;
+    *cnt_slot(c.cnt, 2) = 0;
+    *cnt_slot(c.cnt, 3 + qi) = 0;
+    for (uint32_t k = 6; k < 12; ++k) *cnt_slot(c.cnt, k) = 0;
+    *cnt_slot(c.cnt, 5) = (c.next[0] >= c.next[1] && *cnt_slot(c.cnt, qi ^ 1u) == 0) ? 1u : 0u;
 }
 
 // Paths from caller rays (Integrator::Li): ray i -> position i, film entry first + i,
@@ -288,7 +287,8 @@ __device__ __forceinline__ void count_add1(unsigned long long *stats, int k, uin
 
 // Closest hit of the paths at positions [0, *count).  SORT: the position is
 // also appended to the queue of its shading class (light, surface, hair at
-// q_cls, q_cls + cap, q_cls + 2 cap; counts cls_cnt[0..2]).
+// q_cls, q_cls + cap, q_cls + 2 cap, subsurface at q_cls + 3 cap; counts in the
+// counter slots cnt_slot(cls_cnt, 0..3)).
 // Shading class of a hit surface: emitters with the misses (light), hair,
 // the random-walk subsurface material, every other surface.
 __device__ __forceinline__ int shading_class(const nori_material &mr) {
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend(SceneDev s, PathsDev p, 
     }
     if (SORT) {
         for (int c = CLS_LIGHT; c <= CLS_SSS; ++c) {
-            const uint32_t k = wave_append(cls_cnt + (c - 1), cls == c);
+            const uint32_t k = wave_append(cnt_slot(cls_cnt, c - 1), cls == c);
             if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
         }
     }
@@ -484,7 +484,8 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend_p(SceneDev s, PathsDev p
 
 // Shading class of the hits at positions [0, *count): each position appended
 // to the queue of its class (light, surface, hair at q_cls, q_cls + cap,
-// q_cls + 2 cap; counts cls_cnt[0..2]).
+// q_cls + 2 cap, subsurface at q_cls + 3 cap; counts in the counter slots
+// cnt_slot(cls_cnt, 0..3)).
 __global__ __launch_bounds__(256) void k_classify(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
                                                   uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, n = *count;
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(256) void k_classify(SceneDev s, PathsDev p, const 
         }
     }
     for (int c = CLS_LIGHT; c <= CLS_SSS; ++c) {
-        const uint32_t k = wave_append(cls_cnt + (c - 1), cls == c);
+        const uint32_t k = wave_append(cnt_slot(cls_cnt, c - 1), cls == c);
         if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
     }
 }
@@ -1562,13 +1563,13 @@ __global__ void k_counters(Counters c, uint32_t ext_from, uint32_t add_shadow, u
         c.stats[12] = c.stats[13] = 0; // alpha-test candidates (extension, shadow)
     }
     if (ext_from < 2) {
-        c.stats[0] += c.cnt[ext_from];
+        c.stats[0] += *cnt_slot(c.cnt, ext_from);
         c.stats[3] += 1;
     }
-    if (add_shadow) c.stats[1] += c.cnt[2];
-    for (int k = 0; k < 9; ++k)
-        if (clear_mask & (1u << k)) c.cnt[k] = 0;
-    c.cnt[10] = c.cnt[11] = 0; // persistent traversal work counters
+    if (add_shadow) c.stats[1] += *cnt_slot(c.cnt, 2);
+    for (uint32_t k = 0; k < 9; ++k)
+        if (clear_mask & (1u << k)) *cnt_slot(c.cnt, k) = 0;
+    *cnt_slot(c.cnt, 10) = *cnt_slot(c.cnt, 11) = 0; // persistent traversal work counters
 }
 
 __global__ void k_set_counter(uint32_t *cnt, uint32_t v) {
@@ -1851,14 +1852,14 @@ void launch_shade_sorted(const SceneDev &s, const PathsDev &p, const PathsDev &p
     hipLaunchKernelGGL(shade_kernel<CLS_LIGHT>(guide, sobol), grid, block, 0, st, s, p, p_out, views, q_cls,
                        cls_cnt, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
     hipLaunchKernelGGL(shade_kernel<CLS_SURF>(guide, sobol), grid, block, 0, st, s, p, p_out, views,
-                       q_cls + cs, cls_cnt + 1, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
+                       q_cls + cs, cls_cnt + 1 * CNT_STRIDE, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
     if (hair)
         hipLaunchKernelGGL(shade_kernel<CLS_HAIR>(guide, sobol), grid, block, 0, st, s, p, p_out, views,
-                           q_cls + 2 * cs, cls_cnt + 2, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
+                           q_cls + 2 * cs, cls_cnt + 2 * CNT_STRIDE, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
     if (s.has_sss)
         hipLaunchKernelGGL(shade_kernel<CLS_SSS>(guide, sobol), grid, block,
                            (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2), st, s, p, p_out, views, q_cls + 3 * cs,
-                           cls_cnt + 3, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
+                           cls_cnt + 3 * CNT_STRIDE, cnt_out, q_free, cnt_free, sh, cnt_sh, film, g, aov);
 }
 
 void launch_shadow(const SceneDev &s, const ShadowDev &sh, const uint32_t *cnt_sh, FilmFx *film, uint32_t cap,
@@ -1985,8 +1986,9 @@ LiScratch li_noalloc_layout(void *base) {
     l.shd = ShadowDev{sp, sp + 2 * n, sp + 4 * n, sp + 6 * n};
     l.film = (FilmFx *)(sp + 8 * n);
     char *tail = (char *)(l.film + n);
-    l.cnt = Counters{(uint32_t *)tail, (unsigned long long *)(tail + 64), (unsigned long long *)(tail + 64) + 8};
-    l.view = (ViewTab *)(tail + 256);
+    char *stats = tail + CNT_SLOTS * CNT_STRIDE * sizeof(uint32_t);
+    l.cnt = Counters{(uint32_t *)tail, (unsigned long long *)stats, (unsigned long long *)stats + 8};
+    l.view = (ViewTab *)(stats + 256);
     return l;
 }
 
@@ -1995,11 +1997,11 @@ LiScratch li_noalloc_layout(void *base) {
 __global__ void k_li_setup(Counters c, ViewTab *v, uint32_t n, uint32_t max_depth, uint32_t rr_depth) {
     if (blockIdx.x != 0) return;
     const uint32_t t = threadIdx.x;
-    if (t < 16) c.cnt[t] = 0;
+    if (t < CNT_SLOTS) *cnt_slot(c.cnt, t) = 0;
     if (t < 16) c.stats[t] = 0; // stats[8..15] are Counters::next
     __syncthreads();
     if (t == 0) {
-        c.cnt[0] = n;
+        *cnt_slot(c.cnt, 0) = n;
         ViewTab z{};
         z.max_depth = max_depth;
         z.rr_depth = rr_depth;
