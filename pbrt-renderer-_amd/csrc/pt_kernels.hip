@@ -26,7 +26,12 @@ constexpr uint32_t MISS = 0xFFFFFFFFu;
 // latter also the subsurface material, whose walk costs registers).
 constexpr int CLS_ALL = 0, CLS_LIGHT = 1, CLS_SURF = 2, CLS_HAIR = 3, CLS_SSS = 4, CLS_ALL_SSS = 5;
 constexpr int TRACE_BLOCK = 64;
-constexpr int SHADE_BLOCK = 128;
+// Shading blocks: 256 threads, so one block's four waves share each queue
+// append; 128 where the scene has media (the free-flight loops differ more
+// in length, and a block's waves wait for each other at the append).
+constexpr int SHADE_BLOCK = 256, SHADE_BLOCK_VOL = 128;
+constexpr int shade_block(bool vol) { return vol ? SHADE_BLOCK_VOL : SHADE_BLOCK; }
+constexpr int CLASSIFY_BLOCK = 256;
 
 __device__ __forceinline__ uint64_t rng_load(float4 aux) {
     return ((uint64_t)__float_as_uint(aux.w) << 32) | (uint64_t)__float_as_uint(aux.z);
@@ -100,6 +105,91 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t *counter, uint32_t v)
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(counter, total);
     return __shfl(base, leader) + off;
+}
+
+// Block-aggregated appends to three queues: v0 (0..2) entries per lane to c0,
+// one entry to c1 where p1 holds, one to c2 where p2 holds (c2 may be null).
+// One atomic per queue per block instead of per wave: returning atomics to one
+// counter are serviced one after another, so their number per counter bounds
+// the kernel.  Called by every thread of the grid below n_valid and by no
+// other: the valid threads are a prefix of each block, so thread 0 is valid
+// in any block that calls, and the waves past the last valid thread have
+// ended (s_barrier waits only for waves that have not).  Returns each lane's
+// first slot in each queue.
+template <int BLOCK>
+__device__ __forceinline__ void block_append3(uint32_t n_valid, uint32_t *c0, uint32_t v0, uint32_t *c1, bool p1,
+                                              uint32_t *c2, bool p2, uint32_t &s0, uint32_t &s1, uint32_t &s2) {
+    constexpr uint32_t NW = BLOCK / 64;
+    __shared__ uint32_t wc[3][NW];
+    __shared__ uint32_t base[3];
+    const uint32_t lane = __lane_id(), w = threadIdx.x / 64u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint64_t a1 = __ballot(v0 >= 1u), a2 = __ballot(v0 >= 2u), b1 = __ballot(p1), b2 = __ballot(p2);
+    const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(true));
+    if (lane == leader) {
+        wc[0][w] = (uint32_t)(__popcll(a1) + __popcll(a2));
+        wc[1][w] = (uint32_t)__popcll(b1);
+        wc[2][w] = (uint32_t)__popcll(b2);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t nw = min(NW, (n_valid - blockIdx.x * (uint32_t)BLOCK + 63u) / 64u);
+        uint32_t t0 = 0, t1 = 0, t2 = 0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            t0 += wc[0][k];
+            t1 += wc[1][k];
+            t2 += wc[2][k];
+        }
+        base[0] = t0 ? atomicAdd(c0, t0) : 0u;
+        base[1] = t1 ? atomicAdd(c1, t1) : 0u;
+        base[2] = c2 && t2 ? atomicAdd(c2, t2) : 0u;
+    }
+    __syncthreads();
+    uint32_t o0 = base[0], o1 = base[1], o2 = base[2];
+    for (uint32_t k = 0; k < w; ++k) {
+        o0 += wc[0][k];
+        o1 += wc[1][k];
+        o2 += wc[2][k];
+    }
+    s0 = o0 + (uint32_t)(__popcll(a1 & lt) + __popcll(a2 & lt));
+    s1 = o1 + (uint32_t)__popcll(b1 & lt);
+    s2 = o2 + (uint32_t)__popcll(b2 & lt);
+}
+
+// Block-aggregated append of each lane to the queue of its class cls (1..NC;
+// 0: none), counters cnt_slot(cnt, cls - 1): one atomic per class per block.
+// Called under the conditions of block_append3; returns the lane's slot.
+template <int BLOCK, int NC>
+__device__ __forceinline__ uint32_t block_append_class(uint32_t n_valid, uint32_t *cnt, int cls) {
+    constexpr uint32_t NW = BLOCK / 64;
+    __shared__ uint32_t wc[NC][NW];
+    __shared__ uint32_t base[NC];
+    const uint32_t lane = __lane_id(), w = threadIdx.x / 64u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const uint32_t leader = (uint32_t)__builtin_ctzll(__ballot(true));
+    uint64_t mine = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint64_t m = __ballot(cls == c + 1);
+        if (lane == leader) wc[c][w] = (uint32_t)__popcll(m);
+        if (cls == c + 1) mine = m;
+    }
+    __syncthreads();
+    // thread 0 is the one thread certain to be valid in a calling block
+    if (threadIdx.x == 0) {
+        const uint32_t nw = min(NW, (n_valid - blockIdx.x * (uint32_t)BLOCK + 63u) / 64u);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            uint32_t t = 0;
+            for (uint32_t k = 0; k < nw; ++k) t += wc[c][k];
+            base[c] = t ? atomicAdd(cnt_slot(cnt, (uint32_t)c), t) : 0u;
+        }
+    }
+    __syncthreads();
+    if (cls < 1 || cls > NC) return 0;
+    uint32_t o = base[cls - 1];
+    for (uint32_t k = 0; k < w; ++k) o += wc[cls - 1][k];
+    return o + (uint32_t)__popcll(mine & lt);
 }
 
 // Nori perspective camera, src/cpu/perspective.cpp:86-123, with the
@@ -486,23 +576,19 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_extend_p(SceneDev s, PathsDev p
 // to the queue of its class (light, surface, hair at q_cls, q_cls + cap,
 // q_cls + 2 cap, subsurface at q_cls + 3 cap; counts in the counter slots
 // cnt_slot(cls_cnt, 0..3)).
-__global__ __launch_bounds__(256) void k_classify(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
+__global__ __launch_bounds__(CLASSIFY_BLOCK) void k_classify(SceneDev s, PathsDev p, const uint32_t *__restrict__ count,
                                                   uint32_t *q_cls, uint32_t *cls_cnt, uint32_t cap) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, n = *count;
-    if (i - threadIdx.x % 64u >= n) return;
-    int cls = 0;
-    if (i < n) {
-        const uint32_t prim = __float_as_uint(p.hit[i].y);
-        cls = CLS_LIGHT;
-        if (prim != MISS) {
-            const nori_material &mr = s.mats[s.tri_mesh[prim]];
-            cls = shading_class(mr);
-        }
+    if (i >= n) return;
+    const uint32_t prim = __float_as_uint(p.hit[i].y);
+    int cls = CLS_LIGHT;
+    if (prim != MISS) {
+        const nori_material &mr = s.mats[s.tri_mesh[prim]];
+        cls = shading_class(mr);
     }
-    for (int c = CLS_LIGHT; c <= CLS_SSS; ++c) {
-        const uint32_t k = wave_append(cnt_slot(cls_cnt, c - 1), cls == c);
-        if (cls == c) q_cls[(size_t)(c - 1) * cap + k] = i;
-    }
+    static_assert(CLS_LIGHT == 1 && CLS_SSS == 4, "class numbering");
+    const uint32_t k = block_append_class<CLASSIFY_BLOCK, 4>(n, cls_cnt, cls);
+    q_cls[(size_t)(cls - 1) * cap + k] = i;
 }
 
 // NEE contribution of a vertex: into the path radiance and, when the vertex
@@ -938,17 +1024,17 @@ __device__ bool sss_walk(const SceneDev &s, f3 pos, f3 dir, uint32_t mesh, float
 // (CLS_LIGHT), scattering surfaces but hair (CLS_SURF), hair (CLS_HAIR), the
 // random-walk subsurface material (CLS_SSS).
 template <bool VOL, bool GUIDE, int CLS, bool SOBOL>
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, PathsDev p_out,
+__global__ __launch_bounds__(shade_block(VOL)) void k_shade(SceneDev s, PathsDev p, PathsDev p_out,
                                                        const ViewTab *__restrict__ views,
                                                        const uint32_t *__restrict__ q_cls,
                                                        const uint32_t *__restrict__ cnt_in, uint32_t *cnt_out,
                                                        uint32_t *q_free, uint32_t *cnt_free, ShadowDev sh,
                                                        uint32_t *cnt_sh, FilmFx *film, GuideShadeDev g, AovDev aov) {
     // traversal stacks of the subsurface walks (allocated when the scene has
-    // subsurface materials: SHADE_BLOCK columns of the tree depth)
+    // subsurface materials: one column of the tree depth per thread)
     extern __shared__ uint2 shade_stk[];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *cnt_in) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x, n_in = *cnt_in;
+    if (i >= n_in) return;
     constexpr bool ALL = CLS == CLS_ALL || CLS == CLS_ALL_SSS;
     // only these launches can meet hair, so only they compile the hair BCSDF
     constexpr bool HAIR_CODE = ALL || CLS == CLS_HAIR;
@@ -1381,12 +1467,6 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
             }
         }
     }
-    {
-        // both shadow rays of the vertex with one atomic per wave
-        uint32_t ks = wave_append_n<2>(cnt_sh, (sh_area ? 1u : 0u) + (sh_env ? 1u : 0u));
-        if (sh_area) write_shadow(sh, ks++, ra, pixel, tag);
-        if (sh_env) write_shadow(sh, ks, re, pixel, tag);
-    }
     if (GUIDE && g.rec) g.lpath[id] = make_float4(lpath.x, lpath.y, lpath.z, 0.0f);
 
     // Russian roulette, shaders/Shaders.hlsl:620-626.  The last bounce does not
@@ -1398,9 +1478,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         else T = T * (1.0f / q);
     }
     alive = alive && more;
+    // Slots: both shadow rays of the vertex in the shadow queue, survivors in
+    // the next buffer, ended paths' ids in the free list.
+    uint32_t ks, k, kf;
+    block_append3<shade_block(VOL)>(n_in, cnt_sh, (sh_area ? 1u : 0u) + (sh_env ? 1u : 0u), cnt_out, alive,
+                               q_free ? cnt_free : nullptr, !alive, ks, k, kf);
+    if (sh_area) write_shadow(sh, ks++, ra, pixel, tag);
+    if (sh_env) write_shadow(sh, ks, re, pixel, tag);
     // Survivors move, with their whole state, to the next buffer in queue
     // order (coalesced); ended paths return their id to the free list.
-    uint32_t k = wave_append(cnt_out, alive);
     if (alive) {
         p_out.ray_o[k] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
         p_out.ray_d[k] = make_float4(new_d.x, new_d.y, new_d.z, 1e20f);
@@ -1410,10 +1496,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev s, PathsDev p, P
         p_out.meta[k] = (meta + 1u) | (aov_done ? 0x8000u : 0u);
         p_out.id[k] = id;
     }
-    if (q_free) {
-        const uint32_t kf = wave_append(cnt_free, !alive);
-        if (!alive) q_free[kf] = id;
-    }
+    if (q_free && !alive) q_free[kf] = id;
 }
 
 // Guide mixtures of the guided vertices among the queued paths.  A block
@@ -1795,11 +1878,14 @@ void launch_extend(const SceneDev &s, const PathsDev &p, const uint32_t *count, 
         const auto k = kp[(count_stats ? 2 : 0) | (s.has_alpha ? 1 : 0)];
         hipLaunchKernelGGL(k, dim3(resident_blocks((const void *)k, lds, cap)), dim3(TRACE_BLOCK), lds, st, s, p,
                            count, work, count_stats);
+        // the persistent form sorts in a pass of its own
         if (q_cls)
-            hipLaunchKernelGGL(k_classify, dim3(blocks(cap, 256)), dim3(256), 0, st, s, p, count, q_cls, cls_cnt,
-                               cls_stride);
+            hipLaunchKernelGGL(k_classify, dim3(blocks(cap, CLASSIFY_BLOCK)), dim3(CLASSIFY_BLOCK), 0, st, s, p,
+                               count, q_cls, cls_cnt, cls_stride);
         return;
     }
+    // Class appends stay inside the traversal kernel (one atomic per class per
+    // wave): a separate sorting pass over the hits measured slower.
 #define KX(c, q, a) k_extend<c, q, a, false>, k_extend<c, q, a, true>
     decltype(&k_extend<false, false, false, false>) ks[16] = {
         KX(false, false, false), KX(false, false, true), KX(false, true, false), KX(false, true, true),
@@ -1836,9 +1922,10 @@ void launch_shade(const SceneDev &s, const PathsDev &p, const PathsDev &p_out, c
                   uint32_t *cnt_sh, FilmFx *film, uint32_t cap, const GuideShadeDev &g, const AovDev &aov,
                   hipStream_t st) {
     const bool guide = g.out != nullptr || g.rec != nullptr;
-    const size_t lds = s.has_sss ? (size_t)s.stack_depth * SHADE_BLOCK * sizeof(uint2) : 0;
-    hipLaunchKernelGGL(shade_kernel_all(s.n_vol > 0, guide, s.sobol != 0, s.has_sss != 0), dim3(blocks(cap, SHADE_BLOCK)),
-                       dim3(SHADE_BLOCK), lds, st, s, p, p_out, views, nullptr, cnt_in, cnt_out, q_free, cnt_free, sh,
+    const uint32_t sb = (uint32_t)shade_block(s.n_vol > 0);
+    const size_t lds = s.has_sss ? (size_t)s.stack_depth * sb * sizeof(uint2) : 0;
+    hipLaunchKernelGGL(shade_kernel_all(s.n_vol > 0, guide, s.sobol != 0, s.has_sss != 0), dim3(blocks(cap, sb)),
+                       dim3(sb), lds, st, s, p, p_out, views, nullptr, cnt_in, cnt_out, q_free, cnt_free, sh,
                        cnt_sh, film, g, aov);
 }
 
