@@ -1501,11 +1501,13 @@ __global__ __launch_bounds__(shade_block(VOL)) void k_shade(SceneDev s, PathsDev
 
 // Guide mixtures of the guided vertices among the queued paths.  A block
 // stages the network (BF16 weights, FP32 biases: 33 KB) in LDS once, then each
-// of its four waves takes tiles of 32 rows in turn: both lanes of a row
-// compute their half of its features from the path state
-// (guide_feature_block), the network runs in registers (mlp_forward_wave) and
-// the row's lane 0..31 packs the mixture.  Tiles without a guided vertex are
-// skipped.
+// of its four waves takes 64 rows at a time: every lane gathers its own row's
+// inputs from the path state (hit, ray, triangle frame, material); the network
+// runs twice, on rows 0-31 and 32-63, each row's two lanes taking its inputs
+// by lane exchange and computing half of its features (guide_feature_block),
+// the layers in registers (mlp_forward_wave); each lane keeps the outputs of
+// its own row, and all 64 lanes then pack their rows' mixtures at once.
+// Halves without a guided vertex skip the network.
 constexpr uint32_t GUIDE_INFER_BLOCKS = 1024; // four per CU: the LDS copies allow four
 __global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, const uint32_t *__restrict__ q,
                                                      const uint32_t *__restrict__ count, uint32_t cap,
@@ -1514,7 +1516,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, con
     __shared__ __align__(16) bf16_t sW[G_LAYERS * 4096];
     __shared__ float sB[G_LAYERS * 64];
     const uint32_t rows = min(*count, cap);
-    if (blockIdx.x * 128u >= rows) return; // the same for the whole block: no tile of its own
+    if (blockIdx.x * 256u >= rows) return; // the same for the whole block: no rows of its own
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(W);
         uint4 *dst = reinterpret_cast<uint4 *>(sW);
@@ -1525,8 +1527,8 @@ __global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, con
     __syncthreads();
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t h = lane >> 5;
-    for (uint32_t row0 = (blockIdx.x * 4u + wave) * 32u; row0 < rows; row0 += gridDim.x * 128u) {
-        const uint32_t r = row0 + (lane & 31u);
+    for (uint32_t row0 = (blockIdx.x * 4u + wave) * 64u; row0 < rows; row0 += gridDim.x * 256u) {
+        const uint32_t r = row0 + lane;
         uint32_t pos = 0;
         bool ok = false;
         f3 p01 = mk(0.0f), N = mk(0.0f), wi = mk(0.0f);
@@ -1550,20 +1552,34 @@ __global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, con
                 }
             }
         }
-        if (__ballot(ok) == 0) continue; // the same for every lane of the wave
-        bf16x8 xb[4];
+        const uint64_t okm = __ballot(ok);
+        float mine[G_USED_OUT]; // the outputs of this lane's own row
+#pragma unroll 1
+        for (uint32_t t = 0; t < 2; ++t) {
+            if (((okm >> (32u * t)) & 0xFFFFFFFFull) == 0) continue; // the same for every lane
+            const int src = (int)(32u * t + (lane & 31u)); // this lane's row of the half
+            const f3 tp = mk(__shfl(p01.x, src), __shfl(p01.y, src), __shfl(p01.z, src));
+            const f3 tn = mk(__shfl(N.x, src), __shfl(N.y, src), __shfl(N.z, src));
+            const f3 tw = mk(__shfl(wi.x, src), __shfl(wi.y, src), __shfl(wi.z, src));
+            const float tr = __shfl(rough, src), tm = __shfl(metal, src);
+            bf16x8 xb[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            float f[8];
-            guide_feature_block(p01, N, wi, rough, metal, b, (int)h, f);
+            for (int k = 0; k < 4; ++k) {
+                float f[8];
+                guide_feature_block(tp, tn, tw, tr, tm, k, (int)h, f);
 #pragma unroll
-            for (int k = 0; k < 8; ++k) xb[b][k] = (bf16_t)f[k];
+                for (int j = 0; j < 8; ++j) xb[k][j] = (bf16_t)f[j];
+            }
+            f32x16 acc[2];
+            mlp_forward_wave(xb, sW, sB, 64, acc);
+            float o[64];
+            mlp_row_outputs(acc, o);
+            if (h == t) {
+#pragma unroll
+                for (int j = 0; j < G_USED_OUT; ++j) mine[j] = o[j];
+            }
         }
-        f32x16 acc[2];
-        mlp_forward_wave(xb, sW, sB, 64, acc);
-        float o[64];
-        mlp_row_outputs(acc, o);
-        if (ok && h == 0) guide_pack(o, out[pos]);
+        if (ok) guide_pack(mine, out[pos]);
     }
 }
 
@@ -1825,7 +1841,7 @@ void launch_raygen_rays(const PathsDev &p, const float4 *rays, const uint64_t *s
 void launch_guide_infer(const SceneDev &s, const PathsDev &p, const uint32_t *q, const uint32_t *count, uint32_t cap,
                         const bf16_t *W, const float *params, GuidePacked *out, hipStream_t st) {
     if (cap)
-        hipLaunchKernelGGL(k_guide_infer, dim3(std::min(blocks(cap, 128), GUIDE_INFER_BLOCKS)), dim3(256), 0, st, s, p, q,
+        hipLaunchKernelGGL(k_guide_infer, dim3(std::min(blocks(cap, 256), GUIDE_INFER_BLOCKS)), dim3(256), 0, st, s, p, q,
                            count, cap, W, params, out);
 }
 
