@@ -22,6 +22,14 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf16x8 load8(const bf16_t *p) { return *reinterpret_cast<const bf16x8 *>(p); }
 
+// v_permlane32_swap on (x, y): returns x' = (x of lanes 0-31, y of lanes 0-31)
+// and y' = (x of lanes 32-63, y of lanes 32-63), each half in lane order.
+__device__ __forceinline__ void swap_halves(uint32_t &x, uint32_t &y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+    x = r[0];
+    y = r[1];
+}
+
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     const bf16_t x = (bf16_t)a, y = (bf16_t)b;
     return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
@@ -59,14 +67,15 @@ __device__ __forceinline__ void mlp_forward_wave(bf16x8 xb[4], const bf16_t *__r
         for (int s = 0; s < 4; ++s) {
             const int t = s >> 1, q0 = 2 * (s & 1);
             // q0 block (offsets 0..3 of k block s), q0 + 1 block (offsets 4..7)
-            const uint32_t lo0 = pack_bf16x2(acc[t][4 * q0], acc[t][4 * q0 + 1]);
-            const uint32_t lo1 = pack_bf16x2(acc[t][4 * q0 + 2], acc[t][4 * q0 + 3]);
-            const uint32_t hi0 = pack_bf16x2(acc[t][4 * q0 + 4], acc[t][4 * q0 + 5]);
-            const uint32_t hi1 = pack_bf16x2(acc[t][4 * q0 + 6], acc[t][4 * q0 + 7]);
-            const uint32_t keep0 = h ? hi0 : lo0, keep1 = h ? hi1 : lo1;
-            const uint32_t recv0 = __shfl_xor(h ? lo0 : hi0, 32), recv1 = __shfl_xor(h ? lo1 : hi1, 32);
-            const u32x4 v4 = h ? u32x4{recv0, recv1, keep0, keep1} : u32x4{keep0, keep1, recv0, recv1};
-            xb[s] = __builtin_bit_cast(bf16x8, v4);
+            uint32_t lo0 = pack_bf16x2(acc[t][4 * q0], acc[t][4 * q0 + 1]);
+            uint32_t lo1 = pack_bf16x2(acc[t][4 * q0 + 2], acc[t][4 * q0 + 3]);
+            uint32_t hi0 = pack_bf16x2(acc[t][4 * q0 + 4], acc[t][4 * q0 + 5]);
+            uint32_t hi1 = pack_bf16x2(acc[t][4 * q0 + 6], acc[t][4 * q0 + 7]);
+            // lane l < 32 keeps its lo and takes lane l+32's; lane l+32 takes
+            // lane l's hi and keeps its own: one swap of halves per pair
+            swap_halves(lo0, hi0);
+            swap_halves(lo1, hi1);
+            xb[s] = __builtin_bit_cast(bf16x8, u32x4{lo0, lo1, hi0, hi1});
         }
     }
 }
@@ -74,15 +83,16 @@ __device__ __forceinline__ void mlp_forward_wave(bf16x8 xb[4], const bf16_t *__r
 // The row's 64 outputs gathered into lane l%32's o[] (both lanes of the row
 // call it; lane l^32 gets the same array).
 __device__ __forceinline__ void mlp_row_outputs(const f32x16 acc[2], float o[64]) {
-    const uint32_t h = (threadIdx.x & 63u) >> 5;
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-            const float mine = acc[t][v], other = __shfl_xor(mine, 32);
+            // output n is held by lane l < 32, n + 4 by lane l + 32
+            uint32_t a = __float_as_uint(acc[t][v]), b = a;
+            swap_halves(a, b);
             const int n = 32 * t + 8 * (v >> 2) + (v & 3);
-            o[n] = h ? other : mine;
-            o[n + 4] = h ? mine : other;
+            o[n] = __uint_as_float(a);
+            o[n + 4] = __uint_as_float(b);
         }
 }
 

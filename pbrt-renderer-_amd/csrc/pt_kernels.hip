@@ -1557,11 +1557,18 @@ __global__ __launch_bounds__(256) void k_guide_infer(SceneDev s, PathsDev p, con
 #pragma unroll 1
         for (uint32_t t = 0; t < 2; ++t) {
             if (((okm >> (32u * t)) & 0xFFFFFFFFull) == 0) continue; // the same for every lane
-            const int src = (int)(32u * t + (lane & 31u)); // this lane's row of the half
-            const f3 tp = mk(__shfl(p01.x, src), __shfl(p01.y, src), __shfl(p01.z, src));
-            const f3 tn = mk(__shfl(N.x, src), __shfl(N.y, src), __shfl(N.z, src));
-            const f3 tw = mk(__shfl(wi.x, src), __shfl(wi.y, src), __shfl(wi.z, src));
-            const float tr = __shfl(rough, src), tm = __shfl(metal, src);
+            // both lanes of row 32t + l%32 take that row's inputs: a swap of
+            // halves of a value with itself hands lanes 0-31's values to both
+            // halves (first result) and lanes 32-63's (second)
+            auto half = [t](float v) {
+                uint32_t a = __float_as_uint(v), b = a;
+                swap_halves(a, b);
+                return __uint_as_float(t ? b : a);
+            };
+            const f3 tp = mk(half(p01.x), half(p01.y), half(p01.z));
+            const f3 tn = mk(half(N.x), half(N.y), half(N.z));
+            const f3 tw = mk(half(wi.x), half(wi.y), half(wi.z));
+            const float tr = half(rough), tm = half(metal);
             bf16x8 xb[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
