@@ -20,7 +20,7 @@ longer training, larger batches or a wider range for the guide's sample share
 c: 1.87-1.95x), at 1.35-1.40x the time per sample, so 1.26-1.33x lower
 relMSE at equal GPU time.  The time per sample of plain sampling fell 2.5x
 when the queue appends stopped serialising on their counters (1024 spp: 60
--> 24 ms); the network's own cost per guided vertex (inference 6.8 ms, the
+-> 24 ms); the network's own cost per guided vertex (inference 6-7 ms, the
 guided shading variant 2 ms) did not, so the equal-time gain fell from the
 1.67-1.75x it had against the slower plain path.  With its training
 counted, a network trained from its initialisation during the first 256 of
