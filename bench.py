@@ -54,9 +54,10 @@ def log(*a):
 
 
 def host_cpu():
-    """Host cores this process may use, the machine's count and CPU model.  The
-    GPU box grants one GPU's share of its host (OMP_NUM_THREADS, 16 there); the
-    CPU baseline runs on every core of that share."""
+    """Host cores this process may use, the machine's count and CPU model.  A
+    host shared by several GPUs grants each one a share of its cores
+    (OMP_NUM_THREADS, where set); the CPU baseline runs on every core of that
+    share."""
     nproc = os.cpu_count() or 1
     try:
         usable = len(os.sched_getaffinity(0))
