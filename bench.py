@@ -385,6 +385,45 @@ def sun_room_legs(gpu, ng, args):
     return tt
 
 
+def guide_frame(gpu, ng):
+    """Kernel times of one frame of the sun room (128x128, 1024 spp, depth 8,
+    footprint envmap weights, default pool), plain and with a network trained
+    for four guided passes of 64 spp and then frozen: each frame once as a
+    graph-replayed render and once with HIP events around every launch."""
+    from guide_scenes import sun_room
+
+    sc = sun_room(width=128, height=128)
+    cam = sc.camera
+    gpu.set_option("env_sampling", 1)
+    gpu.upload(sc)
+    gpu.set_option("guide_reset", 1)
+    gpu.set_option("guide_train_steps", 32)
+    gpu.set_option("guiding", 1)
+    for k in range(4):
+        gpu.render(ng.make_view(cam, 64, 8, sample_offset=k * 64), seed=5, readback=False)
+    gpu.set_option("guide_train_steps", 0)
+    out = {}
+    for name, guided in (("plain", 0), ("guided", 1)):
+        gpu.set_option("guiding", guided)
+        gpu.reset_film()
+        gpu.render(ng.make_view(cam, 1024, 8, sample_offset=4096), seed=7, readback=False)
+        graph_ms = gpu.stats().ms_total
+        gpu.set_option("timing_mode", 1)
+        gpu.reset_film()
+        gpu.render(ng.make_view(cam, 1024, 8, sample_offset=4096), seed=7, readback=False)
+        s = gpu.stats()
+        gpu.set_option("timing_mode", 0)
+        out[name] = {"graph_ms": round(graph_ms, 2), "extend_ms": round(s.ms_trace_closest, 2),
+                     "shadow_ms": round(s.ms_trace_shadow, 2), "shade_ms": round(s.ms_shade, 2),
+                     "guide_infer_ms": round(s.ms_guide_infer, 2), "timed_total_ms": round(s.ms_total, 2),
+                     "extension_rays": int(s.extension_rays)}
+        log(f"[bench] guide frame {name}: {out[name]}")
+    gpu.set_option("guiding", 0)
+    gpu.set_option("guide_train_steps", 4)
+    gpu.set_option("env_sampling", 0)  # for later uploads
+    return out
+
+
 CONFIG_LEGS = [
     # name, generator config, generator options, spp, max depth, what it is
     ("cornell", 1, {}, 64, 8, "Cornell box, 2 level-4 icospheres, 10,252 triangles, quad light, 256x256, 64 spp, "
@@ -448,7 +487,7 @@ def main():
     ap.add_argument("--chunk-iters", type=int, default=2, help="pool iterations per done-flag check (--no-autotune)")
     ap.add_argument("--traversal", type=int, default=1, help="triangle schedule of the traversal (--no-autotune)")
     ap.add_argument("--trace-waves", type=int, default=0, help="waves-per-SIMD cap of traversal (--no-autotune)")
-    ap.add_argument("--only", choices=["batched", "disney", "configs"], default=None,
+    ap.add_argument("--only", choices=["batched", "disney", "configs", "sun_room", "guide_frame"], default=None,
                     help="run one workload leg only (profiling); the JSON line then carries that leg as value")
     ap.add_argument("--disney-steps", type=int, default=4)
     ap.add_argument("--timing-steps", type=int, default=1,
@@ -477,6 +516,17 @@ def main():
     import nori_gpu as ng
 
     gpu = ng.NoriGPU(0)
+    if args.only == "sun_room":
+        r = sun_room_legs(gpu, ng, args)
+        print(json.dumps({"metric": "time to relMSE<=1e-3, guided vs plain, sun room", "unit": "ms",
+                          "sun_room": r}), flush=True)
+        gpu.close()
+        return
+    if args.only == "guide_frame":
+        print(json.dumps({"metric": "GPU ms per frame, plain vs guided, sun room 128x128 x 1024 spp", "unit": "ms",
+                          "guide_frame": guide_frame(gpu, ng)}), flush=True)
+        gpu.close()
+        return
     legs = {}
     cpu = None
     if args.only in (None, "batched"):
