@@ -322,6 +322,14 @@ template <typename F> int guard(nori_ctx *c, F &&f) {
     }
 }
 
+// Grid of a shadow launch over a pool of n paths: a vertex appends one shadow
+// ray per next-event technique the scene has (area lights, envmap), so the
+// queue holds at most that many per path.
+static uint32_t shadow_grid(const SceneDev &s, uint32_t n) {
+    const uint32_t k = (s.n_emit > 0 ? 1u : 0u) + (s.has_env ? 1u : 0u);
+    return std::max(k, 1u) * n;
+}
+
 // Deferred kernel timing: records an event pair around a launch on the
 // context stream; the pairs are resolved once, after the render synchronises.
 struct Span {
@@ -353,7 +361,8 @@ void run_bounces(nori_ctx *c, FilmFx *film, const ViewTab *views, uint32_t n, ui
                       c->stream);
         launch_shade(c->sdev, c->pd[qi], c->pd[qo], views, cnt_slot(c->cdev.cnt, qi), cnt_slot(c->cdev.cnt, qo), nullptr,
                      nullptr, c->shdev, cnt_slot(c->cdev.cnt, 2), film, n, none, AovDev{nullptr, nullptr}, c->stream);
-        launch_shadow(c->sdev, c->shdev, cnt_slot(c->cdev.cnt, 2), film, 2 * n, nullptr, none, c->work_shadow(), c->stream);
+        launch_shadow(c->sdev, c->shdev, cnt_slot(c->cdev.cnt, 2), film, shadow_grid(c->sdev, n), nullptr, none,
+                      c->work_shadow(), c->stream);
         st.launches_closest++;
         st.launches_shadow++;
         launch_counters(c->cdev, qi, 1, (1u << qi) | 4u, c->stream);
@@ -399,7 +408,8 @@ void pool_iteration(nori_ctx *c, uint32_t qi, FilmFx *film, const ViewTab *views
     }
     {
         Span t(c, 1);
-        launch_shadow(c->sdev, c->shdev, cnt_slot(cnt, 2), film, 2 * n, cs, gsd, c->work_shadow(), c->stream);
+        launch_shadow(c->sdev, c->shdev, cnt_slot(cnt, 2), film, shadow_grid(c->sdev, n), cs, gsd, c->work_shadow(),
+                      c->stream);
     }
     if (gsd.rec) {
         Span t(c, 4);
@@ -490,7 +500,7 @@ int nori_gpu_li_noalloc(nori_ctx *c, const nori_ray *rays, const uint64_t *seeds
             launch_extend(s, l.pd[qi], cnt_slot(l.cnt.cnt, qi), m, nullptr, nullptr, nullptr, 0, nullptr, c->stream);
             launch_shade(s, l.pd[qi], l.pd[qo], l.view, cnt_slot(l.cnt.cnt, qi), cnt_slot(l.cnt.cnt, qo), nullptr, nullptr,
                          l.shd, cnt_slot(l.cnt.cnt, 2), film, m, none, AovDev{nullptr, nullptr}, c->stream);
-            launch_shadow(s, l.shd, cnt_slot(l.cnt.cnt, 2), film, 2 * m, nullptr, none, nullptr, c->stream);
+            launch_shadow(s, l.shd, cnt_slot(l.cnt.cnt, 2), film, shadow_grid(s, m), nullptr, none, nullptr, c->stream);
             launch_counters(l.cnt, qi, 1, (1u << qi) | 4u, c->stream);
         }
         launch_film_resolve(film, (float4 *)rgba4, (size_t)m, c->stream);
