@@ -30,9 +30,12 @@ __device__ __forceinline__ void swap_halves(uint32_t &x, uint32_t &y) {
     y = r[1];
 }
 
+// Two floats rounded to BF16 (nearest even) in one v_cvt_pk_bf16_f32, a in
+// the low half; the same bits as two separate conversions.
 __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
-    const bf16_t x = (bf16_t)a, y = (bf16_t)b;
-    return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){a, b}, bf16x2_t));
 }
 
 // xb: the lane's input fragments (row l%32, k = 16s + 8(l/32) .. +7 for
