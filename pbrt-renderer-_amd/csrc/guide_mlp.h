@@ -38,6 +38,15 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){a, b}, bf16x2_t));
 }
 
+// ReLU of two packed BF16 values as one v_pk_max_i16 against 0: a BF16 with
+// its sign bit set is a negative 16-bit integer and becomes +0, any other is
+// kept, so max(x, 0) of the rounded value, which is the rounding of
+// max(x, 0) (rounding to nearest keeps the sign and is monotonic).
+__device__ __forceinline__ uint32_t relu_bf16x2(uint32_t x) {
+    typedef short i16x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2_t, x), (i16x2_t){0, 0}));
+}
+
 // xb: the lane's input fragments (row l%32, k = 16s + 8(l/32) .. +7 for
 // s = 0..3).  W: BF16 weights [layer][out][in]; bias: layer l's 64 biases at
 // bias + l * bias_stride (G_LAYER_PARAMS in the FP32 parameter block, 64 in a
@@ -62,8 +71,7 @@ __device__ __forceinline__ void mlp_forward_wave(bf16x8 xb[4], const bf16_t *__r
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const float val = acc[t][v] + bl[32 * t + 8 * (v >> 2) + 4 * h + (v & 3)];
-                acc[t][v] = l < G_LAYERS - 1 ? fmaxf(val, 0.0f) : val;
+                acc[t][v] += bl[32 * t + 8 * (v >> 2) + 4 * h + (v & 3)];
             }
         if (l == G_LAYERS - 1) break;
 #pragma unroll
@@ -74,6 +82,10 @@ __device__ __forceinline__ void mlp_forward_wave(bf16x8 xb[4], const bf16_t *__r
             uint32_t lo1 = pack_bf16x2(acc[t][4 * q0 + 2], acc[t][4 * q0 + 3]);
             uint32_t hi0 = pack_bf16x2(acc[t][4 * q0 + 4], acc[t][4 * q0 + 5]);
             uint32_t hi1 = pack_bf16x2(acc[t][4 * q0 + 6], acc[t][4 * q0 + 7]);
+            lo0 = relu_bf16x2(lo0);
+            lo1 = relu_bf16x2(lo1);
+            hi0 = relu_bf16x2(hi0);
+            hi1 = relu_bf16x2(hi1);
             // lane l < 32 keeps its lo and takes lane l+32's; lane l+32 takes
             // lane l's hi and keeps its own: one swap of halves per pair
             swap_halves(lo0, hi0);
