@@ -17,15 +17,16 @@ against a reference of 2 x 16384 plain spp whose own error (a quarter of the
 relMSE between its halves) is subtracted.  Measured on MI355X at 128 x 128
 with this schedule: 1.74-1.86x lower relMSE at equal spp (earlier, with
 longer training, larger batches or a wider range for the guide's sample share
-c: 1.87-1.95x), at 1.35-1.40x the time per sample, so 1.26-1.33x lower
-relMSE at equal GPU time.  The time per sample of plain sampling fell 2.5x
+c: 1.87-1.95x), at 1.30-1.40x the time per sample, so 1.26-1.41x lower
+relMSE at equal GPU time (the upper end with the cheaper BF16 packing and
+ReLU of the inference kernel: 6.1 -> 5.6 ms per frame).  The time per sample of plain sampling fell 2.5x
 when the queue appends stopped serialising on their counters (1024 spp: 60
 -> 24 ms); the network's own cost per guided vertex (inference 6-7 ms, the
 guided shading variant 2 ms) did not, so the equal-time gain fell from the
 1.67-1.75x it had against the slower plain path.  With its training
 counted, a network trained from its initialisation during the first 256 of
-4096 spp gives 1.5-1.7x lower relMSE at equal spp and 0.96-1.11x at equal
-time (six seeds).
+4096 spp gives 1.5-1.7x lower relMSE at equal spp and 0.96-1.13x at equal
+time (eight runs).
 """
 import sys
 
